@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the drop-in block-sparse multiply on one MI355X.
+
+Default workload (the one the headline metric is quoted on): plain FP64 GPU
+multiply at DZP scale — 4096 atoms at 0.05 atoms/A^3 (seeded, minimum
+separation 2 A), 13 support functions per atom (13x13 blocks), result range
+16 A = (A range 8 A) . (B range 16 A)^T in CONQUEST's reduction form
+(m_kern_min), entries N(0,1)*exp(-r/2 A), FP64.
+
+A "step" is one cqmm_mat_mult over device-resident operands (the plan and the
+HBM copies of a, b, c are built before the timed region). value = useful
+GFLOP/s = (sum over products of 2*n_i*n_k*n_j) / wall time per step.
+The roofline object uses the mean kernel time from HIP events on the library
+stream; cpu_baseline times the oracle's restatement of multiply_kernel_ompGemm
+on a bounded sample of halo partitions on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "conquest-release_amd")]
+
+import numpy as np  # noqa: E402
+
+import cqmm  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
+
+WORKLOADS = {
+    # name: natoms, nsf, mult_type, (r_a, r_b, r_c) as mat_mult's a, b, c
+    "cfg1": dict(natoms=512, nsf=4, mult_type=1, ra=8.0, rb=12.0, rc=20.0,
+                 desc="plain FP64, 512 atoms, 4x4 blocks, C(20 A) = A(8 A).B(12 A), full product (m_kern_max)"),
+    "cfg2": dict(natoms=4096, nsf=13, mult_type=2, ra=16.0, rb=16.0, rc=8.0,
+                 desc="plain FP64 drop-in at DZP scale, 4096 atoms, 13x13 blocks, "
+                      "X(16 A) = A(8 A).B(16 A)^T, C truncated to 16 A (m_kern_min)"),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_for(workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(workload, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(desc, a, b, c, target_s):
+    """Oracle ompGemm restatement on a bounded sample of halo partitions."""
+    import oracle
+    provider = oracle.use_numpy_openblas()
+    nparts = desc.ahalo.np_in_halo
+    # Calibrate on one partition, then size the sample to ~target_s seconds.
+    t = oracle.mat_mult_ompgemm(desc, a, b, c, 1, 2)
+    f = oracle.flops(desc, 1, 2)
+    rate = f / max(t, 1e-6)
+    total = oracle.flops(desc)
+    per_part = total / nparts
+    n = int(max(1, min(nparts, target_s * rate / max(per_part, 1.0))))
+    log(f"cpu baseline: {rate / 1e9:.2f} GFLOP/s on calibration, sampling {n}/{nparts} halo partitions")
+    t = oracle.mat_mult_ompgemm(desc, a, b, c, 1, 1 + n)
+    f = oracle.flops(desc, 1, 1 + n)
+    return {
+        "value": round(f / t / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": oracle.max_threads(),
+        "kind": "port",
+        "sample": f"multiply_kernel_ompGemm restatement (OpenMP schedule(runtime), {provider}); "
+                  f"halo partitions 1..{n} of {nparts} ({f / 1e9:.1f} GFLOP, {t:.1f} s, partition loop only)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    t_setup = time.perf_counter()
+    xyz, edge = cqmm.random_structure(w["natoms"], 0.05, 2.0, args.seed)
+    ra, rb, rc = w["ra"], w["rb"], w["rc"]
+    rcover = max(ra + rb, rc)
+    sysm = cqmm.System(xyz, edge, w["nsf"], rcover, cqmm.default_grid(edge, 4.0))
+    ids = {}
+    for r in sorted({ra, rb, rc}):
+        ids[r] = sysm.add_range(r)
+    log(f"system: {w['natoms']} atoms, box {edge:.2f} A, ranges {sorted(ids)} built in {time.perf_counter() - t_setup:.1f} s")
+    desc = sysm.mult_desc(w["mult_type"], ids[ra], ids[rb], ids[rc])
+    # Operands as mat_mult receives them: type 1 reads a, b -> c; type 2 reads c, b -> a.
+    b = sysm.fill(ids[rb], 0, args.seed, 2, (2.0,))
+    if w["mult_type"] == 1:
+        a = sysm.fill(ids[ra], 0, args.seed, 1, (2.0,))
+        c = np.zeros(sysm.length(ids[rc]))
+    else:
+        c = sysm.fill(ids[rc], 0, args.seed, 3, (2.0,))
+        a = np.zeros(sysm.length(ids[ra]))
+    log(f"operands: lena={a.size} lenb={b.size} lenc={c.size} doubles")
+    cqmm.init(0)
+    t0 = time.perf_counter()
+    plan = cqmm.Plan(desc)
+    t_plan = time.perf_counter() - t0
+    info = plan.info()
+    flops = info.flops
+    log(f"plan: {info.n_triplets} products, {flops / 1e12:.3f} TFLOP/step, built in {t_plan:.2f} s")
+    da, db, dc = (cqmm.DeviceArray.from_host(x) for x in (a, b, c))
+    setup_s = time.perf_counter() - t_setup
+    for _ in range(args.warmup):
+        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC)
+    cqmm.synchronize()
+    log(f"warmup {args.warmup} done; timing {args.steps} steps")
+    t0 = time.perf_counter()
+    cqmm.timer_start()
+    for _ in range(args.steps):
+        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC)
+    kern_ms = cqmm.timer_stop()  # waits for the stream
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    kern_avg_ms = kern_ms / args.steps
+    value = flops / (ms_step / 1e3) / 1e9
+    achieved = flops / (kern_avg_ms / 1e3) / 1e12
+    out = (dc if w["mult_type"] == 1 else da).to_host()
+    log(f"step {ms_step:.3f} ms (kernel {kern_avg_ms:.3f} ms), {value:.1f} GFLOP/s; checksum {float(np.abs(out).sum()):.6e}")
+    line = {
+        "metric": "block-sparse mult GFLOP/s",
+        "value": round(value, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random periodic structure; N(0,1)*exp(-r/2) block entries)",
+        "config": {
+            "workload": w["desc"],
+            "natoms": w["natoms"], "nsf": w["nsf"], "mult_type": w["mult_type"],
+            "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
+            "flop_per_step": flops, "output_blocks": int(info.n_out_total),
+            "plan_build_s": round(t_plan, 3), "setup_s": round(setup_s, 2),
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+            "traffic": traffic_for(args.workload),
+            "kernel": "k_block_products_f64",
+            "kernel_avg_ms": round(kern_avg_ms, 4),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(desc, a.copy() if w["mult_type"] == 2 else a, b,
+                                            c if w["mult_type"] == 2 else c.copy(), args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,339 @@
+"""Python host layer over libcqmm.so (ctypes).
+
+It mirrors the Fortran host API the library replaces:
+  * ``System`` is the single-process stand-in for CONQUEST's index set-up
+    (make_cs / matrix_ini / immi, src/cover_module.f90:133-429,
+    src/matrix_elements_module.f90:67-142, src/mult_module.f90:271-1102).
+  * ``Plan`` + ``Plan.mat_mult`` is ``mat_mult(myid,a,lena,b,lenb,c,lenc,a_b_c)``
+    (src/multiply_module.f90:115) for one process, type 1 or 2.
+
+The library is loaded from ``conquest-release_amd/lib/libcqmm.so``; a missing
+library raises instead of falling back to any CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libcqmm.so")
+
+CQMM_DEVICE_PTRS = 1
+CQMM_FP32 = 2
+CQMM_NO_SYNC = 4
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+
+
+class Halo(C.Structure):
+    _fields_ = [
+        ("np_in_halo", C.c_int32), ("ni_in_halo", C.c_int32),
+        ("nh_part", i32p), ("j_beg", i32p), ("lab_hcell", i32p), ("lab_hcover", i32p),
+        ("j_seq", i32p), ("ndimj", i32p), ("i_h2d", i32p), ("i_halo", i32p), ("i_hbeg", i32p),
+        ("ndimi", i32p), ("len_i_halo", C.c_int64), ("len_i_hbeg", C.c_int64),
+    ]
+
+
+class Trans(C.Structure):
+    _fields_ = [("n_hnab", i32p), ("i_beg", i32p), ("i_nd_beg", i32p), ("i_prim", i32p),
+                ("len_i_prim", C.c_int64)]
+
+
+class BIndex(C.Structure):
+    _fields_ = [("n_parts", C.c_int32), ("array_posn", i32p), ("n_atoms", i32p),
+                ("part_nabs", i32p), ("nd_offset", i32p), ("bindex", i32p),
+                ("len_bindex", C.c_int64)]
+
+
+class MultDesc(C.Structure):
+    _fields_ = [
+        ("mult_type", C.c_int32), ("n_prim", C.c_int32), ("ncovery", C.c_int32), ("ncoverz", C.c_int32),
+        ("ahalo", Halo), ("chalo", Halo), ("ltrans", Trans), ("bmat", BIndex),
+        ("i_cc2seq", i32p), ("n_cell_parts", C.c_int32),
+        ("lena", C.c_int64), ("lenb", C.c_int64), ("lenc", C.c_int64),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("n_triplets", C.c_int64), ("n_out_blocks", C.c_int64), ("n_out_total", C.c_int64),
+                ("flops", C.c_double), ("uniform_nsf", C.c_int32), ("mult_type", C.c_int32)]
+
+
+class RangeInfo(C.Structure):
+    _fields_ = [("rcut", C.c_double), ("length", C.c_int64), ("n_blocks", C.c_int64),
+                ("ni_in_halo", C.c_int32), ("np_in_halo", C.c_int32), ("max_nab", C.c_int32),
+                ("fits_int32", C.c_int32)]
+
+
+class SysInfo(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("n_parts", C.c_int32), ("ng_cover", C.c_int32),
+                ("mx_mcover", C.c_int32), ("ncover", C.c_int32 * 3), ("nspanl", C.c_int32 * 3),
+                ("ngrid", C.c_int32 * 3), ("box", C.c_double * 3)]
+
+
+# Symbols declared in include/cqmm.h and include/cqmm_system.h.
+EXPORTS = [
+    "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
+    "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
+    "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_plan_get_info",
+    "cqmm_plan_get_triplets",
+    "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
+    "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
+    "cqmm_sys_fill", "cqmm_random_structure",
+]
+
+_lib = None
+
+
+class CqmmError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcqmm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CqmmError(f"libcqmm.so not found at {LIB_PATH}; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    L.cqmm_last_error.restype = C.c_char_p
+    L.cqmm_random_structure.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_uint64, f64p, f64p]
+    L.cqmm_sys_create.argtypes = [C.c_int32, f64p, f64p, i32p, i32p, C.c_double, C.POINTER(C.c_void_p)]
+    L.cqmm_sys_destroy.argtypes = [C.c_void_p]
+    L.cqmm_sys_add_range.argtypes = [C.c_void_p, C.c_double, i32p]
+    L.cqmm_sys_range_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(RangeInfo)]
+    L.cqmm_sys_get_info.argtypes = [C.c_void_p, C.POINTER(SysInfo)]
+    L.cqmm_sys_mult_desc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(MultDesc)]
+    L.cqmm_sys_blocks.argtypes = [C.c_void_p, C.c_int32, i32p, i32p, i32p, i32p, i64p, i32p, i32p]
+    L.cqmm_sys_primary_order.argtypes = [C.c_void_p, i32p]
+    L.cqmm_sys_fill.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64, f64p, C.c_int32, f64p]
+    L.cqmm_init.argtypes = [C.c_int]
+    L.cqmm_plan_create.argtypes = [C.POINTER(MultDesc), C.POINTER(C.c_void_p)]
+    L.cqmm_plan_destroy.argtypes = [C.c_void_p]
+    L.cqmm_plan_get_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
+    L.cqmm_plan_get_triplets.argtypes = [C.c_void_p, i64p, i64p, i64p]
+    L.cqmm_mat_mult.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                C.c_int64, C.c_uint32]
+    L.cqmm_device_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_int64]
+    L.cqmm_device_free.argtypes = [C.c_void_p]
+    L.cqmm_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.cqmm_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.cqmm_timer_stop.argtypes = [C.POINTER(C.c_float)]
+    _lib = L
+    return L
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = lib().cqmm_last_error().decode(errors="replace")
+        raise CqmmError(f"{what}: status {status}: {msg}")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def random_structure(natoms, density=0.05, dmin=2.0, seed=1):
+    """Seeded uniform positions in a periodic cube (Angstrom); returns (xyz, edge)."""
+    xyz = np.zeros((natoms, 3), dtype=np.float64)
+    edge = C.c_double(0.0)
+    check(lib().cqmm_random_structure(natoms, density, dmin, seed, _p(xyz, C.c_double), C.byref(edge)),
+          "cqmm_random_structure")
+    return xyz, edge.value
+
+
+def default_grid(edge, target=5.0):
+    n = max(1, int(round(edge / target)))
+    return (n, n, n)
+
+
+class System:
+    """Single-process CONQUEST index set built from atom positions."""
+
+    def __init__(self, xyz, box, nsf, rcut_cover, grid=None):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.natoms = xyz.shape[0]
+        box = np.ascontiguousarray(np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)))
+        nsf = np.ascontiguousarray(np.broadcast_to(np.asarray(nsf, dtype=np.int32), (self.natoms,)))
+        if grid is None:
+            grid = default_grid(float(box.min()))
+        g = np.ascontiguousarray(np.asarray(grid, dtype=np.int32))
+        self.xyz, self.box, self.nsf, self.grid = xyz, box, nsf, g
+        self.rcut_cover = float(rcut_cover)
+        h = C.c_void_p()
+        check(lib().cqmm_sys_create(self.natoms, _p(xyz, C.c_double), _p(box, C.c_double), _p(nsf, C.c_int32),
+                                    _p(g, C.c_int32), self.rcut_cover, C.byref(h)), "cqmm_sys_create")
+        self.h = h
+        self.ranges = []
+
+    def close(self):
+        if self.h:
+            lib().cqmm_sys_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_range(self, rcut):
+        rid = C.c_int32()
+        check(lib().cqmm_sys_add_range(self.h, float(rcut), C.byref(rid)), "cqmm_sys_add_range")
+        self.ranges.append(float(rcut))
+        return rid.value
+
+    def range_info(self, rid):
+        info = RangeInfo()
+        check(lib().cqmm_sys_range_info(self.h, rid, C.byref(info)), "cqmm_sys_range_info")
+        return info
+
+    def info(self):
+        info = SysInfo()
+        check(lib().cqmm_sys_get_info(self.h, C.byref(info)), "cqmm_sys_get_info")
+        return info
+
+    def length(self, rid):
+        return self.range_info(rid).length
+
+    def mult_desc(self, mult_type, ra, rb, rc):
+        d = MultDesc()
+        check(lib().cqmm_sys_mult_desc(self.h, mult_type, ra, rb, rc, C.byref(d)), "cqmm_sys_mult_desc")
+        return d
+
+    def blocks(self, rid):
+        """Block list in storage order as a dict of numpy arrays."""
+        n = self.range_info(rid).n_blocks
+        out = dict(prim=np.zeros(n, np.int32), gid_i=np.zeros(n, np.int32), gid_j=np.zeros(n, np.int32),
+                   shift=np.zeros((n, 3), np.int32), offset=np.zeros(n, np.int64),
+                   nd1=np.zeros(n, np.int32), nd2=np.zeros(n, np.int32))
+        check(lib().cqmm_sys_blocks(self.h, rid, _p(out["prim"], C.c_int32), _p(out["gid_i"], C.c_int32),
+                                    _p(out["gid_j"], C.c_int32), _p(out["shift"], C.c_int32),
+                                    _p(out["offset"], C.c_int64), _p(out["nd1"], C.c_int32),
+                                    _p(out["nd2"], C.c_int32)), "cqmm_sys_blocks")
+        return out
+
+    def primary_order(self):
+        out = np.zeros(self.natoms, np.int32)
+        check(lib().cqmm_sys_primary_order(self.h, _p(out, C.c_int32)), "cqmm_sys_primary_order")
+        return out
+
+    def fill(self, rid, kind=0, seed=1, tag=0, params=(2.0,)):
+        v = np.zeros(self.length(rid), np.float64)
+        p = np.ascontiguousarray(np.asarray(params, np.float64))
+        check(lib().cqmm_sys_fill(self.h, rid, kind, seed, tag, _p(p, C.c_double), len(p), _p(v, C.c_double)),
+              "cqmm_sys_fill")
+        return v
+
+
+class DeviceArray:
+    """float64 buffer in HBM allocated through the library (no torch needed)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = C.c_void_p()
+        check(lib().cqmm_device_alloc(C.byref(p), max(self.n, 1) * 8), "cqmm_device_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        d = cls(arr.size)
+        if arr.size:
+            check(lib().cqmm_memcpy_h2d(d.ptr, arr.ctypes.data, arr.size * 8), "cqmm_memcpy_h2d")
+        return d
+
+    def to_host(self, out=None):
+        if out is None:
+            out = np.empty(self.n, np.float64)
+        if self.n:
+            check(lib().cqmm_memcpy_d2h(out.ctypes.data, self.ptr, self.n * 8), "cqmm_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cqmm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib().cqmm_synchronize(), "cqmm_synchronize")
+
+
+def timer_start():
+    check(lib().cqmm_timer_start(), "cqmm_timer_start")
+
+
+def timer_stop():
+    """Milliseconds since timer_start(), measured with HIP events on the library stream."""
+    ms = C.c_float()
+    check(lib().cqmm_timer_stop(C.byref(ms)), "cqmm_timer_stop")
+    return ms.value
+
+
+def init(device=-1):
+    check(lib().cqmm_init(device), "cqmm_init")
+
+
+class Plan:
+    """A triplet plan for one product (one CONQUEST mult(x) entry)."""
+
+    def __init__(self, desc: MultDesc):
+        init()
+        self.desc = desc
+        h = C.c_void_p()
+        check(lib().cqmm_plan_create(C.byref(desc), C.byref(h)), "cqmm_plan_create")
+        self.h = h
+        self.mult_type = desc.mult_type
+        self.lena, self.lenb, self.lenc = desc.lena, desc.lenb, desc.lenc
+
+    def close(self):
+        if self.h:
+            lib().cqmm_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = PlanInfo()
+        check(lib().cqmm_plan_get_info(self.h, C.byref(i)), "cqmm_plan_get_info")
+        return i
+
+    def triplets(self):
+        n = self.info().n_triplets
+        o = np.zeros(n, np.int64)
+        p = np.zeros(n, np.int64)
+        q = np.zeros(n, np.int64)
+        check(lib().cqmm_plan_get_triplets(self.h, _p(o, C.c_int64), _p(p, C.c_int64), _p(q, C.c_int64)),
+              "cqmm_plan_get_triplets")
+        return o, p, q
+
+    def mat_mult(self, a, b, c, flags=0):
+        """Host arrays (numpy float64). Type 1 overwrites c; type 2 overwrites a."""
+        for name, arr in (("a", a), ("b", b), ("c", c)):
+            if arr.dtype != np.float64 or not arr.flags.c_contiguous:
+                raise CqmmError(f"mat_mult: {name} must be a contiguous float64 array")
+        check(lib().cqmm_mat_mult(self.h, a.ctypes.data, a.size, b.ctypes.data, b.size, c.ctypes.data, c.size,
+                                  flags), "cqmm_mat_mult")
+
+    def mat_mult_dev(self, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags=CQMM_DEVICE_PTRS):
+        """Device pointers (e.g. torch tensors' data_ptr())."""
+        check(lib().cqmm_mat_mult(self.h, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags | CQMM_DEVICE_PTRS),
+              "cqmm_mat_mult")

@@ -1,0 +1,661 @@
+// Device triplet builder: turns CONQUEST's halo / cover-set / B-partition index
+// arrays (one process) into an explicit list of contributing (i,k,j) block
+// products, grouped by output block in the reference's summation order.
+//
+// Reference walk reproduced (src/multiply_kernel_default.f90:169-221 for
+// m_kern_max, :362-409 for m_kern_min, inside the partition loop of mat_mult,
+// src/multiply_module.f90:217-307):
+//   for kpart in A-halo partitions, k in kpart:            (halo order)
+//     k_in_part = ahalo%j_seq(k); B row of k from the B partition
+//       ipart = parts%i_cc2seq(ahalo%lab_hcell(kpart))     (prefetch, :568-623)
+//     jbnab2ch(j) = chalo%i_halo(chalo%i_hbeg(ibpart(j)+k_off)+ibseq(j)-1)
+//       with ibpart from npxyz (end_part_comms, src/matrix_comms_module.f90:67-103)
+//       and k_off = ahalo%lab_hcover(kpart)
+//     for i in ltrans%i_prim(k):                            (ascending primary atoms)
+//       for j in B row of k:
+//         if jbnab2ch(j) /= 0 and chalo%i_h2d(i, jbnab2ch(j)) /= 0: product (i,k,j)
+// Type 1 output block is C(i,j) = chalo%i_h2d(i,j); its products are summed in
+// k order, so the list is stably sorted by that block. Type 2 output block is
+// A(i,k) = ahalo%i_h2d(i,k); its products (j order) are already contiguous.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "../../include/cqmm.h"
+#include "cq_common.h"
+#include "cq_device.h"
+
+namespace cq {
+
+int DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 8;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return hip_check(e, "hipMalloc (plan buffer)");
+  }
+  bytes = n;
+  return CQMM_OK;
+}
+
+}  // namespace cq
+
+struct cqmm_plan {
+  cq::TripletPlan tp;
+  int64_t lena = 0, lenb = 0, lenc = 0;
+  // Staging buffers for host-pointer calls (allocated on first use).
+  cq::DevBuf sa, sb, sc;
+};
+
+namespace {
+
+using cq::DevBuf;
+
+template <class T>
+int upload(DevBuf &d, const T *h, size_t n, hipStream_t s) {
+  int st = d.alloc(n * sizeof(T));
+  if (st) return st;
+  if (n == 0) return CQMM_OK;
+  return cq::hip_check(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpy (plan upload)");
+}
+
+// Error record written by the builder kernels: first failing code wins.
+struct DevErr {
+  int code;
+  int a, b;
+  long long c;
+};
+
+enum : int {
+  E_NONE = 0,
+  E_JPART = 1,     // ibpart + k_off outside the cover set
+  E_IHALO = 2,     // i_hbeg(jpart)+jseq-1 outside i_halo
+  E_JHALO = 3,     // C-halo number out of range
+  E_BRANGE = 4,    // B block past lenb
+  E_PRANGE = 5,    // left operand block past its length
+  E_BIDX = 6,      // B index slice outside bindex
+};
+
+__device__ inline void dev_fail(DevErr *err, int code, int a, int b, long long c) {
+  if (atomicCAS(&err->code, 0, code) == 0) {
+    err->a = a;
+    err->b = b;
+    err->c = c;
+  }
+}
+
+// Per halo atom k (host-computed): B-row location and labels.
+struct KMeta {
+  int32_t k_off;       // ahalo%lab_hcover(kpart)
+  int32_t nbnab;       // B neighbours of k
+  int32_t nd3;         // ahalo%ndimj(k)
+  int32_t kpart;
+  int64_t idx_seq;     // 0-based position of k's first i_seq in bindex
+  int64_t idx_pxyz;    // 0-based position of k's first npxyz triple
+  int64_t idx_ndim;    // 0-based position of k's first ndimj
+  int64_t bdata;       // 0-based offset of k's first B block in b
+  int64_t jrow;        // start of k's entries in the per-(k,j) arrays
+};
+
+// jbnab2ch and the B-block offsets of every (k, j): one thread per halo atom k.
+__global__ void k_jmap(const KMeta *__restrict__ km, int32_t ni_a, const int32_t *__restrict__ bindex,
+                       int64_t len_bindex, const int32_t *__restrict__ c_i_halo, int64_t len_i_halo,
+                       const int32_t *__restrict__ c_i_hbeg, int64_t len_i_hbeg, int32_t ni_c, int32_t ncyz,
+                       int32_t ncz, int64_t lenb, int32_t *__restrict__ jh_out, int32_t *__restrict__ nd2_out,
+                       int64_t *__restrict__ boff_out, DevErr *err) {
+  const int32_t kh = blockIdx.x * blockDim.x + threadIdx.x;
+  if (kh >= ni_a) return;
+  const KMeta m = km[kh];
+  int64_t boff = m.bdata;
+  for (int32_t j = 0; j < m.nbnab; ++j) {
+    const int64_t is = m.idx_seq + j, ip = m.idx_pxyz + 3 * (int64_t)j, id = m.idx_ndim + j;
+    if (is >= len_bindex || ip + 2 >= len_bindex || id >= len_bindex) {
+      dev_fail(err, E_BIDX, kh, j, is);
+      return;
+    }
+    const int32_t jseq = bindex[is];
+    const int32_t ibpart = bindex[ip] * ncyz + bindex[ip + 1] * ncz + bindex[ip + 2];
+    const int32_t jpart = ibpart + m.k_off;
+    const int32_t nd2 = bindex[id];
+    int32_t jh = 0;
+    if (jpart < 1 || jpart > len_i_hbeg) {
+      dev_fail(err, E_JPART, kh, j, jpart);
+      return;
+    }
+    const int64_t ih = (int64_t)c_i_hbeg[jpart - 1] + jseq - 1;
+    if (ih < 1 || ih > len_i_halo) {
+      dev_fail(err, E_IHALO, kh, j, ih);
+      return;
+    }
+    jh = c_i_halo[ih - 1];
+    if (jh < 0 || jh > ni_c) {
+      dev_fail(err, E_JHALO, kh, j, jh);
+      return;
+    }
+    if (boff + (int64_t)m.nd3 * nd2 > lenb) {
+      dev_fail(err, E_BRANGE, kh, j, boff);
+      return;
+    }
+    jh_out[m.jrow + j] = jh;
+    nd2_out[m.jrow + j] = nd2;
+    boff_out[m.jrow + j] = boff;
+    boff += (int64_t)m.nd3 * nd2;
+  }
+}
+
+// One wavefront per (k, i) item: count contributing j (and the item's flops).
+// When `fill` is set, write the products at the item's offset in j order.
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_items(
+    const KMeta *__restrict__ km, const int32_t *__restrict__ item_k, const int32_t *__restrict__ i_prim,
+    int64_t n_items, const int32_t *__restrict__ jh_arr, const int32_t *__restrict__ nd2_arr,
+    const int64_t *__restrict__ boff_arr, const int32_t *__restrict__ c_i_h2d, int32_t ni_c,
+    const int32_t *__restrict__ a_i_h2d, int32_t ni_a, const int32_t *__restrict__ ndimi, int32_t mult_type,
+    int64_t len_p, int32_t *__restrict__ counts, int64_t *__restrict__ flops,
+    const int64_t *__restrict__ item_off, int32_t *__restrict__ key_out, int64_t *__restrict__ p_out,
+    int64_t *__restrict__ q_out, uint8_t *__restrict__ k_out, DevErr *err) {
+  const int lane = threadIdx.x & 63;
+  const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (item >= n_items) return;
+  const int32_t kh = item_k[item];
+  const KMeta m = km[kh];
+  const int32_t i = i_prim[item];  // 1-based primary atom
+  const int32_t nd1 = ndimi[i - 1];
+  const int64_t crow = (int64_t)(i - 1) * ni_c;
+  int64_t base = FILL ? item_off[item] : 0;
+  int32_t cnt = 0;
+  long long fl = 0;
+  // Type 1: left operand is A(i,k), the same for every j of the item.
+  int64_t pa = 0;
+  if (mult_type == 1) pa = (int64_t)a_i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1;
+  for (int32_t j0 = 0; j0 < m.nbnab; j0 += 64) {
+    const int32_t j = j0 + lane;
+    bool valid = false;
+    int32_t jh = 0, pos = 0, nd2 = 0;
+    int64_t ncbeg = 0;
+    if (j < m.nbnab) {
+      jh = jh_arr[m.jrow + j];
+      if (jh != 0) {
+        pos = (int32_t)(crow + jh - 1);
+        ncbeg = c_i_h2d[crow + jh - 1];
+        valid = ncbeg != 0;
+        nd2 = nd2_arr[m.jrow + j];
+      }
+    }
+    if (!FILL && valid && mult_type == 2 && ncbeg - 1 + (int64_t)nd1 * nd2 > len_p)
+      dev_fail(err, E_PRANGE, kh, j, ncbeg);
+    const unsigned long long mask = __ballot(valid);
+    if (FILL && valid) {
+      const int32_t before = __popcll(mask & ((1ull << lane) - 1ull));
+      const int64_t t = base + cnt + before;
+      if (mult_type == 1) {
+        key_out[t] = pos;
+        p_out[t] = pa;
+        if (k_out) k_out[t] = (uint8_t)m.nd3;
+      } else {
+        p_out[t] = ncbeg - 1;
+        if (k_out) k_out[t] = (uint8_t)nd2;
+      }
+      q_out[t] = boff_arr[m.jrow + j];
+    }
+    if (!FILL && valid) fl += 2ll * nd1 * nd2 * m.nd3;
+    cnt += __popcll(mask);
+  }
+  if (!FILL) {
+    // Wave-reduce the flops.
+    for (int off = 32; off > 0; off >>= 1) fl += __shfl_down(fl, off, 64);
+    if (lane == 0) {
+      counts[item] = cnt;
+      flops[item] = fl;
+    }
+  }
+}
+
+// seg_beg[b] = first sorted triplet whose key is >= key of output block b.
+__global__ void k_lower_bound(const int32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ qk,
+                              int64_t nq, int64_t *__restrict__ out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nq) return;
+  if (b == nq) {
+    out[b] = n;
+    return;
+  }
+  const int32_t v = qk[b];
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  out[b] = lo;
+}
+
+template <class T>
+__global__ void k_gather(const int64_t *__restrict__ src, const int32_t *__restrict__ perm, int64_t n,
+                         T *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (T)src[perm[t]];
+}
+
+__global__ void k_gather_u8(const uint8_t *__restrict__ src, const int32_t *__restrict__ perm, int64_t n,
+                            uint8_t *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[perm[t]];
+}
+
+template <class T>
+__global__ void k_narrow(const int64_t *__restrict__ src, int64_t n, T *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (T)src[t];
+}
+
+__global__ void k_iota(int32_t *__restrict__ v, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = (int32_t)t;
+}
+
+inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+const char *err_name(int code) {
+  switch (code) {
+    case E_JPART: return "partition label ibpart+k_off outside the cover set (cover radius below r(A halo)+r(B)?)";
+    case E_IHALO: return "i_hbeg(jpart)+jseq-1 outside i_halo";
+    case E_JHALO: return "C-halo number out of range";
+    case E_BRANGE: return "B block beyond lenb";
+    case E_PRANGE: return "left-operand block beyond its length";
+    case E_BIDX: return "B index slice outside bindex";
+    default: return "unknown";
+  }
+}
+
+int check_halo(const cqmm_halo &h, int32_t n_prim, const char *name) {
+  if (h.np_in_halo <= 0 || h.ni_in_halo <= 0)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s is empty", name);
+  if (!h.nh_part || !h.j_beg || !h.lab_hcell || !h.lab_hcover || !h.j_seq || !h.ndimj || !h.i_h2d ||
+      !h.i_halo || !h.i_hbeg || !h.ndimi)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s has a null array", name);
+  if (h.len_i_halo <= 0 || h.len_i_hbeg <= 0)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s has empty i_halo/i_hbeg", name);
+  int64_t acc = 1;
+  for (int32_t p = 0; p < h.np_in_halo; ++p) {
+    if (h.j_beg[p] != acc || h.nh_part[p] <= 0)
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s j_beg/nh_part inconsistent at partition %d", name, p + 1);
+    acc += h.nh_part[p];
+  }
+  if (acc - 1 != h.ni_in_halo)
+    return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s nh_part sums to %lld, ni_in_halo is %d", name,
+                          (long long)(acc - 1), h.ni_in_halo);
+  for (int32_t p = 0; p < n_prim; ++p)
+    if (h.ndimi[p] <= 0 || h.ndimi[p] > 16)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s ndimi(%d)=%d outside 1..16", name, p + 1, h.ndimi[p]);
+  for (int32_t k = 0; k < h.ni_in_halo; ++k)
+    if (h.ndimj[k] <= 0 || h.ndimj[k] > 16)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s ndimj(%d)=%d outside 1..16", name, k + 1, h.ndimj[k]);
+  return CQMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
+  if (!d || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: null argument");
+  *out = nullptr;
+  int st = cq::ensure_init();
+  if (st) return st;
+  hipStream_t s = cq::runtime().stream;
+  if (d->mult_type != 1 && d->mult_type != 2)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: mult_type %d (must be 1 or 2)", d->mult_type);
+  if (d->n_prim <= 0 || d->ncovery <= 0 || d->ncoverz <= 0 || d->lena <= 0 || d->lenb <= 0 || d->lenc <= 0)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: non-positive size in descriptor");
+  if ((st = check_halo(d->ahalo, d->n_prim, "ahalo"))) return st;
+  if ((st = check_halo(d->chalo, d->n_prim, "chalo"))) return st;
+  const cqmm_halo &A = d->ahalo, &C = d->chalo;
+  const cqmm_trans &T = d->ltrans;
+  const cqmm_bindex &B = d->bmat;
+  if (!T.n_hnab || !T.i_beg || !T.i_prim || T.len_i_prim <= 0)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: ltrans incomplete");
+  if (!B.array_posn || !B.n_atoms || !B.part_nabs || !B.nd_offset || !B.bindex || B.n_parts <= 0 || B.len_bindex <= 0)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: bmat incomplete");
+  if (!d->i_cc2seq || d->n_cell_parts <= 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: i_cc2seq missing");
+  const int32_t ni_a = A.ni_in_halo, ni_c = C.ni_in_halo;
+  if ((int64_t)d->n_prim * ni_a > INT32_MAX || (int64_t)d->n_prim * ni_c > INT32_MAX)
+    return cq::set_error(CQMM_ERR_RANGE, "cqmm_plan_create: n_prim*ni_in_halo exceeds int32");
+
+  // ---- per halo atom k: B-row location (prefetch + the k loop header) ----
+  std::vector<KMeta> km(ni_a);
+  std::vector<int32_t> item_k;
+  int64_t jrow = 0, n_items = 0;
+  for (int32_t kp = 0; kp < A.np_in_halo; ++kp) {
+    const int32_t ind_part = A.lab_hcell[kp];
+    if (ind_part < 1 || ind_part > d->n_cell_parts)
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: lab_hcell(%d)=%d", kp + 1, ind_part);
+    const int32_t ipart = d->i_cc2seq[ind_part - 1];
+    if (ipart < 1 || ipart > B.n_parts)
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: i_cc2seq(%d)=%d", ind_part, ipart);
+    const int32_t na = B.n_atoms[ipart - 1], nabs = B.part_nabs[ipart - 1];
+    const int64_t size1 = na > 0 ? na : 1, size2 = nabs > 0 ? nabs : 1;
+    const int64_t base = (int64_t)B.array_posn[ipart - 1] - 1;
+    if (base < 0 || base + 3 * size1 + 5 * size2 > B.len_bindex)
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: B partition %d slice outside bindex", ipart);
+    const int32_t first_nd_acc = B.bindex[base + 2 * size1];  // i_nd_acc(1) of the partition
+    for (int32_t kk = 0; kk < A.nh_part[kp]; ++kk) {
+      const int32_t kh = A.j_beg[kp] - 1 + kk;
+      const int32_t kin = A.j_seq[kh];
+      if (kin < 1 || kin > na)
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: j_seq(%d)=%d outside partition of %d atoms", kh + 1, kin, na);
+      KMeta &m = km[kh];
+      m.kpart = kp + 1;
+      m.k_off = A.lab_hcover[kp];
+      m.nbnab = B.bindex[base + (kin - 1)];
+      const int32_t ibaddr = B.bindex[base + size1 + (kin - 1)];
+      const int32_t ibndacc = B.bindex[base + 2 * size1 + (kin - 1)];
+      if (m.nbnab < 0 || ibaddr < 1 || ibaddr - 1 + m.nbnab > size2 || ibndacc < 1)
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: B row of halo atom %d inconsistent", kh + 1);
+      m.nd3 = A.ndimj[kh];
+      m.idx_seq = base + 3 * size1 + (ibaddr - 1);
+      m.idx_pxyz = base + 3 * size1 + size2 + 3 * (int64_t)(ibaddr - 1);
+      m.idx_ndim = base + 3 * size1 + 4 * size2 + (ibaddr - 1);
+      m.bdata = (int64_t)B.nd_offset[ipart - 1] + first_nd_acc - 1 + ibndacc - 1;
+      m.jrow = jrow;
+      jrow += m.nbnab;
+    }
+  }
+  // Items (k, i) in the reference loop order = order of ltrans%i_prim.
+  for (int32_t kh = 0; kh < ni_a; ++kh) {
+    const int32_t nh = T.n_hnab[kh], ib = T.i_beg[kh];
+    if (nh < 0 || ib < 1 || (int64_t)ib - 1 + nh > T.len_i_prim)
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: ltrans of halo atom %d outside i_prim", kh + 1);
+    if (ib - 1 != n_items)
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: ltrans i_beg(%d) not cumulative", kh + 1);
+    for (int32_t t = 0; t < nh; ++t) {
+      const int32_t i = T.i_prim[ib - 1 + t];
+      if (i < 1 || i > d->n_prim) return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: i_prim(%d)=%d", ib + t, i);
+      const int64_t aoff = A.i_h2d[(int64_t)(i - 1) * ni_a + kh];
+      if (aoff < 1 || aoff - 1 + (int64_t)A.ndimi[i - 1] * A.ndimj[kh] > d->lena)
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: A block (%d,%d) offset %lld invalid", i, kh + 1, (long long)aoff);
+      item_k.push_back(kh);
+    }
+    n_items += nh;
+  }
+  if (n_items == 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: no (k,i) pairs");
+
+  // Uniform support-function count (fast path of the product kernel).
+  int uniform = A.ndimi[0];
+  for (int32_t p = 0; p < d->n_prim && uniform; ++p)
+    if (A.ndimi[p] != uniform || C.ndimi[p] != uniform) uniform = 0;
+  for (int32_t k = 0; k < ni_a && uniform; ++k) if (A.ndimj[k] != uniform) uniform = 0;
+  for (int32_t k = 0; k < ni_c && uniform; ++k) if (C.ndimj[k] != uniform) uniform = 0;
+  // Only these sizes have compile-time kernel instances; others use per-block sizes.
+  if (uniform != 4 && uniform != 9 && uniform != 13 && uniform != 16) uniform = 0;
+
+  // ---- device work ----
+  DevBuf d_km, d_bindex, d_ci_halo, d_ci_hbeg, d_ci_h2d, d_ai_h2d, d_iprim, d_itemk, d_ndimi, d_err;
+  if ((st = upload(d_km, km.data(), km.size(), s))) return st;
+  if ((st = upload(d_bindex, B.bindex, (size_t)B.len_bindex, s))) return st;
+  if ((st = upload(d_ci_halo, C.i_halo, (size_t)C.len_i_halo, s))) return st;
+  if ((st = upload(d_ci_hbeg, C.i_hbeg, (size_t)C.len_i_hbeg, s))) return st;
+  if ((st = upload(d_ci_h2d, C.i_h2d, (size_t)d->n_prim * ni_c, s))) return st;
+  if ((st = upload(d_ai_h2d, A.i_h2d, (size_t)d->n_prim * ni_a, s))) return st;
+  if ((st = upload(d_iprim, T.i_prim, (size_t)n_items, s))) return st;
+  if ((st = upload(d_itemk, item_k.data(), item_k.size(), s))) return st;
+  if ((st = upload(d_ndimi, A.ndimi, (size_t)d->n_prim, s))) return st;
+  DevErr herr = {0, 0, 0, 0};
+  if ((st = upload(d_err, &herr, 1, s))) return st;
+
+  DevBuf d_jh, d_nd2, d_boff;
+  if ((st = d_jh.alloc(std::max<int64_t>(jrow, 1) * 4))) return st;
+  if ((st = d_nd2.alloc(std::max<int64_t>(jrow, 1) * 4))) return st;
+  if ((st = d_boff.alloc(std::max<int64_t>(jrow, 1) * 8))) return st;
+  const int32_t ncyz = d->ncovery * d->ncoverz;
+  hipLaunchKernelGGL(k_jmap, dim3(grid1(ni_a, 128)), dim3(128), 0, s, d_km.as<KMeta>(), ni_a,
+                     d_bindex.as<int32_t>(), B.len_bindex, d_ci_halo.as<int32_t>(), C.len_i_halo,
+                     d_ci_hbeg.as<int32_t>(), C.len_i_hbeg, ni_c, ncyz, d->ncoverz, d->lenb, d_jh.as<int32_t>(),
+                     d_nd2.as<int32_t>(), d_boff.as<int64_t>(), d_err.as<DevErr>());
+  const int64_t len_p = d->mult_type == 1 ? d->lena : d->lenc;
+  DevBuf d_counts, d_flops;
+  if ((st = d_counts.alloc(n_items * 4))) return st;
+  if ((st = d_flops.alloc(n_items * 8))) return st;
+  const unsigned gi = grid1(n_items, 4);
+  hipLaunchKernelGGL(k_items<false>, dim3(gi), dim3(256), 0, s, d_km.as<KMeta>(), d_itemk.as<int32_t>(),
+                     d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
+                     d_ci_h2d.as<int32_t>(), ni_c, d_ai_h2d.as<int32_t>(), ni_a, d_ndimi.as<int32_t>(),
+                     d->mult_type, len_p, d_counts.as<int32_t>(), d_flops.as<int64_t>(), nullptr, nullptr, nullptr,
+                     nullptr, nullptr, d_err.as<DevErr>());
+  if ((st = cq::hip_check(hipGetLastError(), "builder kernel launch"))) return st;
+  std::vector<int32_t> counts(n_items);
+  std::vector<int64_t> iflops(n_items);
+  if ((st = cq::hip_check(hipMemcpyAsync(counts.data(), d_counts.p, n_items * 4, hipMemcpyDeviceToHost, s), "D2H counts"))) return st;
+  if ((st = cq::hip_check(hipMemcpyAsync(iflops.data(), d_flops.p, n_items * 8, hipMemcpyDeviceToHost, s), "D2H flops"))) return st;
+  if ((st = cq::hip_check(hipMemcpyAsync(&herr, d_err.p, sizeof(herr), hipMemcpyDeviceToHost, s), "D2H err"))) return st;
+  if ((st = cq::hip_check(hipStreamSynchronize(s), "triplet count"))) return st;
+  if (herr.code != E_NONE)
+    return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s (halo atom %d, B neighbour %d, value %lld)",
+                          err_name(herr.code), herr.a + 1, herr.b + 1, herr.c);
+
+  std::vector<int64_t> item_off(n_items + 1, 0);
+  double flops = 0.0;
+  for (int64_t t = 0; t < n_items; ++t) {
+    item_off[t + 1] = item_off[t] + counts[t];
+    flops += (double)iflops[t];
+  }
+  const int64_t n_trip = item_off[n_items];
+  std::unique_ptr<cqmm_plan> plan(new cqmm_plan());
+  cq::TripletPlan &tp = plan->tp;
+  tp.mult_type = d->mult_type;
+  tp.q_trans = d->mult_type == 2;
+  tp.uniform = uniform;
+  tp.n_trip = n_trip;
+  tp.flops = flops;
+  tp.len_out = d->mult_type == 1 ? d->lenc : d->lena;
+  tp.len_p = len_p;
+  tp.len_q = d->lenb;
+  tp.off64 = (tp.len_p > INT32_MAX || tp.len_q > INT32_MAX);
+  plan->lena = d->lena;
+  plan->lenb = d->lenb;
+  plan->lenc = d->lenc;
+  if (n_trip > INT32_MAX) return cq::set_error(CQMM_ERR_RANGE, "cqmm_plan_create: more than 2^31 block products");
+
+  DevBuf d_itemoff, d_key, d_p, d_q, d_k;
+  if ((st = upload(d_itemoff, item_off.data(), item_off.size(), s))) return st;
+  const int64_t nt1 = std::max<int64_t>(n_trip, 1);
+  if ((st = d_p.alloc(nt1 * 8))) return st;
+  if ((st = d_q.alloc(nt1 * 8))) return st;
+  if (d->mult_type == 1 && (st = d_key.alloc(nt1 * 4))) return st;
+  if (!uniform && (st = d_k.alloc(nt1))) return st;
+  hipLaunchKernelGGL(k_items<true>, dim3(gi), dim3(256), 0, s, d_km.as<KMeta>(), d_itemk.as<int32_t>(),
+                     d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
+                     d_ci_h2d.as<int32_t>(), ni_c, d_ai_h2d.as<int32_t>(), ni_a, d_ndimi.as<int32_t>(),
+                     d->mult_type, len_p, nullptr, nullptr, d_itemoff.as<int64_t>(), d_key.as<int32_t>(),
+                     d_p.as<int64_t>(), d_q.as<int64_t>(), uniform ? nullptr : d_k.as<uint8_t>(), d_err.as<DevErr>());
+  if ((st = cq::hip_check(hipGetLastError(), "builder fill launch"))) return st;
+
+  const size_t osz = tp.off64 ? 8 : 4;
+  if ((st = tp.tp.alloc(nt1 * osz))) return st;
+  if ((st = tp.tq.alloc(nt1 * osz))) return st;
+  if (d->mult_type == 2) {
+    // Segments are the items themselves: every block of the result, once.
+    tp.n_seg = n_items;
+    std::vector<int64_t> seg_out(n_items);
+    std::vector<uint8_t> seg_mn(uniform ? 0 : 2 * n_items);
+    for (int64_t t = 0; t < n_items; ++t) {
+      const int32_t kh = item_k[t], i = T.i_prim[t];
+      seg_out[t] = (int64_t)A.i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1;
+      if (!uniform) {
+        seg_mn[2 * t] = (uint8_t)A.ndimi[i - 1];
+        seg_mn[2 * t + 1] = (uint8_t)A.ndimj[kh];
+      }
+    }
+    if ((st = upload(tp.seg_beg, item_off.data(), item_off.size(), s))) return st;
+    if ((st = upload(tp.seg_out, seg_out.data(), seg_out.size(), s))) return st;
+    if (!uniform && (st = upload(tp.seg_mn, seg_mn.data(), seg_mn.size(), s))) return st;
+    if (tp.off64) {
+      if ((st = cq::hip_check(hipMemcpyAsync(tp.tp.p, d_p.p, n_trip * 8, hipMemcpyDeviceToDevice, s), "copy p"))) return st;
+      if ((st = cq::hip_check(hipMemcpyAsync(tp.tq.p, d_q.p, n_trip * 8, hipMemcpyDeviceToDevice, s), "copy q"))) return st;
+    } else if (n_trip > 0) {
+      hipLaunchKernelGGL(k_narrow<int32_t>, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_p.as<int64_t>(), n_trip, tp.tp.as<int32_t>());
+      hipLaunchKernelGGL(k_narrow<int32_t>, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_q.as<int64_t>(), n_trip, tp.tq.as<int32_t>());
+    }
+    if (!uniform) tp.tk = std::move(d_k);
+  } else {
+    // Output blocks of C in storage order: nonzero chalo%i_h2d in row-major order.
+    std::vector<int32_t> qkey;
+    std::vector<int64_t> seg_out;
+    std::vector<uint8_t> seg_mn;
+    const int64_t nc_all = (int64_t)d->n_prim * ni_c;
+    for (int64_t pos = 0; pos < nc_all; ++pos) {
+      const int32_t off = C.i_h2d[pos];
+      if (off == 0) continue;
+      const int32_t i = (int32_t)(pos / ni_c), jh = (int32_t)(pos % ni_c);
+      if (off < 1 || (int64_t)off - 1 + (int64_t)C.ndimi[i] * C.ndimj[jh] > d->lenc)
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: C block (%d,%d) offset %d invalid", i + 1, jh + 1, off);
+      qkey.push_back((int32_t)pos);
+      seg_out.push_back((int64_t)off - 1);
+      if (!uniform) {
+        seg_mn.push_back((uint8_t)C.ndimi[i]);
+        seg_mn.push_back((uint8_t)C.ndimj[jh]);
+      }
+    }
+    tp.n_seg = (int64_t)qkey.size();
+    if ((st = upload(tp.seg_out, seg_out.data(), seg_out.size(), s))) return st;
+    if (!uniform && (st = upload(tp.seg_mn, seg_mn.data(), seg_mn.size(), s))) return st;
+    // Stable radix sort of the products by output block keeps the k order.
+    DevBuf d_idx, d_key2, d_idx2, d_tmp, d_qkey;
+    if ((st = d_idx.alloc(nt1 * 4))) return st;
+    if ((st = d_key2.alloc(nt1 * 4))) return st;
+    if ((st = d_idx2.alloc(nt1 * 4))) return st;
+    if (n_trip > 0) hipLaunchKernelGGL(k_iota, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_idx.as<int32_t>(), n_trip);
+    unsigned end_bit = 1;
+    while (end_bit < 31 && ((int64_t)1 << end_bit) < nc_all) ++end_bit;
+    size_t tmp_bytes = 0;
+    if (n_trip > 0) {
+      hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_key.as<int32_t>(), d_key2.as<int32_t>(),
+                                               d_idx.as<int32_t>(), d_idx2.as<int32_t>(), (size_t)n_trip, 0, end_bit, s);
+      if ((st = cq::hip_check(e, "radix_sort_pairs (size)"))) return st;
+      if ((st = d_tmp.alloc(tmp_bytes))) return st;
+      e = rocprim::radix_sort_pairs(d_tmp.p, tmp_bytes, d_key.as<int32_t>(), d_key2.as<int32_t>(), d_idx.as<int32_t>(),
+                                    d_idx2.as<int32_t>(), (size_t)n_trip, 0, end_bit, s);
+      if ((st = cq::hip_check(e, "radix_sort_pairs"))) return st;
+      if (tp.off64) {
+        hipLaunchKernelGGL(k_gather<int64_t>, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_p.as<int64_t>(), d_idx2.as<int32_t>(), n_trip, tp.tp.as<int64_t>());
+        hipLaunchKernelGGL(k_gather<int64_t>, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_q.as<int64_t>(), d_idx2.as<int32_t>(), n_trip, tp.tq.as<int64_t>());
+      } else {
+        hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_p.as<int64_t>(), d_idx2.as<int32_t>(), n_trip, tp.tp.as<int32_t>());
+        hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_q.as<int64_t>(), d_idx2.as<int32_t>(), n_trip, tp.tq.as<int32_t>());
+      }
+      if (!uniform) {
+        if ((st = tp.tk.alloc(nt1))) return st;
+        hipLaunchKernelGGL(k_gather_u8, dim3(grid1(n_trip, 256)), dim3(256), 0, s, d_k.as<uint8_t>(), d_idx2.as<int32_t>(), n_trip, tp.tk.as<uint8_t>());
+      }
+    }
+    if ((st = upload(d_qkey, qkey.data(), qkey.size(), s))) return st;
+    if ((st = tp.seg_beg.alloc((tp.n_seg + 1) * 8))) return st;
+    hipLaunchKernelGGL(k_lower_bound, dim3(grid1(tp.n_seg + 1, 256)), dim3(256), 0, s, d_key2.as<int32_t>(), n_trip,
+                       d_qkey.as<int32_t>(), tp.n_seg, tp.seg_beg.as<int64_t>());
+    if ((st = cq::hip_check(hipGetLastError(), "segment launch"))) return st;
+    if ((st = cq::hip_check(hipStreamSynchronize(s), "segment build"))) return st;
+  }
+  if ((st = cq::hip_check(hipMemcpyAsync(&herr, d_err.p, sizeof(herr), hipMemcpyDeviceToHost, s), "D2H err"))) return st;
+  if ((st = cq::hip_check(hipStreamSynchronize(s), "triplet fill"))) return st;
+  if (herr.code != E_NONE)
+    return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s (halo atom %d, B neighbour %d, value %lld)",
+                          err_name(herr.code), herr.a + 1, herr.b + 1, herr.c);
+  *out = plan.release();
+  return CQMM_OK;
+}
+
+int cqmm_plan_destroy(cqmm_plan *plan) {
+  if (plan) {
+    hipStreamSynchronize(cq::runtime().stream);
+    delete plan;
+  }
+  return CQMM_OK;
+}
+
+int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info) {
+  if (!plan || !info) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_info: null argument");
+  info->n_triplets = plan->tp.n_trip;
+  info->n_out_total = plan->tp.n_seg;
+  info->flops = plan->tp.flops;
+  info->uniform_nsf = plan->tp.uniform;
+  info->mult_type = plan->tp.mult_type;
+  // Output blocks with at least one product.
+  std::vector<int64_t> sb(plan->tp.n_seg + 1);
+  int st = cq::hip_check(hipMemcpy(sb.data(), plan->tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg");
+  if (st) return st;
+  int64_t nz = 0;
+  for (int64_t b = 0; b < plan->tp.n_seg; ++b) nz += sb[b + 1] > sb[b];
+  info->n_out_blocks = nz;
+  return CQMM_OK;
+}
+
+int cqmm_plan_get_triplets(const cqmm_plan *plan, int64_t *out, int64_t *p, int64_t *q) {
+  if (!plan || !out || !p || !q) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_triplets: null argument");
+  const cq::TripletPlan &tp = plan->tp;
+  std::vector<int64_t> sb(tp.n_seg + 1), so(tp.n_seg);
+  int st;
+  if ((st = cq::hip_check(hipMemcpy(sb.data(), tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg"))) return st;
+  if ((st = cq::hip_check(hipMemcpy(so.data(), tp.seg_out.p, so.size() * 8, hipMemcpyDeviceToHost), "D2H out"))) return st;
+  if (tp.off64) {
+    if ((st = cq::hip_check(hipMemcpy(p, tp.tp.p, tp.n_trip * 8, hipMemcpyDeviceToHost), "D2H p"))) return st;
+    if ((st = cq::hip_check(hipMemcpy(q, tp.tq.p, tp.n_trip * 8, hipMemcpyDeviceToHost), "D2H q"))) return st;
+  } else {
+    std::vector<int32_t> pp(tp.n_trip), qq(tp.n_trip);
+    if ((st = cq::hip_check(hipMemcpy(pp.data(), tp.tp.p, tp.n_trip * 4, hipMemcpyDeviceToHost), "D2H p"))) return st;
+    if ((st = cq::hip_check(hipMemcpy(qq.data(), tp.tq.p, tp.n_trip * 4, hipMemcpyDeviceToHost), "D2H q"))) return st;
+    for (int64_t t = 0; t < tp.n_trip; ++t) { p[t] = pp[t]; q[t] = qq[t]; }
+  }
+  for (int64_t b = 0; b < tp.n_seg; ++b)
+    for (int64_t t = sb[b]; t < sb[b + 1]; ++t) out[t] = so[b];
+  for (int64_t t = 0; t < tp.n_trip; ++t) { out[t] += 1; p[t] += 1; q[t] += 1; }
+  return CQMM_OK;
+}
+
+int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b, int64_t lenb, double *c,
+                  int64_t lenc, uint32_t flags) {
+  if (!plan || !a || !b || !c) return cq::set_error(CQMM_ERR_ARG, "cqmm_mat_mult: null argument");
+  if (lena < plan->lena || lenb < plan->lenb || lenc < plan->lenc)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_mat_mult: lengths (%lld,%lld,%lld) below the plan's (%lld,%lld,%lld)",
+                          (long long)lena, (long long)lenb, (long long)lenc, (long long)plan->lena,
+                          (long long)plan->lenb, (long long)plan->lenc);
+  int st = cq::ensure_init();
+  if (st) return st;
+  hipStream_t s = cq::runtime().stream;
+  const bool dev = flags & CQMM_DEVICE_PTRS;
+  const bool fp32 = flags & CQMM_FP32;
+  double *da = a, *dc = c;
+  const double *db = b;
+  if (!dev) {
+    if (plan->sa.bytes < (size_t)plan->lena * 8 && (st = plan->sa.alloc(plan->lena * 8))) return st;
+    if (plan->sb.bytes < (size_t)plan->lenb * 8 && (st = plan->sb.alloc(plan->lenb * 8))) return st;
+    if (plan->sc.bytes < (size_t)plan->lenc * 8 && (st = plan->sc.alloc(plan->lenc * 8))) return st;
+    da = plan->sa.as<double>();
+    dc = plan->sc.as<double>();
+    db = plan->sb.as<double>();
+    if ((st = cq::hip_check(hipMemcpyAsync((void *)db, b, plan->lenb * 8, hipMemcpyHostToDevice, s), "H2D b"))) return st;
+    if (plan->tp.mult_type == 1) {
+      if ((st = cq::hip_check(hipMemcpyAsync(da, a, plan->lena * 8, hipMemcpyHostToDevice, s), "H2D a"))) return st;
+    } else {
+      if ((st = cq::hip_check(hipMemcpyAsync(dc, c, plan->lenc * 8, hipMemcpyHostToDevice, s), "H2D c"))) return st;
+    }
+  }
+  if (plan->tp.mult_type == 1) st = cq::launch_block_products(plan->tp, da, db, dc, fp32, s);
+  else st = cq::launch_block_products(plan->tp, dc, db, da, fp32, s);
+  if (st) return st;
+  if (!dev) {
+    if (plan->tp.mult_type == 1) st = cq::hip_check(hipMemcpyAsync(c, dc, plan->lenc * 8, hipMemcpyDeviceToHost, s), "D2H c");
+    else st = cq::hip_check(hipMemcpyAsync(a, da, plan->lena * 8, hipMemcpyDeviceToHost, s), "D2H a");
+    if (st) return st;
+  }
+  if (!dev || !(flags & CQMM_NO_SYNC)) return cq::hip_check(hipStreamSynchronize(s), "cqmm_mat_mult");
+  return CQMM_OK;
+}
+
+}  // extern "C"

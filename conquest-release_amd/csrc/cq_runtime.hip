@@ -1,0 +1,148 @@
+// Library state, error reporting and device-memory helpers of libcqmm.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/cqmm.h"
+#include "cq_common.h"
+#include "cq_device.h"
+
+namespace cq {
+
+static thread_local std::string g_last_error;
+
+int set_error(int status, const std::string &msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int set_errorf(int status, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+static Runtime g_rt;
+
+Runtime &runtime() { return g_rt; }
+
+int ensure_init() {
+  if (g_rt.initialised) return CQMM_OK;
+  return cqmm_init(-1);
+}
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return CQMM_OK;
+  return set_errorf(e == hipErrorOutOfMemory ? CQMM_ERR_NOMEM : CQMM_ERR_HIP, "%s: %s", what,
+                    hipGetErrorString(e));
+}
+
+}  // namespace cq
+
+extern "C" {
+
+const char *cqmm_last_error(void) { return cq::g_last_error.c_str(); }
+
+int cqmm_init(int device) {
+  cq::Runtime &rt = cq::g_rt;
+  if (rt.initialised) {
+    if (device < 0 || device == rt.device) return CQMM_OK;
+    return cq::set_error(CQMM_ERR_STATE, "cqmm_init: already initialised on another device");
+  }
+  int n = 0;
+  int st = cq::hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+  if (st) return st;
+  if (n <= 0) return cq::set_error(CQMM_ERR_HIP, "cqmm_init: no HIP device visible");
+  int dev = device;
+  if (dev < 0) {
+    st = cq::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (st) return st;
+  }
+  if (dev >= n) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_init: device %d of %d", dev, n);
+  st = cq::hip_check(hipSetDevice(dev), "hipSetDevice");
+  if (st) return st;
+  // Default-priority stream of the library (no priority other than the default).
+  st = cq::hip_check(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (st) return st;
+  hipDeviceProp_t prop;
+  st = cq::hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  if (st) return st;
+  rt.n_cu = prop.multiProcessorCount;
+  rt.device = dev;
+  rt.initialised = true;
+  return CQMM_OK;
+}
+
+int cqmm_finalize(void) {
+  cq::Runtime &rt = cq::g_rt;
+  if (!rt.initialised) return CQMM_OK;
+  hipStreamSynchronize(rt.stream);
+  hipStreamDestroy(rt.stream);
+  rt.stream = nullptr;
+  rt.initialised = false;
+  return CQMM_OK;
+}
+
+int cqmm_device_alloc(void **ptr, int64_t bytes) {
+  if (!ptr || bytes < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_device_alloc: bad argument");
+  int st = cq::ensure_init();
+  if (st) return st;
+  *ptr = nullptr;
+  if (bytes == 0) return CQMM_OK;
+  return cq::hip_check(hipMalloc(ptr, (size_t)bytes), "hipMalloc");
+}
+
+int cqmm_device_free(void *ptr) {
+  if (!ptr) return CQMM_OK;
+  return cq::hip_check(hipFree(ptr), "hipFree");
+}
+
+int cqmm_memcpy_h2d(void *dst, const void *src, int64_t bytes) {
+  int st = cq::ensure_init();
+  if (st) return st;
+  if (bytes <= 0) return CQMM_OK;
+  st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, cq::g_rt.stream), "hipMemcpy H2D");
+  if (st) return st;
+  return cq::hip_check(hipStreamSynchronize(cq::g_rt.stream), "hipStreamSynchronize");
+}
+
+int cqmm_memcpy_d2h(void *dst, const void *src, int64_t bytes) {
+  int st = cq::ensure_init();
+  if (st) return st;
+  if (bytes <= 0) return CQMM_OK;
+  st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, cq::g_rt.stream), "hipMemcpy D2H");
+  if (st) return st;
+  return cq::hip_check(hipStreamSynchronize(cq::g_rt.stream), "hipStreamSynchronize");
+}
+
+static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+
+int cqmm_timer_start(void) {
+  int st = cq::ensure_init();
+  if (st) return st;
+  if (!g_ev0 && (st = cq::hip_check(hipEventCreate(&g_ev0), "hipEventCreate"))) return st;
+  if (!g_ev1 && (st = cq::hip_check(hipEventCreate(&g_ev1), "hipEventCreate"))) return st;
+  return cq::hip_check(hipEventRecord(g_ev0, cq::g_rt.stream), "hipEventRecord");
+}
+
+int cqmm_timer_stop(float *ms) {
+  if (!g_ev0 || !g_ev1 || !ms) return cq::set_error(CQMM_ERR_STATE, "cqmm_timer_stop: timer not started");
+  int st = cq::hip_check(hipEventRecord(g_ev1, cq::g_rt.stream), "hipEventRecord");
+  if (st) return st;
+  if ((st = cq::hip_check(hipEventSynchronize(g_ev1), "hipEventSynchronize"))) return st;
+  return cq::hip_check(hipEventElapsedTime(ms, g_ev0, g_ev1), "hipEventElapsedTime");
+}
+
+int cqmm_synchronize(void) {
+  int st = cq::ensure_init();
+  if (st) return st;
+  return cq::hip_check(hipStreamSynchronize(cq::g_rt.stream), "hipStreamSynchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+/*
+ * cqmm_system.h — single-process mirror of CONQUEST's sparsity indexing, and
+ * the synthetic matrices of the benchmark workloads.
+ *
+ * CONQUEST builds its index arrays in Fortran at start-up: the cover set
+ * (make_cs, src/cover_module.f90:133-429), neighbour lists per range
+ * (matrix_ini/get_naba, src/matrix_elements_module.f90:67-142,257-384), halos
+ * (make_halo, :560-660), local-transpose indices (get_retr, :517-557) and
+ * partition offsets (make_npxyz, :836-903), then the mult table (immi,
+ * src/mult_module.f90:271-1102). A Fortran host hands those arrays to
+ * cqmm_plan_create unchanged. This header lets a host without CONQUEST (tests,
+ * benchmarks, the request loop) produce the same arrays for one process from
+ * atom positions, and reads them back as a cqmm_mult_desc.
+ *
+ * Units: lengths in Angstrom; the neighbour test is d^2 < rcut^2 - 1e-8
+ * (RD_ERR, src/numbers_module.f90:96) in Angstrom^2.
+ */
+#ifndef CQMM_SYSTEM_H
+#define CQMM_SYSTEM_H
+
+#include <stdint.h>
+#include "cqmm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cqmm_system cqmm_system;
+
+/* Atoms: xyz[3*natoms] cell coordinates in [0,box), nsf[natoms] support
+ * functions per atom, ngrid[3] partitions per cell edge (regular grid, raster
+ * order), rcut_cover the cover-set radius (it must be at least r(A-halo)+r(B)
+ * of every product planned on the system). Atom i keeps global id i. */
+int cqmm_sys_create(int32_t natoms, const double *xyz, const double box[3],
+                    const int32_t *nsf, const int32_t ngrid[3],
+                    double rcut_cover, cqmm_system **sys);
+int cqmm_sys_destroy(cqmm_system *sys);
+
+/* matrix_ini for one range: neighbour lists, halo and transpose index. */
+int cqmm_sys_add_range(cqmm_system *sys, double rcut, int32_t *range_id);
+
+typedef struct cqmm_range_info {
+  double rcut;
+  int64_t length;              /* doubles in the matrix                       */
+  int64_t n_blocks;            /* stored blocks                               */
+  int32_t ni_in_halo;          /* atoms in the halo                           */
+  int32_t np_in_halo;          /* partitions in the halo                      */
+  int32_t max_nab;             /* largest row                                 */
+  int32_t fits_int32;          /* 1 when every offset fits the reference's int32 arrays */
+} cqmm_range_info;
+int cqmm_sys_range_info(const cqmm_system *sys, int32_t range_id,
+                        cqmm_range_info *info);
+
+typedef struct cqmm_sys_info {
+  int32_t natoms, n_parts, ng_cover, mx_mcover;
+  int32_t ncover[3], nspanl[3], ngrid[3];
+  double box[3];
+} cqmm_sys_info;
+int cqmm_sys_get_info(const cqmm_system *sys, cqmm_sys_info *info);
+
+/* Fill a cqmm_mult_desc for product `mult_type` with operand ranges as
+ * CONQUEST's immi assigns them (src/mult_module.f90:583-872): type 1 computes
+ * C(rc) = A(ra).B(rb); type 2 computes A(ra) = C(rc).B(rb)^T (mat_mult's a is
+ * the result). The pointers stay valid while the system lives. */
+int cqmm_sys_mult_desc(const cqmm_system *sys, int32_t mult_type, int32_t ra,
+                       int32_t rb, int32_t rc, cqmm_mult_desc *desc);
+
+/* Block list of a range in storage order: primary atom (0-based, primary
+ * order), global ids of the row and column atoms, the column atom's periodic
+ * image shift in cells, the 0-based data offset, and block dimensions. Any
+ * output pointer may be NULL. */
+int cqmm_sys_blocks(const cqmm_system *sys, int32_t range_id, int32_t *prim,
+                    int32_t *gid_i, int32_t *gid_j, int32_t *shift,
+                    int64_t *offset, int32_t *nd1, int32_t *nd2);
+/* Primary order: prim_gid[p] is the global id of primary atom p. */
+int cqmm_sys_primary_order(const cqmm_system *sys, int32_t *prim_gid);
+
+/* Synthetic block values (counter-based, independent of enumeration order):
+ *  kind 0: entry = N(0,1) * exp(-r_ij / p[0])
+ *  kind 1: overlap S = I on the onsite block, symmetrised p[0]*N(0,1)*exp(-r/p[1])
+ *          on every other block
+ *  kind 2: Hamiltonian H: onsite block diagonal eps_a ~ U(-p[2],p[2]) Ha,
+ *          shifted by -p[3] for the half of all orbitals with the lowest eps
+ *          and by +p[3] for the rest (opens a gap at half filling); every other
+ *          block symmetrised p[0]*N(0,1)*exp(-r/p[1])
+ *  kind 3: identity
+ * `tag` separates matrices drawn with the same seed. values[length]. */
+int cqmm_sys_fill(const cqmm_system *sys, int32_t range_id, int32_t kind,
+                  uint64_t seed, uint64_t tag, const double *p, int32_t np,
+                  double *values);
+
+/* Synthetic structure of the workloads: natoms uniform in a cubic periodic box
+ * of edge (natoms/density)^(1/3) with minimum-image separation >= dmin
+ * (rejection sampling, seeded). Writes xyz[3*natoms]; returns the edge. */
+int cqmm_random_structure(int32_t natoms, double density, double dmin,
+                          uint64_t seed, double *xyz, double *edge);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CQMM_SYSTEM_H */

@@ -1,0 +1,52 @@
+"""Small seeded systems shaped like the benchmark workloads (sizes the CPU oracle
+finishes in seconds). Lengths in Angstrom."""
+from __future__ import annotations
+
+import numpy as np
+
+import cqmm
+
+
+def species_nsf(natoms, mix, seed):
+    """nsf per atom: an int, or a mix like ((4, .5), (9, .25), (13, .25))."""
+    if isinstance(mix, int):
+        return np.full(natoms, mix, np.int32)
+    rng = np.random.default_rng(seed + 7919)
+    vals = np.array([m[0] for m in mix], np.int32)
+    probs = np.array([m[1] for m in mix], np.float64)
+    return vals[rng.choice(len(vals), size=natoms, p=probs / probs.sum())]
+
+
+class Case:
+    """A system with ranges and one product.
+
+    mult_type 1: c(rc) = a(ra).b(rb); mult_type 2: a(ra) = c(rc).b(rb)^T."""
+
+    def __init__(self, natoms, nsf, ra, rb, rc, mult_type=1, seed=1, density=0.05, grid=None, decay=2.0):
+        self.xyz, self.edge = cqmm.random_structure(natoms, density, 2.0, seed)
+        self.nsf = species_nsf(natoms, nsf, seed)
+        rcover = max(ra + rb, rc, ra, rb)
+        if grid is None:
+            grid = cqmm.default_grid(self.edge, 4.0)
+        self.grid = grid
+        self.sys = cqmm.System(self.xyz, self.edge, self.nsf, rcover, grid)
+        self.ra, self.rb, self.rc = ra, rb, rc
+        self.ida = self.sys.add_range(ra)
+        self.idb = self.sys.add_range(rb) if rb != ra else self.ida
+        self.idc = self.sys.add_range(rc) if rc not in (ra, rb) else (self.ida if rc == ra else self.idb)
+        self.mult_type = mult_type
+        self.desc = self.sys.mult_desc(mult_type, self.ida, self.idb, self.idc)
+        self.seed = seed
+        self.decay = decay
+
+    def values(self, tag_a=1, tag_b=2, tag_c=3):
+        """Inputs as mat_mult receives them: (a, b, c) with the output zeroed."""
+        s = self.sys
+        b = s.fill(self.idb, 0, self.seed, tag_b, (self.decay,))
+        if self.mult_type == 1:
+            a = s.fill(self.ida, 0, self.seed, tag_a, (self.decay,))
+            c = np.zeros(s.length(self.idc))
+        else:
+            c = s.fill(self.idc, 0, self.seed, tag_c, (self.decay,))
+            a = np.zeros(s.length(self.ida))
+        return a, b, c

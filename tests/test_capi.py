@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every function include/*.h declares (CPU only)."""
+import ctypes
+import glob
+import os
+import re
+
+import cqmm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(cqmm_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = cqmm.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_layer_binds_every_export():
+    assert declared_functions() <= set(cqmm.EXPORTS)
+
+
+def test_errors_are_reported_not_raised_in_c():
+    lib = cqmm.lib()
+    h = ctypes.c_void_p()
+    # Bad arguments return a status and a message instead of crashing.
+    st = lib.cqmm_sys_create(0, None, None, None, None, 1.0, ctypes.byref(h))
+    assert st != 0
+    assert b"cqmm_sys_create" in lib.cqmm_last_error()

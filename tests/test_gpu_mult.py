@@ -1,0 +1,134 @@
+"""GPU parity of the drop-in mat_mult (HIP triplet builder + FP64 MFMA kernel)
+against the CPU oracle, through the C ABI.
+
+Bar: the triplet list (which (i,k,j) products, which output block, which
+operand blocks, and the per-block summation order) is bit-exact with the
+default kernels' walk; values agree to 1e-12 relative (max |diff| / max |ref|)
+and output blocks without products are exactly zero.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cqmm
+import oracle
+from cases import Case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _stable_by_out(o, p, q):
+    order = np.argsort(o, kind="stable")
+    return o[order], p[order], q[order]
+
+
+def check_case(case, exact_order=True):
+    print(f"case natoms={case.sys.natoms} nsf={np.unique(case.nsf)} ranges={case.ra},{case.rb},{case.rc} "
+          f"type={case.mult_type}", flush=True)
+    plan = cqmm.Plan(case.desc)
+    o_ref, p_ref, q_ref = oracle.triplets(case.desc)
+    o_gpu, p_gpu, q_gpu = plan.triplets()
+    if case.mult_type == 1:
+        o_ref, p_ref, q_ref = _stable_by_out(o_ref, p_ref, q_ref)
+    np.testing.assert_array_equal(o_gpu, o_ref)
+    np.testing.assert_array_equal(p_gpu, p_ref)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    info = plan.info()
+    assert info.n_triplets == len(o_ref)
+    assert info.flops == oracle.flops(case.desc)
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    # Poison the output: the GPU path must overwrite every element.
+    if case.mult_type == 1:
+        c[:] = np.nan
+    else:
+        a[:] = np.nan
+    plan.mat_mult(a, b, c)
+    got, ref = (c, c_r) if case.mult_type == 1 else (a, a_r)
+    assert np.all(np.isfinite(got))
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"  triplets={len(o_ref)} rel err={err:.3e}", flush=True)
+    assert err <= TOL
+    # Blocks with no products are exactly zero.
+    zero = ref == 0.0
+    assert np.all(got[zero] == 0.0)
+    return plan, got
+
+
+SMALL = [
+    (40, 4, 6.0, 8.0, 14.0, 1),
+    (40, 4, 6.0, 8.0, 9.0, 1),
+    (36, 13, 6.0, 8.0, 8.0, 2),
+    (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),
+    (48, ((4, .5), (9, .25), (13, .25)), 7.0, 9.0, 6.0, 2),
+    (30, 9, 5.0, 6.0, 11.0, 1),
+    (30, 16, 5.0, 6.0, 6.0, 2),
+    (40, 5, 6.0, 6.0, 12.0, 1),   # non-instanced uniform size takes the general path
+]
+
+
+@pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", SMALL)
+def test_small_parity(natoms, nsf, ra, rb, rc, mt):
+    check_case(Case(natoms, nsf, ra, rb, rc, mt, seed=21))
+
+
+def test_cfg1_shape_full_size_parity():
+    # 512 atoms, 4x4 blocks, A 8, B 12, C 20 (full product), type 1.
+    check_case(Case(512, 4, 8.0, 12.0, 20.0, 1, seed=1))
+
+
+def test_cfg2_shape_reduced_parity():
+    # DZP shape (13x13, A 8, B 16, C truncated to 16, m_kern_min form) on 128 atoms.
+    check_case(Case(128, 13, 16.0, 16.0, 8.0, 2, seed=2))
+
+
+def test_cfg3_shape_mixed_species_parity():
+    check_case(Case(160, ((4, .5), (9, .25), (13, .25)), 8.0, 12.0, 16.0, 1, seed=3))
+
+
+def test_deterministic_bitwise():
+    case = Case(48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1, seed=8)
+    plan = cqmm.Plan(case.desc)
+    a, b, c = case.values()
+    plan.mat_mult(a, b, c)
+    c1 = c.copy()
+    for _ in range(3):
+        c[:] = 0.0
+        plan.mat_mult(a, b, c)
+        np.testing.assert_array_equal(c, c1)
+
+
+def test_device_pointer_path_matches_host_path():
+    case = Case(40, 13, 6.0, 8.0, 8.0, 2, seed=5)
+    plan = cqmm.Plan(case.desc)
+    a, b, c = case.values()
+    plan.mat_mult(a, b, c)
+    da, db, dc = (cqmm.DeviceArray.from_host(x) for x in (np.zeros_like(a), b, c))
+    plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n)
+    np.testing.assert_array_equal(da.to_host(), a)
+
+
+def test_linearity_in_each_operand():
+    case = Case(64, 9, 6.0, 8.0, 12.0, 1, seed=6)
+    plan = cqmm.Plan(case.desc)
+    a, b, c = case.values()
+    a2 = case.sys.fill(case.ida, 0, 99, 1, (2.0,))
+    c1 = np.zeros_like(c); c2 = np.zeros_like(c); c3 = np.zeros_like(c)
+    plan.mat_mult(a, b, c1)
+    plan.mat_mult(a2, b, c2)
+    plan.mat_mult(2.0 * a - 0.5 * a2, b, c3)
+    ref = 2.0 * c1 - 0.5 * c2
+    assert np.abs(c3 - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_bad_descriptor_is_rejected_before_any_kernel():
+    case = Case(30, 4, 5.0, 6.0, 11.0, 1, seed=3)
+    d = case.desc
+    d.lenb = 10  # B blocks now point past lenb
+    with pytest.raises(cqmm.CqmmError, match="lenb|B block"):
+        cqmm.Plan(d)
