@@ -71,6 +71,38 @@ class RangeInfo(C.Structure):
                 ("fits_int32", C.c_int32)]
 
 
+class MatrixIndex(C.Structure):
+    _fields_ = [("n_parts", C.c_int32), ("n_atoms", i32p), ("nab_offset", i32p), ("nd_offset", i64p),
+                ("n_prim", C.c_int32), ("n_nab", i32p), ("i_acc", i32p), ("i_nd_acc", i32p), ("ndimi", i32p),
+                ("onsite", i64p), ("i_part", i32p), ("i_seq", i32p), ("ndimj", i32p), ("n_nabs", C.c_int64),
+                ("length", C.c_int64)]
+
+
+class McwSetup(C.Structure):
+    _fields_ = [("range_L", C.c_int32), ("range_S", C.c_int32), ("range_H", C.c_int32), ("range_LS", C.c_int32),
+                ("range_LSL", C.c_int32), ("mult_L_S_LS", C.c_int32), ("mult_LS_L_LSL", C.c_int32),
+                ("mult_LSL_SL_L", C.c_int32), ("max_iter", C.c_int32), ("precision", C.c_int32),
+                ("n_final_fp64", C.c_int32), ("fp32_tol", C.c_double)]
+
+
+class McwResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("fp32_iterations", C.c_int32), ("stop_reason", C.c_int32),
+                ("restored", C.c_int32), ("energy", C.c_double), ("electrons", C.c_double), ("c", C.c_double),
+                ("seconds", C.c_double)]
+
+
+class InitMcwSetup(C.Structure):
+    _fields_ = [("range_T", C.c_int32), ("range_S", C.c_int32), ("range_H", C.c_int32), ("range_L", C.c_int32),
+                ("range_TS", C.c_int32), ("range_TH", C.c_int32), ("mult_T_S_TS", C.c_int32),
+                ("mult_T_H_TH", C.c_int32), ("mult_TH_T_L", C.c_int32), ("n_electrons", C.c_double)]
+
+
+class HotellingSetup(C.Structure):
+    _fields_ = [("range_T", C.c_int32), ("range_S", C.c_int32), ("range_TS", C.c_int32),
+                ("mult_T_S_TS", C.c_int32), ("mult_TS_T_T", C.c_int32), ("max_iter", C.c_int32),
+                ("tol", C.c_double)]
+
+
 class SysInfo(C.Structure):
     _fields_ = [("natoms", C.c_int32), ("n_parts", C.c_int32), ("ng_cover", C.c_int32),
                 ("mx_mcover", C.c_int32), ("ncover", C.c_int32 * 3), ("nspanl", C.c_int32 * 3),
@@ -85,7 +117,13 @@ EXPORTS = [
     "cqmm_plan_get_triplets",
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
-    "cqmm_sys_fill", "cqmm_random_structure",
+    "cqmm_sys_fill", "cqmm_random_structure", "cqmm_sys_plan_create", "cqmm_sys_matrix_index",
+    "cqmm_sys_transpose_pairs",
+    "cqmm_ctx_create", "cqmm_ctx_destroy", "cqmm_ctx_add_range", "cqmm_ctx_set_transpose", "cqmm_ctx_add_mult",
+    "cqmm_matrix_allocate", "cqmm_matrix_free", "cqmm_matrix_length", "cqmm_matrix_range",
+    "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
+    "cqmm_matrix_scale", "cqmm_matrix_product_trace", "cqmm_matrix_trace", "cqmm_matrix_symmetrise",
+    "cqmm_mcweeny", "cqmm_init_mcweeny", "cqmm_hotelling",
 ]
 
 _lib = None
@@ -126,6 +164,31 @@ def lib():
     L.cqmm_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.cqmm_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.cqmm_timer_stop.argtypes = [C.POINTER(C.c_float)]
+    vp = C.c_void_p
+    L.cqmm_sys_plan_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.cqmm_sys_matrix_index.argtypes = [vp, C.c_int32, C.POINTER(MatrixIndex)]
+    L.cqmm_sys_transpose_pairs.argtypes = [vp, C.c_int32, i64p, i64p, i64p, i32p, i32p]
+    L.cqmm_ctx_create.argtypes = [C.POINTER(vp)]
+    L.cqmm_ctx_destroy.argtypes = [vp]
+    L.cqmm_ctx_add_range.argtypes = [vp, C.c_int32, C.POINTER(MatrixIndex)]
+    L.cqmm_ctx_set_transpose.argtypes = [vp, C.c_int32, C.c_int64, i64p, i64p, i32p, i32p]
+    L.cqmm_ctx_add_mult.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]
+    L.cqmm_matrix_allocate.argtypes = [vp, C.c_int32, i32p]
+    L.cqmm_matrix_free.argtypes = [vp, C.c_int32]
+    L.cqmm_matrix_length.argtypes = [vp, C.c_int32, i64p]
+    L.cqmm_matrix_range.argtypes = [vp, C.c_int32, i32p]
+    L.cqmm_matrix_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    L.cqmm_matrix_set.argtypes = [vp, C.c_int32, vp]
+    L.cqmm_matrix_get.argtypes = [vp, C.c_int32, vp]
+    L.cqmm_matrix_product.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.cqmm_matrix_sum.argtypes = [vp, C.c_double, C.c_int32, C.c_double, C.c_int32]
+    L.cqmm_matrix_scale.argtypes = [vp, C.c_double, C.c_int32]
+    L.cqmm_matrix_product_trace.argtypes = [vp, C.c_int32, C.c_int32, f64p]
+    L.cqmm_matrix_trace.argtypes = [vp, C.c_int32, f64p]
+    L.cqmm_matrix_symmetrise.argtypes = [vp, C.c_int32]
+    L.cqmm_mcweeny.argtypes = [vp, C.POINTER(McwSetup), C.c_int32, C.c_int32, C.c_int32, C.POINTER(McwResult)]
+    L.cqmm_init_mcweeny.argtypes = [vp, C.POINTER(InitMcwSetup), C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]
+    L.cqmm_hotelling.argtypes = [vp, C.POINTER(HotellingSetup), C.c_int32, C.c_int32, i32p, f64p]
     _lib = L
     return L
 

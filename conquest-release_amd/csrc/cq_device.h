@@ -63,5 +63,16 @@ struct TripletPlan {
 // Launch the FP64 (or FP32) block-product kernel over a plan.
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O,
                           bool fp32, hipStream_t stream);
+// Same products on FP32 copies of the operands with FP32 MFMA; acc64 sums the
+// per-product results of each block in FP64 registers, else in the FP32 accumulator.
+int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, bool acc64,
+                              hipStream_t stream);
 
 }  // namespace cq
+
+struct cqmm_plan {
+  cq::TripletPlan tp;
+  int64_t lena = 0, lenb = 0, lenc = 0;
+  // Staging buffers for host-pointer calls (allocated on first use).
+  cq::DevBuf sa, sb, sc;
+};

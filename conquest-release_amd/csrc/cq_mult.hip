@@ -17,6 +17,9 @@
 // (checked on gfx950 against the CPU oracle).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <string>
+
 #include "../../include/cqmm.h"
 #include "cq_common.h"
 #include "cq_device.h"
@@ -68,6 +71,90 @@ __global__ void __launch_bounds__(64 * kWaves) k_block_products_f64(
   }
 }
 
+// Uniform block size NSF (<= 16): the same products as k_block_products_f64,
+// with
+//  * an XCD-aware workgroup order: hardware dispatches workgroup b to XCD b % 8,
+//    so workgroup b takes segment group (b % 8) * ceil(G/8) + b / 8 (the grid is
+//    padded to 8*ceil(G/8) workgroups so every group runs once). Consecutive
+//    segments share a B row (type 2: same halo atom k) or an A row (type 1:
+//    same primary atom i) and now run on one XCD, whose L2 keeps that row;
+//  * wave-uniform segment and triplet indices (scalar loads);
+//  * a two-stage register pipeline: the operands of product t+1 are loaded
+//    while the MFMAs of product t run.
+template <typename OffT, bool QT, int NSF>
+__global__ void __launch_bounds__(64 * kWaves) k_block_products_f64_u(
+    const int64_t *__restrict__ seg_beg, const int64_t *__restrict__ seg_out, const OffT *__restrict__ tp,
+    const OffT *__restrict__ tq, const double *__restrict__ P, const double *__restrict__ Q,
+    double *__restrict__ O, int64_t n_seg, int32_t n_groups) {
+  constexpr int NS = (NSF + 3) / 4;  // MFMA k-steps per product
+  const int lane = threadIdx.x & 63;
+  const int xcd = blockIdx.x & 7;
+  const int32_t per = (n_groups + 7) >> 3;
+  const int32_t grp = xcd * per + (int32_t)(blockIdx.x >> 3);
+  if (grp >= n_groups) return;
+  const int64_t seg = (int64_t)grp * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (seg >= n_seg) return;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t t0 = seg_beg[seg], t1 = seg_beg[seg + 1];
+  // Per-lane element offsets inside a block for each k-step (constant per lane).
+  int oa[NS], ob[NS];
+  bool va[NS], vb[NS];
+#pragma unroll
+  for (int s4 = 0; s4 < NS; ++s4) {
+    const int qi = 4 * s4 + g;
+    va[s4] = (r < NSF) && (qi < NSF);
+    vb[s4] = va[s4];
+    oa[s4] = r + NSF * qi;
+    ob[s4] = QT ? (r + NSF * qi) : (qi + NSF * r);
+  }
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  double a0[NS], b0[NS];
+  if (t0 < t1) {
+    const int64_t p = (int64_t)tp[t0], q = (int64_t)tq[t0];
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4) {
+      a0[s4] = va[s4] ? P[p + oa[s4]] : 0.0;
+      b0[s4] = vb[s4] ? Q[q + ob[s4]] : 0.0;
+    }
+  }
+  for (int64_t t = t0; t < t1; ++t) {
+    double a1[NS], b1[NS];
+    if (t + 1 < t1) {
+      const int64_t p = (int64_t)tp[t + 1], q = (int64_t)tq[t + 1];
+#pragma unroll
+      for (int s4 = 0; s4 < NS; ++s4) {
+        a1[s4] = va[s4] ? P[p + oa[s4]] : 0.0;
+        b1[s4] = vb[s4] ? Q[q + ob[s4]] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s4], b0[s4], acc, 0, 0, 0);
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4) {
+      a0[s4] = a1[s4];
+      b0[s4] = b1[s4];
+    }
+  }
+  const int64_t o = seg_out[seg];
+  if (r < NSF) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = g + 4 * e;
+      if (row < NSF) O[o + row + (int64_t)NSF * r] = acc[e];
+    }
+  }
+}
+
+// CQMM_KERNEL=simple selects the one-stage kernel for every block size (used to
+// measure what the pipelined, XCD-ordered kernel gains).
+bool force_general() {
+  static const int v = [] {
+    const char *e = getenv("CQMM_KERNEL");
+    return (e && std::string(e) == "simple") ? 1 : 0;
+  }();
+  return v != 0;
+}
+
 template <typename OffT, bool QT>
 int launch_t(const cq::TripletPlan &tp, const double *P, const double *Q, double *O, hipStream_t s) {
   const unsigned grid = (unsigned)((tp.n_seg + kWaves - 1) / kWaves);
@@ -78,6 +165,17 @@ int launch_t(const cq::TripletPlan &tp, const double *P, const double *Q, double
   const OffT *qq = tp.tq.as<OffT>();
   const uint8_t *kk = tp.tk.as<uint8_t>();
   dim3 blk(64 * kWaves);
+  // The XCD-aware order needs every (xcd, slot) pair: pad the grid to 8*ceil(G/8).
+  const int32_t ngrp = (int32_t)grid;
+  const unsigned gridx = 8u * (unsigned)((ngrp + 7) / 8);
+  switch (force_general() ? -1 : tp.uniform) {
+    case 4: hipLaunchKernelGGL((k_block_products_f64_u<OffT, QT, 4>), dim3(gridx), blk, 0, s, sb, so, pp, qq, P, Q, O, tp.n_seg, ngrp); break;
+    case 9: hipLaunchKernelGGL((k_block_products_f64_u<OffT, QT, 9>), dim3(gridx), blk, 0, s, sb, so, pp, qq, P, Q, O, tp.n_seg, ngrp); break;
+    case 13: hipLaunchKernelGGL((k_block_products_f64_u<OffT, QT, 13>), dim3(gridx), blk, 0, s, sb, so, pp, qq, P, Q, O, tp.n_seg, ngrp); break;
+    case 16: hipLaunchKernelGGL((k_block_products_f64_u<OffT, QT, 16>), dim3(gridx), blk, 0, s, sb, so, pp, qq, P, Q, O, tp.n_seg, ngrp); break;
+    default: break;
+  }
+  if (tp.uniform != 0 && !force_general()) return cq::hip_check(hipGetLastError(), "block-product kernel launch");
   switch (tp.uniform) {
     case 4: hipLaunchKernelGGL((k_block_products_f64<OffT, QT, 4>), dim3(grid), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg); break;
     case 9: hipLaunchKernelGGL((k_block_products_f64<OffT, QT, 9>), dim3(grid), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg); break;
@@ -93,9 +191,101 @@ int launch_t(const cq::TripletPlan &tp, const double *P, const double *Q, double
   return cq::hip_check(hipGetLastError(), "block-product kernel launch");
 }
 
+// FP32 variant (v_mfma_f32_16x16x4_f32, FP32 accumulate) for the bulk of the
+// mixed-precision purification. Same segment/XCD/pipeline structure as the
+// FP64 kernel; general block sizes use per-segment/per-product dimensions.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// ACC64: each product's FP32 MFMA result is added into FP64 registers (the
+// MFMA inner sums stay FP32; the sum over products of a block is FP64).
+template <typename OffT, bool QT, int NSF, bool ACC64>
+__global__ void __launch_bounds__(64 * kWaves) k_block_products_f32(
+    const int64_t *__restrict__ seg_beg, const int64_t *__restrict__ seg_out, const uint8_t *__restrict__ seg_mn,
+    const OffT *__restrict__ tp, const OffT *__restrict__ tq, const uint8_t *__restrict__ tk,
+    const float *__restrict__ P, const float *__restrict__ Q, float *__restrict__ O, int64_t n_seg, int32_t n_groups) {
+  const int lane = threadIdx.x & 63;
+  const int xcd = blockIdx.x & 7;
+  const int32_t per = (n_groups + 7) >> 3;
+  const int32_t grp = xcd * per + (int32_t)(blockIdx.x >> 3);
+  if (grp >= n_groups) return;
+  const int64_t seg = (int64_t)grp * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (seg >= n_seg) return;
+  int m, n;
+  if (NSF > 0) {
+    m = NSF;
+    n = NSF;
+  } else {
+    m = seg_mn[2 * seg];
+    n = seg_mn[2 * seg + 1];
+  }
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t t0 = seg_beg[seg], t1 = seg_beg[seg + 1];
+  flt4 acc = {0.f, 0.f, 0.f, 0.f};
+  double a64[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t p = (int64_t)tp[t], q = (int64_t)tq[t];
+    const int kk = NSF > 0 ? NSF : (int)tk[t];
+    const int nsteps = (kk + 3) >> 2;
+    for (int s4 = 0; s4 < nsteps; ++s4) {
+      const int qi = 4 * s4 + g;
+      float a = 0.f, b = 0.f;
+      if (r < m && qi < kk) a = P[p + r + (int64_t)m * qi];
+      if (r < n && qi < kk) b = QT ? Q[q + r + (int64_t)n * qi] : Q[q + qi + (int64_t)kk * r];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+    if (ACC64) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        a64[e] += (double)acc[e];
+        acc[e] = 0.f;
+      }
+    }
+  }
+  const int64_t o = seg_out[seg];
+  if (r < n) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 4 * g + e;  // FP32 accumulator: D[4*(l/16)+e][l%16]
+      if (row < m) O[o + row + (int64_t)m * r] = ACC64 ? (float)a64[e] : acc[e];
+    }
+  }
+}
+
+template <typename OffT, bool QT, bool ACC64>
+int launch_f32_t(const cq::TripletPlan &tp, const float *P, const float *Q, float *O, hipStream_t s) {
+  const int32_t ngrp = (int32_t)((tp.n_seg + kWaves - 1) / kWaves);
+  const unsigned gridx = 8u * (unsigned)((ngrp + 7) / 8);
+  const int64_t *sb = tp.seg_beg.as<int64_t>();
+  const int64_t *so = tp.seg_out.as<int64_t>();
+  const uint8_t *mn = tp.seg_mn.as<uint8_t>();
+  const OffT *pp = tp.tp.as<OffT>();
+  const OffT *qq = tp.tq.as<OffT>();
+  const uint8_t *kk = tp.tk.as<uint8_t>();
+  dim3 blk(64 * kWaves);
+  switch (tp.uniform) {
+    case 4: hipLaunchKernelGGL((k_block_products_f32<OffT, QT, 4, ACC64>), dim3(gridx), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg, ngrp); break;
+    case 9: hipLaunchKernelGGL((k_block_products_f32<OffT, QT, 9, ACC64>), dim3(gridx), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg, ngrp); break;
+    case 13: hipLaunchKernelGGL((k_block_products_f32<OffT, QT, 13, ACC64>), dim3(gridx), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg, ngrp); break;
+    case 16: hipLaunchKernelGGL((k_block_products_f32<OffT, QT, 16, ACC64>), dim3(gridx), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg, ngrp); break;
+    default: hipLaunchKernelGGL((k_block_products_f32<OffT, QT, 0, ACC64>), dim3(gridx), blk, 0, s, sb, so, mn, pp, qq, kk, P, Q, O, tp.n_seg, ngrp); break;
+  }
+  return cq::hip_check(hipGetLastError(), "FP32 block-product kernel launch");
+}
+
 }  // namespace
 
 namespace cq {
+
+int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, bool acc64,
+                              hipStream_t s) {
+  if (tp.n_seg == 0) return CQMM_OK;
+  if (acc64) {
+    if (tp.off64) return tp.q_trans ? launch_f32_t<int64_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int64_t, false, true>(tp, P, Q, O, s);
+    return tp.q_trans ? launch_f32_t<int32_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int32_t, false, true>(tp, P, Q, O, s);
+  }
+  if (tp.off64) return tp.q_trans ? launch_f32_t<int64_t, true, false>(tp, P, Q, O, s) : launch_f32_t<int64_t, false, false>(tp, P, Q, O, s);
+  return tp.q_trans ? launch_f32_t<int32_t, true, false>(tp, P, Q, O, s) : launch_f32_t<int32_t, false, false>(tp, P, Q, O, s);
+}
 
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O, bool fp32,
                           hipStream_t s) {

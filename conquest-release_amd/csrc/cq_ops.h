@@ -1,0 +1,43 @@
+// Maps and launchers for the streaming matrix kernels (cq_ops.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cq_device.h"
+
+namespace cq {
+
+// Matched element runs between two patterns: y[dst+e] op= x[src+e], e < len.
+struct RunMap {
+  int64_t nruns = 0;
+  int64_t nelem = 0;
+  DevBuf dst, src, len;  // int64, int64, int32
+};
+
+// Onsite blocks of a pattern (for matrix_trace).
+struct DiagMap {
+  int64_t nblk = 0;
+  DevBuf off, nsf;  // int64, int32
+};
+
+// Transpose pairs (p <= q) of a pattern (for symmetrise).
+struct PairMap {
+  int64_t npairs = 0;
+  DevBuf poff, qoff, n1, n2;  // int64, int64, uint8, uint8
+};
+
+int op_axpby(int64_t n, double alpha, double *y, double beta, const double *x, hipStream_t s);
+int op_scale(int64_t n, double alpha, double *y, hipStream_t s);
+int op_copy(int64_t n, double *y, const double *x, hipStream_t s);
+int op_runs_axpby(const RunMap &m, double alpha, double *y, double beta, const double *x, hipStream_t s);
+int op_dot(int64_t n, const double *x, const double *y, double *d_partials, double *d_out, hipStream_t s);
+int op_trace(const DiagMap &m, const double *a, double *d_partials, double *d_out, hipStream_t s);
+int op_symmetrise(const PairMap &m, double *a, hipStream_t s);
+int op_to_f32(int64_t n, const double *x, float *y, hipStream_t s);
+int op_to_f64(int64_t n, const float *x, double *y, hipStream_t s);
+int op_set_diag(const DiagMap &m, double v, double *a, hipStream_t s);
+int reduction_partials();
+
+}  // namespace cq

@@ -30,6 +30,7 @@
 #include "../../include/cqmm.h"
 #include "cq_common.h"
 #include "cq_device.h"
+#include "cq_system.h"
 
 namespace cq {
 
@@ -47,12 +48,6 @@ int DevBuf::alloc(size_t n) {
 
 }  // namespace cq
 
-struct cqmm_plan {
-  cq::TripletPlan tp;
-  int64_t lena = 0, lenb = 0, lenc = 0;
-  // Staging buffers for host-pointer calls (allocated on first use).
-  cq::DevBuf sa, sb, sc;
-};
 
 namespace {
 
@@ -90,6 +85,35 @@ __device__ inline void dev_fail(DevErr *err, int code, int a, int b, long long c
     err->c = c;
   }
 }
+
+// Internal view of a product's index arrays. The C ABI passes the reference's
+// int32 arrays (H = int32_t); the host index mirror passes 64-bit block offsets
+// (H = int64_t) for matrices longer than 2^31 elements.
+template <class H>
+struct HaloV {
+  int32_t np_in_halo = 0, ni_in_halo = 0;
+  const int32_t *nh_part = nullptr, *j_beg = nullptr, *lab_hcell = nullptr, *lab_hcover = nullptr;
+  const int32_t *j_seq = nullptr, *ndimj = nullptr, *i_halo = nullptr, *i_hbeg = nullptr, *ndimi = nullptr;
+  const H *i_h2d = nullptr;
+  int64_t len_i_halo = 0, len_i_hbeg = 0;
+};
+
+template <class H>
+struct DescV {
+  int32_t mult_type = 0, n_prim = 0, ncovery = 0, ncoverz = 0;
+  HaloV<H> ahalo, chalo;
+  const int32_t *n_hnab = nullptr, *i_beg = nullptr, *i_prim = nullptr;
+  int64_t len_i_prim = 0;
+  int32_t b_n_parts = 0;
+  const int32_t *b_array_posn = nullptr, *b_n_atoms = nullptr, *b_part_nabs = nullptr, *b_bindex = nullptr;
+  const int32_t *b_nd_offset32 = nullptr;
+  const int64_t *b_nd_offset64 = nullptr;
+  int64_t b_len_bindex = 0;
+  const int32_t *i_cc2seq = nullptr;
+  int32_t n_cell_parts = 0;
+  int64_t lena = 0, lenb = 0, lenc = 0;
+  int64_t b_nd_offset(int32_t p) const { return b_nd_offset64 ? b_nd_offset64[p] : (int64_t)b_nd_offset32[p]; }
+};
 
 // Per halo atom k (host-computed): B-row location and labels.
 struct KMeta {
@@ -152,12 +176,12 @@ __global__ void k_jmap(const KMeta *__restrict__ km, int32_t ni_a, const int32_t
 
 // One wavefront per (k, i) item: count contributing j (and the item's flops).
 // When `fill` is set, write the products at the item's offset in j order.
-template <bool FILL>
+template <bool FILL, class H>
 __global__ void __launch_bounds__(256) k_items(
     const KMeta *__restrict__ km, const int32_t *__restrict__ item_k, const int32_t *__restrict__ i_prim,
     int64_t n_items, const int32_t *__restrict__ jh_arr, const int32_t *__restrict__ nd2_arr,
-    const int64_t *__restrict__ boff_arr, const int32_t *__restrict__ c_i_h2d, int32_t ni_c,
-    const int32_t *__restrict__ a_i_h2d, int32_t ni_a, const int32_t *__restrict__ ndimi, int32_t mult_type,
+    const int64_t *__restrict__ boff_arr, const H *__restrict__ c_i_h2d, int32_t ni_c,
+    const H *__restrict__ a_i_h2d, int32_t ni_a, const int32_t *__restrict__ ndimi, int32_t mult_type,
     int64_t len_p, int32_t *__restrict__ counts, int64_t *__restrict__ flops,
     const int64_t *__restrict__ item_off, int32_t *__restrict__ key_out, int64_t *__restrict__ p_out,
     int64_t *__restrict__ q_out, uint8_t *__restrict__ k_out, DevErr *err) {
@@ -275,7 +299,8 @@ const char *err_name(int code) {
   }
 }
 
-int check_halo(const cqmm_halo &h, int32_t n_prim, const char *name) {
+template <class H>
+int check_halo(const HaloV<H> &h, int32_t n_prim, const char *name) {
   if (h.np_in_halo <= 0 || h.ni_in_halo <= 0)
     return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s is empty", name);
   if (!h.nh_part || !h.j_beg || !h.lab_hcell || !h.lab_hcover || !h.j_seq || !h.ndimj || !h.i_h2d ||
@@ -301,12 +326,8 @@ int check_halo(const cqmm_halo &h, int32_t n_prim, const char *name) {
   return CQMM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
-  if (!d || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: null argument");
+template <class H>
+int build_plan(const DescV<H> *d, cqmm_plan **out) {
   *out = nullptr;
   int st = cq::ensure_init();
   if (st) return st;
@@ -317,12 +338,14 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
     return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: non-positive size in descriptor");
   if ((st = check_halo(d->ahalo, d->n_prim, "ahalo"))) return st;
   if ((st = check_halo(d->chalo, d->n_prim, "chalo"))) return st;
-  const cqmm_halo &A = d->ahalo, &C = d->chalo;
-  const cqmm_trans &T = d->ltrans;
-  const cqmm_bindex &B = d->bmat;
+  const HaloV<H> &A = d->ahalo, &C = d->chalo;
+  struct { const int32_t *n_hnab, *i_beg, *i_prim; int64_t len_i_prim; } T = {d->n_hnab, d->i_beg, d->i_prim, d->len_i_prim};
+  struct { int32_t n_parts; const int32_t *array_posn, *n_atoms, *part_nabs, *bindex; int64_t len_bindex; } B = {
+      d->b_n_parts, d->b_array_posn, d->b_n_atoms, d->b_part_nabs, d->b_bindex, d->b_len_bindex};
   if (!T.n_hnab || !T.i_beg || !T.i_prim || T.len_i_prim <= 0)
     return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: ltrans incomplete");
-  if (!B.array_posn || !B.n_atoms || !B.part_nabs || !B.nd_offset || !B.bindex || B.n_parts <= 0 || B.len_bindex <= 0)
+  if (!B.array_posn || !B.n_atoms || !B.part_nabs || (!d->b_nd_offset32 && !d->b_nd_offset64) || !B.bindex ||
+      B.n_parts <= 0 || B.len_bindex <= 0)
     return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: bmat incomplete");
   if (!d->i_cc2seq || d->n_cell_parts <= 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: i_cc2seq missing");
   const int32_t ni_a = A.ni_in_halo, ni_c = C.ni_in_halo;
@@ -363,7 +386,7 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
       m.idx_seq = base + 3 * size1 + (ibaddr - 1);
       m.idx_pxyz = base + 3 * size1 + size2 + 3 * (int64_t)(ibaddr - 1);
       m.idx_ndim = base + 3 * size1 + 4 * size2 + (ibaddr - 1);
-      m.bdata = (int64_t)B.nd_offset[ipart - 1] + first_nd_acc - 1 + ibndacc - 1;
+      m.bdata = d->b_nd_offset(ipart - 1) + first_nd_acc - 1 + ibndacc - 1;
       m.jrow = jrow;
       jrow += m.nbnab;
     }
@@ -424,9 +447,9 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
   if ((st = d_counts.alloc(n_items * 4))) return st;
   if ((st = d_flops.alloc(n_items * 8))) return st;
   const unsigned gi = grid1(n_items, 4);
-  hipLaunchKernelGGL(k_items<false>, dim3(gi), dim3(256), 0, s, d_km.as<KMeta>(), d_itemk.as<int32_t>(),
+  hipLaunchKernelGGL((k_items<false, H>), dim3(gi), dim3(256), 0, s, d_km.as<KMeta>(), d_itemk.as<int32_t>(),
                      d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
-                     d_ci_h2d.as<int32_t>(), ni_c, d_ai_h2d.as<int32_t>(), ni_a, d_ndimi.as<int32_t>(),
+                     d_ci_h2d.as<H>(), ni_c, d_ai_h2d.as<H>(), ni_a, d_ndimi.as<int32_t>(),
                      d->mult_type, len_p, d_counts.as<int32_t>(), d_flops.as<int64_t>(), nullptr, nullptr, nullptr,
                      nullptr, nullptr, d_err.as<DevErr>());
   if ((st = cq::hip_check(hipGetLastError(), "builder kernel launch"))) return st;
@@ -470,9 +493,9 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
   if ((st = d_q.alloc(nt1 * 8))) return st;
   if (d->mult_type == 1 && (st = d_key.alloc(nt1 * 4))) return st;
   if (!uniform && (st = d_k.alloc(nt1))) return st;
-  hipLaunchKernelGGL(k_items<true>, dim3(gi), dim3(256), 0, s, d_km.as<KMeta>(), d_itemk.as<int32_t>(),
+  hipLaunchKernelGGL((k_items<true, H>), dim3(gi), dim3(256), 0, s, d_km.as<KMeta>(), d_itemk.as<int32_t>(),
                      d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
-                     d_ci_h2d.as<int32_t>(), ni_c, d_ai_h2d.as<int32_t>(), ni_a, d_ndimi.as<int32_t>(),
+                     d_ci_h2d.as<H>(), ni_c, d_ai_h2d.as<H>(), ni_a, d_ndimi.as<int32_t>(),
                      d->mult_type, len_p, nullptr, nullptr, d_itemoff.as<int64_t>(), d_key.as<int32_t>(),
                      d_p.as<int64_t>(), d_q.as<int64_t>(), uniform ? nullptr : d_k.as<uint8_t>(), d_err.as<DevErr>());
   if ((st = cq::hip_check(hipGetLastError(), "builder fill launch"))) return st;
@@ -511,13 +534,14 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
     std::vector<uint8_t> seg_mn;
     const int64_t nc_all = (int64_t)d->n_prim * ni_c;
     for (int64_t pos = 0; pos < nc_all; ++pos) {
-      const int32_t off = C.i_h2d[pos];
+      const int64_t off = (int64_t)C.i_h2d[pos];
       if (off == 0) continue;
       const int32_t i = (int32_t)(pos / ni_c), jh = (int32_t)(pos % ni_c);
       if (off < 1 || (int64_t)off - 1 + (int64_t)C.ndimi[i] * C.ndimj[jh] > d->lenc)
-        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: C block (%d,%d) offset %d invalid", i + 1, jh + 1, off);
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: C block (%d,%d) offset %lld invalid", i + 1, jh + 1,
+                              (long long)off);
       qkey.push_back((int32_t)pos);
-      seg_out.push_back((int64_t)off - 1);
+      seg_out.push_back(off - 1);
       if (!uniform) {
         seg_mn.push_back((uint8_t)C.ndimi[i]);
         seg_mn.push_back((uint8_t)C.ndimj[jh]);
@@ -569,6 +593,65 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
                           err_name(herr.code), herr.a + 1, herr.b + 1, herr.c);
   *out = plan.release();
   return CQMM_OK;
+}
+
+template <class H>
+void halo_view(const cqmm_halo &h, HaloV<H> &v) {
+  v.np_in_halo = h.np_in_halo; v.ni_in_halo = h.ni_in_halo;
+  v.nh_part = h.nh_part; v.j_beg = h.j_beg; v.lab_hcell = h.lab_hcell; v.lab_hcover = h.lab_hcover;
+  v.j_seq = h.j_seq; v.ndimj = h.ndimj; v.i_halo = h.i_halo; v.i_hbeg = h.i_hbeg; v.ndimi = h.ndimi;
+  v.i_h2d = h.i_h2d; v.len_i_halo = h.len_i_halo; v.len_i_hbeg = h.len_i_hbeg;
+}
+
+void halo_view64(const cq::RangeIndex &r, HaloV<int64_t> &v) {
+  v.np_in_halo = r.np_in_halo; v.ni_in_halo = r.ni_in_halo;
+  v.nh_part = r.nh_part.data(); v.j_beg = r.j_beg.data(); v.lab_hcell = r.lab_hcell.data();
+  v.lab_hcover = r.lab_hcover.data(); v.j_seq = r.j_seq.data(); v.ndimj = r.ndimj_h.data();
+  v.i_halo = r.i_halo.data(); v.i_hbeg = r.i_hbeg.data(); v.ndimi = r.ndimi.data();
+  v.i_h2d = r.i_h2d64.data(); v.len_i_halo = (int64_t)r.i_halo.size(); v.len_i_hbeg = (int64_t)r.i_hbeg.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
+  if (!d || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: null argument");
+  DescV<int32_t> v;
+  v.mult_type = d->mult_type; v.n_prim = d->n_prim; v.ncovery = d->ncovery; v.ncoverz = d->ncoverz;
+  halo_view(d->ahalo, v.ahalo);
+  halo_view(d->chalo, v.chalo);
+  v.n_hnab = d->ltrans.n_hnab; v.i_beg = d->ltrans.i_beg; v.i_prim = d->ltrans.i_prim; v.len_i_prim = d->ltrans.len_i_prim;
+  v.b_n_parts = d->bmat.n_parts; v.b_array_posn = d->bmat.array_posn; v.b_n_atoms = d->bmat.n_atoms;
+  v.b_part_nabs = d->bmat.part_nabs; v.b_bindex = d->bmat.bindex; v.b_nd_offset32 = d->bmat.nd_offset;
+  v.b_len_bindex = d->bmat.len_bindex;
+  v.i_cc2seq = d->i_cc2seq; v.n_cell_parts = d->n_cell_parts;
+  v.lena = d->lena; v.lenb = d->lenb; v.lenc = d->lenc;
+  return build_plan(&v, out);
+}
+
+int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
+                         cqmm_plan **out) {
+  if (!sys || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: null argument");
+  const cq::System &s = sys->s;
+  const int32_t nr = (int32_t)s.ranges.size();
+  if (ra < 0 || ra >= nr || rb < 0 || rb >= nr || rc < 0 || rc >= nr)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: range id out of bounds");
+  const cq::RangeIndex &A = s.ranges[ra], &Bm = s.ranges[rb], &Cm = s.ranges[rc];
+  if (A.rcut + Bm.rcut > s.rcut_cover + 1e-9)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_plan_create: cover radius %.6g < r(A halo)+r(B) = %.6g",
+                          s.rcut_cover, A.rcut + Bm.rcut);
+  DescV<int64_t> v;
+  v.mult_type = mult_type; v.n_prim = s.natoms; v.ncovery = s.ncover[1]; v.ncoverz = s.ncover[2];
+  halo_view64(A, v.ahalo);
+  halo_view64(Cm, v.chalo);
+  v.n_hnab = A.n_hnab.data(); v.i_beg = A.i_beg.data(); v.i_prim = A.i_prim.data(); v.len_i_prim = (int64_t)A.i_prim.size();
+  v.b_n_parts = s.n_parts; v.b_array_posn = Bm.array_posn.data(); v.b_n_atoms = Bm.n_atoms_part.data();
+  v.b_part_nabs = Bm.part_nabs.data(); v.b_bindex = Bm.bindex.data(); v.b_nd_offset64 = Bm.nd_offset64.data();
+  v.b_len_bindex = (int64_t)Bm.bindex.size();
+  v.i_cc2seq = s.i_cc2seq.data(); v.n_cell_parts = s.n_parts;
+  v.lena = A.length; v.lenb = Bm.length; v.lenc = Cm.length;
+  return build_plan(&v, out);
 }
 
 int cqmm_plan_destroy(cqmm_plan *plan) {

@@ -13,6 +13,8 @@
 
 #include "cq_common.h"
 #include "cq_system.h"
+#include "../../include/cqmm_matrix.h"
+#include <unordered_map>
 
 using cq::RangeIndex;
 using cq::System;
@@ -581,6 +583,83 @@ int cqmm_sys_blocks(const cqmm_system *sys, int32_t rid, int32_t *prim, int32_t 
       if (nd1) nd1[e] = s.pnsf[p];
       if (nd2) nd2[e] = r.nb_ndimj[e];
     }
+  return CQMM_OK;
+}
+
+int cqmm_sys_matrix_index(const cqmm_system *sys, int32_t rid, cqmm_matrix_index *idx) {
+  if (!sys || !idx || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_matrix_index: bad argument");
+  const System &s = sys->s;
+  RangeIndex &r = const_cast<RangeIndex &>(s.ranges[rid]);
+  if (r.nab_offset.empty()) {
+    r.nab_offset.assign(s.n_parts, 0);
+    int64_t acc = 0;
+    for (int32_t nn = 0; nn < s.n_parts; ++nn) {
+      r.nab_offset[nn] = (int32_t)acc;
+      acc += r.part_nabs[nn];
+    }
+    r.onsite.assign(s.natoms, 0);
+    for (int32_t p = 0; p < s.natoms; ++p)
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
+        const int32_t m = r.nb_cover[e], ic = s.c_part[m];
+        if (s.c_prim[m] == p && s.cs_shift[3 * ic] == 0 && s.cs_shift[3 * ic + 1] == 0 && s.cs_shift[3 * ic + 2] == 0)
+          r.onsite[p] = r.nb_off[e] + 1;
+      }
+  }
+  idx->n_parts = s.n_parts;
+  idx->n_atoms = r.n_atoms_part.data();
+  idx->nab_offset = r.nab_offset.data();
+  idx->nd_offset = r.nd_offset64.data();
+  idx->n_prim = s.natoms;
+  idx->n_nab = r.n_nab.data();
+  idx->i_acc = r.i_acc.data();
+  idx->i_nd_acc = r.i_nd_acc.data();
+  idx->ndimi = r.ndimi.data();
+  idx->onsite = r.onsite.data();
+  idx->i_part = r.nb_part.data();
+  idx->i_seq = r.nb_seq.data();
+  idx->ndimj = r.nb_ndimj.data();
+  idx->n_nabs = (int64_t)r.nb_part.size();
+  idx->length = r.length;
+  return CQMM_OK;
+}
+
+int cqmm_sys_transpose_pairs(const cqmm_system *sys, int32_t rid, int64_t *npairs, int64_t *p, int64_t *q,
+                             int32_t *n1, int32_t *n2) {
+  if (!sys || !npairs || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_transpose_pairs: bad argument");
+  const System &s = sys->s;
+  const RangeIndex &r = s.ranges[rid];
+  auto key = [&](int32_t gi, int32_t gj, int32_t sx, int32_t sy, int32_t sz) {
+    return (((((uint64_t)(uint32_t)gi * (uint64_t)s.natoms + (uint32_t)gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32));
+  };
+  std::unordered_map<uint64_t, int64_t> where;
+  where.reserve((size_t)r.row_beg[s.natoms] * 2);
+  std::vector<uint64_t> kt(r.row_beg[s.natoms]);
+  for (int32_t pp = 0; pp < s.natoms; ++pp)
+    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+      const int32_t m = r.nb_cover[e], ic = s.c_part[m];
+      const int32_t *sh = &s.cs_shift[3 * ic];
+      const int32_t gi = s.prim_gid[pp], gj = s.prim_gid[s.c_prim[m]];
+      where[key(gi, gj, sh[0], sh[1], sh[2])] = e;
+      kt[e] = key(gj, gi, -sh[0], -sh[1], -sh[2]);
+    }
+  int64_t n = 0;
+  for (int32_t pp = 0; pp < s.natoms; ++pp)
+    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+      auto it = where.find(kt[e]);
+      if (it == where.end()) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
+      const int64_t f = it->second;
+      if (f < e) continue;
+      if (p) {
+        p[n] = r.nb_off[e] + 1;
+        q[n] = r.nb_off[f] + 1;
+        n1[n] = s.pnsf[pp];
+        n2[n] = r.nb_ndimj[e];
+      }
+      ++n;
+    }
+  *npairs = n;
   return CQMM_OK;
 }
 

@@ -44,6 +44,9 @@ struct RangeIndex {
   std::vector<int32_t> n_hnab, i_beg, i_prim;
   std::vector<int64_t> i_nd_beg64;
   std::vector<int32_t> i_nd_beg32;
+  // Flattened type(matrix) for cqmm_matrix_index.
+  std::vector<int32_t> nab_offset;
+  std::vector<int64_t> onsite;
   bool fits32 = true;
 };
 

@@ -1,0 +1,148 @@
+/*
+ * cqmm_matrix.h — device-resident matrix API around the product, mirroring
+ * CONQUEST's mult_module (src/mult_module.f90): integer matrix handles, a LIFO
+ * temporary-matrix stack, matrix_product / matrix_sum / matrix_scale /
+ * matrix_product_trace / matrix_trace / symmetrise_L, all on one GPU, and the
+ * density-matrix loops that call them (McWeeny purification, Hotelling S^-1).
+ *
+ * A context holds, per range id, the reference's type(matrix) index of that
+ * range (mat(:,range), src/matrix_data_module.f90:76-133), and per mult id one
+ * cqmm_plan (the mult(x) table, src/mult_module.f90:271-1102). Matrix data live
+ * in HBM for the life of the handle.
+ */
+#ifndef CQMM_MATRIX_H
+#define CQMM_MATRIX_H
+
+#include <stdint.h>
+
+#include "cqmm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* type(matrix) of all primary-set partitions for one process, flattened
+ * (src/matrix_module.f90:63-82; filled by get_naba / make_npxyz,
+ * src/matrix_elements_module.f90:257-384,836-903). Per-atom arrays are in
+ * primary order; i_acc / i_nd_acc stay partition-relative as in the reference;
+ * per-neighbour arrays are the partitions' neighbour lists concatenated. */
+typedef struct cqmm_matrix_index {
+  int32_t n_parts;
+  const int32_t *n_atoms;     /* [n_parts]                                   */
+  const int32_t *nab_offset;  /* [n_parts] amat(nn)%offset                   */
+  const int64_t *nd_offset;   /* [n_parts] amat(nn)%nd_offset (64-bit here: one GPU holds
+                                 matrices longer than 2^31 elements)         */
+  int32_t n_prim;
+  const int32_t *n_nab;       /* [n_prim]                                    */
+  const int32_t *i_acc;       /* [n_prim] partition-relative, 1-based        */
+  const int32_t *i_nd_acc;    /* [n_prim] partition-relative, 1-based        */
+  const int32_t *ndimi;       /* [n_prim]                                    */
+  const int64_t *onsite;      /* [n_prim] 1-based offset of the onsite block, 0 if none */
+  const int32_t *i_part;      /* [n_nabs] cover-set partition of neighbour   */
+  const int32_t *i_seq;       /* [n_nabs] sequence number in that partition  */
+  const int32_t *ndimj;       /* [n_nabs]                                    */
+  int64_t n_nabs;
+  int64_t length;             /* amat(1)%length                              */
+} cqmm_matrix_index;
+
+typedef struct cqmm_ctx cqmm_ctx;
+
+int cqmm_ctx_create(cqmm_ctx **ctx);
+int cqmm_ctx_destroy(cqmm_ctx *ctx);
+/* Register range `range` (0 <= range < 64) with its index (copied). */
+int cqmm_ctx_add_range(cqmm_ctx *ctx, int32_t range, const cqmm_matrix_index *idx);
+/* Transpose pairs of a range for symmetrise_L / matrix_transpose
+ * (src/mult_module.f90:1856-1893, 2609-2673): 1-based offsets p, q of blocks
+ * (i,j,s) and (j,i,-s) and their dimensions n1 x n2 of block p. Every block
+ * appears in exactly one pair (p == q for blocks that are their own transpose). */
+int cqmm_ctx_set_transpose(cqmm_ctx *ctx, int32_t range, int64_t npairs, const int64_t *p, const int64_t *q,
+                           const int32_t *n1, const int32_t *n2);
+/* mult(mult_id) with operand ranges as matrix_product passes them:
+ * type 1 C(rc) = A(ra).B(rb); type 2 A(ra) = C(rc).B(rb)^T. Takes ownership of plan. */
+int cqmm_ctx_add_mult(cqmm_ctx *ctx, int32_t mult_id, int32_t ra, int32_t rb, int32_t rc, cqmm_plan *plan);
+
+/* allocate_temp_matrix / free_temp_matrix (src/mult_module.f90:3269-3377):
+ * zero-filled, handles freed strictly in reverse order of allocation. */
+int cqmm_matrix_allocate(cqmm_ctx *ctx, int32_t range, int32_t *handle);
+int cqmm_matrix_free(cqmm_ctx *ctx, int32_t handle);
+int cqmm_matrix_length(const cqmm_ctx *ctx, int32_t handle, int64_t *length);
+int cqmm_matrix_range(const cqmm_ctx *ctx, int32_t handle, int32_t *range);
+int cqmm_matrix_device_ptr(const cqmm_ctx *ctx, int32_t handle, double **ptr);
+int cqmm_matrix_set(cqmm_ctx *ctx, int32_t handle, const double *host);
+int cqmm_matrix_get(cqmm_ctx *ctx, int32_t handle, double *host);
+
+/* matrix_product(A, B, C, mult) (src/mult_module.f90:2556-2606). */
+int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t mult_id);
+/* matrix_sum(alpha, A, beta, B) (src/mult_module.f90:2677-2715): A = alpha*A + beta*B
+ * on the blocks common to both patterns (matrix_add / addscan,
+ * src/multiply_module.f90:807-943); blocks of A absent from B are untouched. */
+int cqmm_matrix_sum(cqmm_ctx *ctx, double alpha, int32_t A, double beta, int32_t B);
+/* matrix_scale (src/mult_module.f90:2718-2738). */
+int cqmm_matrix_scale(cqmm_ctx *ctx, double alpha, int32_t A);
+/* matrix_product_trace (src/mult_module.f90:2885-2916): sum_p A_p*B_p, same range,
+ * deterministic FP64 reduction. */
+int cqmm_matrix_product_trace(cqmm_ctx *ctx, int32_t A, int32_t B, double *result);
+/* matrix_trace (src/mult_module.f90:2773-2811): sum of onsite diagonals. */
+int cqmm_matrix_trace(cqmm_ctx *ctx, int32_t A, double *result);
+/* symmetrise_L (src/mult_module.f90:1856-1893) for any range with transpose pairs:
+ * A = A/2 + A^T/2, in place. */
+int cqmm_matrix_symmetrise(cqmm_ctx *ctx, int32_t A);
+
+/* ---- McWeeny purification (src/McWeeny.f90) ------------------------------ */
+typedef struct cqmm_mcw_setup {
+  int32_t range_L, range_S, range_H, range_LS, range_LSL;
+  int32_t mult_L_S_LS;     /* type 1: LS  = L.S      (L_S_LS,  src/mult_module.f90:583) */
+  int32_t mult_LS_L_LSL;   /* type 1: LSL = LS.L     (LS_L_LSL, :687)                  */
+  int32_t mult_LSL_SL_L;   /* type 2: LSLSL(L) = LSL.(LS)^T (LSL_SL_L, :729)           */
+  int32_t max_iter;        /* n_L_iterations                                          */
+  int32_t precision;       /* 0: all FP64; bulk products with FP32 MFMA on FP32 operands,
+                              then FP64 refinement: 1 sums each block's products in FP64
+                              registers, 2 sums them in the FP32 MFMA accumulator     */
+  int32_t n_final_fp64;    /* FP64 iterations after the FP32 phase (>= 2)              */
+  double fp32_tol;         /* FP32 phase ends when |dE| < fp32_tol (Ha, per spin)      */
+} cqmm_mcw_setup;
+
+typedef struct cqmm_mcw_result {
+  int32_t iterations;      /* McWeeny steps taken (accepted + the final rejected one)   */
+  int32_t fp32_iterations; /* of which with FP32 products                              */
+  int32_t stop_reason;     /* 1 energy rose, 2 |dE| < 1e-8, 3 c outside [0,1], 4 max_iter */
+  int32_t restored;        /* 1 when fewer than 3 steps ran and L was restored         */
+  double energy;           /* Tr[rho H] of the last accepted rho (per spin)           */
+  double electrons;        /* Tr[rho S] (per spin)                                     */
+  double c;                /* last mid-point c                                         */
+  double seconds;          /* device wall time of the loop                             */
+} cqmm_mcw_result;
+
+/* McWMin (src/McWeeny.f90:107-272) over handles matH, matS, matL (one spin,
+ * fixed population): each step is McW_matrix_multiply_nonds (:620-740). On
+ * return matL holds the purified density matrix. */
+int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *setup, int32_t matH, int32_t matS, int32_t matL,
+                 cqmm_mcw_result *result);
+
+/* InitMcW (src/McWeeny.f90:340-570): L0 = ((lambda*mubar + Ne)/No) T - (lambda/No) THT
+ * from T ~ S^-1, with Gershgorin-type limits of H (GetHLimits, :1020-1109).
+ * Products: T_S_TS (type 1), T_H_TH (type 1), TH_T_L (type 1, result at L range). */
+typedef struct cqmm_initmcw_setup {
+  int32_t range_T, range_S, range_H, range_L, range_TS, range_TH;
+  int32_t mult_T_S_TS, mult_T_H_TH, mult_TH_T_L;
+  double n_electrons;      /* electrons per spin                                      */
+} cqmm_initmcw_setup;
+int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *setup, int32_t matH, int32_t matS, int32_t matT,
+                      int32_t matL, double out_mu_lambda[2]);
+
+/* Hotelling iteration for T ~ S^-1 (Iter_Hott_InvS, src/S_matrix_module.f90:746-963):
+ * T <- 2T - T.S.T with products T_S_TS (type 1) and TS_T_T (type 2, result at T
+ * range); stops when the change of Tr[TS]/n_orb is below tol or after max_iter. */
+typedef struct cqmm_hotelling_setup {
+  int32_t range_T, range_S, range_TS;
+  int32_t mult_T_S_TS, mult_TS_T_T;
+  int32_t max_iter;
+  double tol;
+} cqmm_hotelling_setup;
+int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *setup, int32_t matS, int32_t matT,
+                   int32_t *iterations, double *omega);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CQMM_MATRIX_H */

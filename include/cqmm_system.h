@@ -65,6 +65,21 @@ int cqmm_sys_get_info(const cqmm_system *sys, cqmm_sys_info *info);
 int cqmm_sys_mult_desc(const cqmm_system *sys, int32_t mult_type, int32_t ra,
                        int32_t rb, int32_t rc, cqmm_mult_desc *desc);
 
+/* Plan of a product straight from the system's indices (64-bit block offsets,
+ * for matrices longer than the reference's int32 arrays can address); same
+ * products and order as cqmm_plan_create on cqmm_sys_mult_desc. */
+int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
+                         cqmm_plan **plan);
+
+/* type(matrix) index of a range as cqmm_ctx_add_range takes it (pointers stay
+ * valid while the system lives). Requires cqmm_matrix.h for the struct. */
+struct cqmm_matrix_index;
+int cqmm_sys_matrix_index(const cqmm_system *sys, int32_t range_id, struct cqmm_matrix_index *idx);
+/* Transpose pairs (p <= q, 1-based offsets) of a range: block (i,j,s) and block
+ * (j,i,-s). Pass NULL arrays to get the count in *npairs first. */
+int cqmm_sys_transpose_pairs(const cqmm_system *sys, int32_t range_id, int64_t *npairs, int64_t *p, int64_t *q,
+                             int32_t *n1, int32_t *n2);
+
 /* Block list of a range in storage order: primary atom (0-based, primary
  * order), global ids of the row and column atoms, the column atom's periodic
  * image shift in cells, the 0-based data offset, and block dimensions. Any

@@ -1,0 +1,280 @@
+"""McWeeny purification oracle (test infrastructure only).
+
+Two restatements of McW_matrix_multiply_nonds + McWMin (src/McWeeny.f90:620-740,
+107-272), InitMcW (:340-570, GetHLimits :1020-1109) and the Hotelling S^-1
+iteration (src/S_matrix_module.f90:746-1034):
+
+  * dense_*  — on dense matrices (an isolated cluster, where no range truncates
+    anything), pinned by closed forms: the step is the polynomial map
+    x -> ((1+c)x^2 - x^3 + (1-2c)x)/(1-c) (c < 1/2) or ((1+c)x^2 - x^3)/c on the
+    eigenvalues x of S^{1/2} L S^{1/2}, with c = sum(x^2-x^3)/sum(x-x^2); the
+    converged L is the S-metric projector onto the lowest Ne eigenvectors of
+    H c = e S c (scipy.linalg.eigh).
+  * Sparse    — the same loop on image-resolved block matrices in the reference
+    layout, products through the C oracle's mat_mult, range changes through
+    block-identity maps (matrix_add's semantics: only blocks common to both
+    patterns change).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import brute
+from . import mat_mult as oracle_mat_mult
+
+
+# ---------------------------------------------------------------- dense forms
+def dense_step(L, S, H):
+    """One McWeeny step on dense matrices: returns (rho_new, E, Ne, c)."""
+    LS = L @ S
+    LSL = LS @ L
+    LSLSL = LSL @ S @ L
+    c1 = np.sum(S * (LSL - LSLSL))
+    c2 = np.sum(S * (L - LSL))
+    c = c1 / c2
+    if c < 0.5:
+        rho = ((1 + c) * LSL - LSLSL + (1 - 2 * c) * L) / (1 - c)
+    else:
+        rho = ((1 + c) * LSL - LSLSL) / c
+    E = np.sum(rho * H)
+    Ne = np.sum(rho * S)
+    # A nearly idempotent L makes c2 vanish; the reference then returns c = -0.01,
+    # which ends McWMin (src/McWeeny.f90:737).
+    if abs(c2) < 1e-6:
+        c = -0.01
+    return rho, E, Ne, c
+
+
+def step_polynomial(x):
+    """Closed form of the step on eigenvalues x of S^1/2 L S^1/2."""
+    c = np.sum(x ** 2 - x ** 3) / np.sum(x - x ** 2)
+    if c < 0.5:
+        return ((1 + c) * x ** 2 - x ** 3 + (1 - 2 * c) * x) / (1 - c), c
+    return ((1 + c) * x ** 2 - x ** 3) / c, c
+
+
+def dense_mcwmin(L, S, H, max_iter=100):
+    """McWMin on dense matrices (same stop tests). Returns (L, E, Ne, iterations)."""
+    oldE = 1e30
+    Lold = L.copy()
+    E = Ne = None
+    n = 0
+    for n in range(1, max_iter + 1):
+        rho, omega, ne, c = dense_step(L, S, H)
+        if omega > oldE or abs(omega - oldE) < 1e-8 or c > 1 or c < 0:
+            break
+        L = 0.5 * (rho + rho.T)
+        oldE, E, Ne = omega, omega, ne
+    else:
+        n = max_iter + 1
+    if n < 3:
+        L = Lold
+    return L, E, Ne, n
+
+
+def projector(H, S, n_occ):
+    """S-metric density matrix of the lowest n_occ states of H c = e S c."""
+    import scipy.linalg
+    e, V = scipy.linalg.eigh(H, S)
+    Vo = V[:, :n_occ]
+    return Vo @ Vo.T, e
+
+
+def hotelling_dense(S, tol=1e-10, max_iter=50):
+    """Iter_Hott_InvS on dense matrices; returns T."""
+    n = S.shape[0]
+    I = np.eye(n)
+    T = S / np.sum(S * S)
+    deltaomega = oldomega = 0.0
+    Told = T.copy()
+    for it in range(1, max_iter + 1):
+        deltaomega *= 0.5
+        if not (it < 3 or abs(deltaomega) > tol):
+            break
+        TS = T @ S
+        A = I - TS
+        T1 = 2 * T - TS @ T
+        omega = np.sum(A * A)
+        deltaomega = omega - oldomega
+        if omega > oldomega and oldomega != 0.0:
+            T = Told
+            break
+        oldomega = omega
+        Told = T.copy()
+        T = T1
+    return T
+
+
+def hlimits_dense_blocks(H, nsf):
+    """GetHLimits' sums for a dense H of an isolated cluster (onsite blocks add |H|,
+    others subtract), indexed per column orbital of each neighbour block."""
+    off = np.concatenate([[0], np.cumsum(nsf)])
+    hmin, hmax = 1e30, -1e30
+    for a in range(len(nsf)):
+        lo = np.zeros(64)
+        hi = np.zeros(64)
+        for b in range(len(nsf)):
+            blk = np.abs(H[off[a]:off[a + 1], off[b]:off[b + 1]])
+            col = blk.sum(axis=0)
+            lo[:nsf[b]] += col if a == b else -col
+            hi[:nsf[b]] += col
+        hmin = min(hmin, lo[:nsf[a]].min())
+        hmax = max(hmax, hi[:nsf[a]].max())
+    return hmin, hmax
+
+
+def init_mcw_dense(H, S, T, ne, nsf, r_L_mask=None):
+    """InitMcW on dense matrices: L0 = ((lambda*mubar + Ne)/No) T - (lambda/No) T H T."""
+    n_o = float(np.sum(nsf))
+    SX = np.trace(T @ S)
+    THT = T @ H @ T
+    SXHX = np.sum(S * THT)
+    hmin, hmax = hlimits_dense_blocks(H, nsf)
+    A = 1.0 - SX / n_o
+    mu1 = SXHX / n_o + hmax * A
+    mu2 = (hmin * A * ne / (n_o - ne) - SXHX / n_o) / (n_o * A / (n_o - ne) - 1.0)
+    if ne / (hmax - mu1) < (n_o - ne) / (mu1 - hmin):
+        mubar, lam = mu1, ne / (hmax - mu1)
+    elif ne / (hmax - mu2) > (n_o - ne) / (mu2 - hmin):
+        mubar, lam = mu2, (n_o - ne) / (mu2 - hmin)
+    else:
+        raise RuntimeError("InitMcW: cannot find mubar")
+    return (lam * mubar + ne) / n_o * T - lam / n_o * THT, mubar, lam
+
+
+def densify(blocks, vals, nsf):
+    """Dense matrix of an image-free pattern (all shifts zero)."""
+    if np.any(blocks["shift"] != 0):
+        raise ValueError("densify needs a pattern without periodic images")
+    off = np.concatenate([[0], np.cumsum(nsf)])
+    n = int(off[-1])
+    M = np.zeros((n, n))
+    for b in range(len(blocks["offset"])):
+        i, j = blocks["gid_i"][b], blocks["gid_j"][b]
+        n1, n2 = blocks["nd1"][b], blocks["nd2"][b]
+        o = blocks["offset"][b]
+        M[off[i]:off[i] + n1, off[j]:off[j] + n2] = vals[o:o + n1 * n2].reshape(n2, n1).T
+    return M
+
+
+def sparsify(M, blocks, nsf, length):
+    off = np.concatenate([[0], np.cumsum(nsf)])
+    v = np.zeros(length)
+    for b in range(len(blocks["offset"])):
+        i, j = blocks["gid_i"][b], blocks["gid_j"][b]
+        n1, n2 = blocks["nd1"][b], blocks["nd2"][b]
+        o = blocks["offset"][b]
+        v[o:o + n1 * n2] = M[off[i]:off[i] + n1, off[j]:off[j] + n2].T.reshape(-1)
+    return v
+
+
+# --------------------------------------------------------------- sparse form
+class Sparse:
+    """mult_module operations on host arrays of a cqmm System (oracle products)."""
+
+    def __init__(self, system, rid):
+        self.sys = system
+        self.rid = dict(rid)
+        self.blocks = {r: system.blocks(r) for r in set(rid.values())}
+        self.idx = {r: brute.BlockIndex(self.blocks[r], system.natoms) for r in set(rid.values())}
+        self.len = {r: system.length(r) for r in set(rid.values())}
+        self._maps = {}
+        self._desc = {}
+
+    def zeros(self, name):
+        return np.zeros(self.len[self.rid[name]])
+
+    def sum(self, alpha, a, ra, beta, b, rb):
+        """matrix_sum(alpha, A, beta, B): only blocks common to both patterns change."""
+        ra, rb = self.rid[ra], self.rid[rb]
+        if ra == rb:
+            a *= alpha
+            a += beta * b
+            return a
+        key = (ra, rb)
+        if key not in self._maps:
+            A, B = self.idx[ra], self.idx[rb]
+            ka = brute.keys(A.ident, self.sys.natoms)
+            pos = np.searchsorted(B.ks, ka)
+            pos = np.minimum(pos, len(B.ks) - 1)
+            ok = B.ks[pos] == ka
+            ia = np.nonzero(ok)[0]
+            ib = B.order[pos[ok]]
+            da = np.concatenate([np.arange(A.offset[x], A.offset[x] + A.nd1[x] * A.nd2[x]) for x in ia])
+            db = np.concatenate([np.arange(B.offset[y], B.offset[y] + B.nd1[y] * B.nd2[y]) for y in ib])
+            self._maps[key] = (da, db)
+        da, db = self._maps[key]
+        a[da] = alpha * a[da] + beta * b[db]
+        return a
+
+    def product(self, mult_type, ra, rb, rc, a, b, c):
+        key = (mult_type, self.rid[ra], self.rid[rb], self.rid[rc])
+        if key not in self._desc:
+            self._desc[key] = self.sys.mult_desc(mult_type, *key[1:])
+        oracle_mat_mult(self._desc[key], a, b, c)
+
+    def symmetrise(self, a, name):
+        r = self.rid[name]
+        A = self.idx[r]
+        ident = A.ident.copy()
+        tr = np.stack([ident[:, 1], ident[:, 0], -ident[:, 2], -ident[:, 3], -ident[:, 4]], axis=1)
+        partner = A.find(tr)
+        out = a.copy()
+        for x in range(len(A.offset)):
+            y = partner[x]
+            n1, n2 = A.nd1[x], A.nd2[x]
+            P = a[A.offset[x]:A.offset[x] + n1 * n2].reshape(n2, n1).T
+            Q = a[A.offset[y]:A.offset[y] + n1 * n2].reshape(n1, n2).T
+            out[A.offset[x]:A.offset[x] + n1 * n2] = (0.5 * P + 0.5 * Q.T).T.reshape(-1)
+        return out
+
+    def step(self, L, S, H):
+        """McW_matrix_multiply_nonds on host arrays: returns (rho, E, Ne, c)."""
+        LS = self.zeros("LS")
+        self.product(1, "L", "S", "LS", L, S, LS)
+        LSL = self.zeros("LSL")
+        self.product(1, "LS", "L", "LSL", LS, L, LSL)
+        LSLSL = self.zeros("L")
+        self.product(2, "L", "LS", "LSL", LSLSL, LS, LSL)
+        top = self.zeros("S")
+        top = self.sum(0.0, top, "S", 1.0, LSL, "LSL")
+        top = self.sum(1.0, top, "S", -1.0, LSLSL, "L")
+        c1 = float(np.dot(S, top))
+        bottom = self.zeros("S")
+        bottom = self.sum(0.0, bottom, "S", 1.0, L, "L")
+        bottom = self.sum(1.0, bottom, "S", -1.0, LSL, "LSL")
+        c2 = float(np.dot(S, bottom))
+        c = c1 / c2
+        rho = self.zeros("L")
+        rho = self.sum(0.0, rho, "L", -1.0, LSLSL, "L")
+        rho = self.sum(1.0, rho, "L", 1.0 + c, LSL, "LSL")
+        if c < 0.5:
+            rho = self.sum(1.0, rho, "L", 1.0 - 2.0 * c, L, "L")
+            rho *= 1.0 / (1.0 - c)
+        else:
+            rho *= 1.0 / c
+        botH = self.zeros("H")
+        botH = self.sum(0.0, botH, "H", 1.0, rho, "L")
+        E = float(np.dot(botH, H))
+        top = self.sum(0.0, top, "S", 1.0, rho, "L")
+        Ne = float(np.dot(top, S))
+        if abs(c2) < 1e-6:
+            c = -0.01
+        return rho, E, Ne, c
+
+    def mcwmin(self, L, S, H, max_iter=100):
+        oldE = 1e30
+        Lold = L.copy()
+        E = Ne = None
+        n = 0
+        for n in range(1, max_iter + 1):
+            rho, omega, ne, c = self.step(L, S, H)
+            if omega > oldE or abs(omega - oldE) < 1e-8 or c > 1 or c < 0:
+                break
+            L = self.symmetrise(rho, "L")
+            oldE, E, Ne = omega, omega, ne
+        else:
+            n = max_iter + 1
+        if n < 3:
+            L = Lold
+        return L, E, Ne, n

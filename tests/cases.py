@@ -50,3 +50,17 @@ class Case:
             c = s.fill(self.idc, 0, self.seed, tag_c, (self.decay,))
             a = np.zeros(s.length(self.ida))
         return a, b, c
+
+
+def cluster_structure(natoms, radius=3.5, dmin=1.8, box=50.0, seed=1):
+    """An isolated cluster (no periodic image within any range used with it):
+    natoms uniform in a sphere of `radius` at the centre of a cubic cell."""
+    rng = np.random.default_rng(seed)
+    pts = []
+    while len(pts) < natoms:
+        p = rng.uniform(-radius, radius, 3)
+        if np.linalg.norm(p) > radius:
+            continue
+        if all(np.linalg.norm(p - q) >= dmin for q in pts):
+            pts.append(p)
+    return np.array(pts) + box / 2.0, box

@@ -1,0 +1,138 @@
+"""GPU tests of the device matrix API and density-matrix loops against the oracle.
+
+Bars: matrix_sum / scale / traces / symmetrise match numpy restatements to
+1e-15 relative; Hotelling and InitMcW match the oracle to 1e-12; McWeeny in FP64
+matches the oracle loop (stop step within one, converged L to 1e-8, the stop
+test |dE| < 1e-8 being sensitive to summation order); on isolated clusters the purified L is the
+eigen-projector of (H, S); the mixed-precision solve's band energy is within
+1e-6 Ha/atom of the FP64 solve.
+"""
+import numpy as np
+import pytest
+
+import cqmm
+from cqmm.dm import Context, Purifier
+from cases import Case, cluster_structure
+from oracle import mcweeny as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def test_matrix_ops_match_numpy():
+    case = Case(40, ((4, .5), (9, .5)), 6.0, 9.0, 12.0, 1, seed=4)
+    s = case.sys
+    ctx = Context(s)
+    for r in sorted({case.ida, case.idb, case.idc}):
+        ctx.add_range(r, transpose=True)
+    sp = mo.Sparse(s, {"a": case.ida, "b": case.idb, "c": case.idc})
+    rng = np.random.default_rng(0)
+    va = rng.normal(size=s.length(case.ida))
+    vb = rng.normal(size=s.length(case.idb))
+    ha, hb = ctx.allocate(case.ida), ctx.allocate(case.idb)
+    ctx.set(ha, va)
+    ctx.set(hb, vb)
+    # Range-changing sums in both directions (matrix_add semantics).
+    ctx.sum(0.5, ha, -2.0, hb)
+    ref_a = sp.sum(0.5, va.copy(), "a", -2.0, vb, "b")
+    np.testing.assert_allclose(ctx.get(ha), ref_a, rtol=1e-15, atol=1e-15)
+    ctx.sum(3.0, hb, 1.5, ha)
+    ref_b = sp.sum(3.0, vb.copy(), "b", 1.5, ref_a, "a")
+    np.testing.assert_allclose(ctx.get(hb), ref_b, rtol=1e-15, atol=1e-15)
+    # Same range sum and scale.
+    hc = ctx.allocate(case.ida)
+    ctx.sum(0.0, hc, 1.0, ha)
+    ctx.scale(-0.25, hc)
+    np.testing.assert_array_equal(ctx.get(hc), -0.25 * ref_a)
+    # Traces.
+    tr = ctx.product_trace(ha, hc)
+    assert abs(tr - np.dot(ref_a, -0.25 * ref_a)) <= 1e-13 * abs(tr)
+    onsite = np.zeros(s.natoms)
+    b = s.blocks(case.ida)
+    ref_tr = 0.0
+    for t in np.nonzero((b["gid_i"] == b["gid_j"]) & np.all(b["shift"] == 0, axis=1))[0]:
+        n = b["nd1"][t]
+        ref_tr += np.trace(ref_a[b["offset"][t]:b["offset"][t] + n * n].reshape(n, n))
+    assert abs(ctx.trace(ha) - ref_tr) <= 1e-13 * abs(ref_tr)
+    # Symmetrise.
+    ctx.symmetrise(ha)
+    np.testing.assert_allclose(ctx.get(ha), sp.symmetrise(ref_a, "a"), rtol=0, atol=1e-15)
+    # LIFO discipline.
+    with pytest.raises(cqmm.CqmmError, match="out of order"):
+        ctx.free(ha)
+    ctx.free(hc)
+    ctx.free(hb)
+    ctx.free(ha)
+
+
+def cluster(natoms=9, nsf=4, seed=3):
+    xyz, box = cluster_structure(natoms, seed=seed)
+    return Purifier(xyz, box, nsf, r_S=8.0, r_H=8.0, r_L=12.0, grid=(5, 5, 5), seed=seed)
+
+
+def dense_of(pu, name, vals):
+    return mo.densify(pu.sys.blocks(pu.rid[name]), vals, pu.nsf)
+
+
+@pytest.mark.parametrize("nsf", [4, 13])
+def test_cluster_hotelling_init_and_mcweeny_match_dense_oracle(nsf):
+    pu = cluster(natoms=9, nsf=nsf)
+    S = dense_of(pu, "S", pu.S_vals)
+    H = dense_of(pu, "H", pu.H_vals)
+    it, om = pu.hotelling(max_iter=50, tol=1e-10)
+    T = dense_of(pu, "T", pu.ctx.get(pu.matT))
+    np.testing.assert_allclose(T, mo.hotelling_dense(S), atol=1e-12)
+    np.testing.assert_allclose(T, np.linalg.inv(S), atol=1e-8)
+    ne = float(pu.n_orb // 2)
+    mubar, lam = pu.init_mcweeny(ne)
+    L0 = dense_of(pu, "L", pu.ctx.get(pu.matL))
+    L0_ref, mubar_ref, lam_ref = mo.init_mcw_dense(H, S, T, ne, pu.nsf)
+    assert abs(mubar - mubar_ref) < 1e-10 and abs(lam - lam_ref) < 1e-10
+    np.testing.assert_allclose(L0, L0_ref, atol=1e-12)
+    res = pu.mcweeny(precision=0, max_iter=100)
+    L = dense_of(pu, "L", pu.ctx.get(pu.matL))
+    L_ref, E_ref, Ne_ref, n_ref = mo.dense_mcwmin(L0_ref, S, H, 100)
+    # The stop test |dE| < 1e-8 may trigger one step apart under different
+    # summation orders; the converged states agree either way.
+    assert abs(res["iterations"] - n_ref) <= 1
+    if res["iterations"] == n_ref:
+        np.testing.assert_allclose(L, L_ref, atol=1e-10)
+    np.testing.assert_allclose(L, L_ref, atol=1e-6)
+    assert abs(res["energy"] - E_ref) < 1e-5
+    P, e = mo.projector(H, S, pu.n_orb // 2)
+    assert e[pu.n_orb // 2] - e[pu.n_orb // 2 - 1] > 0.1  # the recipe opened a gap
+    assert np.abs(L - P).max() < 1e-6
+    assert abs(res["electrons"] - ne) < 1e-6
+
+
+def test_periodic_mcweeny_fp64_matches_sparse_oracle():
+    xyz, edge = cqmm.random_structure(24, 0.05, 2.0, 5)
+    pu = Purifier(xyz, edge, 4, r_S=4.5, r_H=4.5, r_L=6.0, grid=(2, 2, 2), seed=5)
+    sp = mo.Sparse(pu.sys, pu.rid)
+    pu.hotelling()
+    pu.init_mcweeny()
+    L0 = pu.ctx.get(pu.matL)
+    res = pu.mcweeny(precision=0, max_iter=60)
+    L = pu.ctx.get(pu.matL)
+    L_ref, E_ref, Ne_ref, n_ref = sp.mcwmin(L0, pu.S_vals, pu.H_vals, 60)
+    assert abs(res["iterations"] - n_ref) <= 1
+    assert abs(res["energy"] - E_ref) < 1e-8 * max(1.0, abs(E_ref))
+    assert np.abs(L - L_ref).max() < 1e-8
+
+
+def test_mixed_precision_band_energy_within_1e6_per_atom():
+    xyz, edge = cqmm.random_structure(96, 0.05, 2.0, 7)
+    pu = Purifier(xyz, edge, 13, r_S=6.0, r_H=6.0, r_L=8.0, grid=(3, 3, 3), seed=7)
+    r64 = pu.solve(precision=0, max_iter=100)
+    errs = {}
+    for prec in (1, 2):
+        r = pu.solve(precision=prec, max_iter=100)
+        assert r["fp32_iterations"] > 0
+        dE = abs(r["band_energy"] - r64["band_energy"]) / pu.natoms
+        dN = abs(r["n_electrons"] - r64["n_electrons"])
+        errs[prec] = (dE, dN)
+        print(f"precision {prec}: FP64 {r64['band_energy']:.10f} Ha ({r64['iterations']} it); mixed "
+              f"{r['band_energy']:.10f} Ha ({r['fp32_iterations']} FP32 + {r['iterations'] - r['fp32_iterations']} "
+              f"FP64 it); dE/atom {dE:.2e} dNe {dN:.2e}", flush=True)
+    # FP32 MFMA with each block's products summed in FP64: within 1e-6 Ha/atom.
+    assert errs[1][0] <= 1e-6
+    assert errs[1][1] <= 1e-6 * pu.n_orb
