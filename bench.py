@@ -83,12 +83,88 @@ def cpu_baseline(desc, a, b, c, target_s):
     }
 
 
+def purification_bench(args):
+    """cfg4: one structure, FP64 vs mixed solves; cfg5: the served request stream."""
+    from cqmm import serve as sv
+    cqmm.init(0)
+    if args.workload == "cfg4":
+        natoms, nsf = 2048, 13
+        out = {}
+        for prec in (0, 1):
+            params = sv.default_params(precision=prec)
+            runs = []
+            for k in range(max(1, args.steps)):
+                r = sv.purify(natoms, nsf, args.seed, params)
+                if r["status"] != 0:
+                    raise SystemExit(f"purify failed: {r}")
+                runs.append(r)
+                log(f"precision {prec} solve {k}: {r['solve_s']:.2f} s, {r['iterations']} it "
+                    f"({r['fp32_iterations']} FP32), E_band {r['band_energy']:.10f} Ha")
+            out[prec] = runs
+        t64 = np.median([r["solve_s"] for r in out[0]])
+        t32 = np.median([r["solve_s"] for r in out[1]])
+        e64, e32 = out[0][-1], out[1][-1]
+        line = {
+            "metric": "purification solves/s", "value": round(1.0 / t32, 5), "unit": "solves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": 0, "ms_per_step": round(t32 * 1e3, 2), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic (seeded random periodic structure; S/H recipe of the workload)",
+            "config": {"workload": "mixed-precision McWeeny, 2048 atoms, 13 SF/atom, S/H/L ranges 8/8/12 A",
+                       "fp64_solves_per_s": round(1.0 / t64, 5), "speedup_vs_fp64": round(t64 / t32, 3),
+                       "band_energy_fp64": e64["band_energy"], "band_energy_mixed": e32["band_energy"],
+                       "band_energy_err_per_atom": abs(e32["band_energy"] - e64["band_energy"]) / natoms,
+                       "n_electrons_err": abs(e32["n_electrons"] - e64["n_electrons"]),
+                       "iterations_fp64": e64["iterations"], "iterations_mixed": e32["iterations"],
+                       "fp32_iterations": e32["fp32_iterations"], "setup_s": round(e32["setup_s"], 2)},
+            "roofline": None, "cpu_baseline": None,
+        }
+        print(json.dumps(line), flush=True)
+        return
+    # cfg5: served stream
+    n = args.requests
+    reqs = sv.poisson_stream(n, args.rate, args.seed)
+    lines = {}
+    for prec in (1, 0):
+        params = sv.default_params(precision=prec)
+        log(f"serving {n} requests at {args.rate}/s, precision {prec}, {args.slots} slots")
+        res, wall = sv.serve(reqs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), params=params)
+        ok = [r for r in res if r["status"] == 0]
+        lat = np.array([r["latency_s"] for r in ok])
+        lines[prec] = dict(solves_per_s=len(ok) / wall, p50_s=float(np.percentile(lat, 50)),
+                           p99_s=float(np.percentile(lat, 99)), wall_s=wall, failed=n - len(ok),
+                           max_ne_error=max(r["ne_error"] for r in ok),
+                           energies=[r["band_energy"] for r in res])
+        log(f"precision {prec}: {len(ok)}/{n} solved in {wall:.1f} s, p50 {lines[prec]['p50_s']:.2f} s, "
+            f"p99 {lines[prec]['p99_s']:.2f} s")
+    e1 = np.array(lines[1].pop("energies"))
+    e0 = np.array(lines[0].pop("energies"))
+    na = np.array([q["natoms"] for q in reqs])
+    err = float(np.max(np.abs(e1 - e0) / na))
+    m = lines[1]
+    line = {
+        "metric": "purification solves/s", "value": round(m["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": n, "warmup": 0, "ms_per_step": round(1e3 / max(m["solves_per_s"], 1e-9), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded Poisson stream; atoms log-uniform [64, 2048]; 4 or 13 SF/atom)",
+        "config": {"workload": f"served stream of {n} purification requests, Poisson {args.rate}/s, "
+                               f"{args.slots} slots, HBM budget {args.hbm_gb} GiB",
+                   "mixed": m, "fp64": lines[0], "max_band_energy_err_per_atom": err,
+                   "speedup_vs_fp64": round(m["solves_per_s"] / lines[0]["solves_per_s"], 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4", "cfg5"])
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=2.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--hbm-gb", type=float, default=200.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
@@ -97,6 +173,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("cfg4", "cfg5"):
+        purification_bench(args)
+        return
     w = WORKLOADS[args.workload]
     t_setup = time.perf_counter()
     xyz, edge = cqmm.random_structure(w["natoms"], 0.05, 2.0, args.seed)

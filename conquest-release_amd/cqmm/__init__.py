@@ -124,6 +124,7 @@ EXPORTS = [
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
     "cqmm_matrix_scale", "cqmm_matrix_product_trace", "cqmm_matrix_trace", "cqmm_matrix_symmetrise",
     "cqmm_mcweeny", "cqmm_init_mcweeny", "cqmm_hotelling",
+    "cqmm_purify_default_params", "cqmm_purify_hbm_estimate", "cqmm_purify", "cqmm_serve",
 ]
 
 _lib = None

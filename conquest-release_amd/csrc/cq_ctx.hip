@@ -122,7 +122,7 @@ int build_summap(const RangeRec &A, const RangeRec &B, cq::RunMap &m) {
       }
     }
   }
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   m.nruns = (int64_t)dst.size();
   int64_t ne = 0;
   for (int32_t l : len) ne += l;
@@ -153,7 +153,7 @@ int summap(cqmm_ctx *ctx, int32_t ra, int32_t rb, const cq::RunMap **out) {
 }
 
 int reduce_result(cqmm_ctx *ctx, double *result) {
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   int st = cq::hip_check(hipMemcpyAsync(result, ctx->red_out.p, 8, hipMemcpyDeviceToHost, s), "D2H reduction");
   if (st) return st;
   return cq::hip_check(hipStreamSynchronize(s), "reduction");
@@ -170,7 +170,7 @@ int ensure_red(cqmm_ctx *ctx) {
 int product_f32(cqmm_ctx *ctx, int32_t mult_id, const float *A, const float *B, float *C, bool acc64) {
   auto it = ctx->mults.find(mult_id);
   if (it == ctx->mults.end()) return cq::set_errorf(CQMM_ERR_ARG, "matrix_product: mult %d not registered", mult_id);
-  return cq::launch_block_products_f32(it->second.plan->tp, A, B, C, acc64, cq::runtime().stream);
+  return cq::launch_block_products_f32(it->second.plan->tp, A, B, C, acc64, cq::stream());
 }
 
 }  // namespace
@@ -187,7 +187,7 @@ int cqmm_ctx_create(cqmm_ctx **ctx) {
 
 int cqmm_ctx_destroy(cqmm_ctx *ctx) {
   if (ctx) {
-    hipStreamSynchronize(cq::runtime().stream);
+    hipStreamSynchronize(cq::stream());
     delete ctx;
   }
   return CQMM_OK;
@@ -235,7 +235,7 @@ int cqmm_ctx_add_range(cqmm_ctx *ctx, int32_t range, const cqmm_matrix_index *id
       dn.push_back(r.ndimi[q]);
     }
   int st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   r.diag.nblk = (int64_t)doff.size();
   if ((st = r.diag.off.alloc(std::max<size_t>(1, doff.size()) * 8))) return st;
   if ((st = r.diag.nsf.alloc(std::max<size_t>(1, dn.size()) * 4))) return st;
@@ -270,7 +270,7 @@ int cqmm_ctx_set_transpose(cqmm_ctx *ctx, int32_t range, int64_t npairs, const i
     b[t] = (uint8_t)n2[t];
   }
   int st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   r.pairs.npairs = npairs;
   if ((st = r.pairs.poff.alloc(std::max<int64_t>(1, npairs) * 8))) return st;
   if ((st = r.pairs.qoff.alloc(std::max<int64_t>(1, npairs) * 8))) return st;
@@ -318,7 +318,7 @@ int cqmm_matrix_allocate(cqmm_ctx *ctx, int32_t range, int32_t *handle) {
     int st = m.data.alloc(bytes);
     if (st) return st;
   }
-  int st = cq::hip_check(hipMemsetAsync(m.data.p, 0, bytes, cq::runtime().stream), "zero matrix");
+  int st = cq::hip_check(hipMemsetAsync(m.data.p, 0, bytes, cq::stream()), "zero matrix");
   if (st) return st;
   m.live = true;
   ctx->mats.push_back(std::move(m));
@@ -363,7 +363,7 @@ int cqmm_matrix_device_ptr(const cqmm_ctx *ctx, int32_t h, double **ptr) {
 int cqmm_matrix_set(cqmm_ctx *ctx, int32_t h, const double *host) {
   int st = ctx_check(ctx, h, "cqmm_matrix_set");
   if (st) return st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   if ((st = cq::hip_check(hipMemcpyAsync(dptr(ctx, h), host, hlen(ctx, h) * 8, hipMemcpyHostToDevice, s), "H2D matrix"))) return st;
   return cq::hip_check(hipStreamSynchronize(s), "cqmm_matrix_set");
 }
@@ -371,7 +371,7 @@ int cqmm_matrix_set(cqmm_ctx *ctx, int32_t h, const double *host) {
 int cqmm_matrix_get(cqmm_ctx *ctx, int32_t h, double *host) {
   int st = ctx_check(ctx, h, "cqmm_matrix_get");
   if (st) return st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   if ((st = cq::hip_check(hipMemcpyAsync(host, dptr(ctx, h), hlen(ctx, h) * 8, hipMemcpyDeviceToHost, s), "D2H matrix"))) return st;
   return cq::hip_check(hipStreamSynchronize(s), "cqmm_matrix_get");
 }
@@ -390,13 +390,13 @@ int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t 
   if (hrange(ctx, a_role) != m.ra || hrange(ctx, B) != m.rb || hrange(ctx, c_role) != m.rc)
     return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_product: operand ranges (%d,%d,%d) do not match mult %d",
                           hrange(ctx, A), hrange(ctx, B), hrange(ctx, C), mult_id);
-  return cq::launch_block_products(tp, dptr(ctx, A), dptr(ctx, B), dptr(ctx, C), false, cq::runtime().stream);
+  return cq::launch_block_products(tp, dptr(ctx, A), dptr(ctx, B), dptr(ctx, C), false, cq::stream());
 }
 
 int cqmm_matrix_sum(cqmm_ctx *ctx, double alpha, int32_t A, double beta, int32_t B) {
   int st;
   if ((st = ctx_check(ctx, A, "cqmm_matrix_sum")) || (st = ctx_check(ctx, B, "cqmm_matrix_sum"))) return st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   const int32_t ra = hrange(ctx, A), rb = hrange(ctx, B);
   if (ra == rb) return cq::op_axpby(hlen(ctx, A), alpha, dptr(ctx, A), beta, dptr(ctx, B), s);
   const cq::RunMap *m = nullptr;
@@ -407,7 +407,7 @@ int cqmm_matrix_sum(cqmm_ctx *ctx, double alpha, int32_t A, double beta, int32_t
 int cqmm_matrix_scale(cqmm_ctx *ctx, double alpha, int32_t A) {
   int st = ctx_check(ctx, A, "cqmm_matrix_scale");
   if (st) return st;
-  return cq::op_scale(hlen(ctx, A), alpha, dptr(ctx, A), cq::runtime().stream);
+  return cq::op_scale(hlen(ctx, A), alpha, dptr(ctx, A), cq::stream());
 }
 
 int cqmm_matrix_product_trace(cqmm_ctx *ctx, int32_t A, int32_t B, double *result) {
@@ -418,7 +418,7 @@ int cqmm_matrix_product_trace(cqmm_ctx *ctx, int32_t A, int32_t B, double *resul
     return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_product_trace: matrices of different ranges");
   if ((st = ensure_red(ctx))) return st;
   if ((st = cq::op_dot(hlen(ctx, A), dptr(ctx, A), dptr(ctx, B), ctx->red_part.as<double>(), ctx->red_out.as<double>(),
-                       cq::runtime().stream)))
+                       cq::stream())))
     return st;
   return reduce_result(ctx, result);
 }
@@ -429,7 +429,7 @@ int cqmm_matrix_trace(cqmm_ctx *ctx, int32_t A, double *result) {
   if (!result) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_trace: null result");
   if ((st = ensure_red(ctx))) return st;
   const RangeRec &r = ctx->ranges[hrange(ctx, A)];
-  if ((st = cq::op_trace(r.diag, dptr(ctx, A), ctx->red_part.as<double>(), ctx->red_out.as<double>(), cq::runtime().stream)))
+  if ((st = cq::op_trace(r.diag, dptr(ctx, A), ctx->red_part.as<double>(), ctx->red_out.as<double>(), cq::stream())))
     return st;
   return reduce_result(ctx, result);
 }
@@ -439,7 +439,7 @@ int cqmm_matrix_symmetrise(cqmm_ctx *ctx, int32_t A) {
   if (st) return st;
   const RangeRec &r = ctx->ranges[hrange(ctx, A)];
   if (!r.has_pairs) return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_symmetrise: range %d has no transpose pairs", hrange(ctx, A));
-  return cq::op_symmetrise(r.pairs, dptr(ctx, A), cq::runtime().stream);
+  return cq::op_symmetrise(r.pairs, dptr(ctx, A), cq::stream());
 }
 
 }  // extern "C"
@@ -464,7 +464,7 @@ struct McwWork {
 
 int mcw_step(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t matS, int32_t matL, int32_t matRho,
              const McwWork &w, bool fp32, double *energy, double *cn_out, double *ne_out) {
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   if (fp32) {
     CQ_TRY(cq::op_to_f32(hlen(ctx, matL), dptr(ctx, matL), w.L32, s));
     const bool acc64 = su->precision == 1;
@@ -524,7 +524,7 @@ int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t 
     return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: handle ranges differ from the setup");
   std::memset(res, 0, sizeof(*res));
   const auto t0 = std::chrono::steady_clock::now();
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   // Temporaries (freed in reverse order at the end).
   int32_t oldL, rho;
   McwWork w;
@@ -623,7 +623,7 @@ int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *su, int32_t matS, 
   if ((st = ctx_check(ctx, matS, "cqmm_hotelling")) || (st = ctx_check(ctx, matT, "cqmm_hotelling"))) return st;
   if (hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T)
     return cq::set_error(CQMM_ERR_ARG, "cqmm_hotelling: handle ranges differ from the setup");
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   int32_t matI, matT1, matTold, matA, matTS, matGrad;
   CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matI));
   CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matT1));
@@ -749,7 +749,7 @@ int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH,
   CQ_TRY(cqmm_matrix_free(ctx, matXHX));
   CQ_TRY(cqmm_matrix_free(ctx, matTH));
   CQ_TRY(cqmm_matrix_free(ctx, matTS));
-  return cq::hip_check(hipStreamSynchronize(cq::runtime().stream), "cqmm_init_mcweeny");
+  return cq::hip_check(hipStreamSynchronize(cq::stream()), "cqmm_init_mcweeny");
 }
 
 }  // extern "C"

@@ -17,6 +17,10 @@ struct Runtime {
 
 Runtime &runtime();
 int ensure_init();
+// Stream for the calling thread's library work: the thread's own stream when
+// one was set with set_thread_stream, else the library stream.
+hipStream_t stream();
+void set_thread_stream(hipStream_t s);
 int hip_check(hipError_t e, const char *what);
 
 // Owning device buffer (freed with the plan that holds it).

@@ -331,7 +331,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   *out = nullptr;
   int st = cq::ensure_init();
   if (st) return st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   if (d->mult_type != 1 && d->mult_type != 2)
     return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: mult_type %d (must be 1 or 2)", d->mult_type);
   if (d->n_prim <= 0 || d->ncovery <= 0 || d->ncoverz <= 0 || d->lena <= 0 || d->lenb <= 0 || d->lenc <= 0)
@@ -656,7 +656,7 @@ int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, 
 
 int cqmm_plan_destroy(cqmm_plan *plan) {
   if (plan) {
-    hipStreamSynchronize(cq::runtime().stream);
+    hipStreamSynchronize(cq::stream());
     delete plan;
   }
   return CQMM_OK;
@@ -710,7 +710,7 @@ int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b, int
                           (long long)plan->lenb, (long long)plan->lenc);
   int st = cq::ensure_init();
   if (st) return st;
-  hipStream_t s = cq::runtime().stream;
+  hipStream_t s = cq::stream();
   const bool dev = flags & CQMM_DEVICE_PTRS;
   const bool fp32 = flags & CQMM_FP32;
   double *da = a, *dc = c;

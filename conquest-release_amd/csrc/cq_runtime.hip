@@ -32,6 +32,10 @@ static Runtime g_rt;
 
 Runtime &runtime() { return g_rt; }
 
+static thread_local hipStream_t t_stream = nullptr;
+hipStream_t stream() { return t_stream ? t_stream : g_rt.stream; }
+void set_thread_stream(hipStream_t s) { t_stream = s; }
+
 int ensure_init() {
   if (g_rt.initialised) return CQMM_OK;
   return cqmm_init(-1);
@@ -107,18 +111,18 @@ int cqmm_memcpy_h2d(void *dst, const void *src, int64_t bytes) {
   int st = cq::ensure_init();
   if (st) return st;
   if (bytes <= 0) return CQMM_OK;
-  st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, cq::g_rt.stream), "hipMemcpy H2D");
+  st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, cq::stream()), "hipMemcpy H2D");
   if (st) return st;
-  return cq::hip_check(hipStreamSynchronize(cq::g_rt.stream), "hipStreamSynchronize");
+  return cq::hip_check(hipStreamSynchronize(cq::stream()), "hipStreamSynchronize");
 }
 
 int cqmm_memcpy_d2h(void *dst, const void *src, int64_t bytes) {
   int st = cq::ensure_init();
   if (st) return st;
   if (bytes <= 0) return CQMM_OK;
-  st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, cq::g_rt.stream), "hipMemcpy D2H");
+  st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, cq::stream()), "hipMemcpy D2H");
   if (st) return st;
-  return cq::hip_check(hipStreamSynchronize(cq::g_rt.stream), "hipStreamSynchronize");
+  return cq::hip_check(hipStreamSynchronize(cq::stream()), "hipStreamSynchronize");
 }
 
 static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
@@ -128,12 +132,12 @@ int cqmm_timer_start(void) {
   if (st) return st;
   if (!g_ev0 && (st = cq::hip_check(hipEventCreate(&g_ev0), "hipEventCreate"))) return st;
   if (!g_ev1 && (st = cq::hip_check(hipEventCreate(&g_ev1), "hipEventCreate"))) return st;
-  return cq::hip_check(hipEventRecord(g_ev0, cq::g_rt.stream), "hipEventRecord");
+  return cq::hip_check(hipEventRecord(g_ev0, cq::stream()), "hipEventRecord");
 }
 
 int cqmm_timer_stop(float *ms) {
   if (!g_ev0 || !g_ev1 || !ms) return cq::set_error(CQMM_ERR_STATE, "cqmm_timer_stop: timer not started");
-  int st = cq::hip_check(hipEventRecord(g_ev1, cq::g_rt.stream), "hipEventRecord");
+  int st = cq::hip_check(hipEventRecord(g_ev1, cq::stream()), "hipEventRecord");
   if (st) return st;
   if ((st = cq::hip_check(hipEventSynchronize(g_ev1), "hipEventSynchronize"))) return st;
   return cq::hip_check(hipEventElapsedTime(ms, g_ev0, g_ev1), "hipEventElapsedTime");
@@ -142,7 +146,7 @@ int cqmm_timer_stop(float *ms) {
 int cqmm_synchronize(void) {
   int st = cq::ensure_init();
   if (st) return st;
-  return cq::hip_check(hipStreamSynchronize(cq::g_rt.stream), "hipStreamSynchronize");
+  return cq::hip_check(hipStreamSynchronize(cq::stream()), "hipStreamSynchronize");
 }
 
 }  // extern "C"

@@ -1,0 +1,77 @@
+/*
+ * cqmm_serve.h — one-call density-matrix solves for synthetic structures and a
+ * request loop that serves many of them on one GPU.
+ *
+ * No reference counterpart: CONQUEST solves one structure per run. Each solve
+ * follows CONQUEST's own sequence for the O(N) path — Hotelling S^-1
+ * (src/S_matrix_module.f90:746-1034), InitMcW (src/McWeeny.f90:340-570),
+ * McWMin (src/McWeeny.f90:107-272) — on a seeded random structure with the
+ * synthetic S and H recipe of include/cqmm_system.h (kinds 1 and 2) plus
+ * h_shift*S.
+ */
+#ifndef CQMM_SERVE_H
+#define CQMM_SERVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cqmm_purify_params {
+  double r_S, r_H, r_L;     /* ranges (Angstrom); T uses r_S                 */
+  double density, dmin;     /* structure: atoms/A^3 and minimum separation    */
+  double s_amp, s_decay;    /* S off-diagonal amplitude and decay length      */
+  double h_amp, h_decay, h_onsite, h_split, h_shift;  /* H recipe             */
+  int32_t precision;        /* 0 FP64; 1 FP32 MFMA + FP64 block sums; 2 FP32 sums */
+  int32_t max_iter;         /* McWeeny iterations at most                     */
+} cqmm_purify_params;
+
+typedef struct cqmm_purify_result {
+  int32_t status;           /* 0, or the failing call's status                 */
+  int32_t iterations, fp32_iterations, stop_reason, hotelling_iterations;
+  double band_energy;       /* 2 Tr[rho H] (Ha), spin-unpolarised               */
+  double n_electrons;       /* 2 Tr[rho S]                                      */
+  double ne_error;          /* |n_electrons - 2 floor(n_orb/2)|                 */
+  double setup_s, solve_s;  /* host+device set-up; Hotelling+InitMcW+McWeeny    */
+  int64_t hbm_estimate;     /* bytes reserved for admission                     */
+  int32_t natoms, n_orb;
+} cqmm_purify_result;
+
+/* Defaults of the served workload: S/H/L 8/8/12 A, 0.05 atoms/A^3, 2 A. */
+void cqmm_purify_default_params(cqmm_purify_params *p);
+/* HBM bytes a solve of natoms x nsf would hold (estimate used for admission). */
+int64_t cqmm_purify_hbm_estimate(int32_t natoms, int32_t nsf, const cqmm_purify_params *p);
+/* One solve on the calling thread (library stream). */
+int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_params *p, cqmm_purify_result *r);
+
+typedef struct cqmm_serve_request {
+  int32_t natoms, nsf;
+  uint64_t seed;
+  double arrival_s;         /* arrival time from the start of the run          */
+} cqmm_serve_request;
+
+typedef struct cqmm_serve_config {
+  int32_t slots;            /* requests in flight at most (worker threads, one HIP stream each) */
+  int64_t hbm_budget;       /* bytes; a request is admitted when its estimate fits */
+  cqmm_purify_params params;
+} cqmm_serve_config;
+
+typedef struct cqmm_serve_result {
+  cqmm_purify_result r;
+  double arrival_s, admit_s, done_s;  /* from the start of the run           */
+  int32_t slot;
+} cqmm_serve_result;
+
+/* Serve `n` requests: each is released at its arrival time, waits (FIFO) until
+ * a slot is free and its HBM estimate fits the budget next to the requests in
+ * flight, then runs on that slot's stream, overlapping on the device with the
+ * others. A failing request records its status and does not stop the others.
+ * wall_s receives the run's duration. */
+int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_config *cfg,
+               cqmm_serve_result *results, double *wall_s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CQMM_SERVE_H */

@@ -1,0 +1,63 @@
+"""GPU tests of the one-call solve and the request loop (continuous batching).
+
+Bars: the native solve equals the Python-driven solve of the same structure
+bit for bit; in the request loop every result equals the same request solved
+alone (bitwise: each request's arithmetic is deterministic and independent of
+what else runs), requests overlap on the device, admission respects the HBM
+budget, and a request that cannot fit fails alone.
+"""
+import numpy as np
+import pytest
+
+import cqmm
+from cqmm import serve as sv
+from cqmm.dm import Purifier
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(r_S=4.5, r_H=4.5, r_L=6.0)
+
+
+def test_native_solve_equals_python_solve():
+    params = sv.default_params(precision=0, **SMALL)
+    r = sv.purify(40, 4, 11, params)
+    assert r["status"] == 0
+    xyz, edge = cqmm.random_structure(40, 0.05, 2.0, 11)
+    pu = Purifier(xyz, edge, 4, seed=11, **SMALL)
+    ref = pu.solve(precision=0)
+    assert r["band_energy"] == ref["band_energy"]
+    assert r["iterations"] == ref["iterations"]
+    assert r["ne_error"] < 1e-8
+
+
+def test_requests_alone_equal_requests_in_the_mix():
+    params = sv.default_params(precision=1, **SMALL)
+    reqs = [dict(natoms=n, nsf=f, seed=s, arrival_s=0.01 * k)
+            for k, (n, f, s) in enumerate([(40, 4, 1), (64, 13, 2), (48, 4, 3), (56, 13, 4), (72, 4, 5), (44, 13, 6)])]
+    alone = [sv.purify(q["natoms"], q["nsf"], q["seed"], params) for q in reqs]
+    res, wall = sv.serve(reqs, slots=3, hbm_budget=64 << 30, params=params)
+    for a, m in zip(alone, res):
+        assert m["status"] == 0 and a["status"] == 0
+        assert m["band_energy"] == a["band_energy"]
+        assert m["n_electrons"] == a["n_electrons"]
+        assert m["iterations"] == a["iterations"]
+        assert m["ne_error"] < 1e-8
+    # Work of different requests overlapped on the device.
+    iv = sorted((m["admit_s"], m["done_s"]) for m in res)
+    assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
+    print(f"served {len(res)} requests in {wall:.2f} s; latencies "
+          f"{[round(m['latency_s'], 3) for m in res]}", flush=True)
+
+
+def test_admission_respects_hbm_budget_and_isolates_failures():
+    params = sv.default_params(precision=0, **SMALL)
+    sizes = [(40, 4, 1), (48, 4, 2), (44, 4, 3)]
+    est = [sv.hbm_estimate(n, f, params) for n, f, _ in sizes]
+    budget = max(est) + 1  # one request at a time fits
+    reqs = [dict(natoms=n, nsf=f, seed=s, arrival_s=0.0) for n, f, s in sizes]
+    reqs.append(dict(natoms=2000, nsf=13, seed=9, arrival_s=0.0))  # larger than the budget
+    res, wall = sv.serve(reqs, slots=3, hbm_budget=budget, params=params)
+    assert all(m["status"] == 0 for m in res[:3])
+    assert res[3]["status"] == -3  # CQMM_ERR_NOMEM: never admitted, the others unaffected
+    iv = sorted((m["admit_s"], m["done_s"]) for m in res[:3])
+    assert all(iv[k + 1][0] >= iv[k][1] - 1e-3 for k in range(2))
