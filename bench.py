@@ -46,14 +46,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_for(workload):
+def traffic_for(workload, kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(workload, {}).get("hbm_bytes_per_launch")
+        return t.get(f"{workload}:{kernel}", {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
 
@@ -168,6 +168,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run the streaming copy kernel once over this many MiB before the timed region "
+                         "(reference kernel for scaling HBM traffic counters)")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "triplet", "tile"],
+                    help="FP64 kernel: auto (tile schedule when present), triplet, tile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -200,11 +205,23 @@ def main():
     t0 = time.perf_counter()
     plan = cqmm.Plan(desc)
     t_plan = time.perf_counter() - t0
+    plan.set_kernel({"auto": cqmm.KERNEL_AUTO, "triplet": cqmm.KERNEL_TRIPLET, "tile": cqmm.KERNEL_TILE}[args.kernel])
     info = plan.info()
     flops = info.flops
-    log(f"plan: {info.n_triplets} products, {flops / 1e12:.3f} TFLOP/step, built in {t_plan:.2f} s")
+    kname = "k_tile_products_f64" if info.kernel == cqmm.KERNEL_TILE else "k_block_products_f64_u"
+    log(f"plan: {info.n_triplets} products, {flops / 1e12:.3f} TFLOP/step, built in {t_plan:.2f} s; "
+        f"kernel {kname}, {info.n_tiles} tiles, {info.n_entries} entries, "
+        f"{info.n_loads / max(info.n_triplets, 1):.3f} operand blocks per product")
     da, db, dc = (cqmm.DeviceArray.from_host(x) for x in (a, b, c))
     setup_s = time.perf_counter() - t_setup
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src, dst = cqmm.DeviceArray(n), cqmm.DeviceArray(n)
+        cqmm.device_copy(dst, src)
+        cqmm.synchronize()
+        log(f"calibration copy: {n * 8} bytes read, {n * 8} bytes written")
+        src.free()
+        dst.free()
     for _ in range(args.warmup):
         plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC)
     cqmm.synchronize()
@@ -240,6 +257,9 @@ def main():
             "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
             "flop_per_step": flops, "output_blocks": int(info.n_out_total),
             "plan_build_s": round(t_plan, 3), "setup_s": round(setup_s, 2),
+            "kernel": kname, "tiles": int(info.n_tiles),
+            "operand_blocks_per_product": round(info.n_loads / max(info.n_triplets, 1), 4)
+            if info.kernel == cqmm.KERNEL_TILE else 2.0,
         },
         "roofline": {
             "bound": "mfma",
@@ -247,8 +267,8 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": traffic_for(args.workload),
-            "kernel": "k_block_products_f64",
+            "traffic": traffic_for(args.workload, kname),
+            "kernel": kname,
             "kernel_avg_ms": round(kern_avg_ms, 4),
         },
         "cpu_baseline": None,

@@ -62,7 +62,11 @@ class MultDesc(C.Structure):
 
 class PlanInfo(C.Structure):
     _fields_ = [("n_triplets", C.c_int64), ("n_out_blocks", C.c_int64), ("n_out_total", C.c_int64),
-                ("flops", C.c_double), ("uniform_nsf", C.c_int32), ("mult_type", C.c_int32)]
+                ("flops", C.c_double), ("uniform_nsf", C.c_int32), ("mult_type", C.c_int32),
+                ("kernel", C.c_int32), ("n_tiles", C.c_int32), ("n_entries", C.c_int64), ("n_loads", C.c_int64)]
+
+
+KERNEL_AUTO, KERNEL_TRIPLET, KERNEL_TILE = 0, 1, 2
 
 
 class RangeInfo(C.Structure):
@@ -112,9 +116,9 @@ class SysInfo(C.Structure):
 # Symbols declared in include/cqmm.h and include/cqmm_system.h.
 EXPORTS = [
     "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
-    "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
+    "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_device_copy", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
     "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_plan_get_info",
-    "cqmm_plan_get_triplets",
+    "cqmm_plan_get_triplets", "cqmm_plan_set_kernel",
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
     "cqmm_sys_fill", "cqmm_random_structure", "cqmm_sys_plan_create", "cqmm_sys_matrix_index",
@@ -158,10 +162,12 @@ def lib():
     L.cqmm_plan_destroy.argtypes = [C.c_void_p]
     L.cqmm_plan_get_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
     L.cqmm_plan_get_triplets.argtypes = [C.c_void_p, i64p, i64p, i64p]
+    L.cqmm_plan_set_kernel.argtypes = [C.c_void_p, C.c_int32]
     L.cqmm_mat_mult.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                 C.c_int64, C.c_uint32]
     L.cqmm_device_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_int64]
     L.cqmm_device_free.argtypes = [C.c_void_p]
+    L.cqmm_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.cqmm_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.cqmm_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.cqmm_timer_stop.argtypes = [C.POINTER(C.c_float)]
@@ -337,6 +343,12 @@ def synchronize():
     check(lib().cqmm_synchronize(), "cqmm_synchronize")
 
 
+def device_copy(dst, src, n=None):
+    """dst[:n] = src[:n] on the library stream (streaming copy kernel)."""
+    n = min(dst.n, src.n) if n is None else int(n)
+    check(lib().cqmm_device_copy(dst.ptr, src.ptr, n), "cqmm_device_copy")
+
+
 def timer_start():
     check(lib().cqmm_timer_start(), "cqmm_timer_start")
 
@@ -379,6 +391,10 @@ class Plan:
         i = PlanInfo()
         check(lib().cqmm_plan_get_info(self.h, C.byref(i)), "cqmm_plan_get_info")
         return i
+
+    def set_kernel(self, kernel):
+        """KERNEL_AUTO, KERNEL_TRIPLET or KERNEL_TILE for the FP64 products."""
+        check(lib().cqmm_plan_set_kernel(self.h, int(kernel)), "cqmm_plan_set_kernel")
 
     def triplets(self):
         n = self.info().n_triplets

@@ -187,7 +187,7 @@ int cqmm_ctx_create(cqmm_ctx **ctx) {
 
 int cqmm_ctx_destroy(cqmm_ctx *ctx) {
   if (ctx) {
-    hipStreamSynchronize(cq::stream());
+    (void)hipStreamSynchronize(cq::stream());
     delete ctx;
   }
   return CQMM_OK;

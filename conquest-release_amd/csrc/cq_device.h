@@ -37,12 +37,32 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
   int alloc(size_t n);
   template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Output-tile schedule of one product with a uniform block size (NSF x NSF
+// blocks stored contiguously, so every offset is a multiple of NSF^2 and is kept
+// as a block index). A tile is kTileRows primary atoms x kTileCols column atoms
+// of the output; its entries are the contraction atoms x with at least one
+// product in the tile, each carrying which of the tile's left-operand blocks
+// (row, x) and right-operand blocks (column, x) exist. Entries are grouped into
+// chunks whose operand blocks fit the kernel's LDS stage.
+constexpr int kTileRows = 8, kTileCols = 8;
+struct TilePlan {
+  int nsf = 0;
+  int32_t n_tiles = 0, n_chunks = 0;
+  int64_t n_entries = 0, n_loads = 0;
+  DevBuf tile_chunk;   // int32 [n_tiles+1]  first chunk of each tile
+  DevBuf chunk_entry;  // int32 [n_chunks+1] first entry of each chunk
+  DevBuf entry_mask;   // uint32 [n_entries] bits 0-7 left blocks (tile row), 8-15 right blocks (tile column)
+  DevBuf entry_lbeg;   // int64 [n_entries+1] first operand of each entry in `list`
+  DevBuf list;         // int32 [n_loads] operand block index; bit 31 set: right operand
+  DevBuf tile_out;     // int32 [n_tiles*64] output block of slot row*8+col, or -1
 };
 
 // Triplet list of one product, grouped by output block (segment), each group in
@@ -61,8 +81,30 @@ struct TripletPlan {
   DevBuf seg_mn;            // uint8 [2*n_seg] (non-uniform only)
   DevBuf tp, tq;            // int32 or int64 [n_trip]
   DevBuf tk;                // uint8 [n_trip] inner dimension (non-uniform only)
-  // Host copies of the segment starts for reports (optional).
+  TilePlan tiles;           // FP64 tile schedule (uniform block sizes), n_tiles == 0 if absent
+  int kernel = 0;           // CQMM_KERNEL_* selection for the FP64 path
 };
+
+// Tile schedule construction from the builder's per-triplet tile keys.
+struct TileBuildInput {
+  int nsf = 0;
+  int64_t n_trip = 0;
+  const uint64_t *key = nullptr;   // device [n_trip]: tile * nx + contraction index
+  const uint8_t *slot = nullptr;   // device [n_trip]: row%8 * 8 + column rank%8
+  const int64_t *p = nullptr, *q = nullptr;  // device [n_trip]: operand element offsets
+  int64_t nx = 0;
+  int32_t n_tiles = 0;
+  const int32_t *tile_out = nullptr;  // host [n_tiles*64]
+};
+bool tiles_enabled();
+// Column order of a halo for tiling: Morton order of the halo partitions'
+// cover-set coordinates (label = (x*ncovery + y)*ncoverz + z + 1), atoms of a
+// partition in halo order. Returns rank[halo atom].
+std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part, const int32_t *lab_hcover,
+                                      int32_t ncovery, int32_t ncoverz);
+int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s);
+int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
+                         hipStream_t s);
 
 // Launch the FP64 (or FP32) block-product kernel over a plan.
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O,

@@ -145,8 +145,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_block_products_f64_u(
   }
 }
 
-// CQMM_KERNEL=simple selects the one-stage kernel for every block size (used to
-// measure what the pipelined, XCD-ordered kernel gains).
+// CQMM_KERNEL=simple selects the one-stage triplet kernel for every block size
+// (used to measure what the pipelined, XCD-ordered and tiled kernels gain).
 bool force_general() {
   static const int v = [] {
     const char *e = getenv("CQMM_KERNEL");
@@ -295,6 +295,8 @@ int launch_block_products(const TripletPlan &tp, const double *P, const double *
     // Other uniform sizes run through the general instance with per-block sizes.
     return set_errorf(CQMM_ERR_ARG, "block products: uniform block size %d unsupported", tp.uniform);
   }
+  if (tp.tiles.n_tiles > 0 && tp.kernel != CQMM_KERNEL_TRIPLET && !force_general())
+    return launch_tile_products(tp.tiles, tp.q_trans, P, Q, O, s);
   if (tp.off64) return tp.q_trans ? launch_t<int64_t, true>(tp, P, Q, O, s) : launch_t<int64_t, false>(tp, P, Q, O, s);
   return tp.q_trans ? launch_t<int32_t, true>(tp, P, Q, O, s) : launch_t<int32_t, false>(tp, P, Q, O, s);
 }

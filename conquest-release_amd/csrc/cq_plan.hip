@@ -184,7 +184,9 @@ __global__ void __launch_bounds__(256) k_items(
     const H *__restrict__ a_i_h2d, int32_t ni_a, const int32_t *__restrict__ ndimi, int32_t mult_type,
     int64_t len_p, int32_t *__restrict__ counts, int64_t *__restrict__ flops,
     const int64_t *__restrict__ item_off, int32_t *__restrict__ key_out, int64_t *__restrict__ p_out,
-    int64_t *__restrict__ q_out, uint8_t *__restrict__ k_out, DevErr *err) {
+    int64_t *__restrict__ q_out, uint8_t *__restrict__ k_out, DevErr *err, const int32_t *__restrict__ crank,
+    const int32_t *__restrict__ tmap, int32_t ncg, int64_t nx, uint64_t *__restrict__ tkey,
+    uint8_t *__restrict__ tslot) {
   const int lane = threadIdx.x & 63;
   const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (item >= n_items) return;
@@ -228,6 +230,16 @@ __global__ void __launch_bounds__(256) k_items(
         if (k_out) k_out[t] = (uint8_t)nd2;
       }
       q_out[t] = boff_arr[m.jrow + j];
+      if (tkey) {
+        // Tile of the output block (row i, column c) and contraction atom x.
+        const int32_t r0 = i - 1;
+        const int32_t c = mult_type == 1 ? jh - 1 : kh;
+        const int64_t x = mult_type == 1 ? kh : jh - 1;
+        const int32_t cr = crank[c];
+        const int32_t tile = tmap[(int64_t)(r0 >> 3) * ncg + (cr >> 3)];
+        tkey[t] = (uint64_t)tile * (uint64_t)nx + (uint64_t)x;
+        tslot[t] = (uint8_t)(((r0 & 7) << 3) | (cr & 7));
+      }
     }
     if (!FILL && valid) fl += 2ll * nd1 * nd2 * m.nd3;
     cnt += __popcll(mask);
@@ -451,7 +463,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
                      d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
                      d_ci_h2d.as<H>(), ni_c, d_ai_h2d.as<H>(), ni_a, d_ndimi.as<int32_t>(),
                      d->mult_type, len_p, d_counts.as<int32_t>(), d_flops.as<int64_t>(), nullptr, nullptr, nullptr,
-                     nullptr, nullptr, d_err.as<DevErr>());
+                     nullptr, nullptr, d_err.as<DevErr>(), nullptr, nullptr, 0, 0, nullptr, nullptr);
   if ((st = cq::hip_check(hipGetLastError(), "builder kernel launch"))) return st;
   std::vector<int32_t> counts(n_items);
   std::vector<int64_t> iflops(n_items);
@@ -486,6 +498,67 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   plan->lenc = d->lenc;
   if (n_trip > INT32_MAX) return cq::set_error(CQMM_ERR_RANGE, "cqmm_plan_create: more than 2^31 block products");
 
+  // ---- output tiles (uniform block sizes): host part ----
+  bool tiles = uniform != 0 && n_trip > 0 && cq::tiles_enabled();
+  const int64_t ns2 = (int64_t)uniform * uniform;
+  std::vector<int32_t> crank, tmap, tout;
+  int32_t ncg = 0, n_tiles = 0;
+  const int64_t nx = d->mult_type == 1 ? ni_a : ni_c;
+  if (tiles) {
+    // Every operand offset must be a whole number of blocks.
+    for (int32_t k = 0; k < ni_a && tiles; ++k) tiles = km[k].bdata % ns2 == 0;
+    const H *ph2d = d->mult_type == 1 ? A.i_h2d : C.i_h2d;
+    const int64_t npn = (int64_t)d->n_prim * (d->mult_type == 1 ? ni_a : ni_c);
+    for (int64_t x = 0; x < npn && tiles; ++x) tiles = ph2d[x] == 0 || (ph2d[x] - 1) % ns2 == 0;
+  }
+  if (tiles) {
+    const HaloV<H> &CH = d->mult_type == 1 ? C : A;  // halo of the output's columns
+    crank = cq::halo_morton_rank(CH.np_in_halo, CH.nh_part, CH.lab_hcover, d->ncovery, d->ncoverz);
+    const int32_t ncol = CH.ni_in_halo;
+    ncg = (ncol + 7) / 8;
+    const int64_t nrg = (d->n_prim + 7) / 8;
+    // Output blocks (row, column, offset) of the product's result.
+    std::vector<int64_t> ob_r, ob_c, ob_o;
+    if (d->mult_type == 2) {
+      for (int64_t t = 0; t < n_items; ++t) {
+        const int32_t kh = item_k[t], i = T.i_prim[t];
+        ob_r.push_back(i - 1);
+        ob_c.push_back(kh);
+        ob_o.push_back((int64_t)A.i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1);
+      }
+    } else {
+      const int64_t nc_all = (int64_t)d->n_prim * ni_c;
+      for (int64_t pos = 0; pos < nc_all; ++pos) {
+        const int64_t off = (int64_t)C.i_h2d[pos];
+        if (off == 0) continue;
+        ob_r.push_back(pos / ni_c);
+        ob_c.push_back(pos % ni_c);
+        ob_o.push_back(off - 1);
+      }
+    }
+    for (int64_t b = 0; b < (int64_t)ob_o.size() && tiles; ++b) tiles = ob_o[b] % ns2 == 0;
+    if (tiles && nrg * ncg > INT32_MAX) tiles = false;
+    if (tiles) {
+      tmap.assign((size_t)(nrg * ncg), -1);
+      for (size_t b = 0; b < ob_o.size(); ++b) tmap[(ob_r[b] >> 3) * ncg + (crank[ob_c[b]] >> 3)] = 0;
+      for (auto &v : tmap)
+        if (v == 0) v = n_tiles++;
+      tout.assign((size_t)n_tiles * 64, -1);
+      for (size_t b = 0; b < ob_o.size(); ++b) {
+        const int32_t cr = crank[ob_c[b]];
+        const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
+        tout[(size_t)tile * 64 + ((ob_r[b] & 7) << 3) + (cr & 7)] = (int32_t)(ob_o[b] / ns2);
+      }
+    }
+  }
+  DevBuf d_crank, d_tmap, d_tkey, d_tslot;
+  if (tiles) {
+    if ((st = upload(d_crank, crank.data(), crank.size(), s))) return st;
+    if ((st = upload(d_tmap, tmap.data(), tmap.size(), s))) return st;
+    if ((st = d_tkey.alloc(n_trip * 8))) return st;
+    if ((st = d_tslot.alloc(n_trip))) return st;
+  }
+
   DevBuf d_itemoff, d_key, d_p, d_q, d_k;
   if ((st = upload(d_itemoff, item_off.data(), item_off.size(), s))) return st;
   const int64_t nt1 = std::max<int64_t>(n_trip, 1);
@@ -497,8 +570,27 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
                      d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
                      d_ci_h2d.as<H>(), ni_c, d_ai_h2d.as<H>(), ni_a, d_ndimi.as<int32_t>(),
                      d->mult_type, len_p, nullptr, nullptr, d_itemoff.as<int64_t>(), d_key.as<int32_t>(),
-                     d_p.as<int64_t>(), d_q.as<int64_t>(), uniform ? nullptr : d_k.as<uint8_t>(), d_err.as<DevErr>());
+                     d_p.as<int64_t>(), d_q.as<int64_t>(), uniform ? nullptr : d_k.as<uint8_t>(), d_err.as<DevErr>(),
+                     tiles ? d_crank.as<int32_t>() : nullptr, tiles ? d_tmap.as<int32_t>() : nullptr, ncg, nx,
+                     tiles ? d_tkey.as<uint64_t>() : nullptr, tiles ? d_tslot.as<uint8_t>() : nullptr);
   if ((st = cq::hip_check(hipGetLastError(), "builder fill launch"))) return st;
+  if (tiles) {
+    cq::TileBuildInput tin;
+    tin.nsf = uniform;
+    tin.n_trip = n_trip;
+    tin.key = d_tkey.as<uint64_t>();
+    tin.slot = d_tslot.as<uint8_t>();
+    tin.p = d_p.as<int64_t>();
+    tin.q = d_q.as<int64_t>();
+    tin.nx = nx;
+    tin.n_tiles = n_tiles;
+    tin.tile_out = tout.data();
+    if ((st = cq::build_tiles(tin, tp.tiles, s))) return st;
+    d_tkey.release();
+    d_tslot.release();
+    d_tmap.release();
+    d_crank.release();
+  }
 
   const size_t osz = tp.off64 ? 8 : 4;
   if ((st = tp.tp.alloc(nt1 * osz))) return st;
@@ -656,7 +748,7 @@ int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, 
 
 int cqmm_plan_destroy(cqmm_plan *plan) {
   if (plan) {
-    hipStreamSynchronize(cq::stream());
+    (void)hipStreamSynchronize(cq::stream());
     delete plan;
   }
   return CQMM_OK;
@@ -669,6 +761,11 @@ int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info) {
   info->flops = plan->tp.flops;
   info->uniform_nsf = plan->tp.uniform;
   info->mult_type = plan->tp.mult_type;
+  info->n_tiles = plan->tp.tiles.n_tiles;
+  info->n_entries = plan->tp.tiles.n_entries;
+  info->n_loads = plan->tp.tiles.n_loads;
+  info->kernel = (plan->tp.tiles.n_tiles > 0 && plan->tp.kernel != CQMM_KERNEL_TRIPLET) ? CQMM_KERNEL_TILE
+                                                                                          : CQMM_KERNEL_TRIPLET;
   // Output blocks with at least one product.
   std::vector<int64_t> sb(plan->tp.n_seg + 1);
   int st = cq::hip_check(hipMemcpy(sb.data(), plan->tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg");
@@ -676,6 +773,16 @@ int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info) {
   int64_t nz = 0;
   for (int64_t b = 0; b < plan->tp.n_seg; ++b) nz += sb[b + 1] > sb[b];
   info->n_out_blocks = nz;
+  return CQMM_OK;
+}
+
+int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel) {
+  if (!plan) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_set_kernel: null plan");
+  if (kernel != CQMM_KERNEL_AUTO && kernel != CQMM_KERNEL_TRIPLET && kernel != CQMM_KERNEL_TILE)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_set_kernel: unknown kernel %d", kernel);
+  if (kernel == CQMM_KERNEL_TILE && plan->tp.tiles.n_tiles == 0)
+    return cq::set_error(CQMM_ERR_STATE, "cqmm_plan_set_kernel: plan has no tile schedule");
+  plan->tp.kernel = kernel;
   return CQMM_OK;
 }
 

@@ -86,8 +86,8 @@ int cqmm_init(int device) {
 int cqmm_finalize(void) {
   cq::Runtime &rt = cq::g_rt;
   if (!rt.initialised) return CQMM_OK;
-  hipStreamSynchronize(rt.stream);
-  hipStreamDestroy(rt.stream);
+  (void)hipStreamSynchronize(rt.stream);
+  (void)hipStreamDestroy(rt.stream);
   rt.stream = nullptr;
   rt.initialised = false;
   return CQMM_OK;
@@ -123,6 +123,35 @@ int cqmm_memcpy_d2h(void *dst, const void *src, int64_t bytes) {
   st = cq::hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, cq::stream()), "hipMemcpy D2H");
   if (st) return st;
   return cq::hip_check(hipStreamSynchronize(cq::stream()), "hipStreamSynchronize");
+}
+
+namespace {
+// Grid-stride copy, 16 bytes per lane per step.
+__global__ void k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, int64_t n2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+}
+__global__ void k_stream_copy_tail(double *__restrict__ dst, const double *__restrict__ src, int64_t i) {
+  dst[i] = src[i];
+}
+}  // namespace
+
+int cqmm_device_copy(double *dst, const double *src, int64_t n) {
+  if (n < 0 || (n > 0 && (!dst || !src))) return cq::set_error(CQMM_ERR_ARG, "cqmm_device_copy: bad argument");
+  if (((uintptr_t)dst | (uintptr_t)src) & 15u)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_device_copy: pointers must be 16-byte aligned");
+  int st = cq::ensure_init();
+  if (st) return st;
+  if (n == 0) return CQMM_OK;
+  const int64_t n2 = n / 2;
+  hipStream_t s = cq::stream();
+  if (n2 > 0) {
+    const int64_t want = (n2 + 255) / 256;
+    const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (double2 *)dst, (const double2 *)src, n2);
+  }
+  if (n & 1) hipLaunchKernelGGL(k_stream_copy_tail, dim3(1), dim3(1), 0, s, dst, src, n - 1);
+  return cq::hip_check(hipGetLastError(), "cqmm_device_copy launch");
 }
 
 static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;

@@ -220,7 +220,7 @@ int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_confi
   const double t_start = now_s();
   const int device = cq::runtime().device;
   auto worker = [&](int slot) {
-    hipSetDevice(device);
+    (void)hipSetDevice(device);
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
     cq::set_thread_stream(s);
@@ -255,7 +255,7 @@ int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_confi
       res[i].admit_s = now_s() - t_start;
       res[i].slot = slot;
       cqmm_purify(reqs[i].natoms, reqs[i].nsf, reqs[i].seed, &cfg->params, &res[i].r);
-      hipStreamSynchronize(s);
+      (void)hipStreamSynchronize(s);
       res[i].done_s = now_s() - t_start;
       {
         std::lock_guard<std::mutex> lk(mu);
@@ -264,7 +264,7 @@ int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_confi
       cv.notify_all();
     }
     cq::set_thread_stream(nullptr);
-    hipStreamDestroy(s);
+    (void)hipStreamDestroy(s);
   };
   std::vector<std::thread> pool;
   for (int32_t k = 0; k < cfg->slots; ++k) pool.emplace_back(worker, k);

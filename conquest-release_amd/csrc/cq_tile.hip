@@ -1,0 +1,436 @@
+// Output-tile schedule and FP64 tile kernel for uniform block sizes.
+//
+// The same block products as the triplet kernels (m_kern_max / m_kern_min,
+// src/multiply_kernel_default.f90:204-214 and :392-402), grouped so that
+// operand blocks are read from HBM once per tile instead of once per product:
+//
+//  * a tile is 8 primary atoms (rows, consecutive in primary order) x 8 column
+//    atoms (consecutive in the Morton order of their halo partitions) of the
+//    output matrix: type 1 C(i,j), columns in the C halo, contraction over the
+//    A-halo atom k; type 2 X(i,k), columns in the A halo, contraction over the
+//    C-halo atom j;
+//  * an entry of a tile is one contraction atom x with its (up to 8) left
+//    operand blocks P(row, x) and (up to 8) right operand blocks Q(column, x);
+//    entries of a tile are in ascending x (type 1: the reference's k order);
+//  * one workgroup (8 wavefronts) runs one tile. It stages a chunk of entries'
+//    operand blocks in LDS, then wavefront w accumulates the 8 output blocks
+//    (row u, column (w-u) mod 8), u = 0..7, with v_mfma_f64_16x16x4 from LDS,
+//    and finally writes each of its output blocks once (zeros where a block
+//    has no products), so the output is fully overwritten as mat_mult
+//    overwrites it (src/multiply_module.f90:204-208). The diagonal ownership
+//    spreads the ragged products of an entry over all wavefronts.
+// Summation order per output block is fixed (ascending x), so results are
+// bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "../../include/cqmm.h"
+#include "cq_common.h"
+#include "cq_device.h"
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileThreads = 512;            // 8 wavefronts
+constexpr int kTileLdsBytes = 64 * 1024;     // operand stage per workgroup (two workgroups per CU)
+
+template <int NSF>
+struct TileCfg {
+  static constexpr int NS2 = NSF * NSF;
+  static constexpr int CAP = kTileLdsBytes / (NS2 * 8);           // operand blocks per chunk
+  static constexpr int KS = (NSF + 3) / 4;                        // MFMA k-steps per product
+  static constexpr int STAGE_IT = (CAP * NS2 + kTileThreads - 1) / kTileThreads;
+};
+
+int tile_cap(int nsf) { return kTileLdsBytes / (nsf * nsf * 8); }
+
+// LDS reads of a block product never mask rows or columns beyond NSF: those
+// lanes only feed accumulator rows/columns >= NSF, which are never written.
+// Only the contraction index 4*s+g >= NSF at the last k-step is zeroed (both
+// operands), so the padding of the 16x16x4 steps adds nothing. kLdsPad doubles
+// after the stage keep the over-reads of the last block inside the array.
+constexpr int kLdsPad = 256;
+
+template <bool QT, int NSF>
+__global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
+    const int32_t *__restrict__ tile_chunk, const int32_t *__restrict__ chunk_entry,
+    const uint32_t *__restrict__ entry_mask, const int64_t *__restrict__ entry_lbeg,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ tile_out, const double *__restrict__ P,
+    const double *__restrict__ Q, double *__restrict__ O, int32_t n_tiles) {
+  using C = TileCfg<NSF>;
+  __shared__ double lds[C::CAP * C::NS2 + kLdsPad];
+  __shared__ int32_t lidx[C::CAP];
+  // XCD-aware order: workgroup b runs on XCD b % 8 and takes tile
+  // (b % 8) * ceil(T/8) + b / 8, so neighbouring tiles (same rows) share an L2.
+  const int32_t per = (n_tiles + 7) >> 3;
+  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  if (tile >= n_tiles) return;  // the whole workgroup leaves together
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, r = lane & 15, g = lane >> 4;
+  constexpr int SA = 4 * NSF;             // A offset step per k-step
+  constexpr int SB = QT ? 4 * NSF : 4;    // B offset step per k-step
+  constexpr int REM = NSF - 4 * (C::KS - 1);  // valid g at the last k-step
+  const int oA = r + NSF * g;
+  const int oB = QT ? (r + NSF * g) : (g + NSF * r);
+  const bool last_ok = g < REM;
+  int32_t oblk[8];
+  uint32_t omask = 0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    oblk[u] = tile_out[(int64_t)tile * 64 + u * 8 + ((w - u) & 7)];
+    if (oblk[u] >= 0) omask |= 1u << u;
+  }
+  dbl4 acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
+  for (int32_t c = c0; c < c1; ++c) {
+    const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];
+    const int64_t l0 = entry_lbeg[e0];
+    const int32_t nb = (int32_t)(entry_lbeg[e1] - l0);  // <= CAP by construction
+    const int32_t nel = nb * C::NS2;
+    for (int i = tid; i < nb; i += kTileThreads) lidx[i] = list[l0 + i];
+    __syncthreads();
+    // Opaque copy of the thread index: keeps the per-element block/offset
+    // arithmetic inside the chunk loop instead of 2*STAGE_IT hoisted registers.
+    int tv = tid;
+    asm volatile("" : "+v"(tv));
+    // Stage the chunk's operand blocks in LDS, 8 loads in flight per thread.
+#pragma unroll
+    for (int it0 = 0; it0 < C::STAGE_IT; it0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int idx = tv + kTileThreads * (it0 + j);
+        v[j] = 0.0;
+        if (it0 + j < C::STAGE_IT && idx < nel) {
+          const int b = idx / C::NS2;
+          const int wi = idx - b * C::NS2;
+          const int32_t lb = lidx[b];
+          const double *src = lb < 0 ? Q : P;
+          v[j] = src[(int64_t)(lb & 0x7fffffff) * C::NS2 + wi];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int idx = tv + kTileThreads * (it0 + j);
+        if (it0 + j < C::STAGE_IT && idx < nel) lds[idx] = v[j];
+      }
+    }
+    __syncthreads();
+    int32_t base = 0;
+    for (int32_t e = e0; e < e1; ++e) {
+      const uint32_t m = entry_mask[e];
+      const uint32_t ym = m & 0xffu, bm = (m >> 8) & 0xffu;
+      const int32_t ny = __popc(ym);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int cl = (w - u) & 7;
+        if (((omask >> u) & 1u) && ((ym >> u) & 1u) && ((bm >> cl) & 1u)) {
+          const double *la = lds + (base + __popc(ym & ((1u << u) - 1u))) * C::NS2 + oA;
+          const double *lq = lds + (base + ny + __popc(bm & ((1u << cl) - 1u))) * C::NS2 + oB;
+          double a[C::KS], bb[C::KS];
+#pragma unroll
+          for (int s4 = 0; s4 < C::KS; ++s4) {
+            a[s4] = la[SA * s4];
+            bb[s4] = lq[SB * s4];
+          }
+          if (REM < 4 && !last_ok) {
+            a[C::KS - 1] = 0.0;
+            bb[C::KS - 1] = 0.0;
+          }
+#pragma unroll
+          for (int s4 = 0; s4 < C::KS; ++s4) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], acc[u], 0, 0, 0);
+        }
+      }
+      base += __popc(m & 0xffffu);
+    }
+    __syncthreads();
+  }
+  if (r < NSF) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (!((omask >> u) & 1u)) continue;
+      double *o = O + (int64_t)oblk[u] * C::NS2;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = g + 4 * e;  // FP64 accumulator: D[(l/16)+4e][l%16]
+        if (row < NSF) o[row + NSF * r] = acc[u][e];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- builder
+
+inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__global__ void k_fill_i32(int32_t *__restrict__ v, int64_t n, int32_t val) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = val;
+}
+
+__global__ void k_iota32(int32_t *__restrict__ v, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = (int32_t)t;
+}
+
+// flag[pos] = 1 where a new (tile, x) key starts in the sorted list.
+__global__ void k_entry_flags(const uint64_t *__restrict__ key, int64_t n, int32_t *__restrict__ flag) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) flag[t] = (t == 0 || key[t] != key[t - 1]) ? 1 : 0;
+}
+
+// Scatter each product's operand blocks into its entry's row / column slots.
+// Products of one entry that share a row (column) write the same block index.
+__global__ void k_entry_fill(const uint64_t *__restrict__ key, const int32_t *__restrict__ perm,
+                             const int32_t *__restrict__ eid, const uint8_t *__restrict__ slot,
+                             const int64_t *__restrict__ p, const int64_t *__restrict__ q, int64_t n, int64_t nx,
+                             int64_t ns2, int32_t *__restrict__ ep, int32_t *__restrict__ eq,
+                             int32_t *__restrict__ etile) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const int64_t e = (int64_t)eid[pos] - 1;
+  const int32_t t = perm[pos];
+  const int sl = slot[t];
+  ep[e * 8 + (sl >> 3)] = (int32_t)(p[t] / ns2);
+  eq[e * 8 + (sl & 7)] = (int32_t)(q[t] / ns2);
+  if (pos == 0 || eid[pos - 1] != eid[pos]) etile[e] = (int32_t)(key[pos] / (uint64_t)nx);
+}
+
+__global__ void k_entry_mask(const int32_t *__restrict__ ep, const int32_t *__restrict__ eq, int64_t ne,
+                             uint32_t *__restrict__ mask, int64_t *__restrict__ cnt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  uint32_t m = 0;
+  for (int u = 0; u < 8; ++u) {
+    if (ep[e * 8 + u] >= 0) m |= 1u << u;
+    if (eq[e * 8 + u] >= 0) m |= 1u << (8 + u);
+  }
+  mask[e] = m;
+  cnt[e] = __popc(m);
+}
+
+// Operand list of each entry: present left blocks by row, then right blocks by column.
+__global__ void k_list_fill(const int32_t *__restrict__ ep, const int32_t *__restrict__ eq,
+                            const int64_t *__restrict__ lbeg, int64_t ne, int32_t *__restrict__ list) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int64_t o = lbeg[e];
+  for (int u = 0; u < 8; ++u)
+    if (ep[e * 8 + u] >= 0) list[o++] = ep[e * 8 + u];
+  for (int u = 0; u < 8; ++u)
+    if (eq[e * 8 + u] >= 0) list[o++] = (int32_t)((uint32_t)eq[e * 8 + u] | 0x80000000u);
+}
+
+// tebeg[t] = first entry of tile t (entries are sorted by tile), t = 0..n_tiles.
+__global__ void k_tile_ebeg(const int32_t *__restrict__ etile, int64_t ne, int32_t n_tiles,
+                            int32_t *__restrict__ tebeg) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tiles) return;
+  int64_t lo = 0, hi = ne;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (etile[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  tebeg[t] = (int32_t)lo;
+}
+
+// Chunks of a tile: consecutive entries whose operand blocks fit `cap`.
+// With FILL unset, counts them; with FILL set, writes each chunk's first entry.
+template <bool FILL>
+__global__ void k_chunks(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask, int32_t n_tiles,
+                         int32_t cap, const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  int32_t nch = 0, sum = 0;
+  for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
+    const int32_t c = __popc(mask[e]);
+    if (nch == 0 || sum + c > cap) {
+      if (FILL) out[tchunk[t] + nch] = e;
+      ++nch;
+      sum = 0;
+    }
+    sum += c;
+  }
+  if (!FILL) out[t] = nch;
+}
+
+template <class T>
+int inclusive_scan(const T *in, T *out, size_t n, hipStream_t s) {
+  if (n == 0) return CQMM_OK;
+  size_t bytes = 0;
+  hipError_t e = rocprim::inclusive_scan(nullptr, bytes, in, out, n, rocprim::plus<T>(), s);
+  if (e != hipSuccess) return cq::hip_check(e, "inclusive_scan (size)");
+  cq::DevBuf tmp;
+  int st = tmp.alloc(bytes);
+  if (st) return st;
+  e = rocprim::inclusive_scan(tmp.p, bytes, in, out, n, rocprim::plus<T>(), s);
+  if (e != hipSuccess) return cq::hip_check(e, "inclusive_scan");
+  // tmp is freed on return: wait for the scan that uses it.
+  return cq::hip_check(hipStreamSynchronize(s), "inclusive_scan");
+}
+
+template <bool QT>
+int launch_nsf(const cq::TilePlan &tp, const double *P, const double *Q, double *O, hipStream_t s) {
+  const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
+  const dim3 blk(kTileThreads);
+  const int32_t *tc = tp.tile_chunk.as<int32_t>(), *ce = tp.chunk_entry.as<int32_t>();
+  const uint32_t *em = tp.entry_mask.as<uint32_t>();
+  const int64_t *lb = tp.entry_lbeg.as<int64_t>();
+  const int32_t *li = tp.list.as<int32_t>(), *to = tp.tile_out.as<int32_t>();
+  switch (tp.nsf) {
+    case 4: hipLaunchKernelGGL((k_tile_products_f64<QT, 4>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 9: hipLaunchKernelGGL((k_tile_products_f64<QT, 9>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 13: hipLaunchKernelGGL((k_tile_products_f64<QT, 13>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 16: hipLaunchKernelGGL((k_tile_products_f64<QT, 16>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: block size %d has no kernel instance", tp.nsf);
+  }
+  return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
+}
+
+}  // namespace
+
+namespace cq {
+
+// CQMM_TILES=0 builds plans without the tile schedule (triplet kernels only).
+bool tiles_enabled() {
+  static const bool v = [] {
+    const char *e = getenv("CQMM_TILES");
+    return !(e && std::string(e) == "0");
+  }();
+  return v;
+}
+
+std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part, const int32_t *lab_hcover,
+                                      int32_t ncovery, int32_t ncoverz) {
+  std::vector<uint64_t> code(np_in_halo);
+  const int64_t nyz = (int64_t)ncovery * ncoverz;
+  for (int32_t p = 0; p < np_in_halo; ++p) {
+    const int64_t l = (int64_t)lab_hcover[p] - 1;
+    const uint64_t x = (uint64_t)(l / nyz), y = (uint64_t)((l / ncoverz) % ncovery), z = (uint64_t)(l % ncoverz);
+    uint64_t c = 0;
+    for (int bit = 0; bit < 20; ++bit)
+      c |= (((x >> bit) & 1u) << (3 * bit)) | (((y >> bit) & 1u) << (3 * bit + 1)) | (((z >> bit) & 1u) << (3 * bit + 2));
+    code[p] = c;
+  }
+  std::vector<int32_t> order(np_in_halo);
+  for (int32_t p = 0; p < np_in_halo; ++p) order[p] = p;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+  std::vector<int64_t> first(np_in_halo + 1, 0);
+  for (int32_t p = 0; p < np_in_halo; ++p) first[p + 1] = first[p] + nh_part[p];
+  std::vector<int32_t> rank(first[np_in_halo]);
+  int32_t next = 0;
+  for (int32_t p : order)
+    for (int64_t k = first[p]; k < first[p + 1]; ++k) rank[k] = next++;
+  return rank;
+}
+
+int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
+  const int64_t n = in.n_trip;
+  const int64_t ns2 = (int64_t)in.nsf * in.nsf;
+  const int32_t cap = tile_cap(in.nsf);
+  int st;
+  tp = TilePlan();
+  tp.nsf = in.nsf;
+  tp.n_tiles = in.n_tiles;
+  if ((st = tp.tile_out.alloc((size_t)in.n_tiles * 64 * 4))) return st;
+  if ((st = hip_check(hipMemcpyAsync(tp.tile_out.p, in.tile_out, (size_t)in.n_tiles * 64 * 4, hipMemcpyHostToDevice, s),
+                      "H2D tile_out")))
+    return st;
+  // Sort the products by (tile, contraction atom).
+  unsigned end_bit = 1;
+  const uint64_t kmax = (uint64_t)in.n_tiles * (uint64_t)in.nx;
+  while (end_bit < 64 && ((uint64_t)1 << end_bit) < kmax) ++end_bit;
+  DevBuf d_perm, d_key2, d_perm2, d_tmp;
+  if ((st = d_perm.alloc(n * 4)) || (st = d_key2.alloc(n * 8)) || (st = d_perm2.alloc(n * 4))) return st;
+  hipLaunchKernelGGL(k_iota32, dim3(grid1(n, 256)), dim3(256), 0, s, d_perm.as<int32_t>(), n);
+  size_t bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, in.key, d_key2.as<uint64_t>(), d_perm.as<int32_t>(),
+                                           d_perm2.as<int32_t>(), (size_t)n, 0, end_bit, s);
+  if ((st = hip_check(e, "radix_sort_pairs (tiles, size)"))) return st;
+  if ((st = d_tmp.alloc(bytes))) return st;
+  e = rocprim::radix_sort_pairs(d_tmp.p, bytes, in.key, d_key2.as<uint64_t>(), d_perm.as<int32_t>(),
+                                d_perm2.as<int32_t>(), (size_t)n, 0, end_bit, s);
+  if ((st = hip_check(e, "radix_sort_pairs (tiles)"))) return st;
+  if ((st = hip_check(hipStreamSynchronize(s), "tile sort"))) return st;
+  d_tmp.release();
+  d_perm.release();
+  // Entries = distinct keys.
+  DevBuf d_flag, d_eid;
+  if ((st = d_flag.alloc(n * 4)) || (st = d_eid.alloc(n * 4))) return st;
+  hipLaunchKernelGGL(k_entry_flags, dim3(grid1(n, 256)), dim3(256), 0, s, d_key2.as<uint64_t>(), n, d_flag.as<int32_t>());
+  if ((st = inclusive_scan(d_flag.as<int32_t>(), d_eid.as<int32_t>(), (size_t)n, s))) return st;
+  d_flag.release();
+  int32_t ne32 = 0;
+  if ((st = hip_check(hipMemcpyAsync(&ne32, d_eid.as<int32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, s), "D2H entries")) ||
+      (st = hip_check(hipStreamSynchronize(s), "entry count")))
+    return st;
+  const int64_t ne = ne32;
+  tp.n_entries = ne;
+  DevBuf d_ep, d_eq, d_etile, d_cnt;
+  if ((st = d_ep.alloc(ne * 8 * 4)) || (st = d_eq.alloc(ne * 8 * 4)) || (st = d_etile.alloc(ne * 4)) ||
+      (st = d_cnt.alloc(ne * 8)))
+    return st;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1(ne * 8, 256)), dim3(256), 0, s, d_ep.as<int32_t>(), ne * 8, -1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1(ne * 8, 256)), dim3(256), 0, s, d_eq.as<int32_t>(), ne * 8, -1);
+  hipLaunchKernelGGL(k_entry_fill, dim3(grid1(n, 256)), dim3(256), 0, s, d_key2.as<uint64_t>(), d_perm2.as<int32_t>(),
+                     d_eid.as<int32_t>(), in.slot, in.p, in.q, n, in.nx, ns2, d_ep.as<int32_t>(), d_eq.as<int32_t>(),
+                     d_etile.as<int32_t>());
+  if ((st = tp.entry_mask.alloc(ne * 4)) || (st = tp.entry_lbeg.alloc((ne + 1) * 8))) return st;
+  hipLaunchKernelGGL(k_entry_mask, dim3(grid1(ne, 256)), dim3(256), 0, s, d_ep.as<int32_t>(), d_eq.as<int32_t>(), ne,
+                     tp.entry_mask.as<uint32_t>(), d_cnt.as<int64_t>());
+  if ((st = hip_check(hipMemsetAsync(tp.entry_lbeg.p, 0, 8, s), "memset lbeg"))) return st;
+  if ((st = inclusive_scan(d_cnt.as<int64_t>(), tp.entry_lbeg.as<int64_t>() + 1, (size_t)ne, s))) return st;
+  int64_t nl = 0;
+  if ((st = hip_check(hipMemcpyAsync(&nl, tp.entry_lbeg.as<int64_t>() + ne, 8, hipMemcpyDeviceToHost, s), "D2H loads")) ||
+      (st = hip_check(hipStreamSynchronize(s), "load count")))
+    return st;
+  tp.n_loads = nl;
+  if ((st = tp.list.alloc(nl * 4))) return st;
+  hipLaunchKernelGGL(k_list_fill, dim3(grid1(ne, 256)), dim3(256), 0, s, d_ep.as<int32_t>(), d_eq.as<int32_t>(),
+                     tp.entry_lbeg.as<int64_t>(), ne, tp.list.as<int32_t>());
+  // Chunks.
+  DevBuf d_tebeg, d_nch;
+  if ((st = d_tebeg.alloc(((size_t)in.n_tiles + 1) * 4)) || (st = d_nch.alloc((size_t)in.n_tiles * 4))) return st;
+  hipLaunchKernelGGL(k_tile_ebeg, dim3(grid1((int64_t)in.n_tiles + 1, 256)), dim3(256), 0, s, d_etile.as<int32_t>(), ne,
+                     in.n_tiles, d_tebeg.as<int32_t>());
+  hipLaunchKernelGGL((k_chunks<false>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
+                     tp.entry_mask.as<uint32_t>(), in.n_tiles, cap, nullptr, d_nch.as<int32_t>());
+  if ((st = tp.tile_chunk.alloc(((size_t)in.n_tiles + 1) * 4))) return st;
+  if ((st = hip_check(hipMemsetAsync(tp.tile_chunk.p, 0, 4, s), "memset tile_chunk"))) return st;
+  if ((st = inclusive_scan(d_nch.as<int32_t>(), tp.tile_chunk.as<int32_t>() + 1, (size_t)in.n_tiles, s))) return st;
+  int32_t nch = 0;
+  if ((st = hip_check(hipMemcpyAsync(&nch, tp.tile_chunk.as<int32_t>() + in.n_tiles, 4, hipMemcpyDeviceToHost, s),
+                      "D2H chunks")) ||
+      (st = hip_check(hipStreamSynchronize(s), "chunk count")))
+    return st;
+  tp.n_chunks = nch;
+  if ((st = tp.chunk_entry.alloc(((size_t)nch + 1) * 4))) return st;
+  hipLaunchKernelGGL((k_chunks<true>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
+                     tp.entry_mask.as<uint32_t>(), in.n_tiles, cap, tp.tile_chunk.as<int32_t>(),
+                     tp.chunk_entry.as<int32_t>());
+  if ((st = hip_check(hipMemcpyAsync(tp.chunk_entry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
+                      "H2D chunk end")))
+    return st;
+  if ((st = hip_check(hipGetLastError(), "tile builder launch"))) return st;
+  return hip_check(hipStreamSynchronize(s), "tile build");
+}
+
+int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
+                         hipStream_t s) {
+  if (tp.n_tiles == 0) return CQMM_OK;
+  return q_trans ? launch_nsf<true>(tp, P, Q, O, s) : launch_nsf<false>(tp, P, Q, O, s);
+}
+
+}  // namespace cq

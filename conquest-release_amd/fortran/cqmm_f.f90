@@ -50,6 +50,8 @@ module cqmm_f
      integer(c_int64_t) :: n_triplets, n_out_blocks, n_out_total
      real(c_double) :: flops
      integer(c_int32_t) :: uniform_nsf, mult_type
+     integer(c_int32_t) :: kernel, n_tiles
+     integer(c_int64_t) :: n_entries, n_loads
   end type cqmm_plan_info
 
   type, bind(c) :: cqmm_mcw_setup

@@ -109,6 +109,10 @@ int cqmm_device_alloc(void **ptr, int64_t bytes);
 int cqmm_device_free(void *ptr);
 int cqmm_memcpy_h2d(void *dst, const void *src, int64_t bytes);
 int cqmm_memcpy_d2h(void *dst, const void *src, int64_t bytes);
+/* Device-to-device copy of n doubles by a streaming kernel on the library
+ * stream (reads 8n bytes, writes 8n bytes; also the reference kernel for
+ * scaling HBM traffic counters). Asynchronous. */
+int cqmm_device_copy(double *dst, const double *src, int64_t n);
 int cqmm_synchronize(void);
 /* HIP-event timer on the library stream: the device-side counterpart of the
  * tmr_std_matmult clock around mat_mult (src/multiply_module.f90:171,356,
@@ -144,8 +148,24 @@ typedef struct cqmm_plan_info {
   double flops;                /* sum over triplets of 2*n_i*n_k*n_j          */
   int32_t uniform_nsf;         /* support functions if uniform, else 0        */
   int32_t mult_type;
+  int32_t kernel;              /* FP64 kernel cqmm_mat_mult runs (CQMM_KERNEL_*) */
+  int32_t n_tiles;             /* output tiles of the tile schedule (0: none)  */
+  int64_t n_entries;           /* (tile, contraction atom) entries            */
+  int64_t n_loads;             /* operand blocks the tile kernel reads        */
 } cqmm_plan_info;
 int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info);
+
+/* FP64 kernel selection (no reference counterpart). For uniform block sizes
+ * the plan also holds an output-tile schedule (8 x 8 output blocks per
+ * workgroup, operand blocks staged once per tile in LDS); AUTO runs it when
+ * present. TRIPLET runs the per-output-block kernel; TILE fails with
+ * CQMM_ERR_STATE when the plan has no tile schedule. Both give the same
+ * products; the summation order of type-2 blocks differs (ascending C-halo
+ * atom in the tile kernel, B-row order in the triplet kernel). */
+#define CQMM_KERNEL_AUTO 0
+#define CQMM_KERNEL_TRIPLET 1
+#define CQMM_KERNEL_TILE 2
+int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel);
 /* Copy the triplet list to host memory, in the order the kernel consumes it
  * (grouped by output block, each group in the reference's summation order).
  * Each triplet is (out, p, q): 1-based offsets of the output block, the left

@@ -132,3 +132,58 @@ def test_bad_descriptor_is_rejected_before_any_kernel():
     d.lenb = 10  # B blocks now point past lenb
     with pytest.raises(cqmm.CqmmError, match="lenb|B block"):
         cqmm.Plan(d)
+
+
+TILE_CASES = [
+    (64, 13, 8.0, 8.0, 12.0, 2),
+    (64, 13, 6.0, 8.0, 12.0, 1),
+    (48, 4, 6.0, 8.0, 14.0, 1),
+    (40, 9, 6.0, 7.0, 10.0, 2),
+    (40, 16, 5.0, 6.0, 9.0, 1),
+    (40, 16, 5.0, 6.0, 6.0, 2),
+    (200, 13, 12.0, 12.0, 6.0, 2),   # many tiles, multi-chunk entries
+]
+
+
+@pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", TILE_CASES)
+def test_tile_kernel_matches_oracle_and_triplet_kernel(natoms, nsf, ra, rb, rc, mt):
+    """Uniform block sizes get the output-tile schedule. Both FP64 kernels meet
+    the 1e-12 bar against the oracle and overwrite every output element. Type 1
+    sums every block in the same (ascending k) order in both kernels with the
+    same MFMA sequence, so there they agree bit for bit."""
+    case = Case(natoms, nsf, ra, rb, rc, mt, seed=31)
+    print(f"tile case natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
+    plan = cqmm.Plan(case.desc)
+    info = plan.info()
+    assert info.n_tiles > 0 and info.kernel == cqmm.KERNEL_TILE
+    assert 0 < info.n_loads <= 2 * info.n_triplets
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    ref = c_r if mt == 1 else a_r
+    got = {}
+    for kernel in (cqmm.KERNEL_TILE, cqmm.KERNEL_TRIPLET):
+        plan.set_kernel(kernel)
+        assert plan.info().kernel == kernel
+        x, y, z = a.copy(), b.copy(), c.copy()
+        out = z if mt == 1 else x
+        out[:] = np.nan
+        plan.mat_mult(x, y, z)
+        assert np.all(np.isfinite(out))
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        print(f"  kernel {kernel}: rel err {err:.3e} (tiles {info.n_tiles}, entries {info.n_entries}, "
+              f"loads/product {info.n_loads / info.n_triplets:.3f})", flush=True)
+        assert err <= TOL
+        assert np.all(out[ref == 0.0] == 0.0)
+        got[kernel] = out.copy()
+    if mt == 1:
+        np.testing.assert_array_equal(got[cqmm.KERNEL_TILE], got[cqmm.KERNEL_TRIPLET])
+
+
+def test_mixed_block_sizes_have_no_tile_schedule():
+    case = Case(40, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1, seed=4)
+    plan = cqmm.Plan(case.desc)
+    info = plan.info()
+    assert info.n_tiles == 0 and info.kernel == cqmm.KERNEL_TRIPLET
+    with pytest.raises(cqmm.CqmmError, match="no tile schedule"):
+        plan.set_kernel(cqmm.KERNEL_TILE)
