@@ -12,8 +12,10 @@
 //  * an entry of a tile is one contraction atom x with its (up to 8) left
 //    operand blocks P(row, x) and (up to 8) right operand blocks Q(column, x);
 //    entries of a tile are in ascending x (type 1: the reference's k order);
-//  * one workgroup (8 wavefronts) runs one tile. It stages a chunk of entries'
-//    operand blocks in LDS, then wavefront w accumulates the 8 output blocks
+//  * one workgroup (8 wavefronts) runs one tile. Chunks of entries have their
+//    operand blocks copied HBM->LDS (global_load_lds, no registers) into one
+//    of two stages while the previous chunk is computed from the other (one
+//    barrier per chunk); wavefront w accumulates the 8 output blocks
 //    (row u, column (w-u) mod 8), u = 0..7, with v_mfma_f64_16x16x4 from LDS,
 //    and finally writes each of its output blocks once (zeros where a block
 //    has no products), so the output is fully overwritten as mat_mult
@@ -40,24 +42,46 @@ namespace {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTileThreads = 512;            // 8 wavefronts
-constexpr int kTileLdsBytes = 64 * 1024;     // operand stage per workgroup (two workgroups per CU)
-
-template <int NSF>
-struct TileCfg {
-  static constexpr int NS2 = NSF * NSF;
-  static constexpr int CAP = kTileLdsBytes / (NS2 * 8);           // operand blocks per chunk
-  static constexpr int KS = (NSF + 3) / 4;                        // MFMA k-steps per product
-  static constexpr int STAGE_IT = (CAP * NS2 + kTileThreads - 1) / kTileThreads;
-};
-
-int tile_cap(int nsf) { return kTileLdsBytes / (nsf * nsf * 8); }
-
+constexpr int kStageBytes = 36 * 1024;       // one of the two operand stages (two workgroups per CU)
 // LDS reads of a block product never mask rows or columns beyond NSF: those
 // lanes only feed accumulator rows/columns >= NSF, which are never written.
 // Only the contraction index 4*s+g >= NSF at the last k-step is zeroed (both
 // operands), so the padding of the 16x16x4 steps adds nothing. kLdsPad doubles
-// after the stage keep the over-reads of the last block inside the array.
+// after each stage keep the over-reads of its last block inside the array.
 constexpr int kLdsPad = 256;
+
+template <int NSF>
+struct TileCfg {
+  static constexpr int NS2 = NSF * NSF;
+  static constexpr int CAP = kStageBytes / (NS2 * 8);   // operand blocks per chunk
+  static constexpr int KS = (NSF + 3) / 4;              // MFMA k-steps per product
+  static constexpr int DW = 2 * NS2;                    // dwords per block
+  static constexpr int NI = (DW + 63) / 64;             // LDS-direct loads per block and wavefront
+};
+
+int tile_cap(int nsf) { return kStageBytes / (nsf * nsf * 8); }
+
+// Issue the HBM->LDS copies of one chunk's operand blocks without waiting:
+// wavefront w copies blocks w, w+8, ... with global_load_lds_dword (lane l
+// moves dword k*64+l of the block to stage + block*DW + k*64 + l).
+template <int NSF>
+__device__ inline void stage_chunk(const int32_t *__restrict__ list, int64_t l0, int32_t nb,
+                                   const double *__restrict__ P, const double *__restrict__ Q, double *stage, int w,
+                                   int lane) {
+  using C = TileCfg<NSF>;
+  for (int32_t b = w; b < nb; b += 8) {
+    const int32_t lb = list[l0 + b];
+    const uint32_t *src = (const uint32_t *)((lb < 0 ? Q : P) + (int64_t)(lb & 0x7fffffff) * C::NS2);
+    uint32_t *dst = (uint32_t *)(stage + b * C::NS2);
+#pragma unroll
+    for (int k = 0; k < C::NI; ++k) {
+      const int d = k * 64 + lane;
+      if (d < C::DW)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + d),
+                                         (__attribute__((address_space(3))) void *)(dst + k * 64), 4, 0, 0);
+    }
+  }
+}
 
 template <bool QT, int NSF>
 __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
@@ -66,8 +90,8 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
     const int32_t *__restrict__ list, const int32_t *__restrict__ tile_out, const double *__restrict__ P,
     const double *__restrict__ Q, double *__restrict__ O, int32_t n_tiles) {
   using C = TileCfg<NSF>;
-  __shared__ double lds[C::CAP * C::NS2 + kLdsPad];
-  __shared__ int32_t lidx[C::CAP];
+  __shared__ double stage0[C::CAP * C::NS2 + kLdsPad];
+  __shared__ double stage1[C::CAP * C::NS2 + kLdsPad];
   // XCD-aware order: workgroup b runs on XCD b % 8 and takes tile
   // (b % 8) * ceil(T/8) + b / 8, so neighbouring tiles (same rows) share an L2.
   const int32_t per = (n_tiles + 7) >> 3;
@@ -93,40 +117,23 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
 #pragma unroll
   for (int u = 0; u < 8; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
+  if (c0 < c1) {
+    const int64_t l0 = entry_lbeg[chunk_entry[c0]];
+    stage_chunk<NSF>(list, l0, (int32_t)(entry_lbeg[chunk_entry[c0 + 1]] - l0), P, Q, stage0, w, lane);
+  }
   for (int32_t c = c0; c < c1; ++c) {
-    const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];
-    const int64_t l0 = entry_lbeg[e0];
-    const int32_t nb = (int32_t)(entry_lbeg[e1] - l0);  // <= CAP by construction
-    const int32_t nel = nb * C::NS2;
-    for (int i = tid; i < nb; i += kTileThreads) lidx[i] = list[l0 + i];
+    // Chunk c has landed (every wavefront waits for its own copies), and no
+    // wavefront still reads the stage that chunk c+1 goes to.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // Opaque copy of the thread index: keeps the per-element block/offset
-    // arithmetic inside the chunk loop instead of 2*STAGE_IT hoisted registers.
-    int tv = tid;
-    asm volatile("" : "+v"(tv));
-    // Stage the chunk's operand blocks in LDS, 8 loads in flight per thread.
-#pragma unroll
-    for (int it0 = 0; it0 < C::STAGE_IT; it0 += 8) {
-      double v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int idx = tv + kTileThreads * (it0 + j);
-        v[j] = 0.0;
-        if (it0 + j < C::STAGE_IT && idx < nel) {
-          const int b = idx / C::NS2;
-          const int wi = idx - b * C::NS2;
-          const int32_t lb = lidx[b];
-          const double *src = lb < 0 ? Q : P;
-          v[j] = src[(int64_t)(lb & 0x7fffffff) * C::NS2 + wi];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int idx = tv + kTileThreads * (it0 + j);
-        if (it0 + j < C::STAGE_IT && idx < nel) lds[idx] = v[j];
-      }
+    const bool odd = ((c - c0) & 1) != 0;
+    const double *cur = odd ? stage1 : stage0;
+    if (c + 1 < c1) {
+      const int64_t l1 = entry_lbeg[chunk_entry[c + 1]];
+      stage_chunk<NSF>(list, l1, (int32_t)(entry_lbeg[chunk_entry[c + 2]] - l1), P, Q, odd ? stage0 : stage1, w,
+                       lane);
     }
-    __syncthreads();
+    const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];
     int32_t base = 0;
     for (int32_t e = e0; e < e1; ++e) {
       const uint32_t m = entry_mask[e];
@@ -136,8 +143,8 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
       for (int u = 0; u < 8; ++u) {
         const int cl = (w - u) & 7;
         if (((omask >> u) & 1u) && ((ym >> u) & 1u) && ((bm >> cl) & 1u)) {
-          const double *la = lds + (base + __popc(ym & ((1u << u) - 1u))) * C::NS2 + oA;
-          const double *lq = lds + (base + ny + __popc(bm & ((1u << cl) - 1u))) * C::NS2 + oB;
+          const double *la = cur + (base + __popc(ym & ((1u << u) - 1u))) * C::NS2 + oA;
+          const double *lq = cur + (base + ny + __popc(bm & ((1u << cl) - 1u))) * C::NS2 + oB;
           double a[C::KS], bb[C::KS];
 #pragma unroll
           for (int s4 = 0; s4 < C::KS; ++s4) {
@@ -154,7 +161,6 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
       }
       base += __popc(m & 0xffffu);
     }
-    __syncthreads();
   }
   if (r < NSF) {
 #pragma unroll
