@@ -88,7 +88,9 @@ int64_t cqmm_purify_hbm_estimate(int32_t natoms, int32_t nsf, const cqmm_purify_
   auto trip = [&](double ra, double rb) { return N * nab_per_row(d, ra) * nab_per_row(d, rb); };
   const double t_big = std::max(trip(rL + rS, rL), trip(rL, rL + rS));
   const double trips = trip(rL, rS) + 2 * t_big + trip(rT, rS) + trip(rT, rT) + trip(rT, rH) + trip(rT + rH, rT);
-  const double plans = 16.0 * trips + 40.0 * t_big;  // resident lists + the largest build's transient buffers
+  // Resident triplet lists and tile schedules (~20 B per product) + the largest
+  // build's transient buffers (triplet fill, tile-key sort and entry scatter).
+  const double plans = 20.0 * trips + 90.0 * t_big;
   return (int64_t)(1.15 * (mats + f32 + plans)) + (64LL << 20);
 }
 

@@ -49,6 +49,7 @@ constexpr int kStageBytes = 36 * 1024;       // one of the two operand stages (t
 // operands), so the padding of the 16x16x4 steps adds nothing. kLdsPad doubles
 // after each stage keep the over-reads of its last block inside the array.
 constexpr int kLdsPad = 256;
+constexpr int kChunkEntries = 8;             // entries per chunk at most (one per lane of a row group)
 
 template <int NSF>
 struct TileCfg {
@@ -133,33 +134,48 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
       stage_chunk<NSF>(list, l1, (int32_t)(entry_lbeg[chunk_entry[c + 2]] - l1), P, Q, odd ? stage0 : stage1, w,
                        lane);
     }
-    const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];
-    int32_t base = 0;
-    for (int32_t e = e0; e < e1; ++e) {
-      const uint32_t m = entry_mask[e];
+    // The wavefront's products in this chunk, found lane-parallel: lane
+    // (u, j) = (lane / 8, lane % 8) looks at entry e0 + j and the wavefront's
+    // output block in tile row u; set ballot bits are products. The packed
+    // value holds the LDS stage slots of the two operand blocks.
+    const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];  // e1 - e0 <= 8
+    const int32_t ej = e0 + (lane & 7);
+    const int uu = lane >> 3;
+    uint32_t pk = 0;
+    bool has = false;
+    if (ej < e1) {
+      const uint32_t m = entry_mask[ej];
+      const int32_t lb = (int32_t)(entry_lbeg[ej] - entry_lbeg[e0]);
       const uint32_t ym = m & 0xffu, bm = (m >> 8) & 0xffu;
-      const int32_t ny = __popc(ym);
+      const int cl = (w - uu) & 7;
+      has = ((omask >> uu) & 1u) && ((ym >> uu) & 1u) && ((bm >> cl) & 1u);
+      const uint32_t ys = (uint32_t)(lb + __popc(ym & ((1u << uu) - 1u)));
+      const uint32_t qs = (uint32_t)(lb + __popc(ym) + __popc(bm & ((1u << cl) - 1u)));
+      pk = ys | (qs << 16);
+    }
+    const uint64_t pm = __ballot(has);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int cl = (w - u) & 7;
-        if (((omask >> u) & 1u) && ((ym >> u) & 1u) && ((bm >> cl) & 1u)) {
-          const double *la = cur + (base + __popc(ym & ((1u << u) - 1u))) * C::NS2 + oA;
-          const double *lq = cur + (base + ny + __popc(bm & ((1u << cl) - 1u))) * C::NS2 + oB;
-          double a[C::KS], bb[C::KS];
+    for (int u = 0; u < 8; ++u) {
+      uint32_t bits = (uint32_t)(pm >> (8 * u)) & 0xffu;
+      while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1u;
+        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)pk, 8 * u + j);
+        const double *la = cur + (int)(v & 0xffffu) * C::NS2 + oA;
+        const double *lq = cur + (int)(v >> 16) * C::NS2 + oB;
+        double a[C::KS], bb[C::KS];
 #pragma unroll
-          for (int s4 = 0; s4 < C::KS; ++s4) {
-            a[s4] = la[SA * s4];
-            bb[s4] = lq[SB * s4];
-          }
-          if (REM < 4 && !last_ok) {
-            a[C::KS - 1] = 0.0;
-            bb[C::KS - 1] = 0.0;
-          }
-#pragma unroll
-          for (int s4 = 0; s4 < C::KS; ++s4) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], acc[u], 0, 0, 0);
+        for (int s4 = 0; s4 < C::KS; ++s4) {
+          a[s4] = la[SA * s4];
+          bb[s4] = lq[SB * s4];
         }
+        if (REM < 4 && !last_ok) {
+          a[C::KS - 1] = 0.0;
+          bb[C::KS - 1] = 0.0;
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < C::KS; ++s4) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], acc[u], 0, 0, 0);
       }
-      base += __popc(m & 0xffffu);
     }
   }
   if (r < NSF) {
@@ -252,22 +268,25 @@ __global__ void k_tile_ebeg(const int32_t *__restrict__ etile, int64_t ne, int32
   tebeg[t] = (int32_t)lo;
 }
 
-// Chunks of a tile: consecutive entries whose operand blocks fit `cap`.
+// Chunks of a tile: at most kChunkEntries consecutive entries whose operand
+// blocks fit `cap`.
 // With FILL unset, counts them; with FILL set, writes each chunk's first entry.
 template <bool FILL>
 __global__ void k_chunks(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask, int32_t n_tiles,
                          int32_t cap, const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tiles) return;
-  int32_t nch = 0, sum = 0;
+  int32_t nch = 0, sum = 0, ne = 0;
   for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
     const int32_t c = __popc(mask[e]);
-    if (nch == 0 || sum + c > cap) {
+    if (nch == 0 || sum + c > cap || ne == kChunkEntries) {
       if (FILL) out[tchunk[t] + nch] = e;
       ++nch;
       sum = 0;
+      ne = 0;
     }
     sum += c;
+    ++ne;
   }
   if (!FILL) out[t] = nch;
 }
