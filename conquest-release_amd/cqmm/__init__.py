@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libcqmm.so")
 
 CQMM_DEVICE_PTRS = 1
 CQMM_FP32 = 2
+CQMM_FP32_ACC64 = 8
 CQMM_NO_SYNC = 4
 
 i32p = C.POINTER(C.c_int32)

@@ -59,8 +59,10 @@ struct TilePlan {
   int64_t n_entries = 0, n_loads = 0;
   DevBuf tile_chunk;   // int32 [n_tiles+1]  first chunk of each tile
   DevBuf chunk_entry;  // int32 [n_chunks+1] first entry of each chunk
+  DevBuf chunk_lbeg;   // int64 [n_chunks+1] first operand of each chunk in `list`
   DevBuf entry_mask;   // uint32 [n_entries] bits 0-7 left blocks (tile row), 8-15 right blocks (tile column)
   DevBuf entry_lbeg;   // int64 [n_entries+1] first operand of each entry in `list`
+  DevBuf entry_coff;   // int32 [n_entries] first operand of each entry relative to its chunk's
   DevBuf list;         // int32 [n_loads] operand block index; bit 31 set: right operand
   DevBuf tile_out;     // int32 [n_tiles*64] output block of slot row*8+col, or -1
 };
@@ -105,6 +107,8 @@ std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part
 int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s);
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
                          hipStream_t s);
+int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, bool acc64,
+                             hipStream_t s);
 
 // Launch the FP64 (or FP32) block-product kernel over a plan.
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O,
@@ -121,4 +125,6 @@ struct cqmm_plan {
   int64_t lena = 0, lenb = 0, lenc = 0;
   // Staging buffers for host-pointer calls (allocated on first use).
   cq::DevBuf sa, sb, sc;
+  // FP32 copies of the two operands and the result for CQMM_FP32 calls.
+  cq::DevBuf fp, fq, fo;
 };

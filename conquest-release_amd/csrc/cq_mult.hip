@@ -279,6 +279,8 @@ namespace cq {
 int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, bool acc64,
                               hipStream_t s) {
   if (tp.n_seg == 0) return CQMM_OK;
+  if (tp.tiles.n_tiles > 0 && tp.kernel != CQMM_KERNEL_TRIPLET && !force_general())
+    return launch_tile_products_f32(tp.tiles, tp.q_trans, P, Q, O, acc64, s);
   if (acc64) {
     if (tp.off64) return tp.q_trans ? launch_f32_t<int64_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int64_t, false, true>(tp, P, Q, O, s);
     return tp.q_trans ? launch_f32_t<int32_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int32_t, false, true>(tp, P, Q, O, s);

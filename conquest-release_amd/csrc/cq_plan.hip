@@ -30,6 +30,7 @@
 #include "../../include/cqmm.h"
 #include "cq_common.h"
 #include "cq_device.h"
+#include "cq_ops.h"
 #include "cq_system.h"
 
 namespace cq {
@@ -836,8 +837,24 @@ int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b, int
       if ((st = cq::hip_check(hipMemcpyAsync(dc, c, plan->lenc * 8, hipMemcpyHostToDevice, s), "H2D c"))) return st;
     }
   }
-  if (plan->tp.mult_type == 1) st = cq::launch_block_products(plan->tp, da, db, dc, fp32, s);
-  else st = cq::launch_block_products(plan->tp, dc, db, da, fp32, s);
+  // Operands as the kernels see them: P (left), Q (right), O (result).
+  const double *dP = plan->tp.mult_type == 1 ? da : dc;
+  double *dO = plan->tp.mult_type == 1 ? dc : da;
+  if (!fp32) {
+    st = cq::launch_block_products(plan->tp, dP, db, dO, false, s);
+  } else {
+    // FP32 products on FP32 copies; the result is widened back into the FP64 array.
+    const cq::TripletPlan &tp = plan->tp;
+    if (plan->fp.bytes < (size_t)tp.len_p * 4 && (st = plan->fp.alloc(tp.len_p * 4))) return st;
+    if (plan->fq.bytes < (size_t)tp.len_q * 4 && (st = plan->fq.alloc(tp.len_q * 4))) return st;
+    if (plan->fo.bytes < (size_t)tp.len_out * 4 && (st = plan->fo.alloc(tp.len_out * 4))) return st;
+    if ((st = cq::op_to_f32(tp.len_p, dP, plan->fp.as<float>(), s))) return st;
+    if ((st = cq::op_to_f32(tp.len_q, db, plan->fq.as<float>(), s))) return st;
+    if ((st = cq::launch_block_products_f32(tp, plan->fp.as<float>(), plan->fq.as<float>(), plan->fo.as<float>(),
+                                            (flags & CQMM_FP32_ACC64) != 0, s)))
+      return st;
+    st = cq::op_to_f64(tp.len_out, plan->fo.as<float>(), dO, s);
+  }
   if (st) return st;
   if (!dev) {
     if (plan->tp.mult_type == 1) st = cq::hip_check(hipMemcpyAsync(c, dc, plan->lenc * 8, hipMemcpyDeviceToHost, s), "D2H c");

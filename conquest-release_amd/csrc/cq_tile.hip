@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -54,45 +55,106 @@ constexpr int kChunkEntries = 8;             // entries per chunk at most (one p
 template <int NSF>
 struct TileCfg {
   static constexpr int NS2 = NSF * NSF;
-  static constexpr int CAP = kStageBytes / (NS2 * 8);   // operand blocks per chunk
+  static constexpr int CAP = kStageBytes / (NS2 * 8);   // operand blocks per chunk (FP64 sizing)
   static constexpr int KS = (NSF + 3) / 4;              // MFMA k-steps per product
-  static constexpr int DW = 2 * NS2;                    // dwords per block
-  static constexpr int NI = (DW + 63) / 64;             // LDS-direct loads per block and wavefront
 };
 
 int tile_cap(int nsf) { return kStageBytes / (nsf * nsf * 8); }
 
-// Issue the HBM->LDS copies of one chunk's operand blocks without waiting:
-// wavefront w copies blocks w, w+8, ... with global_load_lds_dword (lane l
-// moves dword k*64+l of the block to stage + block*DW + k*64 + l).
-template <int NSF>
-__device__ inline void stage_chunk(const int32_t *__restrict__ list, int64_t l0, int32_t nb,
-                                   const double *__restrict__ P, const double *__restrict__ Q, double *stage, int w,
-                                   int lane) {
-  using C = TileCfg<NSF>;
-  for (int32_t b = w; b < nb; b += 8) {
-    const int32_t lb = list[l0 + b];
-    const uint32_t *src = (const uint32_t *)((lb < 0 ? Q : P) + (int64_t)(lb & 0x7fffffff) * C::NS2);
-    uint32_t *dst = (uint32_t *)(stage + b * C::NS2);
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Prefetch loads are unconditional (indices clamped into the chunk) so that
+// no value is consumed before the next barrier: a use right after the load
+// would make the wavefront wait for every outstanding load, including the
+// stage copies that are meant to overlap the compute.
+//
+// Operand list of a chunk with nb > 0 blocks: lane k holds the operand of
+// block min(w + 8k, nb - 1) (lanes past wavefront w's blocks are unused).
+__device__ inline int32_t load_chunk_list(const int32_t *__restrict__ list, int64_t lbeg, int32_t nb, int w,
+                                          int lane) {
+  return list[lbeg + min(w + 8 * lane, nb - 1)];
+}
+
+// Entry masks and chunk-relative operand-list starts of a chunk's ne (1..8)
+// entries: lane with lane % 8 == j holds entry e0 + min(j, ne - 1). (32-bit
+// loads only: a dead half of a wider load would be reused by the register
+// allocator and force a wait for it.)
+__device__ inline void load_chunk_entries(const uint32_t *__restrict__ entry_mask,
+                                          const int32_t *__restrict__ entry_coff, int32_t e0, int32_t ne, int lane,
+                                          uint32_t &em, int32_t &eco) {
+  const int32_t e = e0 + min(lane & 7, ne - 1);
+  em = entry_mask[e];
+  eco = entry_coff[e];
+}
+
+// Issue the HBM->LDS copies of a chunk's operand blocks without waiting:
+// wavefront w copies blocks w, w+8, ... (operands in lv) with
+// global_load_lds_dword; lane l moves dword k*64+l of a block to
+// stage + block*DW + k*64 + l.
+template <int NSF, class T>
+__device__ inline void issue_stage(int32_t lv, int32_t nb, const T *__restrict__ P, const T *__restrict__ Q,
+                                   T *stage, int w, int lane) {
+  constexpr int NS2 = NSF * NSF;
+  constexpr int DW = NS2 * (int)sizeof(T) / 4;
+  constexpr int NI = (DW + 63) / 64;
+  const int32_t nbw = nb > w ? (nb - w + 7) >> 3 : 0;
+  for (int32_t k = 0; k < nbw; ++k) {
+    const int32_t lb = __builtin_amdgcn_readlane(lv, k);
+    const uint32_t *src = (const uint32_t *)((lb < 0 ? Q : P) + (int64_t)(lb & 0x7fffffff) * NS2);
+    uint32_t *dst = (uint32_t *)(stage + (w + 8 * k) * NS2);
 #pragma unroll
-    for (int k = 0; k < C::NI; ++k) {
-      const int d = k * 64 + lane;
-      if (d < C::DW)
+    for (int i = 0; i < NI; ++i) {
+      const int d = i * 64 + lane;
+      if (d < DW)
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + d),
-                                         (__attribute__((address_space(3))) void *)(dst + k * 64), 4, 0, 0);
+                                         (__attribute__((address_space(3))) void *)(dst + i * 64), 4, 0, 0);
     }
   }
 }
 
-template <bool QT, int NSF>
-__global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
+// The wavefront's products in a chunk, found lane-parallel: lane (u, j) =
+// (lane / 8, lane % 8) looks at entry j and the wavefront's output block in
+// tile row u; set ballot bits are products. pk holds the stage slots of the
+// two operand blocks.
+__device__ inline uint64_t chunk_products(uint32_t em, int32_t elb, int32_t ne, uint32_t omask, int w, int lane,
+                                          uint32_t &pk) {
+  // elb: the entry's first operand relative to the chunk's first.
+  const int j = lane & 7, uu = lane >> 3;
+  bool has = false;
+  pk = 0;
+  if (j < ne) {
+    const uint32_t ym = em & 0xffu, bm = (em >> 8) & 0xffu;
+    const int cl = (w - uu) & 7;
+    has = ((omask >> uu) & 1u) && ((ym >> uu) & 1u) && ((bm >> cl) & 1u);
+    const uint32_t ys = (uint32_t)(elb + __popc(ym & ((1u << uu) - 1u)));
+    const uint32_t qs = (uint32_t)(elb + __popc(ym) + __popc(bm & ((1u << cl) - 1u)));
+    pk = ys | (qs << 16);
+  }
+  return __ballot(has);
+}
+
+// MODE 0: FP64 operands, v_mfma_f64_16x16x4 (accumulator D[(l/16)+4e][l%16]).
+// MODE 1: FP32 operands, v_mfma_f32_16x16x4, FP32 accumulation.
+// MODE 2: FP32 operands and MFMA, each product's result added into FP64
+//         registers (accumulator D[4(l/16)+e][l%16]).
+// Pipeline per chunk c (one barrier): wait for chunk c's copies and the
+// prefetched registers, issue chunk c+1's copies into the other stage and the
+// register loads for chunks c+1 (entries) and c+2 (operand list), compute
+// chunk c from LDS, then load the chunk boundaries two chunks ahead (scalar
+// loads, waited for at the next barrier).
+template <int MODE, bool QT, int NSF>
+__global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
     const int32_t *__restrict__ tile_chunk, const int32_t *__restrict__ chunk_entry,
-    const uint32_t *__restrict__ entry_mask, const int64_t *__restrict__ entry_lbeg,
-    const int32_t *__restrict__ list, const int32_t *__restrict__ tile_out, const double *__restrict__ P,
-    const double *__restrict__ Q, double *__restrict__ O, int32_t n_tiles) {
+    const int64_t *__restrict__ chunk_lbeg, const uint32_t *__restrict__ entry_mask,
+    const int32_t *__restrict__ entry_coff, const int32_t *__restrict__ list, const int32_t *__restrict__ tile_out,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
+    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
   using C = TileCfg<NSF>;
-  __shared__ double stage0[C::CAP * C::NS2 + kLdsPad];
-  __shared__ double stage1[C::CAP * C::NS2 + kLdsPad];
+  constexpr int STAGE = C::CAP * C::NS2 + kLdsPad * (int)(8 / sizeof(T));
+  __shared__ T stage0[STAGE];
+  __shared__ T stage1[STAGE];
   // XCD-aware order: workgroup b runs on XCD b % 8 and takes tile
   // (b % 8) * ceil(T/8) + b / 8, so neighbouring tiles (same rows) share an L2.
   const int32_t per = (n_tiles + 7) >> 3;
@@ -114,79 +176,92 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products_f64(
     oblk[u] = tile_out[(int64_t)tile * 64 + u * 8 + ((w - u) & 7)];
     if (oblk[u] >= 0) omask |= 1u << u;
   }
-  dbl4 acc[8];
+  dbl4 a64[8];
+  flt4 a32[8];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int u = 0; u < 8; ++u) {
+    a64[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    a32[u] = flt4{0.f, 0.f, 0.f, 0.f};
+  }
   const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
   if (c0 < c1) {
-    const int64_t l0 = entry_lbeg[chunk_entry[c0]];
-    stage_chunk<NSF>(list, l0, (int32_t)(entry_lbeg[chunk_entry[c0 + 1]] - l0), P, Q, stage0, w, lane);
-  }
-  for (int32_t c = c0; c < c1; ++c) {
-    // Chunk c has landed (every wavefront waits for its own copies), and no
-    // wavefront still reads the stage that chunk c+1 goes to.
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    const bool odd = ((c - c0) & 1) != 0;
-    const double *cur = odd ? stage1 : stage0;
-    if (c + 1 < c1) {
-      const int64_t l1 = entry_lbeg[chunk_entry[c + 1]];
-      stage_chunk<NSF>(list, l1, (int32_t)(entry_lbeg[chunk_entry[c + 2]] - l1), P, Q, odd ? stage0 : stage1, w,
-                       lane);
-    }
-    // The wavefront's products in this chunk, found lane-parallel: lane
-    // (u, j) = (lane / 8, lane % 8) looks at entry e0 + j and the wavefront's
-    // output block in tile row u; set ballot bits are products. The packed
-    // value holds the LDS stage slots of the two operand blocks.
-    const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];  // e1 - e0 <= 8
-    const int32_t ej = e0 + (lane & 7);
-    const int uu = lane >> 3;
-    uint32_t pk = 0;
-    bool has = false;
-    if (ej < e1) {
-      const uint32_t m = entry_mask[ej];
-      const int32_t lb = (int32_t)(entry_lbeg[ej] - entry_lbeg[e0]);
-      const uint32_t ym = m & 0xffu, bm = (m >> 8) & 0xffu;
-      const int cl = (w - uu) & 7;
-      has = ((omask >> uu) & 1u) && ((ym >> uu) & 1u) && ((bm >> cl) & 1u);
-      const uint32_t ys = (uint32_t)(lb + __popc(ym & ((1u << uu) - 1u)));
-      const uint32_t qs = (uint32_t)(lb + __popc(ym) + __popc(bm & ((1u << cl) - 1u)));
-      pk = ys | (qs << 16);
-    }
-    const uint64_t pm = __ballot(has);
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      uint32_t bits = (uint32_t)(pm >> (8 * u)) & 0xffu;
-      while (bits) {
-        const int j = __builtin_ctz(bits);
-        bits &= bits - 1u;
-        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)pk, 8 * u + j);
-        const double *la = cur + (int)(v & 0xffffu) * C::NS2 + oA;
-        const double *lq = cur + (int)(v >> 16) * C::NS2 + oB;
-        double a[C::KS], bb[C::KS];
-#pragma unroll
-        for (int s4 = 0; s4 < C::KS; ++s4) {
-          a[s4] = la[SA * s4];
-          bb[s4] = lq[SB * s4];
-        }
-        if (REM < 4 && !last_ok) {
-          a[C::KS - 1] = 0.0;
-          bb[C::KS - 1] = 0.0;
-        }
-#pragma unroll
-        for (int s4 = 0; s4 < C::KS; ++s4) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], acc[u], 0, 0, 0);
+    // Sliding window of chunk boundaries (indices clamped to c1, in bounds).
+    int32_t ce0 = chunk_entry[c0], ce1 = chunk_entry[min(c0 + 1, c1)], ce2 = chunk_entry[min(c0 + 2, c1)];
+    int64_t cl0 = chunk_lbeg[c0], cl1 = chunk_lbeg[min(c0 + 1, c1)], cl2 = chunk_lbeg[min(c0 + 2, c1)],
+            cl3 = chunk_lbeg[min(c0 + 3, c1)];
+    uint32_t em;
+    int32_t eco;
+    load_chunk_entries(entry_mask, entry_coff, ce0, ce1 - ce0, lane, em, eco);
+    const int32_t lv0 = load_chunk_list(list, cl0, (int32_t)(cl1 - cl0), w, lane);
+    issue_stage<NSF, T>(lv0, (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+    int32_t lvn = 0;
+    if (c0 + 1 < c1) lvn = load_chunk_list(list, cl1, (int32_t)(cl2 - cl1), w, lane);
+    for (int32_t c = c0; c < c1; ++c) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      const bool odd = ((c - c0) & 1) != 0;
+      const T *cur = odd ? stage1 : stage0;
+      uint32_t pk;
+      const uint64_t pm = chunk_products(em, eco, ce1 - ce0, omask, w, lane, pk);
+      if (c + 1 < c1) {
+        issue_stage<NSF, T>(lvn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
+        load_chunk_entries(entry_mask, entry_coff, ce1, ce2 - ce1, lane, em, eco);
+        if (c + 2 < c1) lvn = load_chunk_list(list, cl2, (int32_t)(cl3 - cl2), w, lane);
       }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        uint32_t bits = (uint32_t)(pm >> (8 * u)) & 0xffu;
+        while (bits) {
+          const int j = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)pk, 8 * u + j);
+          const T *la = cur + (int)(v & 0xffffu) * C::NS2 + oA;
+          const T *lq = cur + (int)(v >> 16) * C::NS2 + oB;
+          T a[C::KS], bb[C::KS];
+#pragma unroll
+          for (int s4 = 0; s4 < C::KS; ++s4) {
+            a[s4] = la[SA * s4];
+            bb[s4] = lq[SB * s4];
+          }
+          if (REM < 4 && !last_ok) {
+            a[C::KS - 1] = (T)0;
+            bb[C::KS - 1] = (T)0;
+          }
+          if constexpr (MODE == 0) {
+#pragma unroll
+            for (int s4 = 0; s4 < C::KS; ++s4)
+              a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], a64[u], 0, 0, 0);
+          } else if constexpr (MODE == 1) {
+#pragma unroll
+            for (int s4 = 0; s4 < C::KS; ++s4)
+              a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], a32[u], 0, 0, 0);
+          } else {
+            flt4 t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s4 = 0; s4 < C::KS; ++s4) t = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], t, 0, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) a64[u][e] += (double)t[e];
+          }
+        }
+      }
+      ce0 = ce1;
+      ce1 = ce2;
+      ce2 = chunk_entry[min(c + 3, c1)];
+      cl0 = cl1;
+      cl1 = cl2;
+      cl2 = cl3;
+      cl3 = chunk_lbeg[min(c + 4, c1)];
     }
   }
   if (r < NSF) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       if (!((omask >> u) & 1u)) continue;
-      double *o = O + (int64_t)oblk[u] * C::NS2;
+      T *o = O + (int64_t)oblk[u] * C::NS2;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = g + 4 * e;  // FP64 accumulator: D[(l/16)+4e][l%16]
-        if (row < NSF) o[row + NSF * r] = acc[u][e];
+        const int row = MODE == 0 ? g + 4 * e : 4 * g + e;
+        if (row < NSF) o[row + NSF * r] = MODE == 0 ? (T)a64[u][e] : (MODE == 1 ? (T)a32[u][e] : (T)a64[u][e]);
       }
     }
   }
@@ -199,6 +274,21 @@ inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs);
 __global__ void k_fill_i32(int32_t *__restrict__ v, int64_t n, int32_t val) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) v[t] = val;
+}
+
+__global__ void k_chunk_lbeg(const int32_t *__restrict__ chunk_entry, const int64_t *__restrict__ entry_lbeg,
+                             int64_t nc1, int64_t *__restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nc1) out[c] = entry_lbeg[chunk_entry[c]];
+}
+
+// Operand-list start of each entry relative to its chunk's first operand.
+__global__ void k_entry_coff(const int32_t *__restrict__ chunk_entry, const int64_t *__restrict__ entry_lbeg,
+                             int64_t nc, int32_t *__restrict__ coff) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int64_t base = entry_lbeg[chunk_entry[c]];
+  for (int32_t e = chunk_entry[c]; e < chunk_entry[c + 1]; ++e) coff[e] = (int32_t)(entry_lbeg[e] - base);
 }
 
 __global__ void k_iota32(int32_t *__restrict__ v, int64_t n) {
@@ -306,19 +396,20 @@ int inclusive_scan(const T *in, T *out, size_t n, hipStream_t s) {
   return cq::hip_check(hipStreamSynchronize(s), "inclusive_scan");
 }
 
-template <bool QT>
-int launch_nsf(const cq::TilePlan &tp, const double *P, const double *Q, double *O, hipStream_t s) {
+template <int MODE, bool QT, class T>
+int launch_mode(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
   const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
   const dim3 blk(kTileThreads);
   const int32_t *tc = tp.tile_chunk.as<int32_t>(), *ce = tp.chunk_entry.as<int32_t>();
+  const int64_t *cl = tp.chunk_lbeg.as<int64_t>();
   const uint32_t *em = tp.entry_mask.as<uint32_t>();
-  const int64_t *lb = tp.entry_lbeg.as<int64_t>();
+  const int32_t *lb = tp.entry_coff.as<int32_t>();
   const int32_t *li = tp.list.as<int32_t>(), *to = tp.tile_out.as<int32_t>();
   switch (tp.nsf) {
-    case 4: hipLaunchKernelGGL((k_tile_products_f64<QT, 4>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
-    case 9: hipLaunchKernelGGL((k_tile_products_f64<QT, 9>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
-    case 13: hipLaunchKernelGGL((k_tile_products_f64<QT, 13>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
-    case 16: hipLaunchKernelGGL((k_tile_products_f64<QT, 16>), dim3(gridx), blk, 0, s, tc, ce, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
     default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: block size %d has no kernel instance", tp.nsf);
   }
   return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
@@ -327,6 +418,13 @@ int launch_nsf(const cq::TilePlan &tp, const double *P, const double *Q, double 
 }  // namespace
 
 namespace cq {
+
+int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, bool acc64,
+                             hipStream_t s) {
+  if (tp.n_tiles == 0) return CQMM_OK;
+  if (acc64) return q_trans ? launch_mode<2, true>(tp, P, Q, O, s) : launch_mode<2, false>(tp, P, Q, O, s);
+  return q_trans ? launch_mode<1, true>(tp, P, Q, O, s) : launch_mode<1, false>(tp, P, Q, O, s);
+}
 
 // CQMM_TILES=0 builds plans without the tile schedule (triplet kernels only).
 bool tiles_enabled() {
@@ -448,6 +546,12 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if ((st = hip_check(hipMemcpyAsync(tp.chunk_entry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
                       "H2D chunk end")))
     return st;
+  if ((st = tp.chunk_lbeg.alloc(((size_t)nch + 1) * 8))) return st;
+  hipLaunchKernelGGL(k_chunk_lbeg, dim3(grid1((int64_t)nch + 1, 256)), dim3(256), 0, s, tp.chunk_entry.as<int32_t>(),
+                     tp.entry_lbeg.as<int64_t>(), (int64_t)nch + 1, tp.chunk_lbeg.as<int64_t>());
+  if ((st = tp.entry_coff.alloc((size_t)ne * 4))) return st;
+  hipLaunchKernelGGL(k_entry_coff, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, tp.chunk_entry.as<int32_t>(),
+                     tp.entry_lbeg.as<int64_t>(), (int64_t)nch, tp.entry_coff.as<int32_t>());
   if ((st = hip_check(hipGetLastError(), "tile builder launch"))) return st;
   return hip_check(hipStreamSynchronize(s), "tile build");
 }
@@ -455,7 +559,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
                          hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
-  return q_trans ? launch_nsf<true>(tp, P, Q, O, s) : launch_nsf<false>(tp, P, Q, O, s);
+  return q_trans ? launch_mode<0, true>(tp, P, Q, O, s) : launch_mode<0, false>(tp, P, Q, O, s);
 }
 
 }  // namespace cq

@@ -187,3 +187,43 @@ def test_mixed_block_sizes_have_no_tile_schedule():
     assert info.n_tiles == 0 and info.kernel == cqmm.KERNEL_TRIPLET
     with pytest.raises(cqmm.CqmmError, match="no tile schedule"):
         plan.set_kernel(cqmm.KERNEL_TILE)
+
+
+FP32_CASES = [
+    (64, 13, 6.0, 8.0, 12.0, 1),
+    (64, 13, 8.0, 8.0, 12.0, 2),
+    (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),   # triplet kernels (mixed sizes)
+]
+
+
+@pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", FP32_CASES)
+def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
+    """CQMM_FP32: FP32 MFMA products on FP32 copies, FP32 accumulation
+    (bar 2e-5 relative to the FP64 oracle: inputs rounded to FP32, sums of a few
+    hundred terms in FP32) or, with CQMM_FP32_ACC64, each block's products summed
+    in FP64 (bar 2e-6: only the inputs and each 13-term product are FP32). The tile
+    and triplet kernels sum type-1 blocks in the same order and agree bit for bit."""
+    case = Case(natoms, nsf, ra, rb, rc, mt, seed=41)
+    print(f"fp32 case natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
+    plan = cqmm.Plan(case.desc)
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    ref = c_r if mt == 1 else a_r
+    kernels = (cqmm.KERNEL_TILE, cqmm.KERNEL_TRIPLET) if plan.info().n_tiles else (cqmm.KERNEL_TRIPLET,)
+    for flags, bar in ((cqmm.CQMM_FP32, 2e-5), (cqmm.CQMM_FP32 | cqmm.CQMM_FP32_ACC64, 2e-6)):
+        got = {}
+        for kernel in kernels:
+            plan.set_kernel(kernel)
+            x, y, z = a.copy(), b.copy(), c.copy()
+            out = z if mt == 1 else x
+            out[:] = np.nan
+            plan.mat_mult(x, y, z, flags)
+            assert np.all(np.isfinite(out))
+            err = np.abs(out - ref).max() / np.abs(ref).max()
+            print(f"  flags {flags} kernel {kernel}: rel err {err:.3e}", flush=True)
+            assert err <= bar
+            assert np.all(out[ref == 0.0] == 0.0)
+            got[kernel] = out.copy()
+        if mt == 1 and len(got) == 2:
+            np.testing.assert_array_equal(got[cqmm.KERNEL_TILE], got[cqmm.KERNEL_TRIPLET])
