@@ -65,6 +65,7 @@ struct TilePlan {
   DevBuf entry_coff;   // int32 [n_entries] first operand of each entry relative to its chunk's
   DevBuf list;         // int32 [n_loads] operand block index; bit 31 set: right operand
   DevBuf tile_out;     // int32 [n_tiles*64] output block of slot row*8+col, or -1
+  DevBuf slot_of;      // uint8 [n_tiles*64] [tile][wavefront][u]: slot owned, or 0xff
 };
 
 // Triplet list of one product, grouped by output block (segment), each group in

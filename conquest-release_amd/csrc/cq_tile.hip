@@ -15,12 +15,13 @@
 //  * one workgroup (8 wavefronts) runs one tile. Chunks of entries have their
 //    operand blocks copied HBM->LDS (global_load_lds, no registers) into one
 //    of two stages while the previous chunk is computed from the other (one
-//    barrier per chunk); wavefront w accumulates the 8 output blocks
-//    (row u, column (w-u) mod 8), u = 0..7, with v_mfma_f64_16x16x4 from LDS,
-//    and finally writes each of its output blocks once (zeros where a block
-//    has no products), so the output is fully overwritten as mat_mult
-//    overwrites it (src/multiply_module.f90:204-208). The diagonal ownership
-//    spreads the ragged products of an entry over all wavefronts.
+//    barrier per chunk); each wavefront accumulates up to 8 output blocks of
+//    the tile with v_mfma_f64_16x16x4 from LDS and finally writes each of them
+//    once (zeros where a block has no products), so the output is fully
+//    overwritten as mat_mult overwrites it (src/multiply_module.f90:204-208).
+//    Ownership is a Latin square (one block per tile row and column per
+//    wavefront), chosen when the plan is built to balance the wavefronts'
+//    product counts.
 // Summation order per output block is fixed (ascending x), so results are
 // bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
@@ -113,21 +114,20 @@ __device__ inline void issue_stage(int32_t lv, int32_t nb, const T *__restrict__
 }
 
 // The wavefront's products in a chunk, found lane-parallel: lane (u, j) =
-// (lane / 8, lane % 8) looks at entry j and the wavefront's output block in
-// tile row u; set ballot bits are products. pk holds the stage slots of the
-// two operand blocks.
-__device__ inline uint64_t chunk_products(uint32_t em, int32_t elb, int32_t ne, uint32_t omask, int w, int lane,
-                                          uint32_t &pk) {
-  // elb: the entry's first operand relative to the chunk's first.
-  const int j = lane & 7, uu = lane >> 3;
+// (lane / 8, lane % 8) looks at entry j and the wavefront's u-th output block
+// (tile row `row`, column `col` of this lane's u; valid = the block exists);
+// set ballot bits are products. pk holds the stage slots of the two operand
+// blocks (eco: the entry's first operand relative to the chunk's first).
+__device__ inline uint64_t chunk_products(uint32_t em, int32_t eco, int32_t ne, bool valid, int row, int col,
+                                          int lane, uint32_t &pk) {
+  const int j = lane & 7;
   bool has = false;
   pk = 0;
-  if (j < ne) {
+  if (valid && j < ne) {
     const uint32_t ym = em & 0xffu, bm = (em >> 8) & 0xffu;
-    const int cl = (w - uu) & 7;
-    has = ((omask >> uu) & 1u) && ((ym >> uu) & 1u) && ((bm >> cl) & 1u);
-    const uint32_t ys = (uint32_t)(elb + __popc(ym & ((1u << uu) - 1u)));
-    const uint32_t qs = (uint32_t)(elb + __popc(ym) + __popc(bm & ((1u << cl) - 1u)));
+    has = ((ym >> row) & 1u) && ((bm >> col) & 1u);
+    const uint32_t ys = (uint32_t)(eco + __popc(ym & ((1u << row) - 1u)));
+    const uint32_t qs = (uint32_t)(eco + __popc(ym) + __popc(bm & ((1u << col) - 1u)));
     pk = ys | (qs << 16);
   }
   return __ballot(has);
@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
     const int32_t *__restrict__ tile_chunk, const int32_t *__restrict__ chunk_entry,
     const int64_t *__restrict__ chunk_lbeg, const uint32_t *__restrict__ entry_mask,
     const int32_t *__restrict__ entry_coff, const int32_t *__restrict__ list, const int32_t *__restrict__ tile_out,
+    const uint8_t *__restrict__ slot_of,
     const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
     const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
     typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
@@ -169,13 +170,20 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
   const int oA = r + NSF * g;
   const int oB = QT ? (r + NSF * g) : (g + NSF * r);
   const bool last_ok = g < REM;
+  // The wavefront's output blocks: slot_of[tile][w][u] = tile row * 8 + column,
+  // or 0xff (balanced over the wavefronts when the plan was built).
+  const uint8_t *so = slot_of + (int64_t)tile * 64 + w * 8;
   int32_t oblk[8];
   uint32_t omask = 0;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
-    oblk[u] = tile_out[(int64_t)tile * 64 + u * 8 + ((w - u) & 7)];
+    const int sl = so[u];
+    oblk[u] = sl < 64 ? tile_out[(int64_t)tile * 64 + sl] : -1;
     if (oblk[u] >= 0) omask |= 1u << u;
   }
+  const int my_sl = so[lane >> 3];
+  const bool my_valid = my_sl < 64;
+  const int my_row = (my_sl >> 3) & 7, my_col = my_sl & 7;
   dbl4 a64[8];
   flt4 a32[8];
 #pragma unroll
@@ -202,7 +210,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
       const bool odd = ((c - c0) & 1) != 0;
       const T *cur = odd ? stage1 : stage0;
       uint32_t pk;
-      const uint64_t pm = chunk_products(em, eco, ce1 - ce0, omask, w, lane, pk);
+      const uint64_t pm = chunk_products(em, eco, ce1 - ce0, my_valid, my_row, my_col, lane, pk);
       if (c + 1 < c1) {
         issue_stage<NSF, T>(lvn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
         load_chunk_entries(entry_mask, entry_coff, ce1, ce2 - ce1, lane, em, eco);
@@ -294,6 +302,13 @@ __global__ void k_entry_coff(const int32_t *__restrict__ chunk_entry, const int6
 __global__ void k_iota32(int32_t *__restrict__ v, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) v[t] = (int32_t)t;
+}
+
+// Products per output slot of each tile.
+__global__ void k_slot_counts(const uint64_t *__restrict__ key, const uint8_t *__restrict__ slot, int64_t n,
+                              int64_t nx, int32_t *__restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) atomicAdd(&cnt[(int64_t)(key[t] / (uint64_t)nx) * 64 + slot[t]], 1);
 }
 
 // flag[pos] = 1 where a new (tile, x) key starts in the sorted list.
@@ -405,11 +420,12 @@ int launch_mode(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_
   const uint32_t *em = tp.entry_mask.as<uint32_t>();
   const int32_t *lb = tp.entry_coff.as<int32_t>();
   const int32_t *li = tp.list.as<int32_t>(), *to = tp.tile_out.as<int32_t>();
+  const uint8_t *sl = tp.slot_of.as<uint8_t>();
   switch (tp.nsf) {
-    case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
-    case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
-    case 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
-    case 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, P, Q, O, tp.n_tiles); break;
+    case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
+    case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
+    case 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
+    case 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
     default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: block size %d has no kernel instance", tp.nsf);
   }
   return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
@@ -471,6 +487,60 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if ((st = hip_check(hipMemcpyAsync(tp.tile_out.p, in.tile_out, (size_t)in.n_tiles * 64 * 4, hipMemcpyHostToDevice, s),
                       "H2D tile_out")))
     return st;
+  // Output slots to wavefronts, per tile: a Latin square over (tile row,
+  // column) — every wavefront owns one block in each tile row and each
+  // column, which spreads every entry's rectangle of products evenly — chosen
+  // among kLatinTries column permutations to balance the wavefronts' total
+  // product counts.
+  {
+    DevBuf d_cnt;
+    if ((st = d_cnt.alloc((size_t)in.n_tiles * 64 * 4))) return st;
+    if ((st = hip_check(hipMemsetAsync(d_cnt.p, 0, (size_t)in.n_tiles * 64 * 4, s), "memset slot counts"))) return st;
+    hipLaunchKernelGGL(k_slot_counts, dim3(grid1(n, 256)), dim3(256), 0, s, in.key, in.slot, n, in.nx,
+                       d_cnt.as<int32_t>());
+    std::vector<int32_t> cnt((size_t)in.n_tiles * 64);
+    if ((st = hip_check(hipMemcpyAsync(cnt.data(), d_cnt.p, cnt.size() * 4, hipMemcpyDeviceToHost, s), "D2H slot counts")) ||
+        (st = hip_check(hipStreamSynchronize(s), "slot counts")))
+      return st;
+    constexpr int kLatinTries = 24;
+    int perms[kLatinTries][8];
+    uint64_t lcg = 0x9e3779b97f4a7c15ull;
+    for (int k = 0; k < kLatinTries; ++k) {
+      for (int i = 0; i < 8; ++i) perms[k][i] = i;
+      for (int i = 7; k > 0 && i > 0; --i) {  // k = 0: identity
+        lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+        const int j = (int)((lcg >> 33) % (uint64_t)(i + 1));
+        std::swap(perms[k][i], perms[k][j]);
+      }
+    }
+    std::vector<uint8_t> slot_of((size_t)in.n_tiles * 64, 0xff);
+    for (int64_t t = 0; t < in.n_tiles; ++t) {
+      const int32_t *c = cnt.data() + t * 64;
+      int best = 0;
+      int64_t best_max = -1;
+      for (int k = 0; k < kLatinTries; ++k) {
+        int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int sl = 0; sl < 64; ++sl) load[((sl >> 3) + perms[k][sl & 7]) & 7] += c[sl];
+        int64_t m = 0;
+        for (int w = 0; w < 8; ++w) m = std::max(m, load[w]);
+        if (best_max < 0 || m < best_max) {
+          best_max = m;
+          best = k;
+        }
+      }
+      // wavefront w, row u owns column col with (u + perm[col]) % 8 == w.
+      for (int u = 0; u < 8; ++u)
+        for (int col = 0; col < 8; ++col) {
+          const int w = (u + perms[best][col]) & 7;
+          if (in.tile_out[t * 64 + u * 8 + col] >= 0) slot_of[t * 64 + w * 8 + u] = (uint8_t)(u * 8 + col);
+        }
+    }
+    if ((st = tp.slot_of.alloc(slot_of.size()))) return st;
+    if ((st = hip_check(hipMemcpyAsync(tp.slot_of.p, slot_of.data(), slot_of.size(), hipMemcpyHostToDevice, s),
+                        "H2D slot_of")) ||
+        (st = hip_check(hipStreamSynchronize(s), "slot_of")))
+      return st;
+  }
   // Sort the products by (tile, contraction atom).
   unsigned end_bit = 1;
   const uint64_t kmax = (uint64_t)in.n_tiles * (uint64_t)in.nx;
