@@ -93,7 +93,7 @@ class McwSetup(C.Structure):
 class McwResult(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("fp32_iterations", C.c_int32), ("stop_reason", C.c_int32),
                 ("restored", C.c_int32), ("energy", C.c_double), ("electrons", C.c_double), ("c", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("energy_trace", C.c_double * 256)]
 
 
 class InitMcwSetup(C.Structure):

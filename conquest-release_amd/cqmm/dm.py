@@ -219,7 +219,7 @@ class Purifier:
               "cqmm_mcweeny")
         return dict(iterations=res.iterations, fp32_iterations=res.fp32_iterations, stop_reason=res.stop_reason,
                     restored=res.restored, energy=res.energy, electrons=res.electrons, c=res.c,
-                    seconds=res.seconds)
+                    seconds=res.seconds, energy_trace=list(res.energy_trace[:min(res.iterations, 256)]))
 
     def solve(self, precision=0, max_iter=100, n_electrons=None, n_final_fp64=2, fp32_tol=1e-6):
         """Hotelling -> InitMcW -> McWMin; band energy = 2 Tr[rho H] (spin-unpolarised)."""

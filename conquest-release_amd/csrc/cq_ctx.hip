@@ -556,6 +556,7 @@ int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t 
   const int nfinal = su->n_final_fp64 > 0 ? su->n_final_fp64 : 2;
   for (n = 1; n <= max_iter; ++n) {
     CQ_TRY(mcw_step(ctx, su, matH, matS, matL, rho, w, fp32_phase, &omega, &cn, &ne));
+    if (n - 1 < CQMM_MCW_TRACE) res->energy_trace[n - 1] = omega;
     if (fp32_phase) ++res->fp32_iterations;
     else ++fp64_done;
     // McWMin's stop test (src/McWeeny.f90:166-173).

@@ -139,9 +139,8 @@ __device__ inline uint64_t chunk_products(uint32_t em, int32_t eco, int32_t ne, 
 //         registers (accumulator D[4(l/16)+e][l%16]).
 // Pipeline per chunk c (one barrier): wait for chunk c's copies and the
 // prefetched registers, issue chunk c+1's copies into the other stage and the
-// register loads for chunks c+1 (entries) and c+2 (operand list), compute
-// chunk c from LDS, then load the chunk boundaries two chunks ahead (scalar
-// loads, waited for at the next barrier).
+// register loads for chunks c+1 (entries) and c+2 (operand list), then
+// compute chunk c from LDS.
 template <int MODE, bool QT, int NSF>
 __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
     const int32_t *__restrict__ tile_chunk, const int32_t *__restrict__ chunk_entry,
@@ -172,16 +171,17 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
   const bool last_ok = g < REM;
   // The wavefront's output blocks: slot_of[tile][w][u] = tile row * 8 + column,
   // or 0xff (balanced over the wavefronts when the plan was built).
-  const uint8_t *so = slot_of + (int64_t)tile * 64 + w * 8;
+  // (One 8-byte scalar load: byte loads would be per-lane vector loads.)
+  const uint64_t so8 = *(const uint64_t *)(slot_of + (int64_t)tile * 64 + w * 8);
   int32_t oblk[8];
   uint32_t omask = 0;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
-    const int sl = so[u];
+    const int sl = (int)((so8 >> (8 * u)) & 0xffu);
     oblk[u] = sl < 64 ? tile_out[(int64_t)tile * 64 + sl] : -1;
     if (oblk[u] >= 0) omask |= 1u << u;
   }
-  const int my_sl = so[lane >> 3];
+  const int my_sl = (int)((so8 >> (8 * (lane >> 3))) & 0xffu);
   const bool my_valid = my_sl < 64;
   const int my_row = (my_sl >> 3) & 7, my_col = my_sl & 7;
   dbl4 a64[8];
@@ -193,22 +193,42 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
   }
   const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
   if (c0 < c1) {
-    // Sliding window of chunk boundaries (indices clamped to c1, in bounds).
-    int32_t ce0 = chunk_entry[c0], ce1 = chunk_entry[min(c0 + 1, c1)], ce2 = chunk_entry[min(c0 + 2, c1)];
-    int64_t cl0 = chunk_lbeg[c0], cl1 = chunk_lbeg[min(c0 + 1, c1)], cl2 = chunk_lbeg[min(c0 + 2, c1)],
-            cl3 = chunk_lbeg[min(c0 + 3, c1)];
+    // Chunk boundaries of the tile held in registers, 64 chunks at a time:
+    // lane l has chunk_entry / chunk_lbeg of chunk wb + l (index clamped to
+    // c1). Vector loads, refilled after the compute of a chunk, so the loop
+    // has no scalar loads (a scalar load outstanding during the compute
+    // would make every LDS wait also wait for it).
+    int32_t wb = c0;
+    int32_t wce = chunk_entry[min(wb + lane, c1)];
+    int64_t wcl = chunk_lbeg[min(wb + lane, c1)];
+    auto ce_at = [&](int32_t cc) -> int32_t { return __builtin_amdgcn_readlane(wce, min(cc, c1) - wb); };
+    auto cl_at = [&](int32_t cc) -> int64_t {
+      const int k = min(cc, c1) - wb;
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wcl, k);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)wcl >> 32), k);
+      return (int64_t)(((uint64_t)hi << 32) | lo);
+    };
     uint32_t em;
     int32_t eco;
-    load_chunk_entries(entry_mask, entry_coff, ce0, ce1 - ce0, lane, em, eco);
-    const int32_t lv0 = load_chunk_list(list, cl0, (int32_t)(cl1 - cl0), w, lane);
-    issue_stage<NSF, T>(lv0, (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+    {
+      const int32_t ce0 = ce_at(c0), ce1 = ce_at(c0 + 1);
+      const int64_t cl0 = cl_at(c0), cl1 = cl_at(c0 + 1);
+      load_chunk_entries(entry_mask, entry_coff, ce0, ce1 - ce0, lane, em, eco);
+      const int32_t lv0 = load_chunk_list(list, cl0, (int32_t)(cl1 - cl0), w, lane);
+      issue_stage<NSF, T>(lv0, (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+    }
     int32_t lvn = 0;
-    if (c0 + 1 < c1) lvn = load_chunk_list(list, cl1, (int32_t)(cl2 - cl1), w, lane);
+    if (c0 + 1 < c1) {
+      const int64_t cl1 = cl_at(c0 + 1);
+      lvn = load_chunk_list(list, cl1, (int32_t)(cl_at(c0 + 2) - cl1), w, lane);
+    }
     for (int32_t c = c0; c < c1; ++c) {
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       const bool odd = ((c - c0) & 1) != 0;
       const T *cur = odd ? stage1 : stage0;
+      const int32_t ce0 = ce_at(c), ce1 = ce_at(c + 1), ce2 = ce_at(c + 2);
+      const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2), cl3 = cl_at(c + 3);
       uint32_t pk;
       const uint64_t pm = chunk_products(em, eco, ce1 - ce0, my_valid, my_row, my_col, lane, pk);
       if (c + 1 < c1) {
@@ -252,13 +272,11 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
           }
         }
       }
-      ce0 = ce1;
-      ce1 = ce2;
-      ce2 = chunk_entry[min(c + 3, c1)];
-      cl0 = cl1;
-      cl1 = cl2;
-      cl2 = cl3;
-      cl3 = chunk_lbeg[min(c + 4, c1)];
+      if (c + 4 > wb + 63 && c + 1 < c1) {  // next iteration reads up to chunk c + 4
+        wb = c + 1;
+        wce = chunk_entry[min(wb + lane, c1)];
+        wcl = chunk_lbeg[min(wb + lane, c1)];
+      }
     }
   }
   if (r < NSF) {

@@ -64,6 +64,7 @@ module cqmm_f
   type, bind(c) :: cqmm_mcw_result
      integer(c_int32_t) :: iterations, fp32_iterations, stop_reason, restored
      real(c_double) :: energy, electrons, c, seconds
+     real(c_double) :: energy_trace(256)
   end type cqmm_mcw_result
 
   interface

@@ -102,6 +102,7 @@ typedef struct cqmm_mcw_setup {
   double fp32_tol;         /* FP32 phase ends when |dE| < fp32_tol (Ha, per spin)      */
 } cqmm_mcw_setup;
 
+#define CQMM_MCW_TRACE 256
 typedef struct cqmm_mcw_result {
   int32_t iterations;      /* McWeeny steps taken (accepted + the final rejected one)   */
   int32_t fp32_iterations; /* of which with FP32 products                              */
@@ -111,6 +112,8 @@ typedef struct cqmm_mcw_result {
   double electrons;        /* Tr[rho S] (per spin)                                     */
   double c;                /* last mid-point c                                         */
   double seconds;          /* device wall time of the loop                             */
+  double energy_trace[CQMM_MCW_TRACE]; /* Tr[rho H] after step n (n < CQMM_MCW_TRACE), the
+                              steps in order, FP32 ones included                        */
 } cqmm_mcw_result;
 
 /* McWMin (src/McWeeny.f90:107-272) over handles matH, matS, matL (one spin,
