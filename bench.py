@@ -39,7 +39,20 @@ WORKLOADS = {
     "cfg2": dict(natoms=4096, nsf=13, mult_type=2, ra=16.0, rb=16.0, rc=8.0,
                  desc="plain FP64 drop-in at DZP scale, 4096 atoms, 13x13 blocks, "
                       "X(16 A) = A(8 A).B(16 A)^T, C truncated to 16 A (m_kern_min)"),
+    "cfg3": dict(natoms=8192, nsf=((4, .5), (9, .25), (13, .25)), mult_type=1, ra=8.0, rb=12.0, rc=16.0,
+                 desc="batched uneven blocks, 8192 atoms, species 50% 4 / 25% 9 / 25% 13 SF (rectangular "
+                      "blocks, one launch), C(16 A) = A(8 A).B(12 A), FP64 and FP32-accumulate"),
 }
+
+
+def species_nsf(natoms, mix, seed):
+    """Support functions per atom: an int, or a seeded species draw ((nsf, fraction), ...)."""
+    if isinstance(mix, int):
+        return mix
+    rng = np.random.default_rng(seed + 7919)
+    vals = np.array([m[0] for m in mix], np.int32)
+    probs = np.array([m[1] for m in mix], np.float64)
+    return vals[rng.choice(len(vals), size=natoms, p=probs / probs.sum())]
 
 
 def log(msg):
@@ -186,7 +199,8 @@ def main():
     xyz, edge = cqmm.random_structure(w["natoms"], 0.05, 2.0, args.seed)
     ra, rb, rc = w["ra"], w["rb"], w["rc"]
     rcover = max(ra + rb, rc)
-    sysm = cqmm.System(xyz, edge, w["nsf"], rcover, cqmm.default_grid(edge, 4.0))
+    sysm = cqmm.System(xyz, edge, species_nsf(w["natoms"], w["nsf"], args.seed), rcover,
+                       cqmm.default_grid(edge, 4.0))
     ids = {}
     for r in sorted({ra, rb, rc}):
         ids[r] = sysm.add_range(r)
@@ -238,6 +252,25 @@ def main():
     achieved = flops / (kern_avg_ms / 1e3) / 1e12
     out = (dc if w["mult_type"] == 1 else da).to_host()
     log(f"step {ms_step:.3f} ms (kernel {kern_avg_ms:.3f} ms), {value:.1f} GFLOP/s; checksum {float(np.abs(out).sum()):.6e}")
+    fp32 = None
+    if args.workload == "cfg3":
+        # Same products with FP32 MFMA on FP32 copies, FP32 accumulation; error vs the FP64 result.
+        f32 = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_FP32 | cqmm.CQMM_NO_SYNC
+        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, f32)
+        cqmm.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, f32)
+        cqmm.synchronize()
+        ms32 = (time.perf_counter() - t0) / args.steps * 1e3
+        out32 = (dc if w["mult_type"] == 1 else da).to_host()
+        d = out32 - out
+        fp32 = {"ms_per_step": round(ms32, 4), "gflops": round(flops / (ms32 / 1e3) / 1e9, 3),
+                "max_rel_err": float(np.abs(d).max() / np.abs(out).max()),
+                "frobenius_rel_err": float(np.linalg.norm(d) / np.linalg.norm(out)),
+                "note": "per call: FP64->FP32 operand copies, FP32 MFMA products (FP32 accumulation), FP32->FP64 result"}
+        log(f"FP32-accumulate: {ms32:.3f} ms, {fp32['gflops']:.1f} GFLOP/s, max rel err {fp32['max_rel_err']:.3e}, "
+            f"Frobenius rel err {fp32['frobenius_rel_err']:.3e}")
     line = {
         "metric": "block-sparse mult GFLOP/s",
         "value": round(value, 3),
@@ -253,11 +286,12 @@ def main():
         "data": "synthetic (seeded random periodic structure; N(0,1)*exp(-r/2) block entries)",
         "config": {
             "workload": w["desc"],
-            "natoms": w["natoms"], "nsf": w["nsf"], "mult_type": w["mult_type"],
+            "natoms": w["natoms"], "nsf": w["nsf"] if isinstance(w["nsf"], int) else "4/9/13 mix 0.5/0.25/0.25",
+            "mult_type": w["mult_type"],
             "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
             "flop_per_step": flops, "output_blocks": int(info.n_out_total),
             "plan_build_s": round(t_plan, 3), "setup_s": round(setup_s, 2),
-            "kernel": kname, "tiles": int(info.n_tiles),
+            "kernel": kname, "tiles": int(info.n_tiles), "fp32_accumulate": fp32,
             "operand_blocks_per_product": round(info.n_loads / max(info.n_triplets, 1), 4)
             if info.kernel == cqmm.KERNEL_TILE else 2.0,
         },

@@ -96,6 +96,30 @@ class McwResult(C.Structure):
                 ("seconds", C.c_double), ("energy_trace", C.c_double * 256)]
 
 
+class LnvSetup(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "range_L", "range_S", "range_H", "range_LS", "range_LH", "range_LSL", "range_SLS",
+        "mult_L_S_LS", "mult_L_H_LH", "mult_HL_S_LSL", "mult_HLS_LS_L", "mult_SL_S_SLS", "mult_SLS_LS_L",
+        "mult_LS_L_LSL", "mult_LSL_SL_H")]
+
+
+class DmmSetup(C.Structure):
+    _fields_ = [("lnv", LnvSetup), ("range_T", C.c_int32), ("range_TL", C.c_int32), ("mult_T_L_TL", C.c_int32),
+                ("mult_TL_T_L", C.c_int32), ("n_L_iterations", C.c_int32), ("vary_mu", C.c_int32),
+                ("maxpulay", C.c_int32), ("global_tolerance", C.c_int32), ("n_atoms", C.c_int32),
+                ("tolerance", C.c_double), ("min_pulay_step", C.c_double), ("max_pulay_step", C.c_double),
+                ("lin_tol", C.c_double), ("electrons_target", C.c_double), ("delta_e", C.c_double)]
+
+
+class DmmResult(C.Structure):
+    _fields_ = [("done", C.c_int32), ("early_iterations", C.c_int32), ("late_iterations", C.c_int32),
+                ("inflexions", C.c_int32), ("energy", C.c_double), ("electrons", C.c_double),
+                ("residual", C.c_double), ("seconds", C.c_double)]
+
+
+LNV_K, LNV_E, LNV_M3, LNV_PHI = 1, 2, 4, 8
+
+
 class InitMcwSetup(C.Structure):
     _fields_ = [("range_T", C.c_int32), ("range_S", C.c_int32), ("range_H", C.c_int32), ("range_L", C.c_int32),
                 ("range_TS", C.c_int32), ("range_TH", C.c_int32), ("mult_T_S_TS", C.c_int32),
@@ -128,7 +152,8 @@ EXPORTS = [
     "cqmm_matrix_allocate", "cqmm_matrix_free", "cqmm_matrix_length", "cqmm_matrix_range",
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
     "cqmm_matrix_scale", "cqmm_matrix_product_trace", "cqmm_matrix_trace", "cqmm_matrix_symmetrise",
-    "cqmm_mcweeny", "cqmm_init_mcweeny", "cqmm_hotelling",
+    "cqmm_matrix_transpose", "cqmm_mcweeny", "cqmm_init_mcweeny", "cqmm_hotelling",
+    "cqmm_lnv", "cqmm_correct_electrons", "cqmm_dmm",
     "cqmm_purify_default_params", "cqmm_purify_hbm_estimate", "cqmm_purify", "cqmm_serve",
 ]
 

@@ -12,11 +12,12 @@ import ctypes as C
 
 import numpy as np
 
-from . import (CqmmError, HotellingSetup, InitMcwSetup, MatrixIndex, McwResult, McwSetup, System, check, init,
-               lib)
+from . import (CqmmError, DmmResult, DmmSetup, HotellingSetup, InitMcwSetup, LnvSetup, MatrixIndex, McwResult,
+               McwSetup, System, check, init, lib)
 
 # mult ids (names as in src/mult_module.f90:128-160)
 L_S_LS, LS_L_LSL, LSL_SL_L, T_S_TS, TS_T_T, T_H_TH, TH_T_L = 1, 2, 3, 4, 5, 6, 7
+L_H_LH, HL_S_LSL, HLS_LS_L, SL_S_SLS, SLS_LS_L, LSL_SL_H, T_L_TL, TL_T_L = 8, 9, 10, 11, 12, 13, 14, 15
 
 
 class Context:
@@ -113,6 +114,9 @@ class Context:
     def symmetrise(self, A):
         check(lib().cqmm_matrix_symmetrise(self.h, A), "cqmm_matrix_symmetrise")
 
+    def transpose(self, A, B):
+        check(lib().cqmm_matrix_transpose(self.h, A, B), "cqmm_matrix_transpose")
+
 
 class Purifier:
     """McWeeny purification workload on one System.
@@ -131,17 +135,22 @@ class Purifier:
 
     def __init__(self, xyz, box, nsf, r_S=8.0, r_H=8.0, r_L=12.0, r_T=None, grid=None, seed=1,
                  s_amp=0.1, s_decay=1.5, h_amp=-0.2, h_decay=2.0, h_onsite=0.5, h_split=1.5, h_shift=3.0,
-                 device=True):
+                 device=True, lnv=False):
         r_T = r_S if r_T is None else r_T
         self.r = dict(S=r_S, H=r_H, L=r_L, T=r_T, LS=r_L + r_S, LSL=2 * r_L + r_S, TS=r_T + r_S, TH=r_T + r_H)
+        self.lnv_ranges = lnv
+        if lnv:  # LNV / DMM ranges (src/dimens_module.f90:342-355)
+            self.r.update(LH=r_L + r_H, SLS=2 * r_S + r_L, TL=r_T + r_L)
         rcover = max(self.r["LSL"], self.r["L"] + self.r["LS"], self.r["LS"] + self.r["L"], self.r["TH"] + r_T,
                      2 * r_T, r_T + r_S, r_T + r_H)
+        if lnv:
+            rcover = max(rcover, self.r["LH"] + r_S, r_H + self.r["LS"], self.r["LS"] + r_S, r_T + r_L)
         self.sys = System(xyz, box, nsf, rcover, grid)
         self.natoms = self.sys.natoms
         self.nsf = np.broadcast_to(np.asarray(nsf, np.int32), (self.natoms,))
         self.n_orb = int(self.nsf.sum())
         self.rid = {}
-        for name in ("S", "H", "L", "T", "LS", "LSL", "TS", "TH"):
+        for name in ("S", "H", "L", "T", "LS", "LSL", "TS", "TH") + (("LH", "SLS", "TL") if lnv else ()):
             rc = self.r[name]
             same = [k for k, v in self.rid.items() if self.r[k] == rc]
             self.rid[name] = self.rid[same[0]] if same else self.sys.add_range(rc)
@@ -176,8 +185,10 @@ class Purifier:
     def to_device(self):
         """Create the device context: ranges, plans, and HBM copies of S and H."""
         self.ctx = Context(self.sys)
+        R = self.rid
+        trans = {R["L"]} | ({R["LS"], R["LH"], R["LSL"], R["T"]} if self.lnv_ranges else set())
         for name in sorted(set(self.rid.values())):
-            self.ctx.add_range(name, transpose=(name == self.rid["L"]))
+            self.ctx.add_range(name, transpose=(name in trans))
         R = self.rid
         self.ctx.add_mult(L_S_LS, 1, R["L"], R["S"], R["LS"])
         self.ctx.add_mult(LS_L_LSL, 1, R["LS"], R["L"], R["LSL"])
@@ -186,6 +197,15 @@ class Purifier:
         self.ctx.add_mult(TS_T_T, 2, R["T"], R["T"], R["TS"])
         self.ctx.add_mult(T_H_TH, 1, R["T"], R["H"], R["TH"])
         self.ctx.add_mult(TH_T_L, 1, R["TH"], R["T"], R["L"])
+        if self.lnv_ranges:  # mult table, src/mult_module.f90:583-875
+            self.ctx.add_mult(L_H_LH, 1, R["L"], R["H"], R["LH"])
+            self.ctx.add_mult(HL_S_LSL, 1, R["LH"], R["S"], R["LSL"])
+            self.ctx.add_mult(HLS_LS_L, 2, R["L"], R["LS"], R["LSL"])
+            self.ctx.add_mult(SL_S_SLS, 1, R["LS"], R["S"], R["SLS"])
+            self.ctx.add_mult(SLS_LS_L, 2, R["L"], R["LS"], R["SLS"])
+            self.ctx.add_mult(LSL_SL_H, 2, R["H"], R["LS"], R["LSL"])
+            self.ctx.add_mult(T_L_TL, 1, R["T"], R["L"], R["TL"])
+            self.ctx.add_mult(TL_T_L, 2, R["L"], R["T"], R["TL"])
         self.matS = self.ctx.allocate(R["S"])
         self.matH = self.ctx.allocate(R["H"])
         self.matT = self.ctx.allocate(R["T"])
@@ -220,6 +240,52 @@ class Purifier:
         return dict(iterations=res.iterations, fp32_iterations=res.fp32_iterations, stop_reason=res.stop_reason,
                     restored=res.restored, energy=res.energy, electrons=res.electrons, c=res.c,
                     seconds=res.seconds, energy_trace=list(res.energy_trace[:min(res.iterations, 256)]))
+
+    # -- LNV / DMM (src/mult_module.f90:1478-1823, src/DMMinModule.f90) ----
+    def lnv_setup(self):
+        R = self.rid
+        return LnvSetup(R["L"], R["S"], R["H"], R["LS"], R["LH"], R["LSL"], R["SLS"], L_S_LS, L_H_LH, HL_S_LSL,
+                        HLS_LS_L, SL_S_SLS, SLS_LS_L, LS_L_LSL, LSL_SL_H)
+
+    def lnv(self, flags=15):
+        """LNV_matrix_multiply on the current L: dict(K, M3, phi, energy, electrons) (per spin)."""
+        c = self.ctx
+        K, M3, phi = c.allocate(self.rid["H"]), c.allocate(self.rid["L"]), c.allocate(self.rid["L"])
+        el, en = C.c_double(), C.c_double()
+        su = self.lnv_setup()
+        check(lib().cqmm_lnv(c.h, C.byref(su), flags, self.matH, self.matS, self.matL, K, M3, phi, C.byref(el),
+                             C.byref(en)), "cqmm_lnv")
+        out = dict(K=c.get(K), M3=c.get(M3), phi=c.get(phi), energy=en.value, electrons=el.value)
+        for h in (phi, M3, K):
+            c.free(h)
+        return out
+
+    def dmm_setup(self, n_electrons=None, tolerance=1e-7, n_L_iterations=50, vary_mu=True, maxpulay=5,
+                  delta_e=0.0):
+        ne = float(self.n_orb // 2) if n_electrons is None else n_electrons
+        R = self.rid
+        return DmmSetup(self.lnv_setup(), R["T"], R["TL"], T_L_TL, TL_T_L, n_L_iterations, int(vary_mu), maxpulay, 0,
+                        self.natoms, tolerance, 1e-3, 0.1, 0.1, ne, delta_e)
+
+    def correct_electrons(self, **kw):
+        su = self.dmm_setup(**kw)
+        el = C.c_double()
+        check(lib().cqmm_correct_electrons(self.ctx.h, C.byref(su), self.matH, self.matS, self.matT, self.matL,
+                                           C.byref(el)), "cqmm_correct_electrons")
+        return el.value
+
+    def dmm(self, **kw):
+        """FindMinDM from the current L; K receives 3LSL - 2LSLSL."""
+        su = self.dmm_setup(**kw)
+        res = DmmResult()
+        K = self.ctx.allocate(self.rid["H"])
+        check(lib().cqmm_dmm(self.ctx.h, C.byref(su), self.matH, self.matS, self.matT, self.matL, K, C.byref(res)),
+              "cqmm_dmm")
+        out = dict(done=res.done, early_iterations=res.early_iterations, late_iterations=res.late_iterations,
+                   inflexions=res.inflexions, band_energy=res.energy, n_electrons=res.electrons,
+                   residual=res.residual, seconds=res.seconds, K=self.ctx.get(K))
+        self.ctx.free(K)
+        return out
 
     def solve(self, precision=0, max_iter=100, n_electrons=None, n_final_fp64=2, fp32_tol=1e-6):
         """Hotelling -> InitMcW -> McWMin; band energy = 2 Tr[rho H] (spin-unpolarised)."""

@@ -442,6 +442,17 @@ int cqmm_matrix_symmetrise(cqmm_ctx *ctx, int32_t A) {
   return cq::op_symmetrise(r.pairs, dptr(ctx, A), cq::stream());
 }
 
+int cqmm_matrix_transpose(cqmm_ctx *ctx, int32_t A, int32_t B) {
+  int st;
+  if ((st = ctx_check(ctx, A, "cqmm_matrix_transpose")) || (st = ctx_check(ctx, B, "cqmm_matrix_transpose"))) return st;
+  if (A == B) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_transpose: A and B must differ");
+  if (hrange(ctx, A) != hrange(ctx, B))
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_transpose: A and B must share a range");
+  const RangeRec &r = ctx->ranges[hrange(ctx, A)];
+  if (!r.has_pairs) return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_transpose: range %d has no transpose pairs", hrange(ctx, A));
+  return cq::op_transpose(r.pairs, dptr(ctx, A), dptr(ctx, B), cq::stream());
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

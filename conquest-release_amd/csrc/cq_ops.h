@@ -35,6 +35,7 @@ int op_runs_axpby(const RunMap &m, double alpha, double *y, double beta, const d
 int op_dot(int64_t n, const double *x, const double *y, double *d_partials, double *d_out, hipStream_t s);
 int op_trace(const DiagMap &m, const double *a, double *d_partials, double *d_out, hipStream_t s);
 int op_symmetrise(const PairMap &m, double *a, hipStream_t s);
+int op_transpose(const PairMap &m, const double *a, double *b, hipStream_t s);
 int op_to_f32(int64_t n, const double *x, float *y, hipStream_t s);
 int op_to_f64(int64_t n, const float *x, double *y, hipStream_t s);
 int op_set_diag(const DiagMap &m, double v, double *a, hipStream_t s);

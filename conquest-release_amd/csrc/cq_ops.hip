@@ -122,6 +122,24 @@ __global__ void k_symmetrise(int64_t npairs, const int64_t *__restrict__ poff, c
   }
 }
 
+// B = A^T on a pattern closed under transposition: for each pair (p, q) of
+// blocks (i,j,s) and (j,i,-s), B_q = (A_p)^T and B_p = (A_q)^T.
+__global__ void k_transpose(int64_t npairs, const int64_t *__restrict__ poff, const int64_t *__restrict__ qoff,
+                            const uint8_t *__restrict__ n1, const uint8_t *__restrict__ n2,
+                            const double *__restrict__ A, double *__restrict__ B) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < npairs;
+       r += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int64_t p = poff[r], q = qoff[r];
+    const int m = n1[r], n = n2[r];
+    for (int e = lane; e < m * n; e += 64) {
+      const int a = e % m, b = e / m;  // element (a,b) of the m x n block p
+      B[q + b + (int64_t)n * a] = A[p + a + (int64_t)m * b];
+      B[p + a + (int64_t)m * b] = A[q + b + (int64_t)n * a];
+    }
+  }
+}
+
 // Onsite diagonals set to `v` (matrix must be zero elsewhere beforehand).
 __global__ void k_set_diag(int64_t nblk, const int64_t *__restrict__ off, const int32_t *__restrict__ nsf, double v,
                            double *__restrict__ a) {
@@ -189,6 +207,13 @@ int op_symmetrise(const PairMap &m, double *a, hipStream_t s) {
   hipLaunchKernelGGL(k_symmetrise, dim3(grid_for(m.npairs * 64, 1)), dim3(kBS), 0, s, m.npairs,
                      m.poff.as<int64_t>(), m.qoff.as<int64_t>(), m.n1.as<uint8_t>(), m.n2.as<uint8_t>(), a);
   return hip_check(hipGetLastError(), "symmetrise launch");
+}
+
+int op_transpose(const PairMap &m, const double *a, double *b, hipStream_t s) {
+  if (m.npairs <= 0) return CQMM_OK;
+  hipLaunchKernelGGL(k_transpose, dim3(grid_for(m.npairs * 64, 1)), dim3(kBS), 0, s, m.npairs,
+                     m.poff.as<int64_t>(), m.qoff.as<int64_t>(), m.n1.as<uint8_t>(), m.n2.as<uint8_t>(), a, b);
+  return hip_check(hipGetLastError(), "transpose launch");
 }
 
 int op_to_f32(int64_t n, const double *x, float *y, hipStream_t s) {

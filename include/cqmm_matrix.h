@@ -87,6 +87,9 @@ int cqmm_matrix_trace(cqmm_ctx *ctx, int32_t A, double *result);
 /* symmetrise_L (src/mult_module.f90:1856-1893) for any range with transpose pairs:
  * A = A/2 + A^T/2, in place. */
 int cqmm_matrix_symmetrise(cqmm_ctx *ctx, int32_t A);
+/* matrix_transpose (src/mult_module.f90:2609-2673) on one process: B = A^T, both
+ * of one range with transpose pairs. */
+int cqmm_matrix_transpose(cqmm_ctx *ctx, int32_t A, int32_t B);
 
 /* ---- McWeeny purification (src/McWeeny.f90) ------------------------------ */
 typedef struct cqmm_mcw_setup {
@@ -144,6 +147,71 @@ typedef struct cqmm_hotelling_setup {
 } cqmm_hotelling_setup;
 int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *setup, int32_t matS, int32_t matT,
                    int32_t *iterations, double *omega);
+
+/* ---- LNV density-matrix minimisation (src/mult_module.f90, src/DMMinModule.f90) -- */
+/* Ranges and mult entries of the LNV products, as matrix_product passes them
+ * (mult table, src/mult_module.f90:583-875). Type 1: C(rc) = A(ra).B(rb);
+ * type 2: A(ra) = C(rc).B(rb)^T. Ranges LS, LH, LSL and L need transpose pairs. */
+typedef struct cqmm_lnv_setup {
+  int32_t range_L, range_S, range_H, range_LS, range_LH, range_LSL, range_SLS;
+  int32_t mult_L_S_LS;     /* t1 (L, S, LS)      LS = L.S                     */
+  int32_t mult_L_H_LH;     /* t1 (L, H, LH)      LH = L.H                     */
+  int32_t mult_HL_S_LSL;   /* t1 (LH, S, LSL)    HLS = HL.S                   */
+  int32_t mult_HLS_LS_L;   /* t2 (L, LS, LSL)    B = (2HLS + SLH).(SL)^T      */
+  int32_t mult_SL_S_SLS;   /* t1 (LS, S, SLS)    SLS = SL.S                   */
+  int32_t mult_SLS_LS_L;   /* t2 (L, LS, SLS)    SLSLS = SLS.(SL)^T           */
+  int32_t mult_LS_L_LSL;   /* t1 (LS, L, LSL)    LSL = LS.L                   */
+  int32_t mult_LSL_SL_H;   /* t2 (H, LS, LSL)    LSLSL = LSL.(LS)^T at H range */
+} cqmm_lnv_setup;
+
+#define CQMM_LNV_K 1u      /* K = 3LSL - 2LSLSL (H range)                       */
+#define CQMM_LNV_E 2u      /* energy = Tr[H K] (needs a K handle)               */
+#define CQMM_LNV_M3 4u     /* M3 = dE/dL (L range)                              */
+#define CQMM_LNV_PHI 8u    /* phi = dNe/dL = 6SLS - 6SLSLS and electrons        */
+
+/* LNV_matrix_multiply (src/mult_module.f90:1478-1823), one spin, with doM1 =
+ * doM2 = doM4 = false. Per spin: electrons = 3Tr[L.SLS] - 2Tr[L.SLSLS],
+ * energy = Tr[H K]. Handles not requested may be -1. */
+int cqmm_lnv(cqmm_ctx *ctx, const cqmm_lnv_setup *setup, uint32_t flags, int32_t matH, int32_t matS,
+             int32_t matL, int32_t matK, int32_t matM3, int32_t matphi, double *electrons, double *energy);
+
+typedef struct cqmm_dmm_setup {
+  cqmm_lnv_setup lnv;
+  int32_t range_T, range_TL;
+  int32_t mult_T_L_TL;      /* t1 (T, L, TL)                                     */
+  int32_t mult_TL_T_L;      /* t2 (L, T, TL): X -> T X T at L range              */
+  int32_t n_L_iterations;   /* iterations per stage (n_L_iterations)             */
+  int32_t vary_mu;          /* 1: keep the electron number (correct_electron_number
+                               and the projected search directions)             */
+  int32_t maxpulay;         /* DM.MaxPulay (default 5, at most 16)               */
+  int32_t global_tolerance; /* residual not divided by the atom count            */
+  int32_t n_atoms;          /* ni_in_cell                                       */
+  double tolerance;         /* residual at which lateDM stops                   */
+  double min_pulay_step, max_pulay_step;  /* DM.MinPulayStepSize / DM.MaxPulayStepSize */
+  double lin_tol;           /* DM.LinTol: earlyDM -> lateDM when |zeta| < lin_tol */
+  double electrons_target;  /* electrons per spin (ne_spin_in_cell)            */
+  double delta_e;           /* initial delta_e (from McWMin; 0 is allowed)      */
+} cqmm_dmm_setup;
+
+typedef struct cqmm_dmm_result {
+  int32_t done;             /* residual below tolerance                         */
+  int32_t early_iterations, late_iterations, inflexions;
+  double energy;            /* 2 Tr[H K] (spin-unpolarised total)               */
+  double electrons;         /* 2 (3Tr[LSLS] - 2Tr[LSLSLS])                      */
+  double residual;          /* last g                                           */
+  double seconds;
+} cqmm_dmm_result;
+
+/* correct_electron_number_fixspin (src/DMMinModule.f90:1676-1817): moves L along
+ * T phi T^T to the target electron number (cubic model, SolveCubic :2072-2143). */
+int cqmm_correct_electrons(cqmm_ctx *ctx, const cqmm_dmm_setup *setup, int32_t matH, int32_t matS, int32_t matT,
+                           int32_t matL, double *electrons);
+/* FindMinDM (src/DMMinModule.f90:172-312) for one spin, from the current L:
+ * earlyDM (:366-723, steepest descent with lineMinL :1395-1614) then lateDM
+ * (:809-1339, Pulay mixing with DoPulay, src/Pulay.f90:188-262). K (H range)
+ * receives 3LSL - 2LSLSL of the final L. */
+int cqmm_dmm(cqmm_ctx *ctx, const cqmm_dmm_setup *setup, int32_t matH, int32_t matS, int32_t matT, int32_t matL,
+             int32_t matK, cqmm_dmm_result *result);
 
 #ifdef __cplusplus
 }
