@@ -56,16 +56,15 @@ constexpr int kTileRows = 8, kTileCols = 8;
 struct TilePlan {
   int nsf = 0;
   int32_t n_tiles = 0, n_chunks = 0;
-  int64_t n_entries = 0, n_loads = 0;
+  int64_t n_entries = 0, n_loads = 0, n_prod = 0;
   DevBuf tile_chunk;   // int32 [n_tiles+1]  first chunk of each tile
-  DevBuf chunk_entry;  // int32 [n_chunks+1] first entry of each chunk
   DevBuf chunk_lbeg;   // int64 [n_chunks+1] first operand of each chunk in `list`
-  DevBuf entry_mask;   // uint32 [n_entries] bits 0-7 left blocks (tile row), 8-15 right blocks (tile column)
-  DevBuf entry_lbeg;   // int64 [n_entries+1] first operand of each entry in `list`
-  DevBuf entry_coff;   // int32 [n_entries] first operand of each entry relative to its chunk's
   DevBuf list;         // int32 [n_loads] operand block index; bit 31 set: right operand
+  DevBuf pstart;       // int64 [(n_chunks+1)*4] first product of (chunk, wavefront) in `prod`
+  DevBuf pcnt;         // uint64 [(n_chunks+1)*4] products of (chunk, wavefront) per slot, 4 bits each
+  DevBuf prod;         // uint32 [n_prod+64] LDS element offsets of the two operands (left | right << 16)
   DevBuf tile_out;     // int32 [n_tiles*64] output block of slot row*8+col, or -1
-  DevBuf slot_of;      // uint8 [n_tiles*64] [tile][wavefront][u]: slot owned, or 0xff
+  DevBuf wave_out;     // int32 [n_tiles*4*16] output block of (tile, wavefront, slot), or -1
 };
 
 // Triplet list of one product, grouped by output block (segment), each group in

@@ -12,16 +12,21 @@
 //  * an entry of a tile is one contraction atom x with its (up to 8) left
 //    operand blocks P(row, x) and (up to 8) right operand blocks Q(column, x);
 //    entries of a tile are in ascending x (type 1: the reference's k order);
-//  * one workgroup (8 wavefronts) runs one tile. Chunks of entries have their
-//    operand blocks copied HBM->LDS (global_load_lds, no registers) into one
-//    of two stages while the previous chunk is computed from the other (one
-//    barrier per chunk); each wavefront accumulates up to 8 output blocks of
-//    the tile with v_mfma_f64_16x16x4 from LDS and finally writes each of them
-//    once (zeros where a block has no products), so the output is fully
-//    overwritten as mat_mult overwrites it (src/multiply_module.f90:204-208).
-//    Ownership is a Latin square (one block per tile row and column per
-//    wavefront), chosen when the plan is built to balance the wavefronts'
-//    product counts.
+//  * one workgroup (4 wavefronts, three workgroups per CU) runs one tile.
+//    Chunks of entries have their operand blocks copied HBM->LDS
+//    (global_load_lds, no registers) into one of two stages while the
+//    previous chunk is computed from the other (one barrier per chunk); each
+//    wavefront accumulates 16 output blocks of the tile with
+//    v_mfma_f64_16x16x4 from LDS and finally writes each of them once (zeros
+//    where a block has no products), so the output is fully overwritten as
+//    mat_mult overwrites it (src/multiply_module.f90:204-208). Ownership is a
+//    Latin square (two blocks per tile row and column per wavefront), chosen
+//    when the plan is built to balance the wavefronts' product counts;
+//  * the plan holds each wavefront's products of each chunk as a list of
+//    packed LDS offsets (left | right << 16, in elements), sorted by the
+//    wavefront's output block and, within a block, by entry, with the count per
+//    block in one 64-bit word (16 x 4 bits). The kernel walks that list: no
+//    product discovery at run time.
 // Summation order per output block is fixed (ascending x), so results are
 // bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
@@ -31,6 +36,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -42,54 +48,65 @@
 namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTileThreads = 512;            // 8 wavefronts
-constexpr int kStageBytes = 36 * 1024;       // one of the two operand stages (two workgroups per CU)
+constexpr int kWaves = 4;                    // wavefronts per workgroup (one tile)
+constexpr int kTileThreads = 64 * kWaves;
+constexpr int kSlots = 64 / kWaves;          // output blocks per wavefront
+constexpr int kStageBytes = 24 * 1024;       // one of the two operand stages (three workgroups per CU)
 // LDS reads of a block product never mask rows or columns beyond NSF: those
 // lanes only feed accumulator rows/columns >= NSF, which are never written.
 // Only the contraction index 4*s+g >= NSF at the last k-step is zeroed (both
 // operands), so the padding of the 16x16x4 steps adds nothing. kLdsPad doubles
 // after each stage keep the over-reads of its last block inside the array.
 constexpr int kLdsPad = 256;
-constexpr int kChunkEntries = 8;             // entries per chunk at most (one per lane of a row group)
+constexpr int kChunkEntries = 15;            // entries per chunk at most (4-bit product counts)
+constexpr int kWaveProducts = 64;            // products per wavefront per chunk at most (one per lane)
+
+// Operand blocks per chunk (FP64 sizing): what fits kStageBytes, but at
+// least the 16 blocks one entry can bring (8 left + 8 right), so every entry
+// fits a stage on its own (16x16 blocks: 32 KB stages, two workgroups per CU).
+constexpr int stage_cap(int nsf) {
+  return kStageBytes / (nsf * nsf * 8) > 16 ? kStageBytes / (nsf * nsf * 8) : 16;
+}
 
 template <int NSF>
 struct TileCfg {
   static constexpr int NS2 = NSF * NSF;
-  static constexpr int CAP = kStageBytes / (NS2 * 8);   // operand blocks per chunk (FP64 sizing)
+  static constexpr int CAP = stage_cap(NSF);
   static constexpr int KS = (NSF + 3) / 4;              // MFMA k-steps per product
 };
 
-int tile_cap(int nsf) { return kStageBytes / (nsf * nsf * 8); }
+int tile_cap(int nsf) { return stage_cap(nsf); }
 
-typedef float flt4 __attribute__((ext_vector_type(4)));
-
-// Prefetch loads are unconditional (indices clamped into the chunk) so that
+// Prefetch loads are unconditional (indices clamped or arrays padded) so that
 // no value is consumed before the next barrier: a use right after the load
 // would make the wavefront wait for every outstanding load, including the
 // stage copies that are meant to overlap the compute.
 //
 // Operand list of a chunk with nb > 0 blocks: lane k holds the operand of
-// block min(w + 8k, nb - 1) (lanes past wavefront w's blocks are unused).
+// block min(w + kWaves*k, nb - 1) (lanes past wavefront w's blocks are unused).
 __device__ inline int32_t load_chunk_list(const int32_t *__restrict__ list, int64_t lbeg, int32_t nb, int w,
                                           int lane) {
-  return list[lbeg + min(w + 8 * lane, nb - 1)];
+  return list[lbeg + min(w + kWaves * lane, nb - 1)];
 }
 
-// Entry masks and chunk-relative operand-list starts of a chunk's ne (1..8)
-// entries: lane with lane % 8 == j holds entry e0 + min(j, ne - 1). (32-bit
-// loads only: a dead half of a wider load would be reused by the register
-// allocator and force a wait for it.)
-__device__ inline void load_chunk_entries(const uint32_t *__restrict__ entry_mask,
-                                          const int32_t *__restrict__ entry_coff, int32_t e0, int32_t ne, int lane,
-                                          uint32_t &em, int32_t &eco) {
-  const int32_t e = e0 + min(lane & 7, ne - 1);
-  em = entry_mask[e];
-  eco = entry_coff[e];
+// One 256-byte piece of a block copy: lane l moves dword OFF/4 + l. The
+// immediate offset applies to both the global and the LDS address (one M0 per
+// block).
+template <int OFF>
+__device__ inline void dma_piece(const uint32_t *src, uint32_t *dst) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                   (__attribute__((address_space(3))) void *)dst, 4, OFF, 0);
+}
+
+template <int... I>
+__device__ inline void dma_full_pieces(const uint32_t *src, uint32_t *dst, std::integer_sequence<int, I...>) {
+  (dma_piece<I * 256>(src, dst), ...);
 }
 
 // Issue the HBM->LDS copies of a chunk's operand blocks without waiting:
-// wavefront w copies blocks w, w+8, ... (operands in lv) with
+// wavefront w copies blocks w, w+kWaves, ... (operands in lv) with
 // global_load_lds_dword; lane l moves dword k*64+l of a block to
 // stage + block*DW + k*64 + l.
 template <int NSF, class T>
@@ -97,40 +114,24 @@ __device__ inline void issue_stage(int32_t lv, int32_t nb, const T *__restrict__
                                    T *stage, int w, int lane) {
   constexpr int NS2 = NSF * NSF;
   constexpr int DW = NS2 * (int)sizeof(T) / 4;
-  constexpr int NI = (DW + 63) / 64;
-  const int32_t nbw = nb > w ? (nb - w + 7) >> 3 : 0;
+  constexpr int NF = DW / 64;          // full 64-dword pieces
+  constexpr int TAIL = DW - 64 * NF;   // dwords of the last piece
+  const int32_t nbw = nb > w ? (nb - w + kWaves - 1) / kWaves : 0;
   for (int32_t k = 0; k < nbw; ++k) {
     const int32_t lb = __builtin_amdgcn_readlane(lv, k);
-    const uint32_t *src = (const uint32_t *)((lb < 0 ? Q : P) + (int64_t)(lb & 0x7fffffff) * NS2);
-    uint32_t *dst = (uint32_t *)(stage + (w + 8 * k) * NS2);
-#pragma unroll
-    for (int i = 0; i < NI; ++i) {
-      const int d = i * 64 + lane;
-      if (d < DW)
-        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + d),
-                                         (__attribute__((address_space(3))) void *)(dst + i * 64), 4, 0, 0);
+    const uint32_t *src = (const uint32_t *)((lb < 0 ? Q : P) + (int64_t)(lb & 0x7fffffff) * NS2) + lane;
+    uint32_t *dst = (uint32_t *)(stage + (w + kWaves * k) * NS2);
+    dma_full_pieces(src, dst, std::make_integer_sequence<int, NF>{});
+    if constexpr (TAIL > 0) {
+      if (lane < TAIL) dma_piece<NF * 256>(src, dst);
     }
   }
 }
 
-// The wavefront's products in a chunk, found lane-parallel: lane (u, j) =
-// (lane / 8, lane % 8) looks at entry j and the wavefront's u-th output block
-// (tile row `row`, column `col` of this lane's u; valid = the block exists);
-// set ballot bits are products. pk holds the stage slots of the two operand
-// blocks (eco: the entry's first operand relative to the chunk's first).
-__device__ inline uint64_t chunk_products(uint32_t em, int32_t eco, int32_t ne, bool valid, int row, int col,
-                                          int lane, uint32_t &pk) {
-  const int j = lane & 7;
-  bool has = false;
-  pk = 0;
-  if (valid && j < ne) {
-    const uint32_t ym = em & 0xffu, bm = (em >> 8) & 0xffu;
-    has = ((ym >> row) & 1u) && ((bm >> col) & 1u);
-    const uint32_t ys = (uint32_t)(eco + __popc(ym & ((1u << row) - 1u)));
-    const uint32_t qs = (uint32_t)(eco + __popc(ym) + __popc(bm & ((1u << col) - 1u)));
-    pk = ys | (qs << 16);
-  }
-  return __ballot(has);
+__device__ inline int64_t readlane64(int64_t v, int k) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), k);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // MODE 0: FP64 operands, v_mfma_f64_16x16x4 (accumulator D[(l/16)+4e][l%16]).
@@ -139,14 +140,13 @@ __device__ inline uint64_t chunk_products(uint32_t em, int32_t eco, int32_t ne, 
 //         registers (accumulator D[4(l/16)+e][l%16]).
 // Pipeline per chunk c (one barrier): wait for chunk c's copies and the
 // prefetched registers, issue chunk c+1's copies into the other stage and the
-// register loads for chunks c+1 (entries) and c+2 (operand list), then
-// compute chunk c from LDS.
+// register loads of chunk c+1's product list and chunk c+2's operand list,
+// then compute chunk c from LDS.
 template <int MODE, bool QT, int NSF>
-__global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
-    const int32_t *__restrict__ tile_chunk, const int32_t *__restrict__ chunk_entry,
-    const int64_t *__restrict__ chunk_lbeg, const uint32_t *__restrict__ entry_mask,
-    const int32_t *__restrict__ entry_coff, const int32_t *__restrict__ list, const int32_t *__restrict__ tile_out,
-    const uint8_t *__restrict__ slot_of,
+__global__ void __launch_bounds__(kTileThreads) k_tile_products(
+    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
+    const int32_t *__restrict__ list, const int64_t *__restrict__ pstart, const uint64_t *__restrict__ pcnt,
+    const uint32_t *__restrict__ prod, const int32_t *__restrict__ wave_out,
     const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
     const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
     typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
@@ -169,82 +169,64 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
   const int oA = r + NSF * g;
   const int oB = QT ? (r + NSF * g) : (g + NSF * r);
   const bool last_ok = g < REM;
-  // The wavefront's output blocks: slot_of[tile][w][u] = tile row * 8 + column,
-  // or 0xff (balanced over the wavefronts when the plan was built).
-  // (One 8-byte scalar load: byte loads would be per-lane vector loads.)
-  const uint64_t so8 = *(const uint64_t *)(slot_of + (int64_t)tile * 64 + w * 8);
-  int32_t oblk[8];
-  uint32_t omask = 0;
+  // The wavefront's output blocks (-1: none), set when the plan was built.
+  const int32_t *wo = wave_out + ((int64_t)tile * kWaves + w) * kSlots;
+  int32_t oblk[kSlots];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int sl = (int)((so8 >> (8 * u)) & 0xffu);
-    oblk[u] = sl < 64 ? tile_out[(int64_t)tile * 64 + sl] : -1;
-    if (oblk[u] >= 0) omask |= 1u << u;
-  }
-  const int my_sl = (int)((so8 >> (8 * (lane >> 3))) & 0xffu);
-  const bool my_valid = my_sl < 64;
-  const int my_row = (my_sl >> 3) & 7, my_col = my_sl & 7;
-  dbl4 a64[8];
-  flt4 a32[8];
+  for (int u = 0; u < kSlots; ++u) oblk[u] = wo[u];
+  dbl4 a64[kSlots];
+  flt4 a32[kSlots];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < kSlots; ++u) {
     a64[u] = dbl4{0.0, 0.0, 0.0, 0.0};
     a32[u] = flt4{0.f, 0.f, 0.f, 0.f};
   }
   const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
   if (c0 < c1) {
-    // Chunk boundaries of the tile held in registers, 64 chunks at a time:
-    // lane l has chunk_entry / chunk_lbeg of chunk wb + l (index clamped to
-    // c1). Vector loads, refilled after the compute of a chunk, so the loop
-    // has no scalar loads (a scalar load outstanding during the compute
-    // would make every LDS wait also wait for it).
+    // Per-chunk data of the tile held in registers, 64 chunks at a time: lane
+    // l has chunk_lbeg and this wavefront's product-list start and counts of
+    // chunk wb + l (index clamped to c1). Vector loads, refilled after the
+    // compute of a chunk, so the loop has no scalar loads (a scalar load
+    // outstanding during the compute would make every LDS wait also wait for
+    // it).
     int32_t wb = c0;
-    int32_t wce = chunk_entry[min(wb + lane, c1)];
-    int64_t wcl = chunk_lbeg[min(wb + lane, c1)];
-    auto ce_at = [&](int32_t cc) -> int32_t { return __builtin_amdgcn_readlane(wce, min(cc, c1) - wb); };
-    auto cl_at = [&](int32_t cc) -> int64_t {
-      const int k = min(cc, c1) - wb;
-      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wcl, k);
-      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)wcl >> 32), k);
-      return (int64_t)(((uint64_t)hi << 32) | lo);
-    };
-    uint32_t em;
-    int32_t eco;
+    int32_t cc = min(wb + lane, c1);
+    int64_t wcl = chunk_lbeg[cc];
+    int64_t wps = pstart[(int64_t)cc * kWaves + w];
+    int64_t wpc = (int64_t)pcnt[(int64_t)cc * kWaves + w];
+    auto cl_at = [&](int32_t x) { return readlane64(wcl, min(x, c1) - wb); };
+    auto ps_at = [&](int32_t x) { return readlane64(wps, min(x, c1) - wb); };
+    auto pc_at = [&](int32_t x) { return (uint64_t)readlane64(wpc, min(x, c1) - wb); };
+    uint32_t pl;
+    int32_t lvn = 0;
     {
-      const int32_t ce0 = ce_at(c0), ce1 = ce_at(c0 + 1);
       const int64_t cl0 = cl_at(c0), cl1 = cl_at(c0 + 1);
-      load_chunk_entries(entry_mask, entry_coff, ce0, ce1 - ce0, lane, em, eco);
       const int32_t lv0 = load_chunk_list(list, cl0, (int32_t)(cl1 - cl0), w, lane);
       issue_stage<NSF, T>(lv0, (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
-    }
-    int32_t lvn = 0;
-    if (c0 + 1 < c1) {
-      const int64_t cl1 = cl_at(c0 + 1);
-      lvn = load_chunk_list(list, cl1, (int32_t)(cl_at(c0 + 2) - cl1), w, lane);
+      pl = prod[ps_at(c0) + lane];
+      if (c0 + 1 < c1) lvn = load_chunk_list(list, cl1, (int32_t)(cl_at(c0 + 2) - cl1), w, lane);
     }
     for (int32_t c = c0; c < c1; ++c) {
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       const bool odd = ((c - c0) & 1) != 0;
       const T *cur = odd ? stage1 : stage0;
-      const int32_t ce0 = ce_at(c), ce1 = ce_at(c + 1), ce2 = ce_at(c + 2);
-      const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2), cl3 = cl_at(c + 3);
-      uint32_t pk;
-      const uint64_t pm = chunk_products(em, eco, ce1 - ce0, my_valid, my_row, my_col, lane, pk);
+      const uint32_t plc = pl;
+      const uint64_t cnt = pc_at(c);
       if (c + 1 < c1) {
+        const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2);
         issue_stage<NSF, T>(lvn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
-        load_chunk_entries(entry_mask, entry_coff, ce1, ce2 - ce1, lane, em, eco);
-        if (c + 2 < c1) lvn = load_chunk_list(list, cl2, (int32_t)(cl3 - cl2), w, lane);
+        pl = prod[ps_at(c + 1) + lane];
+        if (c + 2 < c1) lvn = load_chunk_list(list, cl2, (int32_t)(cl_at(c + 3) - cl2), w, lane);
       }
+      int p = 0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        uint32_t bits = (uint32_t)(pm >> (8 * u)) & 0xffu;
-        while (bits) {
-          const int j = __builtin_ctz(bits);
-          bits &= bits - 1u;
-          const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)pk, 8 * u + j);
-          const T *la = cur + (int)(v & 0xffffu) * C::NS2 + oA;
-          const T *lq = cur + (int)(v >> 16) * C::NS2 + oB;
+      for (int u = 0; u < kSlots; ++u) {
+        const int n = (int)((cnt >> (4 * u)) & 15u);
+        for (int i = 0; i < n; ++i, ++p) {
+          const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
+          const T *la = cur + (int)(v & 0xffffu) + oA;
+          const T *lq = cur + (int)(v >> 16) + oB;
           T a[C::KS], bb[C::KS];
 #pragma unroll
           for (int s4 = 0; s4 < C::KS; ++s4) {
@@ -272,17 +254,19 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile_products(
           }
         }
       }
-      if (c + 4 > wb + 63 && c + 1 < c1) {  // next iteration reads up to chunk c + 4
+      if (c + 4 > wb + 63 && c + 1 < c1) {  // next iteration reads up to chunk c + 3
         wb = c + 1;
-        wce = chunk_entry[min(wb + lane, c1)];
-        wcl = chunk_lbeg[min(wb + lane, c1)];
+        cc = min(wb + lane, c1);
+        wcl = chunk_lbeg[cc];
+        wps = pstart[(int64_t)cc * kWaves + w];
+        wpc = (int64_t)pcnt[(int64_t)cc * kWaves + w];
       }
     }
   }
   if (r < NSF) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (!((omask >> u) & 1u)) continue;
+    for (int u = 0; u < kSlots; ++u) {
+      if (oblk[u] < 0) continue;
       T *o = O + (int64_t)oblk[u] * C::NS2;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -391,27 +375,127 @@ __global__ void k_tile_ebeg(const int32_t *__restrict__ etile, int64_t ne, int32
   tebeg[t] = (int32_t)lo;
 }
 
+// Wavefront products of entry e (tile t): rows u of the entry's left blocks x
+// columns of its right blocks whose output block exists (owner != 0xff);
+// owner byte = wavefront << 4 | slot.
+__device__ inline void entry_wave_counts(uint32_t m, const uint8_t *__restrict__ own, int pw[kWaves]) {
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) pw[w] = 0;
+  for (uint32_t ym = m & 0xffu; ym; ym &= ym - 1u) {
+    const int u = __builtin_ctz(ym);
+    for (uint32_t bm = (m >> 8) & 0xffu; bm; bm &= bm - 1u) {
+      const uint8_t o = own[u * 8 + __builtin_ctz(bm)];
+      if (o != 0xff) ++pw[o >> 4];
+    }
+  }
+}
+
 // Chunks of a tile: at most kChunkEntries consecutive entries whose operand
-// blocks fit `cap`.
-// With FILL unset, counts them; with FILL set, writes each chunk's first entry.
+// blocks fit `cap` and that give no wavefront more than kWaveProducts
+// products. With FILL unset, counts them; with FILL set, writes each chunk's
+// first entry.
 template <bool FILL>
-__global__ void k_chunks(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask, int32_t n_tiles,
-                         int32_t cap, const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
+__global__ void k_chunks(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask,
+                         const uint8_t *__restrict__ owner, int32_t n_tiles, int32_t cap,
+                         const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tiles) return;
+  const uint8_t *own = owner + (int64_t)t * 64;
   int32_t nch = 0, sum = 0, ne = 0;
+  int wc[kWaves] = {0, 0, 0, 0};
   for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
-    const int32_t c = __popc(mask[e]);
-    if (nch == 0 || sum + c > cap || ne == kChunkEntries) {
+    const uint32_t m = mask[e];
+    const int32_t c = __popc(m);
+    int pw[kWaves];
+    entry_wave_counts(m, own, pw);
+    bool over = false;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) over |= wc[w] + pw[w] > kWaveProducts;
+    if (nch == 0 || sum + c > cap || ne == kChunkEntries || over) {
       if (FILL) out[tchunk[t] + nch] = e;
       ++nch;
       sum = 0;
       ne = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) wc[w] = 0;
     }
     sum += c;
     ++ne;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wc[w] += pw[w];
   }
   if (!FILL) out[t] = nch;
+}
+
+// Tile of each chunk.
+__global__ void k_chunk_tile(const int32_t *__restrict__ tchunk, int32_t n_tiles, int32_t *__restrict__ ctile) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  for (int32_t c = tchunk[t]; c < tchunk[t + 1]; ++c) ctile[c] = t;
+}
+
+// Product lists of each chunk, sorted by (wavefront, slot, entry). Count pass
+// (FILL unset): products per chunk into np. Fill pass: with pbase = the
+// chunk's first product, writes pstart/pcnt of its wavefronts and the packed
+// LDS offsets (left | right << 16, elements) of every product.
+template <bool FILL>
+__global__ void k_chunk_prods(int32_t nch, const int32_t *__restrict__ chunk_entry, const int32_t *__restrict__ ctile,
+                              const uint32_t *__restrict__ mask, const int32_t *__restrict__ coff,
+                              const uint8_t *__restrict__ owner, int32_t ns2, int64_t *__restrict__ np,
+                              const int64_t *__restrict__ pbase, int64_t *__restrict__ pstart,
+                              uint64_t *__restrict__ pcnt, uint32_t *__restrict__ prod) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  const uint8_t *own = owner + (int64_t)ctile[c] * 64;
+  const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];
+  uint8_t cnt[64];
+  for (int i = 0; i < 64; ++i) cnt[i] = 0;
+  int32_t tot = 0;
+  for (int32_t e = e0; e < e1; ++e) {
+    const uint32_t m = mask[e];
+    for (uint32_t ym = m & 0xffu; ym; ym &= ym - 1u) {
+      const int u = __builtin_ctz(ym);
+      for (uint32_t bm = (m >> 8) & 0xffu; bm; bm &= bm - 1u) {
+        const uint8_t o = own[u * 8 + __builtin_ctz(bm)];
+        if (o != 0xff) {
+          ++cnt[o];
+          ++tot;
+        }
+      }
+    }
+  }
+  if (!FILL) {
+    np[c] = tot;
+    return;
+  }
+  uint16_t off[64];
+  int32_t run = 0;
+  for (int w = 0; w < kWaves; ++w) {
+    pstart[(int64_t)c * kWaves + w] = pbase[c] + run;
+    uint64_t word = 0;
+    for (int l = 0; l < kSlots; ++l) {
+      off[w * kSlots + l] = (uint16_t)run;
+      run += cnt[w * kSlots + l];
+      word |= (uint64_t)cnt[w * kSlots + l] << (4 * l);
+    }
+    pcnt[(int64_t)c * kWaves + w] = word;
+  }
+  for (int32_t e = e0; e < e1; ++e) {
+    const uint32_t m = mask[e];
+    const uint32_t ym = m & 0xffu, bmask = (m >> 8) & 0xffu;
+    const int32_t base = coff[e];
+    for (uint32_t yb = ym; yb; yb &= yb - 1u) {
+      const int u = __builtin_ctz(yb);
+      const uint32_t aoff = (uint32_t)((base + __popc(ym & ((1u << u) - 1u))) * ns2);
+      for (uint32_t bm = bmask; bm; bm &= bm - 1u) {
+        const int col = __builtin_ctz(bm);
+        const uint8_t o = own[u * 8 + col];
+        if (o == 0xff) continue;
+        const uint32_t boff = (uint32_t)((base + __popc(ym) + __popc(bmask & ((1u << col) - 1u))) * ns2);
+        prod[pbase[c] + off[o]++] = aoff | (boff << 16);
+      }
+    }
+  }
 }
 
 template <class T>
@@ -433,17 +517,15 @@ template <int MODE, bool QT, class T>
 int launch_mode(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
   const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
   const dim3 blk(kTileThreads);
-  const int32_t *tc = tp.tile_chunk.as<int32_t>(), *ce = tp.chunk_entry.as<int32_t>();
-  const int64_t *cl = tp.chunk_lbeg.as<int64_t>();
-  const uint32_t *em = tp.entry_mask.as<uint32_t>();
-  const int32_t *lb = tp.entry_coff.as<int32_t>();
-  const int32_t *li = tp.list.as<int32_t>(), *to = tp.tile_out.as<int32_t>();
-  const uint8_t *sl = tp.slot_of.as<uint8_t>();
+  const int32_t *tc = tp.tile_chunk.as<int32_t>(), *li = tp.list.as<int32_t>(), *wo = tp.wave_out.as<int32_t>();
+  const int64_t *cl = tp.chunk_lbeg.as<int64_t>(), *ps = tp.pstart.as<int64_t>();
+  const uint64_t *pc = tp.pcnt.as<uint64_t>();
+  const uint32_t *pr = tp.prod.as<uint32_t>();
   switch (tp.nsf) {
-    case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
-    case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
-    case 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
-    case 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16>), dim3(gridx), blk, 0, s, tc, ce, cl, em, lb, li, to, sl, P, Q, O, tp.n_tiles); break;
+    case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
+    case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
+    case 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
+    case 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
     default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: block size %d has no kernel instance", tp.nsf);
   }
   return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
@@ -506,10 +588,12 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
                       "H2D tile_out")))
     return st;
   // Output slots to wavefronts, per tile: a Latin square over (tile row,
-  // column) — every wavefront owns one block in each tile row and each
-  // column, which spreads every entry's rectangle of products evenly — chosen
-  // among kLatinTries column permutations to balance the wavefronts' total
-  // product counts.
+  // column) — with k = (row + perm[column]) % 8, wavefront k % 4 owns the
+  // block as its slot 2*row + k / 4, so every wavefront owns two blocks of
+  // each tile row and column, which spreads every entry's rectangle of
+  // products evenly — chosen among kLatinTries column permutations to balance
+  // the wavefronts' total product counts.
+  DevBuf d_owner;
   {
     DevBuf d_cnt;
     if ((st = d_cnt.alloc((size_t)in.n_tiles * 64 * 4))) return st;
@@ -531,32 +615,37 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
         std::swap(perms[k][i], perms[k][j]);
       }
     }
-    std::vector<uint8_t> slot_of((size_t)in.n_tiles * 64, 0xff);
+    std::vector<uint8_t> owner((size_t)in.n_tiles * 64, 0xff);
+    std::vector<int32_t> wave_out((size_t)in.n_tiles * 64, -1);
     for (int64_t t = 0; t < in.n_tiles; ++t) {
       const int32_t *c = cnt.data() + t * 64;
       int best = 0;
       int64_t best_max = -1;
       for (int k = 0; k < kLatinTries; ++k) {
-        int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int sl = 0; sl < 64; ++sl) load[((sl >> 3) + perms[k][sl & 7]) & 7] += c[sl];
+        int64_t load[kWaves] = {0, 0, 0, 0};
+        for (int sl = 0; sl < 64; ++sl) load[((sl >> 3) + perms[k][sl & 7]) & (kWaves - 1)] += c[sl];
         int64_t m = 0;
-        for (int w = 0; w < 8; ++w) m = std::max(m, load[w]);
+        for (int w = 0; w < kWaves; ++w) m = std::max(m, load[w]);
         if (best_max < 0 || m < best_max) {
           best_max = m;
           best = k;
         }
       }
-      // wavefront w, row u owns column col with (u + perm[col]) % 8 == w.
       for (int u = 0; u < 8; ++u)
         for (int col = 0; col < 8; ++col) {
-          const int w = (u + perms[best][col]) & 7;
-          if (in.tile_out[t * 64 + u * 8 + col] >= 0) slot_of[t * 64 + w * 8 + u] = (uint8_t)(u * 8 + col);
+          const int32_t ob = in.tile_out[t * 64 + u * 8 + col];
+          if (ob < 0) continue;
+          const int k = (u + perms[best][col]) & 7;
+          const int w = k & (kWaves - 1), l = 2 * u + (k >> 2);
+          owner[t * 64 + u * 8 + col] = (uint8_t)(w << 4 | l);
+          wave_out[(t * kWaves + w) * kSlots + l] = ob;
         }
     }
-    if ((st = tp.slot_of.alloc(slot_of.size()))) return st;
-    if ((st = hip_check(hipMemcpyAsync(tp.slot_of.p, slot_of.data(), slot_of.size(), hipMemcpyHostToDevice, s),
-                        "H2D slot_of")) ||
-        (st = hip_check(hipStreamSynchronize(s), "slot_of")))
+    if ((st = d_owner.alloc(owner.size())) || (st = tp.wave_out.alloc(wave_out.size() * 4))) return st;
+    if ((st = hip_check(hipMemcpyAsync(d_owner.p, owner.data(), owner.size(), hipMemcpyHostToDevice, s), "H2D owner")) ||
+        (st = hip_check(hipMemcpyAsync(tp.wave_out.p, wave_out.data(), wave_out.size() * 4, hipMemcpyHostToDevice, s),
+                        "H2D wave_out")) ||
+        (st = hip_check(hipStreamSynchronize(s), "owner")))
       return st;
   }
   // Sort the products by (tile, contraction atom).
@@ -589,7 +678,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
     return st;
   const int64_t ne = ne32;
   tp.n_entries = ne;
-  DevBuf d_ep, d_eq, d_etile, d_cnt;
+  DevBuf d_ep, d_eq, d_etile, d_cnt, d_emask, d_elbeg, d_ecoff;
   if ((st = d_ep.alloc(ne * 8 * 4)) || (st = d_eq.alloc(ne * 8 * 4)) || (st = d_etile.alloc(ne * 4)) ||
       (st = d_cnt.alloc(ne * 8)))
     return st;
@@ -598,26 +687,29 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   hipLaunchKernelGGL(k_entry_fill, dim3(grid1(n, 256)), dim3(256), 0, s, d_key2.as<uint64_t>(), d_perm2.as<int32_t>(),
                      d_eid.as<int32_t>(), in.slot, in.p, in.q, n, in.nx, ns2, d_ep.as<int32_t>(), d_eq.as<int32_t>(),
                      d_etile.as<int32_t>());
-  if ((st = tp.entry_mask.alloc(ne * 4)) || (st = tp.entry_lbeg.alloc((ne + 1) * 8))) return st;
+  d_eid.release();
+  d_perm2.release();
+  d_key2.release();
+  if ((st = d_emask.alloc(ne * 4)) || (st = d_elbeg.alloc((ne + 1) * 8))) return st;
   hipLaunchKernelGGL(k_entry_mask, dim3(grid1(ne, 256)), dim3(256), 0, s, d_ep.as<int32_t>(), d_eq.as<int32_t>(), ne,
-                     tp.entry_mask.as<uint32_t>(), d_cnt.as<int64_t>());
-  if ((st = hip_check(hipMemsetAsync(tp.entry_lbeg.p, 0, 8, s), "memset lbeg"))) return st;
-  if ((st = inclusive_scan(d_cnt.as<int64_t>(), tp.entry_lbeg.as<int64_t>() + 1, (size_t)ne, s))) return st;
+                     d_emask.as<uint32_t>(), d_cnt.as<int64_t>());
+  if ((st = hip_check(hipMemsetAsync(d_elbeg.p, 0, 8, s), "memset lbeg"))) return st;
+  if ((st = inclusive_scan(d_cnt.as<int64_t>(), d_elbeg.as<int64_t>() + 1, (size_t)ne, s))) return st;
   int64_t nl = 0;
-  if ((st = hip_check(hipMemcpyAsync(&nl, tp.entry_lbeg.as<int64_t>() + ne, 8, hipMemcpyDeviceToHost, s), "D2H loads")) ||
+  if ((st = hip_check(hipMemcpyAsync(&nl, d_elbeg.as<int64_t>() + ne, 8, hipMemcpyDeviceToHost, s), "D2H loads")) ||
       (st = hip_check(hipStreamSynchronize(s), "load count")))
     return st;
   tp.n_loads = nl;
   if ((st = tp.list.alloc(nl * 4))) return st;
   hipLaunchKernelGGL(k_list_fill, dim3(grid1(ne, 256)), dim3(256), 0, s, d_ep.as<int32_t>(), d_eq.as<int32_t>(),
-                     tp.entry_lbeg.as<int64_t>(), ne, tp.list.as<int32_t>());
+                     d_elbeg.as<int64_t>(), ne, tp.list.as<int32_t>());
   // Chunks.
   DevBuf d_tebeg, d_nch;
   if ((st = d_tebeg.alloc(((size_t)in.n_tiles + 1) * 4)) || (st = d_nch.alloc((size_t)in.n_tiles * 4))) return st;
   hipLaunchKernelGGL(k_tile_ebeg, dim3(grid1((int64_t)in.n_tiles + 1, 256)), dim3(256), 0, s, d_etile.as<int32_t>(), ne,
                      in.n_tiles, d_tebeg.as<int32_t>());
   hipLaunchKernelGGL((k_chunks<false>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
-                     tp.entry_mask.as<uint32_t>(), in.n_tiles, cap, nullptr, d_nch.as<int32_t>());
+                     d_emask.as<uint32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, nullptr, d_nch.as<int32_t>());
   if ((st = tp.tile_chunk.alloc(((size_t)in.n_tiles + 1) * 4))) return st;
   if ((st = hip_check(hipMemsetAsync(tp.tile_chunk.p, 0, 4, s), "memset tile_chunk"))) return st;
   if ((st = inclusive_scan(d_nch.as<int32_t>(), tp.tile_chunk.as<int32_t>() + 1, (size_t)in.n_tiles, s))) return st;
@@ -627,19 +719,56 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
       (st = hip_check(hipStreamSynchronize(s), "chunk count")))
     return st;
   tp.n_chunks = nch;
-  if ((st = tp.chunk_entry.alloc(((size_t)nch + 1) * 4))) return st;
+  DevBuf d_centry;
+  if ((st = d_centry.alloc(((size_t)nch + 1) * 4))) return st;
   hipLaunchKernelGGL((k_chunks<true>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
-                     tp.entry_mask.as<uint32_t>(), in.n_tiles, cap, tp.tile_chunk.as<int32_t>(),
-                     tp.chunk_entry.as<int32_t>());
-  if ((st = hip_check(hipMemcpyAsync(tp.chunk_entry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
+                     d_emask.as<uint32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, tp.tile_chunk.as<int32_t>(),
+                     d_centry.as<int32_t>());
+  if ((st = hip_check(hipMemcpyAsync(d_centry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
                       "H2D chunk end")))
     return st;
   if ((st = tp.chunk_lbeg.alloc(((size_t)nch + 1) * 8))) return st;
-  hipLaunchKernelGGL(k_chunk_lbeg, dim3(grid1((int64_t)nch + 1, 256)), dim3(256), 0, s, tp.chunk_entry.as<int32_t>(),
-                     tp.entry_lbeg.as<int64_t>(), (int64_t)nch + 1, tp.chunk_lbeg.as<int64_t>());
-  if ((st = tp.entry_coff.alloc((size_t)ne * 4))) return st;
-  hipLaunchKernelGGL(k_entry_coff, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, tp.chunk_entry.as<int32_t>(),
-                     tp.entry_lbeg.as<int64_t>(), (int64_t)nch, tp.entry_coff.as<int32_t>());
+  hipLaunchKernelGGL(k_chunk_lbeg, dim3(grid1((int64_t)nch + 1, 256)), dim3(256), 0, s, d_centry.as<int32_t>(),
+                     d_elbeg.as<int64_t>(), (int64_t)nch + 1, tp.chunk_lbeg.as<int64_t>());
+  if ((st = d_ecoff.alloc((size_t)ne * 4))) return st;
+  hipLaunchKernelGGL(k_entry_coff, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, d_centry.as<int32_t>(),
+                     d_elbeg.as<int64_t>(), (int64_t)nch, d_ecoff.as<int32_t>());
+  // Product lists: count per chunk, first product of each chunk, fill.
+  DevBuf d_ctile, d_np, d_pbase;
+  if ((st = d_ctile.alloc((size_t)std::max(nch, 1) * 4)) || (st = d_np.alloc((size_t)std::max(nch, 1) * 8)) ||
+      (st = d_pbase.alloc(((size_t)nch + 1) * 8)))
+    return st;
+  hipLaunchKernelGGL(k_chunk_tile, dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, tp.tile_chunk.as<int32_t>(),
+                     in.n_tiles, d_ctile.as<int32_t>());
+  hipLaunchKernelGGL((k_chunk_prods<false>), dim3(grid1(std::max(nch, 1), 256)), dim3(256), 0, s, nch,
+                     d_centry.as<int32_t>(), d_ctile.as<int32_t>(), d_emask.as<uint32_t>(), d_ecoff.as<int32_t>(),
+                     d_owner.as<uint8_t>(), (int32_t)ns2, d_np.as<int64_t>(), nullptr, nullptr, nullptr, nullptr);
+  if ((st = hip_check(hipMemsetAsync(d_pbase.p, 0, 8, s), "memset pbase"))) return st;
+  if ((st = inclusive_scan(d_np.as<int64_t>(), d_pbase.as<int64_t>() + 1, (size_t)nch, s))) return st;
+  int64_t nprod = 0;
+  if ((st = hip_check(hipMemcpyAsync(&nprod, d_pbase.as<int64_t>() + nch, 8, hipMemcpyDeviceToHost, s), "D2H products")) ||
+      (st = hip_check(hipStreamSynchronize(s), "product count")))
+    return st;
+  tp.n_prod = nprod;
+  if (nprod != n)
+    return set_errorf(CQMM_ERR_STATE, "tile plan: %lld listed products for %lld triplets", (long long)nprod,
+                      (long long)n);
+  // pstart/pcnt carry one extra chunk (the clamp target of the kernel's
+  // per-chunk registers) and prod kWaveProducts extra entries (each
+  // wavefront loads a full row of 64 list words per chunk).
+  if ((st = tp.pstart.alloc(((size_t)nch + 1) * kWaves * 8)) || (st = tp.pcnt.alloc(((size_t)nch + 1) * kWaves * 8)) ||
+      (st = tp.prod.alloc(((size_t)nprod + kWaveProducts) * 4)))
+    return st;
+  std::vector<int64_t> tail_start(kWaves, nprod);
+  if ((st = hip_check(hipMemsetAsync(tp.pcnt.as<uint64_t>() + (size_t)nch * kWaves, 0, kWaves * 8, s), "memset pcnt")) ||
+      (st = hip_check(hipMemcpyAsync(tp.pstart.as<int64_t>() + (size_t)nch * kWaves, tail_start.data(), kWaves * 8,
+                                     hipMemcpyHostToDevice, s), "H2D pstart tail")) ||
+      (st = hip_check(hipMemsetAsync(tp.prod.as<uint32_t>() + nprod, 0, kWaveProducts * 4, s), "memset prod tail")))
+    return st;
+  hipLaunchKernelGGL((k_chunk_prods<true>), dim3(grid1(std::max(nch, 1), 256)), dim3(256), 0, s, nch,
+                     d_centry.as<int32_t>(), d_ctile.as<int32_t>(), d_emask.as<uint32_t>(), d_ecoff.as<int32_t>(),
+                     d_owner.as<uint8_t>(), (int32_t)ns2, nullptr, d_pbase.as<int64_t>(), tp.pstart.as<int64_t>(),
+                     tp.pcnt.as<uint64_t>(), tp.prod.as<uint32_t>());
   if ((st = hip_check(hipGetLastError(), "tile builder launch"))) return st;
   return hip_check(hipStreamSynchronize(s), "tile build");
 }
