@@ -140,7 +140,8 @@ def purification_bench(args):
     for prec in (1, 0):
         params = sv.default_params(precision=prec)
         log(f"serving {n} requests at {args.rate}/s, precision {prec}, {args.slots} slots")
-        res, wall = sv.serve(reqs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), params=params)
+        res, wall = sv.serve(reqs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), params=params,
+                             log_progress=True)
         ok = [r for r in res if r["status"] == 0]
         lat = np.array([r["latency_s"] for r in ok])
         lines[prec] = dict(solves_per_s=len(ok) / wall, p50_s=float(np.percentile(lat, 50)),
@@ -252,10 +253,13 @@ def main():
     achieved = flops / (kern_avg_ms / 1e3) / 1e12
     out = (dc if w["mult_type"] == 1 else da).to_host()
     log(f"step {ms_step:.3f} ms (kernel {kern_avg_ms:.3f} ms), {value:.1f} GFLOP/s; checksum {float(np.abs(out).sum()):.6e}")
-    fp32 = None
-    if args.workload == "cfg3":
-        # Same products with FP32 MFMA on FP32 copies, FP32 accumulation; error vs the FP64 result.
-        f32 = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_FP32 | cqmm.CQMM_NO_SYNC
+    # The same products with FP32 MFMA on FP32 copies of the operands (per call:
+    # FP64->FP32 operand copies, products, FP32->FP64 result): FP32 accumulation
+    # (CQMM_FP32) and each product's FP32 result summed in FP64 (CQMM_FP32_ACC64,
+    # the mixed-precision McWeeny's kernel); errors against the FP64 result.
+    fp32 = {}
+    for key, extra in (("fp32_accumulate", 0), ("fp32_acc64", cqmm.CQMM_FP32_ACC64)):
+        f32 = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_FP32 | cqmm.CQMM_NO_SYNC | extra
         plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, f32)
         cqmm.synchronize()
         t0 = time.perf_counter()
@@ -265,12 +269,12 @@ def main():
         ms32 = (time.perf_counter() - t0) / args.steps * 1e3
         out32 = (dc if w["mult_type"] == 1 else da).to_host()
         d = out32 - out
-        fp32 = {"ms_per_step": round(ms32, 4), "gflops": round(flops / (ms32 / 1e3) / 1e9, 3),
-                "max_rel_err": float(np.abs(d).max() / np.abs(out).max()),
-                "frobenius_rel_err": float(np.linalg.norm(d) / np.linalg.norm(out)),
-                "note": "per call: FP64->FP32 operand copies, FP32 MFMA products (FP32 accumulation), FP32->FP64 result"}
-        log(f"FP32-accumulate: {ms32:.3f} ms, {fp32['gflops']:.1f} GFLOP/s, max rel err {fp32['max_rel_err']:.3e}, "
-            f"Frobenius rel err {fp32['frobenius_rel_err']:.3e}")
+        fp32[key] = {"ms_per_step": round(ms32, 4), "gflops": round(flops / (ms32 / 1e3) / 1e9, 3),
+                     "speedup_vs_fp64": round(ms_step / ms32, 3),
+                     "max_rel_err": float(np.abs(d).max() / np.abs(out).max()),
+                     "frobenius_rel_err": float(np.linalg.norm(d) / np.linalg.norm(out))}
+        log(f"{key}: {ms32:.3f} ms, {fp32[key]['gflops']:.1f} GFLOP/s, max rel err {fp32[key]['max_rel_err']:.3e}, "
+            f"Frobenius rel err {fp32[key]['frobenius_rel_err']:.3e}")
     line = {
         "metric": "block-sparse mult GFLOP/s",
         "value": round(value, 3),
@@ -291,7 +295,7 @@ def main():
             "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
             "flop_per_step": flops, "output_blocks": int(info.n_out_total),
             "plan_build_s": round(t_plan, 3), "setup_s": round(setup_s, 2),
-            "kernel": kname, "tiles": int(info.n_tiles), "fp32_accumulate": fp32,
+            "kernel": kname, "tiles": int(info.n_tiles), "fp32": fp32,
             "operand_blocks_per_product": round(info.n_loads / max(info.n_triplets, 1), 4)
             if info.kernel == cqmm.KERNEL_TILE else 2.0,
         },

@@ -12,14 +12,15 @@ class PurifyParams(C.Structure):
     _fields_ = [("r_S", C.c_double), ("r_H", C.c_double), ("r_L", C.c_double), ("density", C.c_double),
                 ("dmin", C.c_double), ("s_amp", C.c_double), ("s_decay", C.c_double), ("h_amp", C.c_double),
                 ("h_decay", C.c_double), ("h_onsite", C.c_double), ("h_split", C.c_double), ("h_shift", C.c_double),
-                ("precision", C.c_int32), ("max_iter", C.c_int32)]
+                ("precision", C.c_int32), ("max_iter", C.c_int32), ("fp32_tol", C.c_double)]
 
 
 class PurifyResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("fp32_iterations", C.c_int32),
                 ("stop_reason", C.c_int32), ("hotelling_iterations", C.c_int32), ("band_energy", C.c_double),
                 ("n_electrons", C.c_double), ("ne_error", C.c_double), ("setup_s", C.c_double),
-                ("solve_s", C.c_double), ("hbm_estimate", C.c_int64), ("natoms", C.c_int32), ("n_orb", C.c_int32)]
+                ("solve_s", C.c_double), ("hbm_estimate", C.c_int64), ("natoms", C.c_int32), ("n_orb", C.c_int32),
+                ("hbm_peak", C.c_int64)]
 
 
 class ServeRequest(C.Structure):
@@ -27,7 +28,7 @@ class ServeRequest(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("params", PurifyParams)]
+    _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("params", PurifyParams), ("log_progress", C.c_int32)]
 
 
 class ServeResult(C.Structure):
@@ -87,7 +88,7 @@ def poisson_stream(n, rate, seed, nmin=64, nmax=2048, nsf_choices=(4, 13)):
             for a, b, s, t in zip(natoms, nsf, seeds, arrival)]
 
 
-def serve(requests, slots=4, hbm_budget=200 << 30, params=None):
+def serve(requests, slots=4, hbm_budget=200 << 30, params=None, log_progress=False):
     """Run the request loop; returns (list of per-request dicts, wall seconds)."""
     init()
     params = params or default_params()
@@ -95,7 +96,7 @@ def serve(requests, slots=4, hbm_budget=200 << 30, params=None):
     reqs = (ServeRequest * max(n, 1))()
     for i, q in enumerate(requests):
         reqs[i] = ServeRequest(q["natoms"], q["nsf"], q["seed"], q["arrival_s"])
-    cfg = ServeConfig(slots, int(hbm_budget), params)
+    cfg = ServeConfig(slots, int(hbm_budget), params, int(bool(log_progress)))
     out = (ServeResult * max(n, 1))()
     wall = C.c_double()
     check(_bind().cqmm_serve(reqs, n, C.byref(cfg), out, C.byref(wall)), "cqmm_serve")

@@ -24,6 +24,13 @@ void set_thread_stream(hipStream_t s);
 int hip_check(hipError_t e, const char *what);
 
 // Owning device buffer (freed with the plan that holds it).
+// Device bytes held through DevBuf by the calling thread, and their high-water
+// mark since the last hbm_peak_reset() (per-request HBM accounting in serving).
+void hbm_track(int64_t delta);
+int64_t hbm_current();
+int64_t hbm_peak();
+void hbm_peak_reset();
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -37,7 +44,10 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      hbm_track(-(int64_t)bytes);
+    }
     p = nullptr;
     bytes = 0;
   }

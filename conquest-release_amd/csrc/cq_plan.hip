@@ -44,8 +44,21 @@ int DevBuf::alloc(size_t n) {
     return hip_check(e, "hipMalloc (plan buffer)");
   }
   bytes = n;
+  hbm_track((int64_t)n);
   return CQMM_OK;
 }
+
+namespace {
+thread_local int64_t t_hbm_bytes = 0, t_hbm_peak = 0;
+}
+
+void hbm_track(int64_t delta) {
+  t_hbm_bytes += delta;
+  if (t_hbm_bytes > t_hbm_peak) t_hbm_peak = t_hbm_bytes;
+}
+int64_t hbm_current() { return t_hbm_bytes; }
+int64_t hbm_peak() { return t_hbm_peak; }
+void hbm_peak_reset() { t_hbm_peak = t_hbm_bytes; }
 
 }  // namespace cq
 

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -71,6 +72,7 @@ void cqmm_purify_default_params(cqmm_purify_params *p) {
   p->h_shift = 3.0;
   p->precision = 1;
   p->max_iter = 100;
+  p->fp32_tol = 0.0;
 }
 
 int64_t cqmm_purify_hbm_estimate(int32_t natoms, int32_t nsf, const cqmm_purify_params *p) {
@@ -88,10 +90,13 @@ int64_t cqmm_purify_hbm_estimate(int32_t natoms, int32_t nsf, const cqmm_purify_
   auto trip = [&](double ra, double rb) { return N * nab_per_row(d, ra) * nab_per_row(d, rb); };
   const double t_big = std::max(trip(rL + rS, rL), trip(rL, rL + rS));
   const double trips = trip(rL, rS) + 2 * t_big + trip(rT, rS) + trip(rT, rT) + trip(rT, rH) + trip(rT + rH, rT);
-  // Resident triplet lists and tile schedules (~20 B per product) + the largest
-  // build's transient buffers (triplet fill, tile-key sort and entry scatter).
-  const double plans = 20.0 * trips + 90.0 * t_big;
-  return (int64_t)(1.15 * (mats + f32 + plans)) + (64LL << 20);
+  // Resident triplet lists and tile schedules (14 B per product) + the largest
+  // build's transient buffers (38 B per product of the biggest product). The
+  // coefficients are fitted to the measured per-solve peaks (hbm_peak) of
+  // 64-2048 atoms with 4 and 13 support functions, where this sum (x1.1) is
+  // 1.18-1.68 times the peak.
+  const double plans = 14.0 * trips + 38.0 * t_big;
+  return (int64_t)(1.1 * (mats + f32 + plans)) + (64LL << 20);
 }
 
 int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_params *p, cqmm_purify_result *r) {
@@ -102,6 +107,8 @@ int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_pa
   r->hbm_estimate = cqmm_purify_hbm_estimate(natoms, nsf, p);
   const double t0 = now_s();
   PTRY(cq::ensure_init());
+  cq::hbm_peak_reset();
+  const int64_t hbm_base = cq::hbm_current();
   Cleanup cl;
   std::vector<double> xyz(3 * (size_t)natoms);
   double edge = 0.0;
@@ -182,7 +189,7 @@ int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_pa
   double ml[2];
   PTRY(cqmm_init_mcweeny(cl.ctx, &is, mH, mS, mT, mL, ml));
   cqmm_mcw_setup ms = {id[Lr], id[S], id[H], id[LS], id[LSL], L_S_LS, LS_L_LSL, LSL_SL_L,
-                       p->max_iter, p->precision, 2, 1e-6};
+                       p->max_iter, p->precision, 2, p->fp32_tol};
   cqmm_mcw_result mr;
   PTRY(cqmm_mcweeny(cl.ctx, &ms, mH, mS, mL, &mr));
   const double t2 = now_s();
@@ -195,6 +202,7 @@ int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_pa
   r->ne_error = std::fabs(r->n_electrons - 2.0 * ne);
   r->setup_s = t1 - t0;
   r->solve_s = t2 - t1;
+  r->hbm_peak = cq::hbm_peak() - hbm_base;
   return CQMM_OK;
 }
 
@@ -262,6 +270,10 @@ int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_confi
       {
         std::lock_guard<std::mutex> lk(mu);
         in_use -= est[i];
+        if (cfg->log_progress)
+          std::fprintf(stderr, "[serve] request %d (%d atoms, %d SF) slot %d: status %d, %.2f s latency, %d iterations\n",
+                       i, reqs[i].natoms, reqs[i].nsf, slot, res[i].r.status, res[i].done_s - res[i].arrival_s,
+                       res[i].r.iterations);
       }
       cv.notify_all();
     }

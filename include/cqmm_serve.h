@@ -25,6 +25,8 @@ typedef struct cqmm_purify_params {
   double h_amp, h_decay, h_onsite, h_split, h_shift;  /* H recipe             */
   int32_t precision;        /* 0 FP64; 1 FP32 MFMA + FP64 block sums; 2 FP32 sums */
   int32_t max_iter;         /* McWeeny iterations at most                     */
+  double fp32_tol;          /* mixed: FP32 phase also ends when |dE| < fp32_tol*|E|
+                               (0: only the McWeeny stop tests end it)        */
 } cqmm_purify_params;
 
 typedef struct cqmm_purify_result {
@@ -36,6 +38,7 @@ typedef struct cqmm_purify_result {
   double setup_s, solve_s;  /* host+device set-up; Hotelling+InitMcW+McWeeny    */
   int64_t hbm_estimate;     /* bytes reserved for admission                     */
   int32_t natoms, n_orb;
+  int64_t hbm_peak;         /* device bytes the solve held at most (measured)   */
 } cqmm_purify_result;
 
 /* Defaults of the served workload: S/H/L 8/8/12 A, 0.05 atoms/A^3, 2 A. */
@@ -55,6 +58,7 @@ typedef struct cqmm_serve_config {
   int32_t slots;            /* requests in flight at most (worker threads, one HIP stream each) */
   int64_t hbm_budget;       /* bytes; a request is admitted when its estimate fits */
   cqmm_purify_params params;
+  int32_t log_progress;     /* nonzero: one line on stderr per finished request */
 } cqmm_serve_config;
 
 typedef struct cqmm_serve_result {

@@ -42,6 +42,8 @@ def test_requests_alone_equal_requests_in_the_mix():
         assert m["n_electrons"] == a["n_electrons"]
         assert m["iterations"] == a["iterations"]
         assert m["ne_error"] < 1e-8
+        # The admission estimate covers what the solve held at most.
+        assert 0 < a["hbm_peak"] <= a["hbm_estimate"]
     # Work of different requests overlapped on the device.
     iv = sorted((m["admit_s"], m["done_s"]) for m in res)
     assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
