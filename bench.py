@@ -223,7 +223,9 @@ def main():
     plan.set_kernel({"auto": cqmm.KERNEL_AUTO, "triplet": cqmm.KERNEL_TRIPLET, "tile": cqmm.KERNEL_TILE}[args.kernel])
     info = plan.info()
     flops = info.flops
-    kname = "k_tile_products_f64" if info.kernel == cqmm.KERNEL_TILE else "k_block_products_f64_u"
+    mixed = not isinstance(w["nsf"], int)
+    kname = (("k_tile_mixed_f64" if mixed else "k_tile_products_f64") if info.kernel == cqmm.KERNEL_TILE
+             else ("k_block_products_f64" if mixed else "k_block_products_f64_u"))
     log(f"plan: {info.n_triplets} products, {flops / 1e12:.3f} TFLOP/step, built in {t_plan:.2f} s; "
         f"kernel {kname}, {info.n_tiles} tiles, {info.n_entries} entries, "
         f"{info.n_loads / max(info.n_triplets, 1):.3f} operand blocks per product")

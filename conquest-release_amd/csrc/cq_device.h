@@ -75,6 +75,12 @@ struct TilePlan {
   DevBuf prod;         // uint32 [n_prod+64] LDS element offsets of the two operands (left | right << 16)
   DevBuf tile_out;     // int32 [n_tiles*64] output block of slot row*8+col, or -1
   DevBuf wave_out;     // int32 [n_tiles*4*16] output block of (tile, wavefront, slot), or -1
+  // Mixed block sizes (maxn > 0): operands and outputs by element offset.
+  int maxn = 0;        // largest block dimension; 0: uniform schedule above
+  DevBuf list64;       // uint64 [n_loads] operand element offset; bit 63 set: right operand
+  DevBuf listls;       // uint32 [n_loads] LDS element offset in the chunk | elements << 16
+  DevBuf wave_out64;   // int64 [n_tiles*4*16] output element offset of (tile, wavefront, slot), or -1
+  DevBuf wave_dim;     // uint8 [n_tiles*4*16] (rows - 1) | (columns - 1) << 4 of that output block
 };
 
 // Triplet list of one product, grouped by output block (segment), each group in
@@ -106,7 +112,11 @@ struct TileBuildInput {
   const int64_t *p = nullptr, *q = nullptr;  // device [n_trip]: operand element offsets
   int64_t nx = 0;
   int32_t n_tiles = 0;
-  const int32_t *tile_out = nullptr;  // host [n_tiles*64]
+  const int64_t *tile_out = nullptr;  // host [n_tiles*64]: output element offset, or -1
+  int maxn = 0;                        // largest block dimension
+  const uint8_t *tile_rdim = nullptr;  // host [n_tiles*8]: block rows of each tile row (0: none)
+  const uint8_t *tile_cdim = nullptr;  // host [n_tiles*8]: block columns of each tile column
+  const uint8_t *xdim = nullptr;       // host [nx]: contraction dimension of each contraction atom
 };
 bool tiles_enabled();
 // Column order of a halo for tiling: Morton order of the halo partitions'

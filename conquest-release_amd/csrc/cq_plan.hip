@@ -444,6 +444,10 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   for (int32_t k = 0; k < ni_c && uniform; ++k) if (C.ndimj[k] != uniform) uniform = 0;
   // Only these sizes have compile-time kernel instances; others use per-block sizes.
   if (uniform != 4 && uniform != 9 && uniform != 13 && uniform != 16) uniform = 0;
+  int maxn = 0;  // largest block dimension (the mixed-size tile kernel takes up to 16)
+  for (int32_t p = 0; p < d->n_prim; ++p) maxn = std::max(maxn, std::max(A.ndimi[p], C.ndimi[p]));
+  for (int32_t k = 0; k < ni_a; ++k) maxn = std::max(maxn, A.ndimj[k]);
+  for (int32_t k = 0; k < ni_c; ++k) maxn = std::max(maxn, C.ndimj[k]);
 
   // ---- device work ----
   DevBuf d_km, d_bindex, d_ci_halo, d_ci_hbeg, d_ci_h2d, d_ai_h2d, d_iprim, d_itemk, d_ndimi, d_err;
@@ -512,13 +516,17 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   plan->lenc = d->lenc;
   if (n_trip > INT32_MAX) return cq::set_error(CQMM_ERR_RANGE, "cqmm_plan_create: more than 2^31 block products");
 
-  // ---- output tiles (uniform block sizes): host part ----
-  bool tiles = uniform != 0 && n_trip > 0 && cq::tiles_enabled();
+  // ---- output tiles: host part ----
+  // Uniform instanced sizes take the fixed-size tile kernel; other sizes up to
+  // 16 (mixed species) the mixed-size one.
+  bool tiles = (uniform != 0 || (maxn > 0 && maxn <= 16)) && n_trip > 0 && cq::tiles_enabled();
   const int64_t ns2 = (int64_t)uniform * uniform;
-  std::vector<int32_t> crank, tmap, tout;
+  std::vector<int32_t> crank, tmap;
+  std::vector<int64_t> tout;
+  std::vector<uint8_t> trdim, tcdim, xdim;
   int32_t ncg = 0, n_tiles = 0;
   const int64_t nx = d->mult_type == 1 ? ni_a : ni_c;
-  if (tiles) {
+  if (tiles && uniform) {
     // Every operand offset must be a whole number of blocks.
     for (int32_t k = 0; k < ni_a && tiles; ++k) tiles = km[k].bdata % ns2 == 0;
     const H *ph2d = d->mult_type == 1 ? A.i_h2d : C.i_h2d;
@@ -550,7 +558,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
         ob_o.push_back(off - 1);
       }
     }
-    for (int64_t b = 0; b < (int64_t)ob_o.size() && tiles; ++b) tiles = ob_o[b] % ns2 == 0;
+    for (int64_t b = 0; b < (int64_t)ob_o.size() && tiles && uniform; ++b) tiles = ob_o[b] % ns2 == 0;
     if (tiles && nrg * ncg > INT32_MAX) tiles = false;
     if (tiles) {
       tmap.assign((size_t)(nrg * ncg), -1);
@@ -558,11 +566,21 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
       for (auto &v : tmap)
         if (v == 0) v = n_tiles++;
       tout.assign((size_t)n_tiles * 64, -1);
+      // Block dimensions of each tile row and column, and of each contraction
+      // atom (the mixed-size schedule packs blocks by their sizes).
+      trdim.assign((size_t)n_tiles * 8, 0);
+      tcdim.assign((size_t)n_tiles * 8, 0);
+      const int32_t *cdims = d->mult_type == 1 ? C.ndimj : A.ndimj;
       for (size_t b = 0; b < ob_o.size(); ++b) {
         const int32_t cr = crank[ob_c[b]];
         const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
-        tout[(size_t)tile * 64 + ((ob_r[b] & 7) << 3) + (cr & 7)] = (int32_t)(ob_o[b] / ns2);
+        tout[(size_t)tile * 64 + ((ob_r[b] & 7) << 3) + (cr & 7)] = ob_o[b];
+        trdim[(size_t)tile * 8 + (ob_r[b] & 7)] = (uint8_t)A.ndimi[ob_r[b]];
+        tcdim[(size_t)tile * 8 + (cr & 7)] = (uint8_t)cdims[ob_c[b]];
       }
+      const int32_t *xd = d->mult_type == 1 ? A.ndimj : C.ndimj;
+      xdim.resize((size_t)nx);
+      for (int64_t x = 0; x < nx; ++x) xdim[x] = (uint8_t)xd[x];
     }
   }
   DevBuf d_crank, d_tmap, d_tkey, d_tslot;
@@ -591,6 +609,10 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   if (tiles) {
     cq::TileBuildInput tin;
     tin.nsf = uniform;
+    tin.maxn = maxn;
+    tin.tile_rdim = trdim.data();
+    tin.tile_cdim = tcdim.data();
+    tin.xdim = xdim.data();
     tin.n_trip = n_trip;
     tin.key = d_tkey.as<uint64_t>();
     tin.slot = d_tslot.as<uint8_t>();

@@ -531,6 +531,403 @@ int launch_mode(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_
   return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
 }
 
+// ------------------------------------------------ mixed block sizes
+// Same schedule with per-atom block dimensions (up to 16): operands are
+// addressed by element offset and packed in LDS by size; each product word
+// carries its contraction dimension; the k-steps of a product (1-4) and the
+// lane offsets follow the dimensions of its blocks.
+
+// Operand elements a chunk stage holds: STG_SMALL for blocks up to 13x13
+// (three workgroups per CU), STG_LARGE up to 16x16 (an entry brings at most
+// 16 blocks, so it always fits).
+constexpr int kStgSmall = 3072;
+constexpr int kStgLarge = 4096;
+int stage_elems(int maxn) { return maxn <= 13 ? kStgSmall : kStgLarge; }
+
+__global__ void k_entry_fill_mx(const uint64_t *__restrict__ key, const int32_t *__restrict__ perm,
+                                const int32_t *__restrict__ eid, const uint8_t *__restrict__ slot,
+                                const int64_t *__restrict__ p, const int64_t *__restrict__ q, int64_t n, int64_t nx,
+                                int64_t *__restrict__ ep, int64_t *__restrict__ eq, int32_t *__restrict__ etile,
+                                int32_t *__restrict__ ex) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const int64_t e = (int64_t)eid[pos] - 1;
+  const int32_t t = perm[pos];
+  const int sl = slot[t];
+  ep[e * 8 + (sl >> 3)] = p[t];
+  eq[e * 8 + (sl & 7)] = q[t];
+  if (pos == 0 || eid[pos - 1] != eid[pos]) {
+    etile[e] = (int32_t)(key[pos] / (uint64_t)nx);
+    ex[e] = (int32_t)(key[pos] % (uint64_t)nx);
+  }
+}
+
+// Mask, operand elements and operand count of each entry.
+__global__ void k_entry_mask_mx(const int64_t *__restrict__ ep, const int64_t *__restrict__ eq,
+                                const int32_t *__restrict__ etile, const int32_t *__restrict__ ex,
+                                const uint8_t *__restrict__ rdim, const uint8_t *__restrict__ cdim,
+                                const uint8_t *__restrict__ xdim, int64_t ne, uint32_t *__restrict__ mask,
+                                int32_t *__restrict__ esz, int64_t *__restrict__ cnt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t t = etile[e];
+  const int32_t nxd = xdim[ex[e]];
+  uint32_t m = 0;
+  int32_t sz = 0;
+  for (int u = 0; u < 8; ++u) {
+    if (ep[e * 8 + u] >= 0) {
+      m |= 1u << u;
+      sz += rdim[t * 8 + u] * nxd;
+    }
+    if (eq[e * 8 + u] >= 0) {
+      m |= 1u << (8 + u);
+      sz += nxd * cdim[t * 8 + u];
+    }
+  }
+  mask[e] = m;
+  esz[e] = sz;
+  cnt[e] = __popc(m);
+}
+
+// Operand list of each entry (left blocks by row, then right blocks by
+// column): element offsets (bit 63: right operand) and element counts.
+__global__ void k_list_fill_mx(const int64_t *__restrict__ ep, const int64_t *__restrict__ eq,
+                               const int32_t *__restrict__ etile, const int32_t *__restrict__ ex,
+                               const uint8_t *__restrict__ rdim, const uint8_t *__restrict__ cdim,
+                               const uint8_t *__restrict__ xdim, const int64_t *__restrict__ lbeg, int64_t ne,
+                               uint64_t *__restrict__ list, uint16_t *__restrict__ lsz) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t t = etile[e];
+  const int32_t nxd = xdim[ex[e]];
+  int64_t o = lbeg[e];
+  for (int u = 0; u < 8; ++u)
+    if (ep[e * 8 + u] >= 0) {
+      list[o] = (uint64_t)ep[e * 8 + u];
+      lsz[o++] = (uint16_t)(rdim[t * 8 + u] * nxd);
+    }
+  for (int u = 0; u < 8; ++u)
+    if (eq[e * 8 + u] >= 0) {
+      list[o] = (uint64_t)eq[e * 8 + u] | (1ull << 63);
+      lsz[o++] = (uint16_t)(nxd * cdim[t * 8 + u]);
+    }
+}
+
+// Chunks of a tile by operand elements (at most cap), entries and
+// wavefront products (as k_chunks).
+template <bool FILL>
+__global__ void k_chunks_mx(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask,
+                            const int32_t *__restrict__ esz, const uint8_t *__restrict__ owner, int32_t n_tiles,
+                            int32_t cap, const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  const uint8_t *own = owner + (int64_t)t * 64;
+  int32_t nch = 0, sum = 0, ne = 0;
+  int wc[kWaves] = {0, 0, 0, 0};
+  for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
+    const uint32_t m = mask[e];
+    const int32_t c = esz[e];
+    int pw[kWaves];
+    entry_wave_counts(m, own, pw);
+    bool over = false;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) over |= wc[w] + pw[w] > kWaveProducts;
+    if (nch == 0 || sum + c > cap || ne == kChunkEntries || over) {
+      if (FILL) out[tchunk[t] + nch] = e;
+      ++nch;
+      sum = 0;
+      ne = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) wc[w] = 0;
+    }
+    sum += c;
+    ++ne;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wc[w] += pw[w];
+  }
+  if (!FILL) out[t] = nch;
+}
+
+// First LDS element of each entry in its chunk's stage.
+__global__ void k_entry_coff_mx(const int32_t *__restrict__ chunk_entry, const int32_t *__restrict__ esz, int64_t nc,
+                                int32_t *__restrict__ coff) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int32_t run = 0;
+  for (int32_t e = chunk_entry[c]; e < chunk_entry[c + 1]; ++e) {
+    coff[e] = run;
+    run += esz[e];
+  }
+}
+
+// LDS element offset | elements << 16 of each operand of each chunk.
+__global__ void k_list_lds_mx(const int64_t *__restrict__ chunk_lbeg, const uint16_t *__restrict__ lsz, int64_t nc,
+                              uint32_t *__restrict__ listls) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  uint32_t run = 0;
+  for (int64_t o = chunk_lbeg[c]; o < chunk_lbeg[c + 1]; ++o) {
+    listls[o] = run | ((uint32_t)lsz[o] << 16);
+    run += lsz[o];
+  }
+}
+
+// Product lists (as k_chunk_prods): word = left LDS offset | right LDS offset
+// << 12 | (contraction dimension - 1) << 24.
+template <bool FILL>
+__global__ void k_chunk_prods_mx(int32_t nch, const int32_t *__restrict__ chunk_entry,
+                                 const int32_t *__restrict__ ctile, const uint32_t *__restrict__ mask,
+                                 const int32_t *__restrict__ coff, const int32_t *__restrict__ ex,
+                                 const uint8_t *__restrict__ owner, const uint8_t *__restrict__ rdim,
+                                 const uint8_t *__restrict__ cdim, const uint8_t *__restrict__ xdim,
+                                 int64_t *__restrict__ np, const int64_t *__restrict__ pbase,
+                                 int64_t *__restrict__ pstart, uint64_t *__restrict__ pcnt,
+                                 uint32_t *__restrict__ prod) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  const int64_t t = ctile[c];
+  const uint8_t *own = owner + t * 64;
+  const int32_t e0 = chunk_entry[c], e1 = chunk_entry[c + 1];
+  uint8_t cnt[64];
+  for (int i = 0; i < 64; ++i) cnt[i] = 0;
+  int32_t tot = 0;
+  for (int32_t e = e0; e < e1; ++e) {
+    const uint32_t m = mask[e];
+    for (uint32_t ym = m & 0xffu; ym; ym &= ym - 1u) {
+      const int u = __builtin_ctz(ym);
+      for (uint32_t bm = (m >> 8) & 0xffu; bm; bm &= bm - 1u) {
+        const uint8_t o = own[u * 8 + __builtin_ctz(bm)];
+        if (o != 0xff) {
+          ++cnt[o];
+          ++tot;
+        }
+      }
+    }
+  }
+  if (!FILL) {
+    np[c] = tot;
+    return;
+  }
+  uint16_t off[64];
+  int32_t run = 0;
+  for (int w = 0; w < kWaves; ++w) {
+    pstart[(int64_t)c * kWaves + w] = pbase[c] + run;
+    uint64_t word = 0;
+    for (int l = 0; l < kSlots; ++l) {
+      off[w * kSlots + l] = (uint16_t)run;
+      run += cnt[w * kSlots + l];
+      word |= (uint64_t)cnt[w * kSlots + l] << (4 * l);
+    }
+    pcnt[(int64_t)c * kWaves + w] = word;
+  }
+  for (int32_t e = e0; e < e1; ++e) {
+    const uint32_t m = mask[e];
+    const uint32_t ym = m & 0xffu, bmask = (m >> 8) & 0xffu;
+    const uint32_t nxd = xdim[ex[e]];
+    uint32_t rows = 0;  // left elements of the entry
+    for (uint32_t yb = ym; yb; yb &= yb - 1u) rows += rdim[t * 8 + __builtin_ctz(yb)] * nxd;
+    uint32_t aoff = (uint32_t)coff[e];
+    for (uint32_t yb = ym; yb; yb &= yb - 1u) {
+      const int u = __builtin_ctz(yb);
+      uint32_t boff = (uint32_t)coff[e] + rows;
+      for (uint32_t bm = bmask; bm; bm &= bm - 1u) {
+        const int col = __builtin_ctz(bm);
+        const uint8_t o = own[u * 8 + col];
+        if (o != 0xff) prod[pbase[c] + off[o]++] = aoff | (boff << 12) | ((nxd - 1u) << 24);
+        boff += nxd * cdim[t * 8 + col];
+      }
+      aoff += rdim[t * 8 + u] * nxd;
+    }
+  }
+}
+
+// Copies of one operand block (up to 16x16) with one M0: pieces I < NP of
+// 256 bytes, the last masked to the block's dwords.
+template <int... I>
+__device__ inline void dma_block_pieces(const uint32_t *src, uint32_t *dst, int dwords, int lane,
+                                        std::integer_sequence<int, I...>) {
+  ((I * 64 < dwords ? (I * 64 + lane < dwords ? (dma_piece<I * 256>(src, dst), 0) : 0) : 0), ...);
+}
+
+template <class T>
+__device__ inline void issue_stage_mx(uint64_t lv, uint32_t lls, int32_t nb, const T *__restrict__ P,
+                                      const T *__restrict__ Q, T *stage, int w, int lane) {
+  constexpr int MAXP = 256 * (int)sizeof(T) / 4 / 64;  // pieces of a 16x16 block
+  const int32_t nbw = nb > w ? (nb - w + kWaves - 1) / kWaves : 0;
+  for (int32_t k = 0; k < nbw; ++k) {
+    const uint64_t g = (uint64_t)readlane64((int64_t)lv, k);
+    const uint32_t ls = (uint32_t)__builtin_amdgcn_readlane((int)lls, k);
+    const T *base = (g >> 63) ? Q : P;
+    const uint32_t *src = (const uint32_t *)(base + (int64_t)(g & 0x7fffffffffffffffull)) + lane;
+    uint32_t *dst = (uint32_t *)(stage + (ls & 0xffffu));
+    const int dwords = (int)(ls >> 16) * (int)sizeof(T) / 4;
+    dma_block_pieces(src, dst, dwords, lane, std::make_integer_sequence<int, MAXP>{});
+  }
+}
+
+// Mixed-size tile kernel: as k_tile_products with per-slot output dimensions
+// (wave_dim) and per-product contraction dimension (product word bits 24-27).
+template <int MODE, bool QT, int STG>
+__global__ void __launch_bounds__(kTileThreads) k_tile_mixed(
+    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
+    const uint64_t *__restrict__ list64, const uint32_t *__restrict__ listls, const int64_t *__restrict__ pstart,
+    const uint64_t *__restrict__ pcnt, const uint32_t *__restrict__ prod, const int64_t *__restrict__ wave_out,
+    const uint8_t *__restrict__ wave_dim,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
+    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
+  constexpr int STAGE = STG + kLdsPad * (int)(8 / sizeof(T));
+  __shared__ T stage0[STAGE];
+  __shared__ T stage1[STAGE];
+  const int32_t per = (n_tiles + 7) >> 3;
+  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  if (tile >= n_tiles) return;
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, r = lane & 15, g = lane >> 4;
+  const int64_t wbase = ((int64_t)tile * kWaves + w) * kSlots;
+  int64_t oblk[kSlots];
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) oblk[u] = wave_out[wbase + u];
+  const uint32_t *wd32 = (const uint32_t *)(wave_dim + wbase);
+  uint32_t wdim[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wdim[k] = wd32[k];
+  dbl4 a64[kSlots];
+  flt4 a32[kSlots];
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) {
+    a64[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    a32[u] = flt4{0.f, 0.f, 0.f, 0.f};
+  }
+  const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
+  if (c0 < c1) {
+    int32_t wb = c0;
+    int32_t cc = min(wb + lane, c1);
+    int64_t wcl = chunk_lbeg[cc];
+    int64_t wps = pstart[(int64_t)cc * kWaves + w];
+    int64_t wpc = (int64_t)pcnt[(int64_t)cc * kWaves + w];
+    auto cl_at = [&](int32_t x) { return readlane64(wcl, min(x, c1) - wb); };
+    auto ps_at = [&](int32_t x) { return readlane64(wps, min(x, c1) - wb); };
+    auto pc_at = [&](int32_t x) { return (uint64_t)readlane64(wpc, min(x, c1) - wb); };
+    auto list_idx = [&](int64_t lbeg, int32_t nb) { return lbeg + min(w + kWaves * lane, nb - 1); };
+    uint32_t pl;
+    uint64_t lvn = 0;
+    uint32_t lsn = 0;
+    {
+      const int64_t cl0 = cl_at(c0), cl1 = cl_at(c0 + 1);
+      const int64_t i0 = list_idx(cl0, (int32_t)(cl1 - cl0));
+      const uint64_t lv0 = list64[i0];
+      const uint32_t ls0 = listls[i0];
+      issue_stage_mx<T>(lv0, ls0, (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+      pl = prod[ps_at(c0) + lane];
+      if (c0 + 1 < c1) {
+        const int64_t i1 = list_idx(cl1, (int32_t)(cl_at(c0 + 2) - cl1));
+        lvn = list64[i1];
+        lsn = listls[i1];
+      }
+    }
+    for (int32_t c = c0; c < c1; ++c) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      const bool odd = ((c - c0) & 1) != 0;
+      const T *cur = odd ? stage1 : stage0;
+      const uint32_t plc = pl;
+      const uint64_t cnt = pc_at(c);
+      if (c + 1 < c1) {
+        const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2);
+        issue_stage_mx<T>(lvn, lsn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
+        pl = prod[ps_at(c + 1) + lane];
+        if (c + 2 < c1) {
+          const int64_t i2 = list_idx(cl2, (int32_t)(cl_at(c + 3) - cl2));
+          lvn = list64[i2];
+          lsn = listls[i2];
+        }
+      }
+      // Lane coordinates made opaque per chunk, so the per-slot lane offsets
+      // below are computed where used instead of being kept for all 16 slots
+      // across the chunk loop (which costs 32 registers).
+      int lr = r, lg = g;
+      asm volatile("" : "+v"(lr), "+v"(lg));
+      int p = 0;
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const int n = (int)((cnt >> (4 * u)) & 15u);
+        if (n == 0) continue;
+        const uint32_t dm = (wdim[u >> 2] >> (8 * (u & 3))) & 0xffu;
+        const int nrow = (int)(dm & 15u) + 1, ncol = (int)(dm >> 4) + 1;
+        const int oA = lr + nrow * lg, SA = 4 * nrow;
+        for (int i = 0; i < n; ++i, ++p) {
+          const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
+          const int nx = (int)((v >> 24) & 15u) + 1;
+          const int ks = (nx + 3) >> 2, rem = nx - 4 * (ks - 1);
+          const T *la = cur + (int)(v & 0xfffu) + oA;
+          const T *lq = cur + (int)((v >> 12) & 0xfffu) + (QT ? lr + ncol * lg : lg + nx * lr);
+          const int SB = QT ? 4 * ncol : 4;
+          const bool zl = lg >= rem;
+          flt4 t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            if (s4 < ks) {
+              T a = la[SA * s4], bb = lq[SB * s4];
+              if (s4 == ks - 1 && zl) {
+                a = (T)0;
+                bb = (T)0;
+              }
+              if constexpr (MODE == 0) a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, a64[u], 0, 0, 0);
+              else if constexpr (MODE == 1) a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, a32[u], 0, 0, 0);
+              else t = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, t, 0, 0, 0);
+            }
+          }
+          if constexpr (MODE == 2) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) a64[u][e] += (double)t[e];
+          }
+        }
+      }
+      if (c + 4 > wb + 63 && c + 1 < c1) {
+        wb = c + 1;
+        cc = min(wb + lane, c1);
+        wcl = chunk_lbeg[cc];
+        wps = pstart[(int64_t)cc * kWaves + w];
+        wpc = (int64_t)pcnt[(int64_t)cc * kWaves + w];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) {
+    if (oblk[u] < 0) continue;
+    const uint32_t dm = (wdim[u >> 2] >> (8 * (u & 3))) & 0xffu;
+    const int nrow = (int)(dm & 15u) + 1, ncol = (int)(dm >> 4) + 1;
+    if (r >= ncol) continue;
+    T *o = O + oblk[u];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = MODE == 0 ? g + 4 * e : 4 * g + e;
+      if (row < nrow) o[row + nrow * r] = MODE == 0 ? (T)a64[u][e] : (MODE == 1 ? (T)a32[u][e] : (T)a64[u][e]);
+    }
+  }
+}
+
+template <int MODE, bool QT, class T>
+int launch_mixed(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
+  const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
+  const dim3 blk(kTileThreads);
+  const int32_t *tc = tp.tile_chunk.as<int32_t>();
+  const int64_t *cl = tp.chunk_lbeg.as<int64_t>(), *ps = tp.pstart.as<int64_t>(), *wo = tp.wave_out64.as<int64_t>();
+  const uint64_t *l64 = tp.list64.as<uint64_t>(), *pc = tp.pcnt.as<uint64_t>();
+  const uint32_t *lls = tp.listls.as<uint32_t>(), *pr = tp.prod.as<uint32_t>();
+  const uint8_t *wd = tp.wave_dim.as<uint8_t>();
+  if (stage_elems(tp.maxn) == kStgSmall)
+    hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgSmall>), dim3(gridx), blk, 0, s, tc, cl, l64, lls, ps, pc, pr, wo, wd,
+                       P, Q, O, tp.n_tiles);
+  else
+    hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgLarge>), dim3(gridx), blk, 0, s, tc, cl, l64, lls, ps, pc, pr, wo, wd,
+                       P, Q, O, tp.n_tiles);
+  return cq::hip_check(hipGetLastError(), "mixed tile-product kernel launch");
+}
+
 }  // namespace
 
 namespace cq {
@@ -538,6 +935,10 @@ namespace cq {
 int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, bool acc64,
                              hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
+  if (tp.maxn) {
+    if (acc64) return q_trans ? launch_mixed<2, true>(tp, P, Q, O, s) : launch_mixed<2, false>(tp, P, Q, O, s);
+    return q_trans ? launch_mixed<1, true>(tp, P, Q, O, s) : launch_mixed<1, false>(tp, P, Q, O, s);
+  }
   if (acc64) return q_trans ? launch_mode<2, true>(tp, P, Q, O, s) : launch_mode<2, false>(tp, P, Q, O, s);
   return q_trans ? launch_mode<1, true>(tp, P, Q, O, s) : launch_mode<1, false>(tp, P, Q, O, s);
 }
@@ -575,7 +976,64 @@ std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part
   return rank;
 }
 
+// Output slots to wavefronts, per tile: a Latin square over (tile row,
+// column) — with k = (row + perm[column]) % 8, wavefront k % 4 owns the block
+// as its slot 2*row + k / 4, so every wavefront owns two blocks of each tile row
+// and column, which spreads every entry's rectangle of products evenly — chosen
+// among kLatinTries column permutations to balance the wavefronts' total
+// product counts. owner[tile*64 + row*8 + col] = wavefront << 4 | slot, or
+// 0xff where the tile has no such output block.
+int assign_slots(const TileBuildInput &in, std::vector<uint8_t> &owner, hipStream_t s) {
+  const int64_t n = in.n_trip;
+  int st;
+  DevBuf d_cnt;
+  if ((st = d_cnt.alloc((size_t)in.n_tiles * 64 * 4))) return st;
+  if ((st = hip_check(hipMemsetAsync(d_cnt.p, 0, (size_t)in.n_tiles * 64 * 4, s), "memset slot counts"))) return st;
+  hipLaunchKernelGGL(k_slot_counts, dim3(grid1(n, 256)), dim3(256), 0, s, in.key, in.slot, n, in.nx, d_cnt.as<int32_t>());
+  std::vector<int32_t> cnt((size_t)in.n_tiles * 64);
+  if ((st = hip_check(hipMemcpyAsync(cnt.data(), d_cnt.p, cnt.size() * 4, hipMemcpyDeviceToHost, s), "D2H slot counts")) ||
+      (st = hip_check(hipStreamSynchronize(s), "slot counts")))
+    return st;
+  constexpr int kLatinTries = 24;
+  int perms[kLatinTries][8];
+  uint64_t lcg = 0x9e3779b97f4a7c15ull;
+  for (int k = 0; k < kLatinTries; ++k) {
+    for (int i = 0; i < 8; ++i) perms[k][i] = i;
+    for (int i = 7; k > 0 && i > 0; --i) {  // k = 0: identity
+      lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+      const int j = (int)((lcg >> 33) % (uint64_t)(i + 1));
+      std::swap(perms[k][i], perms[k][j]);
+    }
+  }
+  owner.assign((size_t)in.n_tiles * 64, 0xff);
+  for (int64_t t = 0; t < in.n_tiles; ++t) {
+    const int32_t *c = cnt.data() + t * 64;
+    int best = 0;
+    int64_t best_max = -1;
+    for (int k = 0; k < kLatinTries; ++k) {
+      int64_t load[kWaves] = {0, 0, 0, 0};
+      for (int sl = 0; sl < 64; ++sl) load[((sl >> 3) + perms[k][sl & 7]) & (kWaves - 1)] += c[sl];
+      int64_t m = 0;
+      for (int w = 0; w < kWaves; ++w) m = std::max(m, load[w]);
+      if (best_max < 0 || m < best_max) {
+        best_max = m;
+        best = k;
+      }
+    }
+    for (int u = 0; u < 8; ++u)
+      for (int col = 0; col < 8; ++col) {
+        if (in.tile_out[t * 64 + u * 8 + col] < 0) continue;
+        const int k = (u + perms[best][col]) & 7;
+        owner[t * 64 + u * 8 + col] = (uint8_t)((k & (kWaves - 1)) << 4 | (2 * u + (k >> 2)));
+      }
+  }
+  return CQMM_OK;
+}
+
+int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s);
+
 int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
+  if (in.nsf == 0) return build_tiles_mixed(in, tp, s);
   const int64_t n = in.n_trip;
   const int64_t ns2 = (int64_t)in.nsf * in.nsf;
   const int32_t cap = tile_cap(in.nsf);
@@ -583,64 +1041,24 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   tp = TilePlan();
   tp.nsf = in.nsf;
   tp.n_tiles = in.n_tiles;
+  // Output blocks by block index (element offset / NSF^2).
+  std::vector<int32_t> tout32((size_t)in.n_tiles * 64);
+  for (size_t k = 0; k < tout32.size(); ++k) tout32[k] = in.tile_out[k] < 0 ? -1 : (int32_t)(in.tile_out[k] / ns2);
   if ((st = tp.tile_out.alloc((size_t)in.n_tiles * 64 * 4))) return st;
-  if ((st = hip_check(hipMemcpyAsync(tp.tile_out.p, in.tile_out, (size_t)in.n_tiles * 64 * 4, hipMemcpyHostToDevice, s),
-                      "H2D tile_out")))
+  if ((st = hip_check(hipMemcpyAsync(tp.tile_out.p, tout32.data(), (size_t)in.n_tiles * 64 * 4, hipMemcpyHostToDevice, s),
+                      "H2D tile_out")) ||
+      (st = hip_check(hipStreamSynchronize(s), "tile_out")))
     return st;
-  // Output slots to wavefronts, per tile: a Latin square over (tile row,
-  // column) — with k = (row + perm[column]) % 8, wavefront k % 4 owns the
-  // block as its slot 2*row + k / 4, so every wavefront owns two blocks of
-  // each tile row and column, which spreads every entry's rectangle of
-  // products evenly — chosen among kLatinTries column permutations to balance
-  // the wavefronts' total product counts.
   DevBuf d_owner;
   {
-    DevBuf d_cnt;
-    if ((st = d_cnt.alloc((size_t)in.n_tiles * 64 * 4))) return st;
-    if ((st = hip_check(hipMemsetAsync(d_cnt.p, 0, (size_t)in.n_tiles * 64 * 4, s), "memset slot counts"))) return st;
-    hipLaunchKernelGGL(k_slot_counts, dim3(grid1(n, 256)), dim3(256), 0, s, in.key, in.slot, n, in.nx,
-                       d_cnt.as<int32_t>());
-    std::vector<int32_t> cnt((size_t)in.n_tiles * 64);
-    if ((st = hip_check(hipMemcpyAsync(cnt.data(), d_cnt.p, cnt.size() * 4, hipMemcpyDeviceToHost, s), "D2H slot counts")) ||
-        (st = hip_check(hipStreamSynchronize(s), "slot counts")))
-      return st;
-    constexpr int kLatinTries = 24;
-    int perms[kLatinTries][8];
-    uint64_t lcg = 0x9e3779b97f4a7c15ull;
-    for (int k = 0; k < kLatinTries; ++k) {
-      for (int i = 0; i < 8; ++i) perms[k][i] = i;
-      for (int i = 7; k > 0 && i > 0; --i) {  // k = 0: identity
-        lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
-        const int j = (int)((lcg >> 33) % (uint64_t)(i + 1));
-        std::swap(perms[k][i], perms[k][j]);
-      }
-    }
-    std::vector<uint8_t> owner((size_t)in.n_tiles * 64, 0xff);
+    std::vector<uint8_t> owner;
+    if ((st = assign_slots(in, owner, s))) return st;
     std::vector<int32_t> wave_out((size_t)in.n_tiles * 64, -1);
-    for (int64_t t = 0; t < in.n_tiles; ++t) {
-      const int32_t *c = cnt.data() + t * 64;
-      int best = 0;
-      int64_t best_max = -1;
-      for (int k = 0; k < kLatinTries; ++k) {
-        int64_t load[kWaves] = {0, 0, 0, 0};
-        for (int sl = 0; sl < 64; ++sl) load[((sl >> 3) + perms[k][sl & 7]) & (kWaves - 1)] += c[sl];
-        int64_t m = 0;
-        for (int w = 0; w < kWaves; ++w) m = std::max(m, load[w]);
-        if (best_max < 0 || m < best_max) {
-          best_max = m;
-          best = k;
-        }
+    for (int64_t t = 0; t < in.n_tiles; ++t)
+      for (int sl = 0; sl < 64; ++sl) {
+        const uint8_t o = owner[t * 64 + sl];
+        if (o != 0xff) wave_out[(t * kWaves + (o >> 4)) * kSlots + (o & 15)] = tout32[t * 64 + sl];
       }
-      for (int u = 0; u < 8; ++u)
-        for (int col = 0; col < 8; ++col) {
-          const int32_t ob = in.tile_out[t * 64 + u * 8 + col];
-          if (ob < 0) continue;
-          const int k = (u + perms[best][col]) & 7;
-          const int w = k & (kWaves - 1), l = 2 * u + (k >> 2);
-          owner[t * 64 + u * 8 + col] = (uint8_t)(w << 4 | l);
-          wave_out[(t * kWaves + w) * kSlots + l] = ob;
-        }
-    }
     if ((st = d_owner.alloc(owner.size())) || (st = tp.wave_out.alloc(wave_out.size() * 4))) return st;
     if ((st = hip_check(hipMemcpyAsync(d_owner.p, owner.data(), owner.size(), hipMemcpyHostToDevice, s), "H2D owner")) ||
         (st = hip_check(hipMemcpyAsync(tp.wave_out.p, wave_out.data(), wave_out.size() * 4, hipMemcpyHostToDevice, s),
@@ -776,7 +1194,184 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
                          hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
+  if (tp.maxn) return q_trans ? launch_mixed<0, true>(tp, P, Q, O, s) : launch_mixed<0, false>(tp, P, Q, O, s);
   return q_trans ? launch_mode<0, true>(tp, P, Q, O, s) : launch_mode<0, false>(tp, P, Q, O, s);
+}
+
+// Mixed block sizes: the uniform builder's steps with element offsets and
+// per-row/column/contraction dimensions (see the kernels above).
+int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
+  const int64_t n = in.n_trip;
+  const int32_t cap = stage_elems(in.maxn);
+  int st;
+  tp = TilePlan();
+  tp.nsf = 0;
+  tp.maxn = in.maxn;
+  tp.n_tiles = in.n_tiles;
+  DevBuf d_owner, d_rdim, d_cdim, d_xdim;
+  {
+    std::vector<uint8_t> owner;
+    if ((st = assign_slots(in, owner, s))) return st;
+    std::vector<int64_t> wave_out((size_t)in.n_tiles * 64, -1);
+    std::vector<uint8_t> wave_dim((size_t)in.n_tiles * 64, 0);
+    for (int64_t t = 0; t < in.n_tiles; ++t)
+      for (int sl = 0; sl < 64; ++sl) {
+        const uint8_t o = owner[t * 64 + sl];
+        if (o == 0xff) continue;
+        const int64_t k = (t * kWaves + (o >> 4)) * kSlots + (o & 15);
+        wave_out[k] = in.tile_out[t * 64 + sl];
+        const int nr = in.tile_rdim[t * 8 + (sl >> 3)], nc = in.tile_cdim[t * 8 + (sl & 7)];
+        if (nr < 1 || nr > 16 || nc < 1 || nc > 16)
+          return set_errorf(CQMM_ERR_STATE, "tile plan: output block %d x %d outside 1..16", nr, nc);
+        wave_dim[k] = (uint8_t)((nr - 1) | (nc - 1) << 4);
+      }
+    if ((st = d_owner.alloc(owner.size())) || (st = tp.wave_out64.alloc(wave_out.size() * 8)) ||
+        (st = tp.wave_dim.alloc(wave_dim.size())) || (st = d_rdim.alloc((size_t)in.n_tiles * 8)) ||
+        (st = d_cdim.alloc((size_t)in.n_tiles * 8)) || (st = d_xdim.alloc((size_t)std::max<int64_t>(in.nx, 1))))
+      return st;
+    if ((st = hip_check(hipMemcpyAsync(d_owner.p, owner.data(), owner.size(), hipMemcpyHostToDevice, s), "H2D owner")) ||
+        (st = hip_check(hipMemcpyAsync(tp.wave_out64.p, wave_out.data(), wave_out.size() * 8, hipMemcpyHostToDevice, s),
+                        "H2D wave_out")) ||
+        (st = hip_check(hipMemcpyAsync(tp.wave_dim.p, wave_dim.data(), wave_dim.size(), hipMemcpyHostToDevice, s),
+                        "H2D wave_dim")) ||
+        (st = hip_check(hipMemcpyAsync(d_rdim.p, in.tile_rdim, (size_t)in.n_tiles * 8, hipMemcpyHostToDevice, s),
+                        "H2D rdim")) ||
+        (st = hip_check(hipMemcpyAsync(d_cdim.p, in.tile_cdim, (size_t)in.n_tiles * 8, hipMemcpyHostToDevice, s),
+                        "H2D cdim")) ||
+        (st = hip_check(hipMemcpyAsync(d_xdim.p, in.xdim, (size_t)in.nx, hipMemcpyHostToDevice, s), "H2D xdim")) ||
+        (st = hip_check(hipStreamSynchronize(s), "owner")))
+      return st;
+  }
+  // Sort the products by (tile, contraction atom).
+  unsigned end_bit = 1;
+  const uint64_t kmax = (uint64_t)in.n_tiles * (uint64_t)in.nx;
+  while (end_bit < 64 && ((uint64_t)1 << end_bit) < kmax) ++end_bit;
+  DevBuf d_perm, d_key2, d_perm2, d_tmp;
+  if ((st = d_perm.alloc(n * 4)) || (st = d_key2.alloc(n * 8)) || (st = d_perm2.alloc(n * 4))) return st;
+  hipLaunchKernelGGL(k_iota32, dim3(grid1(n, 256)), dim3(256), 0, s, d_perm.as<int32_t>(), n);
+  size_t bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, in.key, d_key2.as<uint64_t>(), d_perm.as<int32_t>(),
+                                           d_perm2.as<int32_t>(), (size_t)n, 0, end_bit, s);
+  if ((st = hip_check(e, "radix_sort_pairs (tiles, size)"))) return st;
+  if ((st = d_tmp.alloc(bytes))) return st;
+  e = rocprim::radix_sort_pairs(d_tmp.p, bytes, in.key, d_key2.as<uint64_t>(), d_perm.as<int32_t>(),
+                                d_perm2.as<int32_t>(), (size_t)n, 0, end_bit, s);
+  if ((st = hip_check(e, "radix_sort_pairs (tiles)"))) return st;
+  if ((st = hip_check(hipStreamSynchronize(s), "tile sort"))) return st;
+  d_tmp.release();
+  d_perm.release();
+  DevBuf d_flag, d_eid;
+  if ((st = d_flag.alloc(n * 4)) || (st = d_eid.alloc(n * 4))) return st;
+  hipLaunchKernelGGL(k_entry_flags, dim3(grid1(n, 256)), dim3(256), 0, s, d_key2.as<uint64_t>(), n, d_flag.as<int32_t>());
+  if ((st = inclusive_scan(d_flag.as<int32_t>(), d_eid.as<int32_t>(), (size_t)n, s))) return st;
+  d_flag.release();
+  int32_t ne32 = 0;
+  if ((st = hip_check(hipMemcpyAsync(&ne32, d_eid.as<int32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, s), "D2H entries")) ||
+      (st = hip_check(hipStreamSynchronize(s), "entry count")))
+    return st;
+  const int64_t ne = ne32;
+  tp.n_entries = ne;
+  DevBuf d_ep, d_eq, d_etile, d_ex, d_cnt, d_emask, d_esz, d_elbeg, d_ecoff;
+  if ((st = d_ep.alloc(ne * 8 * 8)) || (st = d_eq.alloc(ne * 8 * 8)) || (st = d_etile.alloc(ne * 4)) ||
+      (st = d_ex.alloc(ne * 4)) || (st = d_cnt.alloc(ne * 8)))
+    return st;
+  if ((st = hip_check(hipMemsetAsync(d_ep.p, 0xff, ne * 8 * 8, s), "memset ep")) ||
+      (st = hip_check(hipMemsetAsync(d_eq.p, 0xff, ne * 8 * 8, s), "memset eq")))
+    return st;
+  hipLaunchKernelGGL(k_entry_fill_mx, dim3(grid1(n, 256)), dim3(256), 0, s, d_key2.as<uint64_t>(), d_perm2.as<int32_t>(),
+                     d_eid.as<int32_t>(), in.slot, in.p, in.q, n, in.nx, d_ep.as<int64_t>(), d_eq.as<int64_t>(),
+                     d_etile.as<int32_t>(), d_ex.as<int32_t>());
+  d_eid.release();
+  d_perm2.release();
+  d_key2.release();
+  if ((st = d_emask.alloc(ne * 4)) || (st = d_esz.alloc(ne * 4)) || (st = d_elbeg.alloc((ne + 1) * 8))) return st;
+  hipLaunchKernelGGL(k_entry_mask_mx, dim3(grid1(ne, 256)), dim3(256), 0, s, d_ep.as<int64_t>(), d_eq.as<int64_t>(),
+                     d_etile.as<int32_t>(), d_ex.as<int32_t>(), d_rdim.as<uint8_t>(), d_cdim.as<uint8_t>(),
+                     d_xdim.as<uint8_t>(), ne, d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_cnt.as<int64_t>());
+  if ((st = hip_check(hipMemsetAsync(d_elbeg.p, 0, 8, s), "memset lbeg"))) return st;
+  if ((st = inclusive_scan(d_cnt.as<int64_t>(), d_elbeg.as<int64_t>() + 1, (size_t)ne, s))) return st;
+  int64_t nl = 0;
+  if ((st = hip_check(hipMemcpyAsync(&nl, d_elbeg.as<int64_t>() + ne, 8, hipMemcpyDeviceToHost, s), "D2H loads")) ||
+      (st = hip_check(hipStreamSynchronize(s), "load count")))
+    return st;
+  tp.n_loads = nl;
+  DevBuf d_lsz;
+  if ((st = tp.list64.alloc(nl * 8)) || (st = d_lsz.alloc(nl * 2)) || (st = tp.listls.alloc(nl * 4))) return st;
+  hipLaunchKernelGGL(k_list_fill_mx, dim3(grid1(ne, 256)), dim3(256), 0, s, d_ep.as<int64_t>(), d_eq.as<int64_t>(),
+                     d_etile.as<int32_t>(), d_ex.as<int32_t>(), d_rdim.as<uint8_t>(), d_cdim.as<uint8_t>(),
+                     d_xdim.as<uint8_t>(), d_elbeg.as<int64_t>(), ne, tp.list64.as<uint64_t>(), d_lsz.as<uint16_t>());
+  d_ep.release();
+  d_eq.release();
+  // Chunks.
+  DevBuf d_tebeg, d_nch;
+  if ((st = d_tebeg.alloc(((size_t)in.n_tiles + 1) * 4)) || (st = d_nch.alloc((size_t)in.n_tiles * 4))) return st;
+  hipLaunchKernelGGL(k_tile_ebeg, dim3(grid1((int64_t)in.n_tiles + 1, 256)), dim3(256), 0, s, d_etile.as<int32_t>(), ne,
+                     in.n_tiles, d_tebeg.as<int32_t>());
+  hipLaunchKernelGGL((k_chunks_mx<false>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
+                     d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, nullptr,
+                     d_nch.as<int32_t>());
+  if ((st = tp.tile_chunk.alloc(((size_t)in.n_tiles + 1) * 4))) return st;
+  if ((st = hip_check(hipMemsetAsync(tp.tile_chunk.p, 0, 4, s), "memset tile_chunk"))) return st;
+  if ((st = inclusive_scan(d_nch.as<int32_t>(), tp.tile_chunk.as<int32_t>() + 1, (size_t)in.n_tiles, s))) return st;
+  int32_t nch = 0;
+  if ((st = hip_check(hipMemcpyAsync(&nch, tp.tile_chunk.as<int32_t>() + in.n_tiles, 4, hipMemcpyDeviceToHost, s),
+                      "D2H chunks")) ||
+      (st = hip_check(hipStreamSynchronize(s), "chunk count")))
+    return st;
+  tp.n_chunks = nch;
+  DevBuf d_centry;
+  if ((st = d_centry.alloc(((size_t)nch + 1) * 4))) return st;
+  hipLaunchKernelGGL((k_chunks_mx<true>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
+                     d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap,
+                     tp.tile_chunk.as<int32_t>(), d_centry.as<int32_t>());
+  if ((st = hip_check(hipMemcpyAsync(d_centry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
+                      "H2D chunk end")))
+    return st;
+  if ((st = tp.chunk_lbeg.alloc(((size_t)nch + 1) * 8))) return st;
+  hipLaunchKernelGGL(k_chunk_lbeg, dim3(grid1((int64_t)nch + 1, 256)), dim3(256), 0, s, d_centry.as<int32_t>(),
+                     d_elbeg.as<int64_t>(), (int64_t)nch + 1, tp.chunk_lbeg.as<int64_t>());
+  if ((st = d_ecoff.alloc((size_t)ne * 4))) return st;
+  hipLaunchKernelGGL(k_entry_coff_mx, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, d_centry.as<int32_t>(),
+                     d_esz.as<int32_t>(), (int64_t)nch, d_ecoff.as<int32_t>());
+  hipLaunchKernelGGL(k_list_lds_mx, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, tp.chunk_lbeg.as<int64_t>(),
+                     d_lsz.as<uint16_t>(), (int64_t)nch, tp.listls.as<uint32_t>());
+  // Product lists.
+  DevBuf d_ctile, d_np, d_pbase;
+  if ((st = d_ctile.alloc((size_t)std::max(nch, 1) * 4)) || (st = d_np.alloc((size_t)std::max(nch, 1) * 8)) ||
+      (st = d_pbase.alloc(((size_t)nch + 1) * 8)))
+    return st;
+  hipLaunchKernelGGL(k_chunk_tile, dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, tp.tile_chunk.as<int32_t>(),
+                     in.n_tiles, d_ctile.as<int32_t>());
+  hipLaunchKernelGGL((k_chunk_prods_mx<false>), dim3(grid1(std::max(nch, 1), 256)), dim3(256), 0, s, nch,
+                     d_centry.as<int32_t>(), d_ctile.as<int32_t>(), d_emask.as<uint32_t>(), d_ecoff.as<int32_t>(),
+                     d_ex.as<int32_t>(), d_owner.as<uint8_t>(), d_rdim.as<uint8_t>(), d_cdim.as<uint8_t>(),
+                     d_xdim.as<uint8_t>(), d_np.as<int64_t>(), nullptr, nullptr, nullptr, nullptr);
+  if ((st = hip_check(hipMemsetAsync(d_pbase.p, 0, 8, s), "memset pbase"))) return st;
+  if ((st = inclusive_scan(d_np.as<int64_t>(), d_pbase.as<int64_t>() + 1, (size_t)nch, s))) return st;
+  int64_t nprod = 0;
+  if ((st = hip_check(hipMemcpyAsync(&nprod, d_pbase.as<int64_t>() + nch, 8, hipMemcpyDeviceToHost, s), "D2H products")) ||
+      (st = hip_check(hipStreamSynchronize(s), "product count")))
+    return st;
+  tp.n_prod = nprod;
+  if (nprod != n)
+    return set_errorf(CQMM_ERR_STATE, "tile plan: %lld listed products for %lld triplets", (long long)nprod,
+                      (long long)n);
+  if ((st = tp.pstart.alloc(((size_t)nch + 1) * kWaves * 8)) || (st = tp.pcnt.alloc(((size_t)nch + 1) * kWaves * 8)) ||
+      (st = tp.prod.alloc(((size_t)nprod + kWaveProducts) * 4)))
+    return st;
+  std::vector<int64_t> tail_start(kWaves, nprod);
+  if ((st = hip_check(hipMemsetAsync(tp.pcnt.as<uint64_t>() + (size_t)nch * kWaves, 0, kWaves * 8, s), "memset pcnt")) ||
+      (st = hip_check(hipMemcpyAsync(tp.pstart.as<int64_t>() + (size_t)nch * kWaves, tail_start.data(), kWaves * 8,
+                                     hipMemcpyHostToDevice, s), "H2D pstart tail")) ||
+      (st = hip_check(hipMemsetAsync(tp.prod.as<uint32_t>() + nprod, 0, kWaveProducts * 4, s), "memset prod tail")))
+    return st;
+  hipLaunchKernelGGL((k_chunk_prods_mx<true>), dim3(grid1(std::max(nch, 1), 256)), dim3(256), 0, s, nch,
+                     d_centry.as<int32_t>(), d_ctile.as<int32_t>(), d_emask.as<uint32_t>(), d_ecoff.as<int32_t>(),
+                     d_ex.as<int32_t>(), d_owner.as<uint8_t>(), d_rdim.as<uint8_t>(), d_cdim.as<uint8_t>(),
+                     d_xdim.as<uint8_t>(), nullptr, d_pbase.as<int64_t>(), tp.pstart.as<int64_t>(),
+                     tp.pcnt.as<uint64_t>(), tp.prod.as<uint32_t>());
+  if ((st = hip_check(hipGetLastError(), "mixed tile builder launch"))) return st;
+  return hip_check(hipStreamSynchronize(s), "mixed tile build");
 }
 
 }  // namespace cq

@@ -180,19 +180,54 @@ def test_tile_kernel_matches_oracle_and_triplet_kernel(natoms, nsf, ra, rb, rc, 
         np.testing.assert_array_equal(got[cqmm.KERNEL_TILE], got[cqmm.KERNEL_TRIPLET])
 
 
-def test_mixed_block_sizes_have_no_tile_schedule():
-    case = Case(40, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1, seed=4)
+MIXED_TILE_CASES = [
+    (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),
+    (48, ((4, .5), (9, .25), (13, .25)), 7.0, 9.0, 6.0, 2),
+    (40, ((16, .3), (4, .4), (9, .3)), 6.0, 7.0, 9.0, 1),    # 16x16 blocks: the larger stage
+    (40, ((16, .3), (4, .4), (9, .3)), 6.0, 6.0, 6.0, 2),
+    (40, 5, 6.0, 6.0, 12.0, 1),                              # a uniform size without a fixed-size instance
+    (160, ((4, .5), (9, .25), (13, .25)), 10.0, 12.0, 8.0, 2),  # many tiles, multi-chunk entries
+]
+
+
+@pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", MIXED_TILE_CASES)
+def test_mixed_size_tile_kernel_matches_oracle_and_triplet_kernel(natoms, nsf, ra, rb, rc, mt):
+    """Mixed block sizes (and uniform sizes without a fixed-size instance) get
+    the mixed-size tile schedule: operands packed in LDS by size, per-product
+    contraction dimension. Both FP64 kernels meet the 1e-12 bar against the
+    oracle, overwrite every output element, and agree with each other to 1e-13."""
+    case = Case(natoms, nsf, ra, rb, rc, mt, seed=37)
+    print(f"mixed tile case natoms={natoms} nsf={np.unique(case.nsf)} ranges={ra},{rb},{rc} type={mt}", flush=True)
     plan = cqmm.Plan(case.desc)
     info = plan.info()
-    assert info.n_tiles == 0 and info.kernel == cqmm.KERNEL_TRIPLET
-    with pytest.raises(cqmm.CqmmError, match="no tile schedule"):
-        plan.set_kernel(cqmm.KERNEL_TILE)
+    assert info.n_tiles > 0 and info.kernel == cqmm.KERNEL_TILE
+    assert 0 < info.n_loads <= 2 * info.n_triplets
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    ref = c_r if mt == 1 else a_r
+    got = {}
+    for kernel in (cqmm.KERNEL_TILE, cqmm.KERNEL_TRIPLET):
+        plan.set_kernel(kernel)
+        x, y, z = a.copy(), b.copy(), c.copy()
+        out = z if mt == 1 else x
+        out[:] = np.nan
+        plan.mat_mult(x, y, z)
+        assert np.all(np.isfinite(out))
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        print(f"  kernel {kernel}: rel err {err:.3e} (tiles {info.n_tiles}, entries {info.n_entries}, "
+              f"loads/product {info.n_loads / info.n_triplets:.3f})", flush=True)
+        assert err <= TOL
+        assert np.all(out[ref == 0.0] == 0.0)
+        got[kernel] = out.copy()
+    d = np.abs(got[cqmm.KERNEL_TILE] - got[cqmm.KERNEL_TRIPLET]).max() / np.abs(ref).max()
+    assert d <= 1e-13
 
 
 FP32_CASES = [
     (64, 13, 6.0, 8.0, 12.0, 1),
     (64, 13, 8.0, 8.0, 12.0, 2),
-    (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),   # triplet kernels (mixed sizes)
+    (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),   # mixed-size tile kernel
 ]
 
 
@@ -202,7 +237,8 @@ def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
     (bar 2e-5 relative to the FP64 oracle: inputs rounded to FP32, sums of a few
     hundred terms in FP32) or, with CQMM_FP32_ACC64, each block's products summed
     in FP64 (bar 2e-6: only the inputs and each 13-term product are FP32). The tile
-    and triplet kernels sum type-1 blocks in the same order and agree bit for bit."""
+    and triplet kernels sum type-1 blocks of uniform size in the same order and agree bit
+    for bit."""
     case = Case(natoms, nsf, ra, rb, rc, mt, seed=41)
     print(f"fp32 case natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
     plan = cqmm.Plan(case.desc)
@@ -225,5 +261,5 @@ def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
             assert err <= bar
             assert np.all(out[ref == 0.0] == 0.0)
             got[kernel] = out.copy()
-        if mt == 1 and len(got) == 2:
+        if mt == 1 and len(got) == 2 and np.ndim(nsf) == 0:
             np.testing.assert_array_equal(got[cqmm.KERNEL_TILE], got[cqmm.KERNEL_TRIPLET])
