@@ -673,7 +673,8 @@ __global__ void k_list_lds_mx(const int64_t *__restrict__ chunk_lbeg, const uint
 }
 
 // Product lists (as k_chunk_prods): word = left LDS offset | right LDS offset
-// << 12 | (contraction dimension - 1) << 24.
+// << 12 | (contraction dimension nx - 1) << 24 | (k-steps - 1) << 28 |
+// (contraction indices of the last k-step - 1) << 30.
 template <bool FILL>
 __global__ void k_chunk_prods_mx(int32_t nch, const int32_t *__restrict__ chunk_entry,
                                  const int32_t *__restrict__ ctile, const uint32_t *__restrict__ mask,
@@ -733,7 +734,10 @@ __global__ void k_chunk_prods_mx(int32_t nch, const int32_t *__restrict__ chunk_
       for (uint32_t bm = bmask; bm; bm &= bm - 1u) {
         const int col = __builtin_ctz(bm);
         const uint8_t o = own[u * 8 + col];
-        if (o != 0xff) prod[pbase[c] + off[o]++] = aoff | (boff << 12) | ((nxd - 1u) << 24);
+        if (o != 0xff) {
+          const uint32_t ks = (nxd + 3u) >> 2, rem = nxd - 4u * (ks - 1u);
+          prod[pbase[c] + off[o]++] = aoff | (boff << 12) | ((nxd - 1u) << 24) | ((ks - 1u) << 28) | ((rem - 1u) << 30);
+        }
         boff += nxd * cdim[t * 8 + col];
       }
       aoff += rdim[t * 8 + u] * nxd;
@@ -857,28 +861,38 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_mixed(
         if (n == 0) continue;
         const uint32_t dm = (wdim[u >> 2] >> (8 * (u & 3))) & 0xffu;
         const int nrow = (int)(dm & 15u) + 1, ncol = (int)(dm >> 4) + 1;
-        const int oA = lr + nrow * lg, SA = 4 * nrow;
+        // Lanes past the block's rows (left) or columns (right) read the last
+        // valid row/column (a broadcast, no bank conflicts); they feed only
+        // accumulator rows/columns that are never written.
+        const int rA = min(lr, nrow - 1), rB = min(lr, ncol - 1);
+        const int oA = rA + nrow * lg, SA = 4 * nrow;
         for (int i = 0; i < n; ++i, ++p) {
           const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
           const int nx = (int)((v >> 24) & 15u) + 1;
-          const int ks = (nx + 3) >> 2, rem = nx - 4 * (ks - 1);
+          const int ks1 = (int)((v >> 28) & 3u), rem = (int)(v >> 30) + 1;
           const T *la = cur + (int)(v & 0xfffu) + oA;
-          const T *lq = cur + (int)((v >> 12) & 0xfffu) + (QT ? lr + ncol * lg : lg + nx * lr);
+          const T *lq = cur + (int)((v >> 12) & 0xfffu) + (QT ? rB + ncol * lg : lg + nx * rB);
           const int SB = QT ? 4 * ncol : 4;
-          const bool zl = lg >= rem;
           flt4 t = {0.f, 0.f, 0.f, 0.f};
+          // Full k-steps, then the last one (contraction indices >= nx zeroed).
 #pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4) {
-            if (s4 < ks) {
-              T a = la[SA * s4], bb = lq[SB * s4];
-              if (s4 == ks - 1 && zl) {
-                a = (T)0;
-                bb = (T)0;
-              }
+          for (int s4 = 0; s4 < 3; ++s4) {
+            if (s4 < ks1) {
+              const T a = la[SA * s4], bb = lq[SB * s4];
               if constexpr (MODE == 0) a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, a64[u], 0, 0, 0);
               else if constexpr (MODE == 1) a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, a32[u], 0, 0, 0);
               else t = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, t, 0, 0, 0);
             }
+          }
+          {
+            T a = la[SA * ks1], bb = lq[SB * ks1];
+            if (lg >= rem) {
+              a = (T)0;
+              bb = (T)0;
+            }
+            if constexpr (MODE == 0) a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, a64[u], 0, 0, 0);
+            else if constexpr (MODE == 1) a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, a32[u], 0, 0, 0);
+            else t = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, t, 0, 0, 0);
           }
           if constexpr (MODE == 2) {
 #pragma unroll
