@@ -1,7 +1,9 @@
 ! cqmm_f — iso_c_binding interface to libcqmm.so for a Fortran (CONQUEST) host.
 !
 ! Interoperable copies of the C descriptor types of include/cqmm.h and
-! include/cqmm_matrix.h, the bind(C) interfaces, and two Fortran-level
+! include/cqmm_matrix.h, the bind(C) interfaces (the drop-in multiply, the
+! one-process index builder, and the mult_module-style handle API with the
+! McWeeny, InitMcW, Hotelling, LNV and FindMinDM loops), and two Fortran-level
 ! wrappers a CONQUEST build would call:
 !   cq_gpu_mat_mult  — same argument list as mat_mult
 !                      (src/multiply_module.f90:115) with a prebuilt plan in
@@ -53,6 +55,50 @@ module cqmm_f
      integer(c_int32_t) :: kernel, n_tiles
      integer(c_int64_t) :: n_entries, n_loads
   end type cqmm_plan_info
+
+  ! type(matrix) index of one range, src/matrix_module.f90:63-82 (include/cqmm_matrix.h)
+  type, bind(c) :: cqmm_matrix_index
+     integer(c_int32_t) :: n_parts
+     type(c_ptr) :: n_atoms, nab_offset, nd_offset
+     integer(c_int32_t) :: n_prim
+     type(c_ptr) :: n_nab, i_acc, i_nd_acc, ndimi, onsite, i_part, i_seq, ndimj
+     integer(c_int64_t) :: n_nabs, length
+  end type cqmm_matrix_index
+
+  type, bind(c) :: cqmm_hotelling_setup
+     integer(c_int32_t) :: range_T, range_S, range_TS
+     integer(c_int32_t) :: mult_T_S_TS, mult_TS_T_T
+     integer(c_int32_t) :: max_iter
+     real(c_double) :: tol
+  end type cqmm_hotelling_setup
+
+  type, bind(c) :: cqmm_initmcw_setup
+     integer(c_int32_t) :: range_T, range_S, range_H, range_L, range_TS, range_TH
+     integer(c_int32_t) :: mult_T_S_TS, mult_T_H_TH, mult_TH_T_L
+     real(c_double) :: n_electrons
+  end type cqmm_initmcw_setup
+
+  ! LNV_matrix_multiply's ranges and mult entries (src/mult_module.f90:1478-1823)
+  type, bind(c) :: cqmm_lnv_setup
+     integer(c_int32_t) :: range_L, range_S, range_H, range_LS, range_LH, range_LSL, range_SLS
+     integer(c_int32_t) :: mult_L_S_LS, mult_L_H_LH, mult_HL_S_LSL, mult_HLS_LS_L, mult_SL_S_SLS, &
+                           mult_SLS_LS_L, mult_LS_L_LSL, mult_LSL_SL_H
+  end type cqmm_lnv_setup
+
+  ! FindMinDM's parameters (src/DMMinModule.f90:172-312)
+  type, bind(c) :: cqmm_dmm_setup
+     type(cqmm_lnv_setup) :: lnv
+     integer(c_int32_t) :: range_T, range_TL, mult_T_L_TL, mult_TL_T_L
+     integer(c_int32_t) :: n_L_iterations, vary_mu, maxpulay, global_tolerance, n_atoms
+     real(c_double) :: tolerance, min_pulay_step, max_pulay_step, lin_tol, electrons_target, delta_e
+  end type cqmm_dmm_setup
+
+  type, bind(c) :: cqmm_dmm_result
+     integer(c_int32_t) :: done, early_iterations, late_iterations, inflexions
+     real(c_double) :: energy, electrons, residual, seconds
+  end type cqmm_dmm_result
+
+  integer(c_int32_t), parameter :: CQMM_LNV_K = 1, CQMM_LNV_E = 2, CQMM_LNV_M3 = 4, CQMM_LNV_PHI = 8
 
   type, bind(c) :: cqmm_mcw_setup
      integer(c_int32_t) :: range_L, range_S, range_H, range_LS, range_LSL
@@ -149,6 +195,160 @@ module cqmm_f
        real(c_double), intent(out) :: values(*)
        integer(c_int) :: st
      end function cqmm_sys_fill
+     ! ---- mult_module handle API (include/cqmm_matrix.h) ----
+     function cqmm_sys_plan_create(sys, mult_type, ra, rb, rc, plan) bind(c, name="cqmm_sys_plan_create") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: sys
+       integer(c_int32_t), value :: mult_type, ra, rb, rc
+       type(c_ptr) :: plan
+       integer(c_int) :: st
+     end function cqmm_sys_plan_create
+     function cqmm_sys_matrix_index(sys, range_id, idx) bind(c, name="cqmm_sys_matrix_index") result(st)
+       import :: c_ptr, c_int, c_int32_t, cqmm_matrix_index
+       type(c_ptr), value :: sys
+       integer(c_int32_t), value :: range_id
+       type(cqmm_matrix_index) :: idx
+       integer(c_int) :: st
+     end function cqmm_sys_matrix_index
+     function cqmm_sys_transpose_pairs(sys, range_id, npairs, p, q, n1, n2) &
+         bind(c, name="cqmm_sys_transpose_pairs") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_int64_t
+       type(c_ptr), value :: sys
+       integer(c_int32_t), value :: range_id
+       integer(c_int64_t) :: npairs
+       type(c_ptr), value :: p, q, n1, n2
+       integer(c_int) :: st
+     end function cqmm_sys_transpose_pairs
+     function cqmm_ctx_create(ctx) bind(c, name="cqmm_ctx_create") result(st)
+       import :: c_ptr, c_int
+       type(c_ptr) :: ctx
+       integer(c_int) :: st
+     end function cqmm_ctx_create
+     function cqmm_ctx_destroy(ctx) bind(c, name="cqmm_ctx_destroy") result(st)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+       integer(c_int) :: st
+     end function cqmm_ctx_destroy
+     function cqmm_ctx_add_range(ctx, range_id, idx) bind(c, name="cqmm_ctx_add_range") result(st)
+       import :: c_ptr, c_int, c_int32_t, cqmm_matrix_index
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: range_id
+       type(cqmm_matrix_index), intent(in) :: idx
+       integer(c_int) :: st
+     end function cqmm_ctx_add_range
+     function cqmm_ctx_set_transpose(ctx, range_id, npairs, p, q, n1, n2) bind(c, name="cqmm_ctx_set_transpose") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: range_id
+       integer(c_int64_t), value :: npairs
+       type(c_ptr), value :: p, q, n1, n2
+       integer(c_int) :: st
+     end function cqmm_ctx_set_transpose
+     function cqmm_ctx_add_mult(ctx, mult_id, ra, rb, rc, plan) bind(c, name="cqmm_ctx_add_mult") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: mult_id, ra, rb, rc
+       type(c_ptr), value :: plan
+       integer(c_int) :: st
+     end function cqmm_ctx_add_mult
+     function cqmm_matrix_allocate(ctx, range_id, handle) bind(c, name="cqmm_matrix_allocate") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: range_id
+       integer(c_int32_t) :: handle
+       integer(c_int) :: st
+     end function cqmm_matrix_allocate
+     function cqmm_matrix_free(ctx, handle) bind(c, name="cqmm_matrix_free") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: handle
+       integer(c_int) :: st
+     end function cqmm_matrix_free
+     function cqmm_matrix_set(ctx, handle, host) bind(c, name="cqmm_matrix_set") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: handle
+       real(c_double), intent(in) :: host(*)
+       integer(c_int) :: st
+     end function cqmm_matrix_set
+     function cqmm_matrix_get(ctx, handle, host) bind(c, name="cqmm_matrix_get") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: handle
+       real(c_double) :: host(*)
+       integer(c_int) :: st
+     end function cqmm_matrix_get
+     function cqmm_matrix_product(ctx, a, b, c, mult_id) bind(c, name="cqmm_matrix_product") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: a, b, c, mult_id
+       integer(c_int) :: st
+     end function cqmm_matrix_product
+     function cqmm_matrix_sum(ctx, alpha, a, beta, b) bind(c, name="cqmm_matrix_sum") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: alpha, beta
+       integer(c_int32_t), value :: a, b
+       integer(c_int) :: st
+     end function cqmm_matrix_sum
+     function cqmm_matrix_product_trace(ctx, a, b, res) bind(c, name="cqmm_matrix_product_trace") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: a, b
+       real(c_double) :: res
+       integer(c_int) :: st
+     end function cqmm_matrix_product_trace
+     function cqmm_hotelling(ctx, setup, mats, matt, iterations, omega) bind(c, name="cqmm_hotelling") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double, cqmm_hotelling_setup
+       type(c_ptr), value :: ctx
+       type(cqmm_hotelling_setup), intent(in) :: setup
+       integer(c_int32_t), value :: mats, matt
+       integer(c_int32_t) :: iterations
+       real(c_double) :: omega
+       integer(c_int) :: st
+     end function cqmm_hotelling
+     function cqmm_init_mcweeny(ctx, setup, math, mats, matt, matl, mu_lambda) bind(c, name="cqmm_init_mcweeny") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double, cqmm_initmcw_setup
+       type(c_ptr), value :: ctx
+       type(cqmm_initmcw_setup), intent(in) :: setup
+       integer(c_int32_t), value :: math, mats, matt, matl
+       real(c_double) :: mu_lambda(2)
+       integer(c_int) :: st
+     end function cqmm_init_mcweeny
+     function cqmm_mcweeny(ctx, setup, math, mats, matl, res) bind(c, name="cqmm_mcweeny") result(st)
+       import :: c_ptr, c_int, c_int32_t, cqmm_mcw_setup, cqmm_mcw_result
+       type(c_ptr), value :: ctx
+       type(cqmm_mcw_setup), intent(in) :: setup
+       integer(c_int32_t), value :: math, mats, matl
+       type(cqmm_mcw_result) :: res
+       integer(c_int) :: st
+     end function cqmm_mcweeny
+     function cqmm_lnv(ctx, setup, flags, math, mats, matl, matk, matm3, matphi, electrons, energy) &
+         bind(c, name="cqmm_lnv") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double, cqmm_lnv_setup
+       type(c_ptr), value :: ctx
+       type(cqmm_lnv_setup), intent(in) :: setup
+       integer(c_int32_t), value :: flags, math, mats, matl, matk, matm3, matphi
+       real(c_double) :: electrons, energy
+       integer(c_int) :: st
+     end function cqmm_lnv
+     function cqmm_correct_electrons(ctx, setup, math, mats, matt, matl, electrons) &
+         bind(c, name="cqmm_correct_electrons") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double, cqmm_dmm_setup
+       type(c_ptr), value :: ctx
+       type(cqmm_dmm_setup), intent(in) :: setup
+       integer(c_int32_t), value :: math, mats, matt, matl
+       real(c_double) :: electrons
+       integer(c_int) :: st
+     end function cqmm_correct_electrons
+     function cqmm_dmm(ctx, setup, math, mats, matt, matl, matk, res) bind(c, name="cqmm_dmm") result(st)
+       import :: c_ptr, c_int, c_int32_t, cqmm_dmm_setup, cqmm_dmm_result
+       type(c_ptr), value :: ctx
+       type(cqmm_dmm_setup), intent(in) :: setup
+       integer(c_int32_t), value :: math, mats, matt, matl, matk
+       type(cqmm_dmm_result) :: res
+       integer(c_int) :: st
+     end function cqmm_dmm
   end interface
 
 contains

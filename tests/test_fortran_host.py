@@ -12,12 +12,15 @@ from cases import Case
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "conquest-release_amd", "fortran", "build", "cq_mult_driver")
+DM_DRIVER = os.path.join(ROOT, "conquest-release_amd", "fortran", "build", "cq_dm_driver")
 
 
 def test_fortran_host_is_built():
     assert os.path.exists(DRIVER), "run __graft_entry__.build() (flang builds the Fortran host)"
+    assert os.path.exists(DM_DRIVER)
     src = open(os.path.join(ROOT, "conquest-release_amd", "fortran", "cqmm_f.f90")).read()
-    assert "bind(c, name=\"cqmm_mat_mult\")" in src
+    for name in ("cqmm_mat_mult", "cqmm_ctx_add_mult", "cqmm_mcweeny", "cqmm_lnv", "cqmm_dmm"):
+        assert f"bind(c, name=\"{name}\")" in src
 
 
 @pytest.mark.gpu
@@ -38,3 +41,35 @@ def test_fortran_host_matches_oracle(tmp_path, args):
     assert ntrip == len(oracle.triplets(case.desc)[0])
     assert got.shape == ref.shape
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_fortran_density_matrix_host_matches_python_host():
+    """The Fortran host drives Hotelling -> InitMcW -> McWMin -> FindMinDM through
+    the handle API; the same sequence from Python (cqmm.dm.Purifier, the same
+    system, ranges, mult table and parameters) gives the same iterations and
+    energies (same library calls; host arithmetic of H + 3S done without FMA in both)."""
+    import cqmm
+    from cqmm.dm import Purifier
+    natoms, nsf, seed = 24, 4, 5
+    out = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"cq_dm_driver_{os.getpid()}.txt")
+    print("fortran dm driver", natoms, nsf, seed, flush=True)
+    r = subprocess.run([DM_DRIVER, str(natoms), str(nsf), str(seed), out], capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stderr
+    lines = open(out).read().split("\n")
+    os.remove(out)
+    hot_it, mcw_it, mcw_e, mcw_n = lines[0].split()
+    done, early, late, e_band, n_el, resid = lines[1].split()
+    xyz, edge = cqmm.random_structure(natoms, 0.05, 2.0, seed)
+    pu = Purifier(xyz, edge, nsf, r_S=4.5, r_H=4.5, r_L=6.0, grid=(2, 2, 2), seed=seed, lnv=True)
+    ne = float(pu.n_orb // 2)
+    it, _ = pu.hotelling()
+    pu.init_mcweeny(ne)
+    mcw = pu.mcweeny(precision=0, max_iter=100)
+    res = pu.dmm(n_electrons=ne, tolerance=1e-7, n_L_iterations=100)
+    assert int(hot_it) == it and int(mcw_it) == mcw["iterations"]
+    assert abs(float(mcw_e) - mcw["energy"]) <= 1e-12 * abs(mcw["energy"])
+    assert int(done) == res["done"] == 1
+    assert (int(early), int(late)) == (res["early_iterations"], res["late_iterations"])
+    assert abs(float(e_band) - res["band_energy"]) <= 1e-12 * abs(res["band_energy"])
+    assert abs(float(n_el) - res["n_electrons"]) <= 1e-12 * abs(res["n_electrons"])
