@@ -239,6 +239,17 @@ def main():
         log(f"calibration copy: {n * 8} bytes read, {n * 8} bytes written")
         src.free()
         dst.free()
+    tune = None
+    if args.kernel == "auto":
+        # Kernel autotuning on these operands: the faster of the tile and per-block kernels stays.
+        ms_tile, ms_trip = plan.tune_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, reps=2)
+        info = plan.info()
+        mixed = not isinstance(w["nsf"], int)
+        kname = (("k_tile_mixed_f64" if mixed else "k_tile_products_f64") if info.kernel == cqmm.KERNEL_TILE
+                 else ("k_block_products_f64" if mixed else "k_block_products_f64_u"))
+        tune = {"tile_ms": None if ms_tile is None else round(ms_tile, 3), "per_block_ms": round(ms_trip, 3),
+                "chosen": kname}
+        log(f"autotune: tile {ms_tile} ms, per-block {ms_trip:.3f} ms -> {kname}")
     for _ in range(args.warmup):
         plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC)
     cqmm.synchronize()
@@ -297,7 +308,7 @@ def main():
             "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
             "flop_per_step": flops, "output_blocks": int(info.n_out_total),
             "plan_build_s": round(t_plan, 3), "setup_s": round(setup_s, 2),
-            "kernel": kname, "tiles": int(info.n_tiles), "fp32": fp32,
+            "kernel": kname, "tiles": int(info.n_tiles), "autotune": tune, "fp32": fp32,
             "operand_blocks_per_product": round(info.n_loads / max(info.n_triplets, 1), 4)
             if info.kernel == cqmm.KERNEL_TILE else 2.0,
         },

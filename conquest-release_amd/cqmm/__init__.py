@@ -143,7 +143,7 @@ EXPORTS = [
     "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
     "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_device_copy", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
     "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_plan_get_info",
-    "cqmm_plan_get_triplets", "cqmm_plan_set_kernel",
+    "cqmm_plan_get_triplets", "cqmm_plan_set_kernel", "cqmm_plan_tune",
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
     "cqmm_sys_fill", "cqmm_random_structure", "cqmm_sys_plan_create", "cqmm_sys_matrix_index",
@@ -189,6 +189,8 @@ def lib():
     L.cqmm_plan_get_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
     L.cqmm_plan_get_triplets.argtypes = [C.c_void_p, i64p, i64p, i64p]
     L.cqmm_plan_set_kernel.argtypes = [C.c_void_p, C.c_int32]
+    L.cqmm_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                 C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.cqmm_mat_mult.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                 C.c_int64, C.c_uint32]
     L.cqmm_device_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_int64]
@@ -421,6 +423,14 @@ class Plan:
     def set_kernel(self, kernel):
         """KERNEL_AUTO, KERNEL_TRIPLET or KERNEL_TILE for the FP64 products."""
         check(lib().cqmm_plan_set_kernel(self.h, int(kernel)), "cqmm_plan_set_kernel")
+
+    def tune_dev(self, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags=CQMM_DEVICE_PTRS, reps=2):
+        """Time the plan's kernels on these device operands and keep the faster;
+        returns (ms_tile or None, ms_triplet)."""
+        mt, mp = C.c_float(), C.c_float()
+        check(lib().cqmm_plan_tune(self.h, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags | CQMM_DEVICE_PTRS, reps,
+                                   C.byref(mt), C.byref(mp)), "cqmm_plan_tune")
+        return (mt.value if mt.value >= 0 else None), mp.value
 
     def triplets(self):
         n = self.info().n_triplets

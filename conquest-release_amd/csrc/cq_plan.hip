@@ -822,6 +822,51 @@ int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel) {
   return CQMM_OK;
 }
 
+int cqmm_plan_tune(cqmm_plan *plan, double *a, int64_t lena, const double *b, int64_t lenb, double *c,
+                   int64_t lenc, uint32_t flags, int32_t reps, float *ms_tile, float *ms_triplet) {
+  if (!plan || !ms_tile || !ms_triplet || reps < 1)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_tune: bad argument");
+  int st = cq::ensure_init();
+  if (st) return st;
+  hipStream_t s = cq::stream();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((st = cq::hip_check(hipEventCreate(&e0), "event")) || (st = cq::hip_check(hipEventCreate(&e1), "event"))) {
+    if (e0) (void)hipEventDestroy(e0);
+    return st;
+  }
+  auto time_kernel = [&](int32_t kernel, float *ms) -> int {
+    plan->tp.kernel = kernel;
+    int r = cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
+    if (r) return r;
+    if ((r = cq::hip_check(hipEventRecord(e0, s), "event record"))) return r;
+    for (int32_t k = 0; k < reps; ++k)
+      if ((r = cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags | CQMM_NO_SYNC))) return r;
+    if ((r = cq::hip_check(hipEventRecord(e1, s), "event record")) ||
+        (r = cq::hip_check(hipEventSynchronize(e1), "event sync")) ||
+        (r = cq::hip_check(hipEventElapsedTime(ms, e0, e1), "event time")))
+      return r;
+    *ms /= (float)reps;
+    return CQMM_OK;
+  };
+  *ms_tile = -1.0f;
+  *ms_triplet = -1.0f;
+  const int32_t before = plan->tp.kernel;
+  st = time_kernel(CQMM_KERNEL_TRIPLET, ms_triplet);
+  if (!st && plan->tp.tiles.n_tiles > 0) st = time_kernel(CQMM_KERNEL_TILE, ms_tile);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {
+    plan->tp.kernel = before;
+    return st;
+  }
+  plan->tp.kernel = (*ms_tile >= 0.0f && *ms_tile <= *ms_triplet) ? CQMM_KERNEL_TILE : CQMM_KERNEL_TRIPLET;
+  // The output must hold a result: re-run when the last timed kernel is not the chosen one
+  // (both are correct; this keeps the call's result from the kernel that stays).
+  if (plan->tp.kernel == CQMM_KERNEL_TRIPLET && *ms_tile >= 0.0f)
+    return cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
+  return CQMM_OK;
+}
+
 int cqmm_plan_get_triplets(const cqmm_plan *plan, int64_t *out, int64_t *p, int64_t *q) {
   if (!plan || !out || !p || !q) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_triplets: null argument");
   const cq::TripletPlan &tp = plan->tp;

@@ -167,6 +167,16 @@ int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info);
 #define CQMM_KERNEL_TRIPLET 1
 #define CQMM_KERNEL_TILE 2
 int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel);
+/* Kernel autotuning on the caller's operands (no reference counterpart: the
+ * reference picks its kernel at compile time, MULT_KERN in src/matrix.obj):
+ * runs each kernel the plan has (tile schedule if built, per-output-block
+ * always) once, then `reps` more times timed with HIP events on the library
+ * stream, and fixes the faster one as the plan's kernel. Same arguments and
+ * overwrite semantics as cqmm_mat_mult (the output holds the last run's,
+ * correct, result). Average milliseconds per call go to *ms_tile (-1 when
+ * the plan has no tile schedule) and *ms_triplet. */
+int cqmm_plan_tune(cqmm_plan *plan, double *a, int64_t lena, const double *b, int64_t lenb, double *c,
+                   int64_t lenc, uint32_t flags, int32_t reps, float *ms_tile, float *ms_triplet);
 /* Copy the triplet list to host memory, in the order the kernel consumes it
  * (grouped by output block, each group in the reference's summation order).
  * Each triplet is (out, p, q): 1-based offsets of the output block, the left

@@ -263,3 +263,23 @@ def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
             got[kernel] = out.copy()
         if mt == 1 and len(got) == 2 and np.ndim(nsf) == 0:
             np.testing.assert_array_equal(got[cqmm.KERNEL_TILE], got[cqmm.KERNEL_TRIPLET])
+
+
+@pytest.mark.parametrize("nsf,mt", [(13, 2), (((4, .5), (9, .25), (13, .25)), 1)])
+def test_plan_tune_keeps_the_faster_kernel_and_a_correct_result(nsf, mt):
+    case = Case(64, nsf, 6.0, 8.0, 12.0, mt, seed=43)
+    plan = cqmm.Plan(case.desc)
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    ref = c_r if mt == 1 else a_r
+    da, db, dc = (cqmm.DeviceArray.from_host(x) for x in (a, b, c))
+    ms_tile, ms_trip = plan.tune_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, reps=2)
+    print(f"tune: tile {ms_tile} ms, per-block {ms_trip} ms -> kernel {plan.info().kernel}", flush=True)
+    assert ms_tile is not None and ms_tile > 0 and ms_trip > 0
+    want = cqmm.KERNEL_TILE if ms_tile <= ms_trip else cqmm.KERNEL_TRIPLET
+    assert plan.info().kernel == want
+    out = (dc if mt == 1 else da).to_host()
+    assert np.abs(out - ref).max() <= TOL * np.abs(ref).max()
+    for x in (da, db, dc):
+        x.free()
