@@ -147,7 +147,7 @@ EXPORTS = [
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
     "cqmm_sys_fill", "cqmm_random_structure", "cqmm_sys_plan_create", "cqmm_sys_matrix_index",
-    "cqmm_sys_transpose_pairs",
+    "cqmm_sys_transpose_pairs", "cqmm_matrix2_file_name", "cqmm_sys_dump_matrix2", "cqmm_sys_load_matrix2",
     "cqmm_ctx_create", "cqmm_ctx_destroy", "cqmm_ctx_add_range", "cqmm_ctx_set_transpose", "cqmm_ctx_add_mult",
     "cqmm_matrix_allocate", "cqmm_matrix_free", "cqmm_matrix_length", "cqmm_matrix_range",
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
@@ -183,6 +183,10 @@ def lib():
     L.cqmm_sys_blocks.argtypes = [C.c_void_p, C.c_int32, i32p, i32p, i32p, i32p, i64p, i32p, i32p]
     L.cqmm_sys_primary_order.argtypes = [C.c_void_p, i32p]
     L.cqmm_sys_fill.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.c_uint64, f64p, C.c_int32, f64p]
+    L.cqmm_matrix2_file_name.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    L.cqmm_sys_dump_matrix2.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(f64p), C.c_char_p, C.c_int32]
+    L.cqmm_sys_load_matrix2.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(f64p),
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.cqmm_init.argtypes = [C.c_int]
     L.cqmm_plan_create.argtypes = [C.POINTER(MultDesc), C.POINTER(C.c_void_p)]
     L.cqmm_plan_destroy.argtypes = [C.c_void_p]
@@ -236,6 +240,13 @@ def check(status, what=""):
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def matrix2_file_name(root, index=0, inode=1):
+    """dump_matrix2's file name: <root>matrix2.i<index>.p<inode>."""
+    buf = C.create_string_buffer(4096)
+    check(lib().cqmm_matrix2_file_name(str(root).encode(), index, inode, buf, 4096), "cqmm_matrix2_file_name")
+    return buf.value.decode()
 
 
 def random_structure(natoms, density=0.05, dmin=2.0, seed=1):
@@ -322,6 +333,25 @@ class System:
         out = np.zeros(self.natoms, np.int32)
         check(lib().cqmm_sys_primary_order(self.h, _p(out, C.c_int32)), "cqmm_sys_primary_order")
         return out
+
+    def dump_matrix2(self, rid, values, path, binary=True):
+        """Write range `rid` arrays (one per spin) in CONQUEST's dump_matrix2 format."""
+        arrs = [np.ascontiguousarray(v, np.float64) for v in values]
+        for a in arrs:
+            if a.size != self.length(rid):
+                raise CqmmError("dump_matrix2: length mismatch")
+        ptrs = (C.POINTER(C.c_double) * len(arrs))(*[_p(a, C.c_double) for a in arrs])
+        check(lib().cqmm_sys_dump_matrix2(self.h, rid, len(arrs), ptrs, str(path).encode(), int(bool(binary))),
+              "cqmm_sys_dump_matrix2")
+
+    def load_matrix2(self, rid, path, binary=True, nspin_max=2):
+        """Read a dump_matrix2 file into range `rid`: (list of arrays, blocks loaded, blocks skipped)."""
+        arrs = [np.zeros(self.length(rid), np.float64) for _ in range(nspin_max)]
+        ptrs = (C.POINTER(C.c_double) * nspin_max)(*[_p(a, C.c_double) for a in arrs])
+        ns, nl, nk = C.c_int32(), C.c_int64(), C.c_int64()
+        check(lib().cqmm_sys_load_matrix2(self.h, rid, str(path).encode(), int(bool(binary)), nspin_max, ptrs,
+                                          C.byref(ns), C.byref(nl), C.byref(nk)), "cqmm_sys_load_matrix2")
+        return arrs[:ns.value], nl.value, nk.value
 
     def fill(self, rid, kind=0, seed=1, tag=0, params=(2.0,)):
         v = np.zeros(self.length(rid), np.float64)

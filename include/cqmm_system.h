@@ -107,6 +107,26 @@ int cqmm_sys_fill(const cqmm_system *sys, int32_t range_id, int32_t kind,
 /* Synthetic structure of the workloads: natoms uniform in a cubic periodic box
  * of edge (natoms/density)^(1/3) with minimum-image separation >= dmin
  * (rejection sampling, seeded). Writes xyz[3*natoms]; returns the edge. */
+/* ---- Matrix files (CONQUEST's store_matrix format, one process) ----
+ * File name of dump_matrix2 / grab_matrix2 (get_file_name_2rank,
+ * src/io_module.f90:3134-3164): <root>matrix2.i<index:2>.p<inode:6>. */
+int cqmm_matrix2_file_name(const char *root, int32_t index, int32_t inode, char *out, int32_t out_len);
+/* dump_matrix2 (src/store_matrix_module.f90:340-470) of range `range_id`:
+ * header, per primary atom its global id (1-based), neighbours' support
+ * functions and global ids, (R_i - R_j image)/cell per neighbour, and the
+ * row's values for each of `nspin` arrays. binary: Fortran unformatted
+ * sequential records (4-byte markers); else list-directed text, values e30.20. */
+int cqmm_sys_dump_matrix2(const cqmm_system *sys, int32_t range_id, int32_t nspin, const double *const *values,
+                          const char *path, int32_t binary);
+/* Read such a file into range `range_id` (outputs are not cleared first):
+ * every file block whose atoms i, j and image (j at R_i - vec*cell) form a
+ * block of the range is copied there, as UpdateMatrix_local does
+ * (src/UpdateInfo_module.f90:1851-2162) for unmoved atoms; other file blocks
+ * are skipped and counted. */
+int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *path, int32_t binary,
+                          int32_t nspin_max, double *const *out, int32_t *nspin, int64_t *blocks_loaded,
+                          int64_t *blocks_skipped);
+
 int cqmm_random_structure(int32_t natoms, double density, double dmin,
                           uint64_t seed, double *xyz, double *edge);
 
