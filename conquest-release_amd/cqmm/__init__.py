@@ -138,6 +138,12 @@ class SysInfo(C.Structure):
                 ("ngrid", C.c_int32 * 3), ("box", C.c_double * 3)]
 
 
+class InfoGlobal(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("numprocs", C.c_int32), ("npcell", C.c_int32 * 3),
+                ("rcell", C.c_double * 3), ("md_step", C.c_int32), ("has_velocity", C.c_int32),
+                ("has_md_step", C.c_int32), ("index", C.c_int32)]
+
+
 # Symbols declared in include/cqmm.h and include/cqmm_system.h.
 EXPORTS = [
     "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
@@ -148,6 +154,8 @@ EXPORTS = [
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
     "cqmm_sys_fill", "cqmm_random_structure", "cqmm_sys_plan_create", "cqmm_sys_matrix_index",
     "cqmm_sys_transpose_pairs", "cqmm_matrix2_file_name", "cqmm_sys_dump_matrix2", "cqmm_sys_load_matrix2",
+    "cqmm_info_global_file_name", "cqmm_sys_dump_info_global", "cqmm_load_info_global",
+    "cqmm_matrix_file_name", "cqmm_dump_matrix_text", "cqmm_grab_matrix_text",
     "cqmm_ctx_create", "cqmm_ctx_destroy", "cqmm_ctx_add_range", "cqmm_ctx_set_transpose", "cqmm_ctx_add_mult",
     "cqmm_matrix_allocate", "cqmm_matrix_free", "cqmm_matrix_length", "cqmm_matrix_range",
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
@@ -187,6 +195,12 @@ def lib():
     L.cqmm_sys_dump_matrix2.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(f64p), C.c_char_p, C.c_int32]
     L.cqmm_sys_load_matrix2.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(f64p),
                                         C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.cqmm_info_global_file_name.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    L.cqmm_sys_dump_info_global.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, f64p]
+    L.cqmm_load_info_global.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(InfoGlobal), i32p, f64p, f64p]
+    L.cqmm_matrix_file_name.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+    L.cqmm_dump_matrix_text.argtypes = [C.c_char_p, f64p, C.c_int64]
+    L.cqmm_grab_matrix_text.argtypes = [C.c_char_p, f64p, C.c_int64]
     L.cqmm_init.argtypes = [C.c_int]
     L.cqmm_plan_create.argtypes = [C.POINTER(MultDesc), C.POINTER(C.c_void_p)]
     L.cqmm_plan_destroy.argtypes = [C.c_void_p]
@@ -247,6 +261,54 @@ def matrix2_file_name(root, index=0, inode=1):
     buf = C.create_string_buffer(4096)
     check(lib().cqmm_matrix2_file_name(str(root).encode(), index, inode, buf, 4096), "cqmm_matrix2_file_name")
     return buf.value.decode()
+
+
+def info_global_file_name(index=0):
+    """dump_InfoMatGlobal's file name: InfoGlobal.i<index>."""
+    buf = C.create_string_buffer(256)
+    check(lib().cqmm_info_global_file_name(index, buf, 256), "cqmm_info_global_file_name")
+    return buf.value.decode()
+
+
+def load_info_global(path, index=0, natoms_max=1 << 24):
+    """Read an InfoGlobal file: dict of its scalars plus glob_to_node, coords
+    (natoms, 3) and velocity (natoms, 3, or None when the file has none)."""
+    head = InfoGlobal()
+    check(lib().cqmm_load_info_global(str(path).encode(), index, 0, C.byref(head), None, None, None),
+          "cqmm_load_info_global")
+    natoms = head.natoms
+    if natoms > natoms_max:
+        raise CqmmError(f"load_info_global: {path} holds {natoms} atoms, room for {natoms_max}")
+    nodes = np.zeros(max(natoms, 1), np.int32)
+    xyz = np.zeros((max(natoms, 1), 3), np.float64)
+    vel = np.zeros((max(natoms, 1), 3), np.float64)
+    check(lib().cqmm_load_info_global(str(path).encode(), index, natoms, C.byref(head), _p(nodes, C.c_int32),
+                                      _p(xyz, C.c_double), _p(vel, C.c_double)), "cqmm_load_info_global")
+    n = head.natoms
+    return dict(natoms=n, numprocs=head.numprocs, npcell=tuple(head.npcell), rcell=tuple(head.rcell),
+                md_step=head.md_step, has_md_step=bool(head.has_md_step), index=head.index,
+                glob_to_node=nodes[:n].copy(), coords=xyz[:n].copy(),
+                velocity=vel[:n].copy() if head.has_velocity else None)
+
+
+def matrix_file_name(stub, numprocs=1, inode=1):
+    """dump_matrix's file name: <stub>matrix.<inode, zero-padded>."""
+    buf = C.create_string_buffer(256)
+    check(lib().cqmm_matrix_file_name(str(stub).encode(), numprocs, inode, buf, 256), "cqmm_matrix_file_name")
+    return buf.value.decode()
+
+
+def dump_matrix_text(path, values):
+    """dump_matrix: one f25.18 value per line, in storage order."""
+    v = np.ascontiguousarray(values, np.float64)
+    check(lib().cqmm_dump_matrix_text(str(path).encode(), _p(v, C.c_double), v.size), "cqmm_dump_matrix_text")
+
+
+def grab_matrix_text(path, length):
+    """grab_matrix: `length` values from a dump_matrix file."""
+    v = np.zeros(length, np.float64)
+    check(lib().cqmm_grab_matrix_text(str(path).encode(), _p(v, C.c_double), length), "cqmm_grab_matrix_text")
+    return v
 
 
 def random_structure(natoms, density=0.05, dmin=2.0, seed=1):
@@ -352,6 +414,17 @@ class System:
         check(lib().cqmm_sys_load_matrix2(self.h, rid, str(path).encode(), int(bool(binary)), nspin_max, ptrs,
                                           C.byref(ns), C.byref(nl), C.byref(nk)), "cqmm_sys_load_matrix2")
         return arrs[:ns.value], nl.value, nk.value
+
+    def dump_info_global(self, path, index=0, md_step=None, velocity=None):
+        """Write InfoGlobal (dump_InfoMatGlobal) for this system; velocity (natoms, 3) by global id."""
+        vp = None
+        if velocity is not None:
+            vel = np.ascontiguousarray(velocity, np.float64)
+            if vel.shape != (self.natoms, 3):
+                raise CqmmError("dump_info_global: velocity must be (natoms, 3)")
+            vp = _p(vel, C.c_double)
+        check(lib().cqmm_sys_dump_info_global(self.h, str(path).encode(), index, int(md_step is not None),
+                                              int(md_step or 0), vp), "cqmm_sys_dump_info_global")
 
     def fill(self, rid, kind=0, seed=1, tag=0, params=(2.0,)):
         v = np.zeros(self.length(rid), np.float64)

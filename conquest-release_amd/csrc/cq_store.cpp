@@ -19,9 +19,11 @@
 //     per spin, the atom's jbeta_max_i * nsf_spec_i values (one record each)
 // File name: <root>matrix2.i<index, 2 digits>.p<inode, 6 digits> (get_file_name_2rank,
 // src/io_module.f90:3134-3164); one process writes inode 1.
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -68,23 +70,53 @@ std::string ints_line(const int32_t *v, int64_t n) {
   return s + "\n";
 }
 
-// Fortran Ew.d with w = 30, d = 20: 0.<20 digits>E<sign><2 digits>, right-aligned.
-std::string e30_20(double x) {
-  char m[64], out[64];
+// Fortran Ew.d: [-]0.<d digits>E<sign><2 digits>, right-aligned in w columns; an
+// exponent beyond two digits drops the E (0.<d digits><sign><3 digits>), and a
+// field that does not fit is w asterisks, as a Fortran write produces.
+std::string e_format(double x, int w, int d) {
+  char m[64], out[80], pad[128];
   if (x == 0.0 || !std::isfinite(x)) {
-    snprintf(out, sizeof out, "%30.20E", x);
-    return std::string(out) + "\n";
+    snprintf(out, sizeof out, "%*.*E", w, d, x);
+  } else {
+    snprintf(m, sizeof m, "%.*E", d - 1, std::fabs(x));  // d.ddd...E+xx: d significant digits
+    const char *e = strchr(m, 'E');
+    const int ex = atoi(e + 1) + 1;
+    std::string digits;
+    for (const char *c = m; c < e; ++c)
+      if (*c != '.') digits += *c;
+    if (std::abs(ex) <= 99)
+      snprintf(out, sizeof out, "%s0.%sE%c%02d", x < 0 ? "-" : "", digits.c_str(), ex < 0 ? '-' : '+', std::abs(ex));
+    else
+      snprintf(out, sizeof out, "%s0.%s%c%03d", x < 0 ? "-" : "", digits.c_str(), ex < 0 ? '-' : '+', std::abs(ex));
   }
-  snprintf(m, sizeof m, "%.19E", std::fabs(x));  // d.ddd...E+xx: 20 significant digits
-  const char *e = strchr(m, 'E');
-  const int ex = atoi(e + 1) + 1;
-  std::string digits;
-  for (const char *c = m; c < e; ++c)
-    if (*c != '.') digits += *c;
-  snprintf(out, sizeof out, "%s0.%sE%c%02d", x < 0 ? "-" : "", digits.c_str(), ex < 0 ? '-' : '+', std::abs(ex));
-  char pad[128];
-  snprintf(pad, sizeof pad, "%30s\n", out);
+  if ((int)strlen(out) > w) return std::string((size_t)w, '*');
+  snprintf(pad, sizeof pad, "%*s", w, out);
   return pad;
+}
+
+std::string e30_20(double x) { return e_format(x, 30, 20) + "\n"; }
+
+// Fortran Fw.d; a value that does not fit is w asterisks.
+std::string f_format(double x, int w, int d) {
+  char out[400];
+  snprintf(out, sizeof out, "%*.*f", w, d, x);
+  if ((int)strlen(out) > w) return std::string((size_t)w, '*');
+  return out;
+}
+
+// A real as a Fortran read accepts it: D or d exponent letters, or an
+// exponent sign straight after the digits (0.123+100). False on anything else.
+bool fortran_real(std::string t, double *v) {
+  for (char &c : t)
+    if (c == 'D' || c == 'd' || c == 'e') c = 'E';
+  if (t.find('E') == std::string::npos) {
+    const size_t k = t.find_first_of("+-", 1);
+    if (k != std::string::npos) t.insert(k, "E");
+  }
+  if (t.empty() || t.find('*') != std::string::npos) return false;
+  char *end = nullptr;
+  *v = strtod(t.c_str(), &end);
+  return end && *end == '\0';
 }
 
 // Tokens of a formatted file or records of a binary one, read in order.
@@ -133,15 +165,106 @@ struct Reader {
       return true;
     }
     std::string t;
-    for (int64_t k = 0; k < n; ++k) {
-      if (!token(t)) return false;
-      for (char &c : t)
-        if (c == 'D' || c == 'd') c = 'E';
-      v[k] = strtod(t.c_str(), nullptr);
-    }
+    for (int64_t k = 0; k < n; ++k)
+      if (!token(t) || !fortran_real(t, &v[k])) return false;
     return true;
   }
 };
+
+// List-directed reads of a text file: each read takes its items from the
+// current line on (continuing onto the next lines when one runs short),
+// accepts r*c repeat counts, and then moves past the rest of the line, as a
+// Fortran list-directed READ does.
+struct ListReader {
+  std::vector<std::string> lines;
+  size_t line = 0;
+  bool open(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    std::string cur;
+    int ch;
+    while ((ch = fgetc(f)) != EOF) {
+      if (ch == '\n') {
+        lines.push_back(cur);
+        cur.clear();
+      } else if (ch != '\r') {
+        cur += (char)ch;
+      }
+    }
+    if (!cur.empty()) lines.push_back(cur);
+    fclose(f);
+    return true;
+  }
+  // The next n items as strings (repeat counts expanded).
+  bool items(std::vector<std::string> &out, size_t n) {
+    out.clear();
+    while (out.size() < n) {
+      if (line >= lines.size()) return false;
+      const std::string &l = lines[line++];
+      size_t k = 0;
+      while (out.size() < n && k < l.size()) {
+        while (k < l.size() && (l[k] == ' ' || l[k] == ',' || l[k] == '\t')) ++k;
+        if (k >= l.size()) break;
+        const size_t b = k;
+        while (k < l.size() && !(l[k] == ' ' || l[k] == ',' || l[k] == '\t')) ++k;
+        std::string t = l.substr(b, k - b);
+        const size_t star = t.find('*');
+        long rep = 1;
+        if (star != std::string::npos && star > 0) {
+          rep = strtol(t.substr(0, star).c_str(), nullptr, 10);
+          t = t.substr(star + 1);
+          if (rep < 1) return false;
+        }
+        for (long r = 0; r < rep && out.size() < n; ++r) out.push_back(t);
+      }
+    }
+    return true;
+  }
+  bool ints(int32_t *v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(t, n)) return false;
+    for (size_t k = 0; k < n; ++k) {
+      char *end = nullptr;
+      const long x = strtol(t[k].c_str(), &end, 10);
+      if (!end || *end) return false;
+      v[k] = (int32_t)x;
+    }
+    return true;
+  }
+  bool reals(double *v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(t, n)) return false;
+    for (size_t k = 0; k < n; ++k)
+      if (!fortran_real(t[k], &v[k])) return false;
+    return true;
+  }
+  // Logicals: T, F, .TRUE., .false. and the like (the letter after an optional dot).
+  bool logicals(int32_t *v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(t, n)) return false;
+    for (size_t k = 0; k < n; ++k) {
+      const std::string &s = t[k];
+      const char c = (char)toupper(s[0] == '.' && s.size() > 1 ? s[1] : s[0]);
+      if (c != 'T' && c != 'F') return false;
+      v[k] = c == 'T';
+    }
+    return true;
+  }
+  // The current line as is (an explicitly formatted record), then move past it.
+  bool record(std::string &out) {
+    if (line >= lines.size()) return false;
+    out = lines[line++];
+    return true;
+  }
+};
+
+// Fixed-column field [b, b + w) of a formatted record (blank past its end).
+std::string field(const std::string &rec, size_t b, size_t w) {
+  if (b >= rec.size()) return std::string();
+  std::string t = rec.substr(b, w);
+  const size_t s = t.find_first_not_of(' '), e = t.find_last_not_of(' ');
+  return s == std::string::npos ? std::string() : t.substr(s, e - s + 1);
+}
 
 struct FileCloser {
   FILE *f;
@@ -319,6 +442,143 @@ int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *
   if (nspin) *nspin = ns;
   if (blocks_loaded) *blocks_loaded = loaded;
   if (blocks_skipped) *blocks_skipped = skipped;
+  return CQMM_OK;
+}
+
+int cqmm_info_global_file_name(int32_t index, char *out, int32_t out_len) {
+  if (!out || out_len <= 0 || index < 0 || index > 99)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_info_global_file_name: bad argument");
+  const int n = snprintf(out, (size_t)out_len, "InfoGlobal.i%02d", index);
+  if (n < 0 || n >= out_len) return cq::set_error(CQMM_ERR_ARG, "cqmm_info_global_file_name: buffer too short");
+  return CQMM_OK;
+}
+
+int cqmm_sys_dump_info_global(const cqmm_system *sys, const char *path, int32_t index, int32_t has_md_step,
+                              int32_t md_step, const double *velocity) {
+  if (!sys || !path || index < 0 || index > 99)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: bad argument");
+  const System &s = sys->s;
+  FILE *f = fopen(path, "w");
+  if (!f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: cannot open %s", path);
+  FileCloser fc{f};
+  // Coordinates by global id (atom_coord(1:3, id_glob)); one process owns every atom.
+  std::vector<double> xyz(3 * (size_t)s.natoms);
+  for (int32_t p = 0; p < s.natoms; ++p) {
+    const size_t g = (size_t)s.prim_gid[p];
+    xyz[3 * g] = s.px[p];
+    xyz[3 * g + 1] = s.py[p];
+    xyz[3 * g + 2] = s.pz[p];
+  }
+  const int32_t step = has_md_step ? md_step : 0;
+  bool ok = fprintf(f, " %c %c  = flag_velocity, flag_MDstep\n", velocity ? 'T' : 'F', has_md_step ? 'T' : 'F') > 0;
+  ok = ok && fprintf(f, " %11d %11d  # of atoms, # of process \n", s.natoms, 1) > 0;
+  ok = ok && fprintf(f, " %11d %11d %11d  npcellx,y,z\n", s.ng[0], s.ng[1], s.ng[2]) > 0;
+  ok = ok && fprintf(f, "%s%s%s rcellx,y,z\n", f_format(s.box[0], 25, 15).c_str(), f_format(s.box[1], 25, 15).c_str(),
+                     f_format(s.box[2], 25, 15).c_str()) > 0;
+  std::string nodes;
+  for (int32_t g = 0; g < s.natoms && ok; ++g) nodes += "           1";
+  ok = ok && fprintf(f, "%s\n", nodes.c_str()) >= 0;
+  ok = ok && fprintf(f, " %11d  MD step\n", step) > 0;
+  auto atom_lines = [&](const double *v) {
+    for (int32_t g = 0; g < s.natoms && ok; ++g)
+      ok = fprintf(f, "     %8d   %s%s%s\n", g + 1, e_format(v[3 * (size_t)g], 22, 13).c_str(),
+                   e_format(v[3 * (size_t)g + 1], 22, 13).c_str(), e_format(v[3 * (size_t)g + 2], 22, 13).c_str()) > 0;
+  };
+  atom_lines(xyz.data());
+  if (velocity) {
+    ok = ok && fprintf(f, " velocity\n") > 0;
+    atom_lines(velocity);
+  }
+  ok = ok && fprintf(f, " %11d  = index_local in dump_InfoMatGlobal\n", index) > 0;
+  if (!ok) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: write to %s failed", path);
+  return CQMM_OK;
+}
+
+int cqmm_load_info_global(const char *path, int32_t index, int32_t natoms_max, cqmm_info_global *info,
+                          int32_t *glob_to_node, double *coords, double *velocity) {
+  if (!path || !info || natoms_max < 0 || index < 0)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_load_info_global: bad argument");
+  ListReader rd;
+  if (!rd.open(path)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: cannot open %s", path);
+  auto bad = [&](const char *what) {
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s: malformed or truncated (%s)", path, what);
+  };
+  cqmm_info_global g;
+  memset(&g, 0, sizeof g);
+  int32_t flags[2], counts[2];
+  if (!rd.logicals(flags, 2)) return bad("flag_velocity, flag_MDstep");
+  g.has_velocity = flags[0];
+  g.has_md_step = flags[1];
+  if (!rd.ints(counts, 2)) return bad("ni_in_cell, numprocs");
+  g.natoms = counts[0];
+  g.numprocs = counts[1];
+  if (g.natoms < 0) return bad("ni_in_cell");
+  if (g.natoms > natoms_max && (glob_to_node || coords || velocity))
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds %d atoms, room for %d", path, g.natoms,
+                          natoms_max);
+  if (!rd.ints(g.npcell, 3)) return bad("npcellx,y,z");
+  if (!rd.reals(g.rcell, 3)) return bad("rcellx,y,z");
+  std::vector<int32_t> nodes((size_t)g.natoms);
+  if (g.natoms && !rd.ints(nodes.data(), nodes.size())) return bad("glob_to_node");
+  if (!rd.ints(&g.md_step, 1)) return bad("MD step");
+  // (5x, i8, 3x, 3e22.13) records: global id in columns 6-13, values from column 17.
+  auto atom_lines = [&](double *v) {
+    std::string rec;
+    for (int32_t a = 0; a < g.natoms; ++a) {
+      if (!rd.record(rec)) return false;
+      double x[3];
+      for (int c = 0; c < 3; ++c)
+        if (!fortran_real(field(rec, 16 + 22 * (size_t)c, 22), &x[c])) return false;
+      if (v)
+        for (int c = 0; c < 3; ++c) v[3 * (size_t)a + c] = x[c];
+    }
+    return true;
+  };
+  if (!atom_lines(coords)) return bad("atom_coord");
+  if (g.has_velocity) {
+    std::string skip;
+    if (!rd.record(skip) || !atom_lines(velocity)) return bad("atom_veloc");
+  }
+  if (!rd.ints(&g.index, 1)) return bad("index");
+  if (g.index != index)
+    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds index %d, expected %d", path, g.index, index);
+  if (glob_to_node && g.natoms) memcpy(glob_to_node, nodes.data(), nodes.size() * 4);
+  *info = g;
+  return CQMM_OK;
+}
+
+int cqmm_matrix_file_name(const char *stub, int32_t numprocs, int32_t inode, char *out, int32_t out_len) {
+  if (!stub || !out || out_len <= 0 || numprocs < 1 || inode < 1)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_file_name: bad argument");
+  int digits = 0;
+  for (int32_t n = numprocs; n > 0; n /= 10) ++digits;
+  const int width = digits > 3 ? digits : 3;
+  const int n = snprintf(out, (size_t)out_len, "%smatrix.%0*d", stub, width, inode);
+  if (n < 0 || n >= out_len || strlen(stub) + 1 > 80 || n > 80)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_file_name: name too long");
+  return CQMM_OK;
+}
+
+int cqmm_dump_matrix_text(const char *path, const double *values, int64_t len) {
+  if (!path || len < 0 || (len && !values)) return cq::set_error(CQMM_ERR_ARG, "cqmm_dump_matrix_text: bad argument");
+  FILE *f = fopen(path, "w");
+  if (!f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_dump_matrix_text: cannot open %s", path);
+  FileCloser fc{f};
+  for (int64_t k = 0; k < len; ++k)
+    if (fprintf(f, "%s\n", f_format(values[k], 25, 18).c_str()) < 0)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_dump_matrix_text: write to %s failed", path);
+  return CQMM_OK;
+}
+
+int cqmm_grab_matrix_text(const char *path, double *values, int64_t len) {
+  if (!path || len < 0 || (len && !values)) return cq::set_error(CQMM_ERR_ARG, "cqmm_grab_matrix_text: bad argument");
+  ListReader rd;
+  if (!rd.open(path)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_grab_matrix_text: failed to open input file %s", path);
+  std::string rec;
+  for (int64_t k = 0; k < len; ++k)
+    if (!rd.record(rec) || !fortran_real(field(rec, 0, 25), &values[k]))
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_grab_matrix_text: %s: value %lld missing or unreadable", path,
+                            (long long)k + 1);
   return CQMM_OK;
 }
 

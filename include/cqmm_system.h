@@ -104,9 +104,6 @@ int cqmm_sys_fill(const cqmm_system *sys, int32_t range_id, int32_t kind,
                   uint64_t seed, uint64_t tag, const double *p, int32_t np,
                   double *values);
 
-/* Synthetic structure of the workloads: natoms uniform in a cubic periodic box
- * of edge (natoms/density)^(1/3) with minimum-image separation >= dmin
- * (rejection sampling, seeded). Writes xyz[3*natoms]; returns the edge. */
 /* ---- Matrix files (CONQUEST's store_matrix format, one process) ----
  * File name of dump_matrix2 / grab_matrix2 (get_file_name_2rank,
  * src/io_module.f90:3134-3164): <root>matrix2.i<index:2>.p<inode:6>. */
@@ -127,6 +124,50 @@ int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *
                           int32_t nspin_max, double *const *out, int32_t *nspin, int64_t *blocks_loaded,
                           int64_t *blocks_skipped);
 
+/* Global atom information written beside the matrix files
+ * (dump_InfoMatGlobal, src/store_matrix_module.f90:700-781; read back as
+ * grab_InfoMatGlobal does, :937-1040). */
+typedef struct cqmm_info_global {
+  int32_t natoms, numprocs;
+  int32_t npcell[3];     /* partitions along a, b, c */
+  double rcell[3];       /* cell lengths */
+  int32_t md_step;
+  int32_t has_velocity, has_md_step;
+  int32_t index;         /* the index written last in the file */
+} cqmm_info_global;
+/* File name InfoGlobal.i<index:2> (get_file_name_2rank without a process
+ * number, src/io_module.f90:3134-3164). */
+int cqmm_info_global_file_name(int32_t index, char *out, int32_t out_len);
+/* Write the file for this system as one process: flags, atom and process
+ * counts, partitions, cell (3f25.15), glob_to_node (all 1), MD step,
+ * coordinates by global id as (5x,i8,3x,3e22.13) records, velocities (same
+ * records, after a "velocity" line) when `velocity` (3*natoms, by global id)
+ * is not NULL, and `index`. has_md_step = 0 writes MD step 0 with the flag F. */
+int cqmm_sys_dump_info_global(const cqmm_system *sys, const char *path, int32_t index, int32_t has_md_step,
+                              int32_t md_step, const double *velocity);
+/* Read such a file (list-directed lines as a Fortran READ takes them,
+ * including r*c repeats; coordinate records by column). Fails, as
+ * grab_InfoMatGlobal aborts, when the index at its end is not `index`, and
+ * when it holds more than natoms_max atoms and an output array is given.
+ * glob_to_node[natoms], coords[3*natoms] and velocity[3*natoms] may each be
+ * NULL (all three NULL: a sizing call that fills `info` only); velocities are
+ * written only when the file holds them. */
+int cqmm_load_info_global(const char *path, int32_t index, int32_t natoms_max, cqmm_info_global *info,
+                          int32_t *glob_to_node, double *coords, double *velocity);
+
+/* The older per-process dump of a matrix's storage array (dump_matrix /
+ * grab_matrix, src/io_module.f90:1960-2075): one value per line as f25.18,
+ * in storage order. File name <stub>matrix.<inode zero-padded to
+ * max(3, digits of numprocs)> (get_file_name, src/io_module.f90:3071-3105). */
+int cqmm_matrix_file_name(const char *stub, int32_t numprocs, int32_t inode, char *out, int32_t out_len);
+int cqmm_dump_matrix_text(const char *path, const double *values, int64_t len);
+/* Reads `len` values; fails when the file holds fewer or a line is not a
+ * number (a value too wide for f25.18 is written as asterisks). */
+int cqmm_grab_matrix_text(const char *path, double *values, int64_t len);
+
+/* Synthetic structure of the workloads: natoms uniform in a cubic periodic box
+ * of edge (natoms/density)^(1/3) with minimum-image separation >= dmin
+ * (rejection sampling, seeded). Writes xyz[3*natoms]; returns the edge. */
 int cqmm_random_structure(int32_t natoms, double density, double dmin,
                           uint64_t seed, double *xyz, double *edge);
 
