@@ -53,6 +53,7 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;                    // wavefronts per workgroup (one tile)
 constexpr int kTileThreads = 64 * kWaves;
 constexpr int kSlots = 64 / kWaves;          // output blocks per wavefront
+static_assert(8 % kWaves == 0, "a tile row of 8 blocks is split evenly over the wavefronts");
 constexpr int kStageBytes = 24 * 1024;       // one of the two operand stages (three workgroups per CU)
 // LDS reads of a block product never mask rows or columns beyond NSF: those
 // lanes only feed accumulator rows/columns >= NSF, which are never written.
@@ -377,7 +378,7 @@ __global__ void k_tile_ebeg(const int32_t *__restrict__ etile, int64_t ne, int32
 
 // Wavefront products of entry e (tile t): rows u of the entry's left blocks x
 // columns of its right blocks whose output block exists (owner != 0xff);
-// owner byte = wavefront << 4 | slot.
+// owner byte = wavefront * kSlots + slot.
 __device__ inline void entry_wave_counts(uint32_t m, const uint8_t *__restrict__ own, int pw[kWaves]) {
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) pw[w] = 0;
@@ -385,7 +386,7 @@ __device__ inline void entry_wave_counts(uint32_t m, const uint8_t *__restrict__
     const int u = __builtin_ctz(ym);
     for (uint32_t bm = (m >> 8) & 0xffu; bm; bm &= bm - 1u) {
       const uint8_t o = own[u * 8 + __builtin_ctz(bm)];
-      if (o != 0xff) ++pw[o >> 4];
+      if (o != 0xff) ++pw[o / kSlots];
     }
   }
 }
@@ -402,7 +403,7 @@ __global__ void k_chunks(const int32_t *__restrict__ tebeg, const uint32_t *__re
   if (t >= n_tiles) return;
   const uint8_t *own = owner + (int64_t)t * 64;
   int32_t nch = 0, sum = 0, ne = 0;
-  int wc[kWaves] = {0, 0, 0, 0};
+  int wc[kWaves] = {};
   for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
     const uint32_t m = mask[e];
     const int32_t c = __popc(m);
@@ -623,7 +624,7 @@ __global__ void k_chunks_mx(const int32_t *__restrict__ tebeg, const uint32_t *_
   if (t >= n_tiles) return;
   const uint8_t *own = owner + (int64_t)t * 64;
   int32_t nch = 0, sum = 0, ne = 0;
-  int wc[kWaves] = {0, 0, 0, 0};
+  int wc[kWaves] = {};
   for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
     const uint32_t m = mask[e];
     const int32_t c = esz[e];
@@ -991,12 +992,13 @@ std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part
 }
 
 // Output slots to wavefronts, per tile: a Latin square over (tile row,
-// column) — with k = (row + perm[column]) % 8, wavefront k % 4 owns the block
-// as its slot 2*row + k / 4, so every wavefront owns two blocks of each tile row
-// and column, which spreads every entry's rectangle of products evenly — chosen
-// among kLatinTries column permutations to balance the wavefronts' total
-// product counts. owner[tile*64 + row*8 + col] = wavefront << 4 | slot, or
-// 0xff where the tile has no such output block.
+// column) — with k = (row + perm[column]) % 8, wavefront k % kWaves owns the
+// block as its slot (8 / kWaves) * row + k / kWaves, so every wavefront owns
+// 8 / kWaves blocks of each tile row and column, which spreads every entry's
+// rectangle of products evenly — chosen among kLatinTries column permutations
+// to balance the wavefronts' total product counts.
+// owner[tile*64 + row*8 + col] = wavefront * kSlots + slot, or 0xff where the
+// tile has no such output block.
 int assign_slots(const TileBuildInput &in, std::vector<uint8_t> &owner, hipStream_t s) {
   const int64_t n = in.n_trip;
   int st;
@@ -1025,8 +1027,8 @@ int assign_slots(const TileBuildInput &in, std::vector<uint8_t> &owner, hipStrea
     int best = 0;
     int64_t best_max = -1;
     for (int k = 0; k < kLatinTries; ++k) {
-      int64_t load[kWaves] = {0, 0, 0, 0};
-      for (int sl = 0; sl < 64; ++sl) load[((sl >> 3) + perms[k][sl & 7]) & (kWaves - 1)] += c[sl];
+      int64_t load[kWaves] = {};
+      for (int sl = 0; sl < 64; ++sl) load[(((sl >> 3) + perms[k][sl & 7]) & 7) % kWaves] += c[sl];
       int64_t m = 0;
       for (int w = 0; w < kWaves; ++w) m = std::max(m, load[w]);
       if (best_max < 0 || m < best_max) {
@@ -1038,7 +1040,7 @@ int assign_slots(const TileBuildInput &in, std::vector<uint8_t> &owner, hipStrea
       for (int col = 0; col < 8; ++col) {
         if (in.tile_out[t * 64 + u * 8 + col] < 0) continue;
         const int k = (u + perms[best][col]) & 7;
-        owner[t * 64 + u * 8 + col] = (uint8_t)((k & (kWaves - 1)) << 4 | (2 * u + (k >> 2)));
+        owner[t * 64 + u * 8 + col] = (uint8_t)((k % kWaves) * kSlots + (8 / kWaves) * u + k / kWaves);
       }
   }
   return CQMM_OK;
@@ -1071,7 +1073,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
     for (int64_t t = 0; t < in.n_tiles; ++t)
       for (int sl = 0; sl < 64; ++sl) {
         const uint8_t o = owner[t * 64 + sl];
-        if (o != 0xff) wave_out[(t * kWaves + (o >> 4)) * kSlots + (o & 15)] = tout32[t * 64 + sl];
+        if (o != 0xff) wave_out[t * 64 + o] = tout32[t * 64 + sl];
       }
     if ((st = d_owner.alloc(owner.size())) || (st = tp.wave_out.alloc(wave_out.size() * 4))) return st;
     if ((st = hip_check(hipMemcpyAsync(d_owner.p, owner.data(), owner.size(), hipMemcpyHostToDevice, s), "H2D owner")) ||
@@ -1232,7 +1234,7 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
       for (int sl = 0; sl < 64; ++sl) {
         const uint8_t o = owner[t * 64 + sl];
         if (o == 0xff) continue;
-        const int64_t k = (t * kWaves + (o >> 4)) * kSlots + (o & 15);
+        const int64_t k = t * 64 + o;
         wave_out[k] = in.tile_out[t * 64 + sl];
         const int nr = in.tile_rdim[t * 8 + (sl >> 3)], nc = in.tile_cdim[t * 8 + (sl & 7)];
         if (nr < 1 || nr > 16 || nc < 1 || nc > 16)
