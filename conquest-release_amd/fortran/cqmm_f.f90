@@ -113,6 +113,14 @@ module cqmm_f
      real(c_double) :: energy_trace(256)
   end type cqmm_mcw_result
 
+  ! InfoGlobal file contents (dump_InfoMatGlobal / grab_InfoMatGlobal).
+  type, bind(c) :: cqmm_info_global
+     integer(c_int32_t) :: natoms, numprocs
+     integer(c_int32_t) :: npcell(3)
+     real(c_double) :: rcell(3)
+     integer(c_int32_t) :: md_step, has_velocity, has_md_step, index
+  end type cqmm_info_global
+
   interface
      function cqmm_last_error() bind(c, name="cqmm_last_error") result(p)
        import :: c_ptr
@@ -349,6 +357,128 @@ module cqmm_f
        type(cqmm_dmm_result) :: res
        integer(c_int) :: st
      end function cqmm_dmm
+     ! Matrix operations of mult_module (matrix_scale, matrix_trace, symmetrise_L,
+     ! matrix_transpose) on context handles.
+     function cqmm_matrix_length(ctx, a, length) bind(c, name="cqmm_matrix_length") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: a
+       integer(c_int64_t) :: length
+       integer(c_int) :: st
+     end function cqmm_matrix_length
+     function cqmm_matrix_scale(ctx, alpha, a) bind(c, name="cqmm_matrix_scale") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: alpha
+       integer(c_int32_t), value :: a
+       integer(c_int) :: st
+     end function cqmm_matrix_scale
+     function cqmm_matrix_trace(ctx, a, res) bind(c, name="cqmm_matrix_trace") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: a
+       real(c_double) :: res
+       integer(c_int) :: st
+     end function cqmm_matrix_trace
+     function cqmm_matrix_symmetrise(ctx, a) bind(c, name="cqmm_matrix_symmetrise") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: a
+       integer(c_int) :: st
+     end function cqmm_matrix_symmetrise
+     function cqmm_matrix_transpose(ctx, a, b) bind(c, name="cqmm_matrix_transpose") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: a, b
+       integer(c_int) :: st
+     end function cqmm_matrix_transpose
+     ! Kernel choice for a plan: fixed, or timed on the caller's operands.
+     function cqmm_plan_set_kernel(plan, kernel) bind(c, name="cqmm_plan_set_kernel") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: plan
+       integer(c_int32_t), value :: kernel
+       integer(c_int) :: st
+     end function cqmm_plan_set_kernel
+     function cqmm_plan_tune(plan, a, lena, b, lenb, c, lenc, flags, reps, ms_tile, ms_triplet) &
+          bind(c, name="cqmm_plan_tune") result(st)
+       import :: c_ptr, c_int, c_int64_t, c_int32_t, c_float
+       type(c_ptr), value :: plan, a, b, c
+       integer(c_int64_t), value :: lena, lenb, lenc
+       integer(c_int32_t), value :: flags, reps
+       real(c_float) :: ms_tile, ms_triplet
+       integer(c_int) :: st
+     end function cqmm_plan_tune
+     ! Matrix files (store_matrix_module / io_module formats). Strings are
+     ! NUL-terminated: pass trim(name)//c_null_char.
+     function cqmm_matrix2_file_name(root, index, inode, out, out_len) bind(c, name="cqmm_matrix2_file_name") result(st)
+       import :: c_int, c_int32_t, c_char
+       character(kind=c_char), dimension(*), intent(in) :: root
+       integer(c_int32_t), value :: index, inode, out_len
+       character(kind=c_char), dimension(*) :: out
+       integer(c_int) :: st
+     end function cqmm_matrix2_file_name
+     function cqmm_sys_dump_matrix2(sys, range_id, nspin, values, path, binary) &
+          bind(c, name="cqmm_sys_dump_matrix2") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_char
+       type(c_ptr), value :: sys, values
+       integer(c_int32_t), value :: range_id, nspin, binary
+       character(kind=c_char), dimension(*), intent(in) :: path
+       integer(c_int) :: st
+     end function cqmm_sys_dump_matrix2
+     function cqmm_sys_load_matrix2(sys, range_id, path, binary, nspin_max, outp, nspin, blocks_loaded, &
+          blocks_skipped) bind(c, name="cqmm_sys_load_matrix2") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_int64_t, c_char
+       type(c_ptr), value :: sys, outp
+       integer(c_int32_t), value :: range_id, binary, nspin_max
+       character(kind=c_char), dimension(*), intent(in) :: path
+       integer(c_int32_t) :: nspin
+       integer(c_int64_t) :: blocks_loaded, blocks_skipped
+       integer(c_int) :: st
+     end function cqmm_sys_load_matrix2
+     function cqmm_info_global_file_name(index, out, out_len) bind(c, name="cqmm_info_global_file_name") result(st)
+       import :: c_int, c_int32_t, c_char
+       integer(c_int32_t), value :: index, out_len
+       character(kind=c_char), dimension(*) :: out
+       integer(c_int) :: st
+     end function cqmm_info_global_file_name
+     function cqmm_sys_dump_info_global(sys, path, index, has_md_step, md_step, velocity) &
+          bind(c, name="cqmm_sys_dump_info_global") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_char
+       type(c_ptr), value :: sys, velocity
+       character(kind=c_char), dimension(*), intent(in) :: path
+       integer(c_int32_t), value :: index, has_md_step, md_step
+       integer(c_int) :: st
+     end function cqmm_sys_dump_info_global
+     function cqmm_load_info_global(path, index, natoms_max, info, glob_to_node, coords, velocity) &
+          bind(c, name="cqmm_load_info_global") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_char, cqmm_info_global
+       character(kind=c_char), dimension(*), intent(in) :: path
+       integer(c_int32_t), value :: index, natoms_max
+       type(cqmm_info_global) :: info
+       type(c_ptr), value :: glob_to_node, coords, velocity
+       integer(c_int) :: st
+     end function cqmm_load_info_global
+     function cqmm_matrix_file_name(stub, numprocs, inode, out, out_len) bind(c, name="cqmm_matrix_file_name") result(st)
+       import :: c_int, c_int32_t, c_char
+       character(kind=c_char), dimension(*), intent(in) :: stub
+       integer(c_int32_t), value :: numprocs, inode, out_len
+       character(kind=c_char), dimension(*) :: out
+       integer(c_int) :: st
+     end function cqmm_matrix_file_name
+     function cqmm_dump_matrix_text(path, values, length) bind(c, name="cqmm_dump_matrix_text") result(st)
+       import :: c_int, c_int64_t, c_double, c_char
+       character(kind=c_char), dimension(*), intent(in) :: path
+       real(c_double), dimension(*), intent(in) :: values
+       integer(c_int64_t), value :: length
+       integer(c_int) :: st
+     end function cqmm_dump_matrix_text
+     function cqmm_grab_matrix_text(path, values, length) bind(c, name="cqmm_grab_matrix_text") result(st)
+       import :: c_int, c_int64_t, c_double, c_char
+       character(kind=c_char), dimension(*), intent(in) :: path
+       real(c_double), dimension(*) :: values
+       integer(c_int64_t), value :: length
+       integer(c_int) :: st
+     end function cqmm_grab_matrix_text
   end interface
 
 contains
