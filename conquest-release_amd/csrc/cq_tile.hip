@@ -226,6 +226,33 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
         const int n = (int)((cnt >> (4 * u)) & 15u);
         for (int i = 0; i < n; ++i, ++p) {
           const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
+          if constexpr (MODE == 0 && REM == 1) {
+            // NSF = 4m + 1: the last contraction index k = NSF - 1 is a
+            // rank-1 update on the VALU, D[g+4e][r] += A[g+4e][k] * B[k][r],
+            // instead of an MFMA step with three zeroed k (a quarter fewer
+            // MFMAs; accumulators in VGPRs, built with the VGPR MFMA form).
+            // Rows g+4e >= NSF read past the block (inside the stage and its
+            // pad) and only feed accumulator rows that are never written.
+            const T *la = cur + (int)(v & 0xffffu) + oA;
+            const T *lq = cur + (int)(v >> 16) + oB;
+            T a[C::KS - 1], bb[C::KS - 1];
+#pragma unroll
+            for (int s4 = 0; s4 < C::KS - 1; ++s4) {
+              a[s4] = la[SA * s4];
+              bb[s4] = lq[SB * s4];
+            }
+            const T *ca = cur + (int)(v & 0xffffu) + NSF * (NSF - 1) + g;
+            const T bl = QT ? cur[(int)(v >> 16) + NSF * (NSF - 1) + r] : cur[(int)(v >> 16) + (NSF - 1) + NSF * r];
+            T ce[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) ce[e] = ca[4 * e];
+#pragma unroll
+            for (int s4 = 0; s4 < C::KS - 1; ++s4)
+              a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], a64[u], 0, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) a64[u][e] = __builtin_fma(ce[e], bl, a64[u][e]);
+            continue;
+          }
           const T *la = cur + (int)(v & 0xffffu) + oA;
           const T *lq = cur + (int)(v >> 16) + oB;
           T a[C::KS], bb[C::KS];
