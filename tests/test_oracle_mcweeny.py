@@ -100,3 +100,38 @@ def test_sparse_symmetrise_matches_dense():
     M = mo.densify(bL, v, pu.nsf)
     got = mo.densify(bL, sp.symmetrise(v, "L"), pu.nsf)
     np.testing.assert_allclose(got, 0.5 * (M + M.T), atol=0)
+
+
+def test_hlimits_known_answer():
+    """GetHLimits (src/McWeeny.f90:1020-1109) on a two-atom H worked through by
+    hand from the reference loop. For primary atom a and each neighbour block,
+    HMin(i) / HMax(i) collect |H(j of a, i of the neighbour)| summed over j —
+    i is the neighbour's orbital (return_matrix_value(..., j, i) is element
+    (i-1)*nsf_a + j of the block, src/mult_module.f90:3016), added to both
+    for the onsite block, subtracted from HMin otherwise — and the extremes are
+    taken over i = 1..nsf_a only. Atoms a (2 orbitals), b (3); rows/columns
+    [a1 a2 b1 b2 b3]; the blocks are not symmetric so that a transposed block
+    or a row sum gives other numbers.
+      atom a: onsite columns |1|+|0.2| = 1.2, |0.5|+|-2| = 2.5; block ab
+              columns 0.1, 0.3, 0.7 -> HMin = [1.1, 2.2, (-0.7 not counted)],
+              HMax = [1.3, 2.8] -> min 1.1, max 2.8
+      atom b: block ba columns 0, 0.9 -> HMin = [-0, -0.9, .], onsite 3, 4, 5
+              -> HMin = [3, 3.1, 5], HMax = [3, 4.9, 5] -> min 3, max 5
+      result (1.1, 5.0); with row sums instead, atom a gives 0.7; with signed
+      onsite values, 2.2 becomes -1.8; with the extremes over all neighbour
+      orbitals, -0.7."""
+    H = np.zeros((5, 5))
+    H[:2, :2] = [[1.0, 0.5], [0.2, -2.0]]
+    H[2:, 2:] = np.diag([3.0, 4.0, 5.0])
+    H[:2, 2:] = [[0.1, 0.0, 0.7], [0.0, 0.3, 0.0]]
+    H[2:, :2] = [[0.0, 0.0], [0.0, 0.0], [0.0, 0.9]]
+    hmin, hmax = mo.hlimits_dense_blocks(H, [2, 3])
+    assert hmin == pytest.approx(1.1, abs=1e-15) and hmax == pytest.approx(5.0, abs=1e-15)
+
+
+def test_hlimits_diagonal_closed_form():
+    """A diagonal H: every orbital's sums hold only |H_ii|, so the limits are
+    the smallest and largest |H_ii| (the onsite block adds absolute values)."""
+    d = np.array([2.5, -0.75, 1.0, 4.0, -3.0, 0.5])
+    hmin, hmax = mo.hlimits_dense_blocks(np.diag(d), [2, 1, 3])
+    assert (hmin, hmax) == (0.5, 4.0)
