@@ -14,6 +14,9 @@ iteration (src/S_matrix_module.f90:746-1034):
     layout, products through the C oracle's mat_mult, range changes through
     block-identity maps (matrix_add's semantics: only blocks common to both
     patterns change).
+  * hlimits_dense_blocks (GetHLimits) — pinned by a two-atom known answer
+    worked by hand from the reference loop (non-symmetric blocks, mixed sizes)
+    and by the diagonal case (min/max of |H_ii|).
 """
 from __future__ import annotations
 
