@@ -90,6 +90,7 @@ struct TripletPlan {
   bool q_trans = false;     // right operand used transposed (type 2)
   bool off64 = false;       // operand offsets stored as int64
   int uniform = 0;          // support functions per atom when uniform, else 0
+  int maxn = 0;             // largest block dimension (up to 64; above 16: sub-block kernels)
   int64_t n_seg = 0;        // output blocks (every block of the output matrix)
   int64_t n_trip = 0;
   int64_t len_out = 0, len_p = 0, len_q = 0;

@@ -5,7 +5,9 @@
 // Type 2:  X_ik(n1,n3) += sum_n2 Y_ij(n1,n2) * B_kj(n3,n2)
 // Both are  O(m,n) = sum_t sum_q P_t(m,q) * Q_t(q,n)  with P column-major
 // (m fastest) and Q either column-major (q fastest, type 1) or used transposed
-// (n fastest, type 2). All blocks are at most 16x16.
+// (n fastest, type 2). Blocks are at most 64x64: up to 16x16 one wavefront
+// owns an output block; larger blocks are split into 16x16 sub-blocks, one
+// wavefront each (k_block_products_big).
 //
 // One wavefront owns one output block (a segment of the plan) and accumulates
 // its products with v_mfma_f64_16x16x4 in registers, then writes the block
@@ -27,6 +29,7 @@
 namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;  // wavefronts (output blocks) per workgroup
 
@@ -145,6 +148,79 @@ __global__ void __launch_bounds__(64 * kWaves) k_block_products_f64_u(
   }
 }
 
+// Blocks larger than 16x16 (up to 64x64): wavefront w of the grid takes
+// sub-block (w % nsub) of output block (w / nsub), rows 16*(st / nsc) .. +15 and
+// columns 16*(st % nsc) .. +15 (nsc = sub-blocks per row of the largest block,
+// nsub = nsc^2; sub-blocks past a smaller block leave at once). The
+// contraction runs over the product's full inner dimension, 4 per MFMA step,
+// in the same order for every sub-block. FP64, or FP32 operands with FP32
+// accumulation (ACC64 unset) or each product's FP32 result added in FP64.
+template <typename OffT, bool QT, typename T, bool ACC64>
+__global__ void __launch_bounds__(64 * kWaves) k_block_products_big(
+    const int64_t *__restrict__ seg_beg, const int64_t *__restrict__ seg_out, const uint8_t *__restrict__ seg_mn,
+    const OffT *__restrict__ tp, const OffT *__restrict__ tq, const uint8_t *__restrict__ tk,
+    const T *__restrict__ P, const T *__restrict__ Q, T *__restrict__ O, int64_t n_seg, int32_t nsc) {
+  const int lane = threadIdx.x & 63;
+  const int nsub = nsc * nsc;
+  const int64_t wv = (int64_t)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t seg = wv / nsub;
+  if (seg >= n_seg) return;
+  const int st = (int)(wv - seg * nsub);
+  const int r0 = 16 * (st / nsc), c0 = 16 * (st % nsc);
+  const int m = seg_mn[2 * seg], n = seg_mn[2 * seg + 1];
+  if (r0 >= m || c0 >= n) return;
+  const int r = lane & 15, g = lane >> 4;
+  const bool ra = r0 + r < m, rb = c0 + r < n;
+  const int64_t t0 = seg_beg[seg], t1 = seg_beg[seg + 1];
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  flt4 acc32 = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t p = (int64_t)tp[t], q = (int64_t)tq[t];
+    const int kk = (int)tk[t];
+    const int nsteps = (kk + 3) >> 2;
+    for (int s4 = 0; s4 < nsteps; ++s4) {
+      const int qi = 4 * s4 + g;
+      T a = (T)0, b = (T)0;
+      if (ra && qi < kk) a = P[p + (r0 + r) + (int64_t)m * qi];
+      if (rb && qi < kk) b = QT ? Q[q + (c0 + r) + (int64_t)n * qi] : Q[q + qi + (int64_t)kk * (c0 + r)];
+      if constexpr (sizeof(T) == 8) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      else acc32 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc32, 0, 0, 0);
+    }
+    if constexpr (sizeof(T) == 4 && ACC64) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        acc[e] += (double)acc32[e];
+        acc32[e] = 0.f;
+      }
+    }
+  }
+  if (!rb) return;
+  const int64_t o = seg_out[seg];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    // FP64 accumulator: D[(l/16)+4e][l%16]; FP32: D[4(l/16)+e][l%16].
+    const int row = r0 + (sizeof(T) == 8 ? g + 4 * e : 4 * g + e);
+    if (row < m) {
+      T v;
+      if constexpr (sizeof(T) == 8) v = acc[e];
+      else v = ACC64 ? (T)acc[e] : acc32[e];
+      O[o + row + (int64_t)m * (c0 + r)] = v;
+    }
+  }
+}
+
+template <typename OffT, bool QT, typename T, bool ACC64>
+int launch_big(const cq::TripletPlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
+  const int32_t nsc = (tp.maxn + 15) / 16;
+  const int64_t waves = tp.n_seg * (int64_t)nsc * nsc;
+  const int64_t grid = (waves + kWaves - 1) / kWaves;
+  if (grid > INT32_MAX) return cq::set_error(CQMM_ERR_ARG, "block products: too many output sub-blocks for one launch");
+  hipLaunchKernelGGL((k_block_products_big<OffT, QT, T, ACC64>), dim3((unsigned)grid), dim3(64 * kWaves), 0, s,
+                     tp.seg_beg.as<int64_t>(), tp.seg_out.as<int64_t>(), tp.seg_mn.as<uint8_t>(), tp.tp.as<OffT>(),
+                     tp.tq.as<OffT>(), tp.tk.as<uint8_t>(), P, Q, O, tp.n_seg, nsc);
+  return cq::hip_check(hipGetLastError(), "block-product kernel launch (large blocks)");
+}
+
 // CQMM_KERNEL=simple selects the one-stage triplet kernel for every block size
 // (used to measure what the pipelined, XCD-ordered and tiled kernels gain).
 bool force_general() {
@@ -157,6 +233,7 @@ bool force_general() {
 
 template <typename OffT, bool QT>
 int launch_t(const cq::TripletPlan &tp, const double *P, const double *Q, double *O, hipStream_t s) {
+  if (tp.maxn > 16) return launch_big<OffT, QT, double, false>(tp, P, Q, O, s);
   const unsigned grid = (unsigned)((tp.n_seg + kWaves - 1) / kWaves);
   const int64_t *sb = tp.seg_beg.as<int64_t>();
   const int64_t *so = tp.seg_out.as<int64_t>();
@@ -194,7 +271,6 @@ int launch_t(const cq::TripletPlan &tp, const double *P, const double *Q, double
 // FP32 variant (v_mfma_f32_16x16x4_f32, FP32 accumulate) for the bulk of the
 // mixed-precision purification. Same segment/XCD/pipeline structure as the
 // FP64 kernel; general block sizes use per-segment/per-product dimensions.
-typedef float flt4 __attribute__((ext_vector_type(4)));
 
 // ACC64: each product's FP32 MFMA result is added into FP64 registers (the
 // MFMA inner sums stay FP32; the sum over products of a block is FP64).
@@ -253,6 +329,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_block_products_f32(
 
 template <typename OffT, bool QT, bool ACC64>
 int launch_f32_t(const cq::TripletPlan &tp, const float *P, const float *Q, float *O, hipStream_t s) {
+  if (tp.maxn > 16) return launch_big<OffT, QT, float, ACC64>(tp, P, Q, O, s);
   const int32_t ngrp = (int32_t)((tp.n_seg + kWaves - 1) / kWaves);
   const unsigned gridx = 8u * (unsigned)((ngrp + 7) / 8);
   const int64_t *sb = tp.seg_beg.as<int64_t>();

@@ -344,11 +344,11 @@ int check_halo(const HaloV<H> &h, int32_t n_prim, const char *name) {
     return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s nh_part sums to %lld, ni_in_halo is %d", name,
                           (long long)(acc - 1), h.ni_in_halo);
   for (int32_t p = 0; p < n_prim; ++p)
-    if (h.ndimi[p] <= 0 || h.ndimi[p] > 16)
-      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s ndimi(%d)=%d outside 1..16", name, p + 1, h.ndimi[p]);
+    if (h.ndimi[p] <= 0 || h.ndimi[p] > 64)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s ndimi(%d)=%d outside 1..64", name, p + 1, h.ndimi[p]);
   for (int32_t k = 0; k < h.ni_in_halo; ++k)
-    if (h.ndimj[k] <= 0 || h.ndimj[k] > 16)
-      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s ndimj(%d)=%d outside 1..16", name, k + 1, h.ndimj[k]);
+    if (h.ndimj[k] <= 0 || h.ndimj[k] > 64)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_create: %s ndimj(%d)=%d outside 1..64", name, k + 1, h.ndimj[k]);
   return CQMM_OK;
 }
 
@@ -505,6 +505,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   tp.mult_type = d->mult_type;
   tp.q_trans = d->mult_type == 2;
   tp.uniform = uniform;
+  tp.maxn = maxn;
   tp.n_trip = n_trip;
   tp.flops = flops;
   tp.len_out = d->mult_type == 1 ? d->lenc : d->lena;

@@ -129,7 +129,8 @@ int cqmm_timer_stop(float *ms);
  * (src/matrix_comms_module.f90:67-103) and the B-partition slices of
  * prefetch / Mquest_get_local (src/multiply_module.f90:568-623,
  * src/matrix_comms_module.f90:130-169). Reused for every product with the
- * same patterns (CONQUEST's mult(x) table entry, src/mult_module.f90:271-1102). */
+ * same patterns (CONQUEST's mult(x) table entry, src/mult_module.f90:271-1102).
+ * Block dimensions (support functions per atom) 1..64; CQMM_ERR_ARG outside. */
 int cqmm_plan_create(const cqmm_mult_desc *desc, cqmm_plan **plan);
 int cqmm_plan_destroy(cqmm_plan *plan);
 

@@ -69,6 +69,13 @@ SMALL = [
     (30, 9, 5.0, 6.0, 11.0, 1),
     (30, 16, 5.0, 6.0, 6.0, 2),
     (40, 5, 6.0, 6.0, 12.0, 1),   # non-instanced uniform size takes the general path
+    # Blocks above 16x16 (sub-block kernel): uniform 20, 4/22 mix, 64 (the limit), odd 33.
+    (24, 20, 5.0, 6.0, 8.0, 1),
+    (24, 20, 5.0, 6.0, 6.0, 2),
+    (30, ((4, .5), (22, .5)), 5.0, 6.0, 7.0, 1),
+    (30, ((4, .5), (22, .5)), 5.0, 6.0, 5.0, 2),
+    (12, 64, 4.0, 4.0, 5.0, 1),
+    (16, ((33, .5), (9, .5)), 4.0, 5.0, 4.0, 2),
 ]
 
 
@@ -228,6 +235,8 @@ FP32_CASES = [
     (64, 13, 6.0, 8.0, 12.0, 1),
     (64, 13, 8.0, 8.0, 12.0, 2),
     (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),   # mixed-size tile kernel
+    (30, ((4, .5), (22, .5)), 5.0, 6.0, 7.0, 1),               # blocks above 16x16: sub-block kernel
+    (24, 20, 5.0, 6.0, 6.0, 2),
 ]
 
 
