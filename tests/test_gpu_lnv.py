@@ -69,7 +69,7 @@ def random_L(pu, seed, scale=0.05):
     return 0.5 * onsite + v
 
 
-@pytest.mark.parametrize("nsf", [4, 13, [4, 9, 13, 4]])
+@pytest.mark.parametrize("nsf", [4, 13, [4, 9, 13, 4], [4, 20]])
 def test_lnv_matches_sparse_oracle_periodic(nsf):
     pu = periodic(nsf=nsf)
     print(f"lnv periodic nsf={nsf} natoms={pu.natoms}", flush=True)
