@@ -76,6 +76,10 @@ SMALL = [
     (30, ((4, .5), (22, .5)), 5.0, 6.0, 5.0, 2),
     (12, 64, 4.0, 4.0, 5.0, 1),
     (16, ((33, .5), (9, .5)), 4.0, 5.0, 4.0, 2),
+    # One atom (every neighbour a periodic image of itself) and two atoms.
+    (1, 4, 4.0, 5.0, 6.0, 1),
+    (1, 13, 4.0, 5.0, 3.0, 2),
+    (2, 4, 3.0, 3.0, 3.0, 1),
 ]
 
 
@@ -149,6 +153,7 @@ TILE_CASES = [
     (40, 16, 5.0, 6.0, 9.0, 1),
     (40, 16, 5.0, 6.0, 6.0, 2),
     (200, 13, 12.0, 12.0, 6.0, 2),   # many tiles, multi-chunk entries
+    (1, 13, 4.0, 5.0, 3.0, 2),       # one atom: one tile of self-images
 ]
 
 
@@ -156,8 +161,10 @@ TILE_CASES = [
 def test_tile_kernel_matches_oracle_and_triplet_kernel(natoms, nsf, ra, rb, rc, mt):
     """Uniform block sizes get the output-tile schedule. Both FP64 kernels meet
     the 1e-12 bar against the oracle and overwrite every output element. Type 1
-    sums every block in the same (ascending k) order in both kernels with the
-    same MFMA sequence, so there they agree bit for bit."""
+    sums every block in the same (ascending k) order in both kernels, so there
+    they agree bit for bit — also for NSF = 4m+1, where the tile kernel's last
+    contraction index is a VALU fused multiply-add and the triplet kernel's an
+    MFMA step with three zero terms."""
     case = Case(natoms, nsf, ra, rb, rc, mt, seed=31)
     print(f"tile case natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
     plan = cqmm.Plan(case.desc)

@@ -77,6 +77,13 @@ CASES = [
     (36, 13, 6.0, 8.0, 8.0, 2),                   # reduction, output truncated
     (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),
     (48, ((4, .5), (9, .25), (13, .25)), 7.0, 9.0, 6.0, 2),
+    # Edge cases: one atom (every neighbour is a periodic image of itself), two
+    # atoms with ranges shorter than their separation images, a 4/22 mix
+    # (blocks above 16x16).
+    (1, 4, 4.0, 5.0, 6.0, 1),
+    (1, 13, 4.0, 5.0, 3.0, 2),
+    (2, 4, 3.0, 3.0, 3.0, 1),
+    (20, ((4, .5), (22, .5)), 5.0, 6.0, 7.0, 1),
 ]
 
 
