@@ -55,6 +55,14 @@ def species_nsf(natoms, mix, seed):
     return vals[rng.choice(len(vals), size=natoms, p=probs / probs.sum())]
 
 
+def kernel_name(info, nsf):
+    """Name of the FP64 product kernel a plan runs (as rocprofv3 lists it, shortened)."""
+    mixed = not isinstance(nsf, int)
+    if info.kernel == cqmm.KERNEL_TILE:
+        return "k_tile_mixed_f64" if mixed else ("k_tile_dense4_f64" if nsf == 4 else "k_tile_products_f64")
+    return "k_block_products_f64" if mixed else "k_block_products_f64_u"
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -223,9 +231,7 @@ def main():
     plan.set_kernel({"auto": cqmm.KERNEL_AUTO, "triplet": cqmm.KERNEL_TRIPLET, "tile": cqmm.KERNEL_TILE}[args.kernel])
     info = plan.info()
     flops = info.flops
-    mixed = not isinstance(w["nsf"], int)
-    kname = (("k_tile_mixed_f64" if mixed else "k_tile_products_f64") if info.kernel == cqmm.KERNEL_TILE
-             else ("k_block_products_f64" if mixed else "k_block_products_f64_u"))
+    kname = kernel_name(info, w["nsf"])
     log(f"plan: {info.n_triplets} products, {flops / 1e12:.3f} TFLOP/step, built in {t_plan:.2f} s; "
         f"kernel {kname}, {info.n_tiles} tiles, {info.n_entries} entries, "
         f"{info.n_loads / max(info.n_triplets, 1):.3f} operand blocks per product")
@@ -244,9 +250,7 @@ def main():
         # Kernel autotuning on these operands: the faster of the tile and per-block kernels stays.
         ms_tile, ms_trip = plan.tune_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, reps=2)
         info = plan.info()
-        mixed = not isinstance(w["nsf"], int)
-        kname = (("k_tile_mixed_f64" if mixed else "k_tile_products_f64") if info.kernel == cqmm.KERNEL_TILE
-                 else ("k_block_products_f64" if mixed else "k_block_products_f64_u"))
+        kname = kernel_name(info, w["nsf"])
         tune = {"tile_ms": None if ms_tile is None else round(ms_tile, 3), "per_block_ms": round(ms_trip, 3),
                 "chosen": kname}
         log(f"autotune: tile {ms_tile} ms, per-block {ms_trip:.3f} ms -> {kname}")

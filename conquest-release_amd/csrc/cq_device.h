@@ -75,6 +75,13 @@ struct TilePlan {
   DevBuf prod;         // uint32 [n_prod+64] LDS element offsets of the two operands (left | right << 16)
   DevBuf tile_out;     // int32 [n_tiles*64] output block of slot row*8+col, or -1
   DevBuf wave_out;     // int32 [n_tiles*4*16] output block of (tile, wavefront, slot), or -1
+  // 4x4 blocks (dense): each wavefront computes a 4x4-block quadrant of the
+  // tile per entry with one MFMA; no product lists. wave_out then holds the
+  // quadrant's blocks (row * 4 + column within the quadrant).
+  bool dense = false;
+  DevBuf chunk_entry;  // int32 [n_chunks+1] first entry of each chunk
+  DevBuf entry_mask;   // uint32 [n_entries] left-operand rows | right-operand columns << 8 present
+  DevBuf entry_coff;   // int32 [n_entries] first operand block of the entry in its chunk's stage
   // Mixed block sizes (maxn > 0): operands and outputs by element offset.
   int maxn = 0;        // largest block dimension; 0: uniform schedule above
   DevBuf list64;       // uint64 [n_loads] operand element offset; bit 63 set: right operand

@@ -63,6 +63,8 @@ constexpr int kStageBytes = 24 * 1024;       // one of the two operand stages (t
 constexpr int kLdsPad = 256;
 constexpr int kChunkEntries = 15;            // entries per chunk at most (4-bit product counts)
 constexpr int kWaveProducts = 64;            // products per wavefront per chunk at most (one per lane)
+constexpr int kDenseNsf = 4;                 // block size of the dense-quadrant kernel
+constexpr int kDenseEntries = 64;            // entries per chunk at most there (one per lane)
 
 // Operand blocks per chunk (FP64 sizing): what fits kStageBytes, but at
 // least the 16 blocks one entry can bring (8 left + 8 right), so every entry
@@ -305,6 +307,124 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
   }
 }
 
+// 4x4 blocks: a tile is 32x32 elements, and wavefront w owns its 16x16
+// quadrant (block rows 4*(w/2)..+3, block columns 4*(w%2)..+3). One
+// 16x16x4 MFMA per entry covers the entry's whole contraction (4) for all 16
+// blocks of the quadrant at once: lane (r, g) supplies P(row 4*(w/2)+r/4,
+// x)[r%4][g] and Q(column 4*(w%2)+r/4, x)[g][r%4], zero where the entry has no
+// such block. A block pair whose output block does not exist only feeds
+// accumulator elements that are never written; every output block sums its
+// products in entry (x) order, one fused step per product, as the triplet
+// kernel does. Same chunks, stages and copies as k_tile_products; the chunk's
+// entries (at most 64, one per lane) replace the product lists.
+template <int MODE, bool QT>
+__global__ void __launch_bounds__(kTileThreads) k_tile_dense4(
+    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ chunk_entry, const uint32_t *__restrict__ emask,
+    const int32_t *__restrict__ ecoff, const int32_t *__restrict__ wave_out,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
+    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
+  constexpr int NS2 = kDenseNsf * kDenseNsf;
+  constexpr int STAGE = stage_cap(kDenseNsf) * NS2 + kLdsPad * (int)(8 / sizeof(T));
+  __shared__ T stage0[STAGE];
+  __shared__ T stage1[STAGE];
+  const int32_t per = (n_tiles + 7) >> 3;
+  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  if (tile >= n_tiles) return;  // the whole workgroup leaves together
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, r = lane & 15, g = lane >> 4;
+  const int qr = w >> 1, qc = w & 1;
+  const int ua = 4 * qr + (r >> 2);  // tile row of this lane's left operand
+  const int cb = 4 * qc + (r >> 2);  // tile column of this lane's right operand
+  const int oa = (r & 3) + 4 * g;    // P(row, x)[r % 4][g]
+  const int ob = QT ? (r & 3) + 4 * g : g + 4 * (r & 3);  // Q[g][r % 4]
+  const uint32_t below_a = (1u << ua) - 1u, below_b = (1u << cb) - 1u;
+  const uint32_t quad_rows = 0xfu << (4 * qr), quad_cols = 0xfu << (8 + 4 * qc);
+  dbl4 a64 = {0.0, 0.0, 0.0, 0.0};
+  flt4 a32 = {0.f, 0.f, 0.f, 0.f};
+  const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
+  if (c0 < c1) {
+    // Per-chunk operand-list and entry bounds of the tile in registers, 64
+    // chunks at a time (lane l: chunk wb + l, clamped to c1).
+    int32_t wb = c0;
+    int32_t cc = min(wb + lane, c1);
+    int64_t wcl = chunk_lbeg[cc];
+    int32_t wce = chunk_entry[cc];
+    auto cl_at = [&](int32_t x) { return readlane64(wcl, min(x, c1) - wb); };
+    auto ce_at = [&](int32_t x) { return __builtin_amdgcn_readlane(wce, min(x, c1) - wb); };
+    // The entries of a chunk, lane i = entry i (index clamped to the chunk).
+    auto entry_regs = [&](int32_t e0, int32_t e1, uint32_t &m, int32_t &o) {
+      const int32_t e = min(e0 + lane, e1 - 1);
+      m = emask[e];
+      o = ecoff[e];
+    };
+    uint32_t em, emn = 0;
+    int32_t eo, eon = 0;
+    int32_t lvn = 0;
+    {
+      const int64_t cl0 = cl_at(c0), cl1 = cl_at(c0 + 1);
+      const int32_t lv0 = load_chunk_list(list, cl0, (int32_t)(cl1 - cl0), w, lane);
+      issue_stage<kDenseNsf, T>(lv0, (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+      entry_regs(ce_at(c0), ce_at(c0 + 1), emn, eon);
+      if (c0 + 1 < c1) lvn = load_chunk_list(list, cl1, (int32_t)(cl_at(c0 + 2) - cl1), w, lane);
+    }
+    for (int32_t c = c0; c < c1; ++c) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      const bool odd = ((c - c0) & 1) != 0;
+      const T *cur = odd ? stage1 : stage0;
+      em = emn;
+      eo = eon;
+      const int32_t ne = ce_at(c + 1) - ce_at(c);
+      if (c + 1 < c1) {
+        const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2);
+        issue_stage<kDenseNsf, T>(lvn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
+        entry_regs(ce_at(c + 1), ce_at(c + 2), emn, eon);
+        if (c + 2 < c1) lvn = load_chunk_list(list, cl2, (int32_t)(cl_at(c + 3) - cl2), w, lane);
+      }
+      for (int i = 0; i < ne; ++i) {
+        const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)em, i);
+        if ((m & quad_rows) == 0 || (m & quad_cols) == 0) continue;  // no block pair in this quadrant
+        const int32_t base = __builtin_amdgcn_readlane(eo, i);
+        const uint32_t ym = m & 0xffu, bm = (m >> 8) & 0xffu;
+        const int ia = base + __popc(ym & below_a);
+        const int ib = base + __popc(ym) + __popc(bm & below_b);
+        T a = cur[ia * NS2 + oa], b = cur[ib * NS2 + ob];
+        if (((ym >> ua) & 1u) == 0) a = (T)0;
+        if (((bm >> cb) & 1u) == 0) b = (T)0;
+        if constexpr (MODE == 0) {
+          a64 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a64, 0, 0, 0);
+        } else if constexpr (MODE == 1) {
+          a32 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, a32, 0, 0, 0);
+        } else {
+          const flt4 t = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) a64[e] += (double)t[e];
+        }
+      }
+      if (c + 4 > wb + 63 && c + 1 < c1) {  // next iteration reads up to chunk c + 3
+        wb = c + 1;
+        cc = min(wb + lane, c1);
+        wcl = chunk_lbeg[cc];
+        wce = chunk_entry[cc];
+      }
+    }
+  }
+  // FP64 accumulator D[g+4e][r], FP32 D[4g+e][r]: quadrant row R, column r is
+  // element (R % 4, r % 4) of block (R / 4, r / 4).
+  const int32_t *wo = wave_out + ((int64_t)tile * kWaves + w) * 16;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int rq = MODE == 0 ? e : g, ri = MODE == 0 ? g : e;
+    const int32_t blk = wo[rq * 4 + (r >> 2)];
+    if (blk >= 0)
+      O[(int64_t)blk * NS2 + ri + 4 * (r & 3)] = MODE == 1 ? (T)a32[e] : (T)a64[e];
+  }
+}
+
 // ---------------------------------------------------------------- builder
 
 inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -418,28 +538,27 @@ __device__ inline void entry_wave_counts(uint32_t m, const uint8_t *__restrict__
   }
 }
 
-// Chunks of a tile: at most kChunkEntries consecutive entries whose operand
-// blocks fit `cap` and that give no wavefront more than kWaveProducts
-// products. With FILL unset, counts them; with FILL set, writes each chunk's
+// Chunks of a tile: at most max_entries consecutive entries whose operand
+// blocks fit `cap` and (with an owner table) that give no wavefront more than
+// kWaveProducts products. With FILL unset, counts them; with FILL set, writes each chunk's
 // first entry.
 template <bool FILL>
 __global__ void k_chunks(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask,
-                         const uint8_t *__restrict__ owner, int32_t n_tiles, int32_t cap,
+                         const uint8_t *__restrict__ owner, int32_t n_tiles, int32_t cap, int32_t max_entries,
                          const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tiles) return;
-  const uint8_t *own = owner + (int64_t)t * 64;
   int32_t nch = 0, sum = 0, ne = 0;
   int wc[kWaves] = {};
   for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
     const uint32_t m = mask[e];
     const int32_t c = __popc(m);
-    int pw[kWaves];
-    entry_wave_counts(m, own, pw);
+    int pw[kWaves] = {};
+    if (owner) entry_wave_counts(m, owner + (int64_t)t * 64, pw);  // no owner: no per-wavefront limit
     bool over = false;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) over |= wc[w] + pw[w] > kWaveProducts;
-    if (nch == 0 || sum + c > cap || ne == kChunkEntries || over) {
+    if (nch == 0 || sum + c > cap || ne == max_entries || over) {
       if (FILL) out[tchunk[t] + nch] = e;
       ++nch;
       sum = 0;
@@ -549,6 +668,11 @@ int launch_mode(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_
   const int64_t *cl = tp.chunk_lbeg.as<int64_t>(), *ps = tp.pstart.as<int64_t>();
   const uint64_t *pc = tp.pcnt.as<uint64_t>();
   const uint32_t *pr = tp.prod.as<uint32_t>();
+  if (tp.dense) {
+    hipLaunchKernelGGL((k_tile_dense4<MODE, QT>), dim3(gridx), blk, 0, s, tc, cl, li, tp.chunk_entry.as<int32_t>(),
+                       tp.entry_mask.as<uint32_t>(), tp.entry_coff.as<int32_t>(), wo, P, Q, O, tp.n_tiles);
+    return cq::hip_check(hipGetLastError(), "dense tile-product kernel launch");
+  }
   switch (tp.nsf) {
     case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
     case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
@@ -1093,7 +1217,21 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
       (st = hip_check(hipStreamSynchronize(s), "tile_out")))
     return st;
   DevBuf d_owner;
-  {
+  tp.dense = in.nsf == kDenseNsf;
+  if (tp.dense) {
+    // Wavefront w computes quadrant (w / 2, w % 2) of the tile: rows
+    // 4*(w/2)..+3, columns 4*(w%2)..+3.
+    std::vector<int32_t> wave_out((size_t)in.n_tiles * 64, -1);
+    for (int64_t t = 0; t < in.n_tiles; ++t)
+      for (int u = 0; u < 8; ++u)
+        for (int c = 0; c < 8; ++c)
+          wave_out[(t * kWaves + (u >> 2) * 2 + (c >> 2)) * 16 + (u & 3) * 4 + (c & 3)] = tout32[t * 64 + u * 8 + c];
+    if ((st = tp.wave_out.alloc(wave_out.size() * 4))) return st;
+    if ((st = hip_check(hipMemcpyAsync(tp.wave_out.p, wave_out.data(), wave_out.size() * 4, hipMemcpyHostToDevice, s),
+                        "H2D wave_out")) ||
+        (st = hip_check(hipStreamSynchronize(s), "wave_out")))
+      return st;
+  } else {
     std::vector<uint8_t> owner;
     if ((st = assign_slots(in, owner, s))) return st;
     std::vector<int32_t> wave_out((size_t)in.n_tiles * 64, -1);
@@ -1109,6 +1247,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
         (st = hip_check(hipStreamSynchronize(s), "owner")))
       return st;
   }
+  const int32_t max_entries = tp.dense ? kDenseEntries : kChunkEntries;
   // Sort the products by (tile, contraction atom).
   unsigned end_bit = 1;
   const uint64_t kmax = (uint64_t)in.n_tiles * (uint64_t)in.nx;
@@ -1170,7 +1309,8 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   hipLaunchKernelGGL(k_tile_ebeg, dim3(grid1((int64_t)in.n_tiles + 1, 256)), dim3(256), 0, s, d_etile.as<int32_t>(), ne,
                      in.n_tiles, d_tebeg.as<int32_t>());
   hipLaunchKernelGGL((k_chunks<false>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
-                     d_emask.as<uint32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, nullptr, d_nch.as<int32_t>());
+                     d_emask.as<uint32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, max_entries, nullptr,
+                     d_nch.as<int32_t>());
   if ((st = tp.tile_chunk.alloc(((size_t)in.n_tiles + 1) * 4))) return st;
   if ((st = hip_check(hipMemsetAsync(tp.tile_chunk.p, 0, 4, s), "memset tile_chunk"))) return st;
   if ((st = inclusive_scan(d_nch.as<int32_t>(), tp.tile_chunk.as<int32_t>() + 1, (size_t)in.n_tiles, s))) return st;
@@ -1183,8 +1323,8 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   DevBuf d_centry;
   if ((st = d_centry.alloc(((size_t)nch + 1) * 4))) return st;
   hipLaunchKernelGGL((k_chunks<true>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
-                     d_emask.as<uint32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, tp.tile_chunk.as<int32_t>(),
-                     d_centry.as<int32_t>());
+                     d_emask.as<uint32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, max_entries,
+                     tp.tile_chunk.as<int32_t>(), d_centry.as<int32_t>());
   if ((st = hip_check(hipMemcpyAsync(d_centry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
                       "H2D chunk end")))
     return st;
@@ -1194,6 +1334,15 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if ((st = d_ecoff.alloc((size_t)ne * 4))) return st;
   hipLaunchKernelGGL(k_entry_coff, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, d_centry.as<int32_t>(),
                      d_elbeg.as<int64_t>(), (int64_t)nch, d_ecoff.as<int32_t>());
+  if (tp.dense) {
+    // The dense kernel reads the entries themselves; no product lists.
+    tp.chunk_entry = std::move(d_centry);
+    tp.entry_mask = std::move(d_emask);
+    tp.entry_coff = std::move(d_ecoff);
+    tp.n_prod = n;
+    if ((st = hip_check(hipGetLastError(), "tile builder launch"))) return st;
+    return hip_check(hipStreamSynchronize(s), "tile build");
+  }
   // Product lists: count per chunk, first product of each chunk, fill.
   DevBuf d_ctile, d_np, d_pbase;
   if ((st = d_ctile.alloc((size_t)std::max(nch, 1) * 4)) || (st = d_np.alloc((size_t)std::max(nch, 1) * 8)) ||

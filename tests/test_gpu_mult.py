@@ -154,6 +154,8 @@ TILE_CASES = [
     (40, 16, 5.0, 6.0, 6.0, 2),
     (200, 13, 12.0, 12.0, 6.0, 2),   # many tiles, multi-chunk entries
     (1, 13, 4.0, 5.0, 3.0, 2),       # one atom: one tile of self-images
+    (40, 4, 6.0, 8.0, 6.0, 2),       # 4x4 blocks (dense-quadrant kernel), type 2
+    (1, 4, 4.0, 5.0, 6.0, 1),        # 4x4 blocks, one atom
 ]
 
 
@@ -242,6 +244,8 @@ FP32_CASES = [
     (64, 13, 6.0, 8.0, 12.0, 1),
     (64, 13, 8.0, 8.0, 12.0, 2),
     (48, ((4, .5), (9, .25), (13, .25)), 6.0, 7.0, 10.0, 1),   # mixed-size tile kernel
+    (48, 4, 6.0, 8.0, 14.0, 1),                               # 4x4 blocks: dense-quadrant tile kernel
+    (40, 4, 6.0, 8.0, 6.0, 2),
     (30, ((4, .5), (22, .5)), 5.0, 6.0, 7.0, 1),               # blocks above 16x16: sub-block kernel
     (24, 20, 5.0, 6.0, 6.0, 2),
 ]
