@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -355,6 +356,7 @@ int check_halo(const HaloV<H> &h, int32_t n_prim, const char *name) {
 template <class H>
 int build_plan(const DescV<H> *d, cqmm_plan **out) {
   *out = nullptr;
+  cq::PhaseTimer pt;
   int st = cq::ensure_init();
   if (st) return st;
   hipStream_t s = cq::stream();
@@ -449,6 +451,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   for (int32_t k = 0; k < ni_a; ++k) maxn = std::max(maxn, A.ndimj[k]);
   for (int32_t k = 0; k < ni_c; ++k) maxn = std::max(maxn, C.ndimj[k]);
 
+  pt.mark("host index checks");
   // ---- device work ----
   DevBuf d_km, d_bindex, d_ci_halo, d_ci_hbeg, d_ci_h2d, d_ai_h2d, d_iprim, d_itemk, d_ndimi, d_err;
   if ((st = upload(d_km, km.data(), km.size(), s))) return st;
@@ -492,6 +495,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   if (herr.code != E_NONE)
     return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s (halo atom %d, B neighbour %d, value %lld)",
                           err_name(herr.code), herr.a + 1, herr.b + 1, herr.c);
+  pt.mark("upload + triplet count");
 
   std::vector<int64_t> item_off(n_items + 1, 0);
   double flops = 0.0;
@@ -532,8 +536,17 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     for (int32_t k = 0; k < ni_a && tiles; ++k) tiles = km[k].bdata % ns2 == 0;
     const H *ph2d = d->mult_type == 1 ? A.i_h2d : C.i_h2d;
     const int64_t npn = (int64_t)d->n_prim * (d->mult_type == 1 ? ni_a : ni_c);
-    for (int64_t x = 0; x < npn && tiles; ++x) tiles = ph2d[x] == 0 || (ph2d[x] - 1) % ns2 == 0;
+    std::atomic<bool> whole(true);
+    cq::parallel_for(npn, [&](int64_t b, int64_t e) {
+      for (int64_t x = b; x < e; ++x)
+        if (ph2d[x] != 0 && (ph2d[x] - 1) % ns2 != 0) {
+          whole = false;
+          return;
+        }
+    });
+    tiles = whole;
   }
+  pt.mark("  tile map: layout checks");
   if (tiles) {
     const HaloV<H> &CH = d->mult_type == 1 ? C : A;  // halo of the output's columns
     crank = cq::halo_morton_rank(CH.np_in_halo, CH.nh_part, CH.lab_hcover, d->ncovery, d->ncoverz);
@@ -543,22 +556,46 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     // Output blocks (row, column, offset) of the product's result.
     std::vector<int64_t> ob_r, ob_c, ob_o;
     if (d->mult_type == 2) {
-      for (int64_t t = 0; t < n_items; ++t) {
-        const int32_t kh = item_k[t], i = T.i_prim[t];
-        ob_r.push_back(i - 1);
-        ob_c.push_back(kh);
-        ob_o.push_back((int64_t)A.i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1);
-      }
+      // One output block per (k, i) item.
+      ob_r.resize(n_items);
+      ob_c.resize(n_items);
+      ob_o.resize(n_items);
+      cq::parallel_for(n_items, [&](int64_t b, int64_t e) {
+        for (int64_t t = b; t < e; ++t) {
+          const int32_t kh = item_k[t], i = T.i_prim[t];
+          ob_r[t] = i - 1;
+          ob_c[t] = kh;
+          ob_o[t] = (int64_t)A.i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1;
+        }
+      });
     } else {
-      const int64_t nc_all = (int64_t)d->n_prim * ni_c;
-      for (int64_t pos = 0; pos < nc_all; ++pos) {
-        const int64_t off = (int64_t)C.i_h2d[pos];
-        if (off == 0) continue;
-        ob_r.push_back(pos / ni_c);
-        ob_c.push_back(pos % ni_c);
-        ob_o.push_back(off - 1);
-      }
+      // The blocks of C row by row: count per row, then fill at the row's offset.
+      std::vector<int64_t> rbeg((size_t)d->n_prim + 1, 0);
+      cq::parallel_for(d->n_prim, [&](int64_t b, int64_t e) {
+        for (int64_t p = b; p < e; ++p) {
+          int64_t c = 0;
+          for (int32_t j = 0; j < ni_c; ++j) c += C.i_h2d[p * ni_c + j] != 0;
+          rbeg[p + 1] = c;
+        }
+      }, 64);
+      for (int32_t p = 0; p < d->n_prim; ++p) rbeg[p + 1] += rbeg[p];
+      ob_r.resize(rbeg[d->n_prim]);
+      ob_c.resize(rbeg[d->n_prim]);
+      ob_o.resize(rbeg[d->n_prim]);
+      cq::parallel_for(d->n_prim, [&](int64_t b, int64_t e) {
+        for (int64_t p = b; p < e; ++p) {
+          int64_t k = rbeg[p];
+          for (int32_t j = 0; j < ni_c; ++j) {
+            const int64_t off = (int64_t)C.i_h2d[p * ni_c + j];
+            if (off == 0) continue;
+            ob_r[k] = p;
+            ob_c[k] = j;
+            ob_o[k++] = off - 1;
+          }
+        }
+      }, 64);
     }
+    pt.mark("  tile map: output blocks");
     for (int64_t b = 0; b < (int64_t)ob_o.size() && tiles && uniform; ++b) tiles = ob_o[b] % ns2 == 0;
     if (tiles && nrg * ncg > INT32_MAX) tiles = false;
     if (tiles) {
@@ -566,24 +603,34 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
       for (size_t b = 0; b < ob_o.size(); ++b) tmap[(ob_r[b] >> 3) * ncg + (crank[ob_c[b]] >> 3)] = 0;
       for (auto &v : tmap)
         if (v == 0) v = n_tiles++;
+      pt.mark("  tile map: numbering");
       tout.assign((size_t)n_tiles * 64, -1);
       // Block dimensions of each tile row and column, and of each contraction
       // atom (the mixed-size schedule packs blocks by their sizes).
       trdim.assign((size_t)n_tiles * 8, 0);
       tcdim.assign((size_t)n_tiles * 8, 0);
       const int32_t *cdims = d->mult_type == 1 ? C.ndimj : A.ndimj;
-      for (size_t b = 0; b < ob_o.size(); ++b) {
-        const int32_t cr = crank[ob_c[b]];
-        const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
-        tout[(size_t)tile * 64 + ((ob_r[b] & 7) << 3) + (cr & 7)] = ob_o[b];
-        trdim[(size_t)tile * 8 + (ob_r[b] & 7)] = (uint8_t)A.ndimi[ob_r[b]];
-        tcdim[(size_t)tile * 8 + (cr & 7)] = (uint8_t)cdims[ob_c[b]];
-      }
+      // Every output block has its own slot: the fill runs in parallel.
+      cq::parallel_for((int64_t)ob_o.size(), [&](int64_t b0, int64_t b1) {
+        for (int64_t b = b0; b < b1; ++b) {
+          const int32_t cr = crank[ob_c[b]];
+          const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
+          tout[(size_t)tile * 64 + ((ob_r[b] & 7) << 3) + (cr & 7)] = ob_o[b];
+        }
+      });
+      if (!uniform)
+        for (size_t b = 0; b < ob_o.size(); ++b) {
+          const int32_t cr = crank[ob_c[b]];
+          const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
+          trdim[(size_t)tile * 8 + (ob_r[b] & 7)] = (uint8_t)A.ndimi[ob_r[b]];
+          tcdim[(size_t)tile * 8 + (cr & 7)] = (uint8_t)cdims[ob_c[b]];
+        }
       const int32_t *xd = d->mult_type == 1 ? A.ndimj : C.ndimj;
       xdim.resize((size_t)nx);
       for (int64_t x = 0; x < nx; ++x) xdim[x] = (uint8_t)xd[x];
     }
   }
+  pt.mark("host tile map");
   DevBuf d_crank, d_tmap, d_tkey, d_tslot;
   if (tiles) {
     if ((st = upload(d_crank, crank.data(), crank.size(), s))) return st;
@@ -622,7 +669,9 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     tin.nx = nx;
     tin.n_tiles = n_tiles;
     tin.tile_out = tout.data();
+    pt.mark("triplet fill + tile keys");
     if ((st = cq::build_tiles(tin, tp.tiles, s))) return st;
+    pt.mark("tile schedule");
     d_tkey.release();
     d_tslot.release();
     d_tmap.release();
@@ -720,6 +769,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   if (herr.code != E_NONE)
     return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s (halo atom %d, B neighbour %d, value %lld)",
                           err_name(herr.code), herr.a + 1, herr.b + 1, herr.c);
+  pt.mark("segments");
   *out = plan.release();
   return CQMM_OK;
 }

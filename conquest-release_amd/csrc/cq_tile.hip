@@ -1147,60 +1147,91 @@ std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part
 // block as its slot (8 / kWaves) * row + k / kWaves, so every wavefront owns
 // 8 / kWaves blocks of each tile row and column, which spreads every entry's
 // rectangle of products evenly — chosen among kLatinTries column permutations
-// to balance the wavefronts' total product counts.
+// (the first with the smallest largest wavefront product count).
 // owner[tile*64 + row*8 + col] = wavefront * kSlots + slot, or 0xff where the
-// tile has no such output block.
-int assign_slots(const TileBuildInput &in, std::vector<uint8_t> &owner, hipStream_t s) {
-  const int64_t n = in.n_trip;
-  int st;
-  DevBuf d_cnt;
-  if ((st = d_cnt.alloc((size_t)in.n_tiles * 64 * 4))) return st;
-  if ((st = hip_check(hipMemsetAsync(d_cnt.p, 0, (size_t)in.n_tiles * 64 * 4, s), "memset slot counts"))) return st;
-  hipLaunchKernelGGL(k_slot_counts, dim3(grid1(n, 256)), dim3(256), 0, s, in.key, in.slot, n, in.nx, d_cnt.as<int32_t>());
-  std::vector<int32_t> cnt((size_t)in.n_tiles * 64);
-  if ((st = hip_check(hipMemcpyAsync(cnt.data(), d_cnt.p, cnt.size() * 4, hipMemcpyDeviceToHost, s), "D2H slot counts")) ||
-      (st = hip_check(hipStreamSynchronize(s), "slot counts")))
-    return st;
-  constexpr int kLatinTries = 24;
-  int perms[kLatinTries][8];
+// tile has no such output block; wave_out[tile*64 + owner] = that block.
+constexpr int kLatinTries = 24;
+struct LatinPerms {
+  int8_t p[kLatinTries][8];
+};
+
+__global__ void k_assign_slots(const int32_t *__restrict__ cnt, const int32_t *__restrict__ tile_out,
+                               LatinPerms perms, int32_t n_tiles, uint8_t *__restrict__ owner,
+                               int32_t *__restrict__ wave_out) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  const int32_t *c = cnt + (int64_t)t * 64;
+  int32_t cs[64];
+  for (int sl = 0; sl < 64; ++sl) cs[sl] = c[sl];
+  int best = 0;
+  int64_t best_max = -1;
+  for (int k = 0; k < kLatinTries; ++k) {
+    int64_t load[kWaves] = {};
+    for (int sl = 0; sl < 64; ++sl) load[(((sl >> 3) + perms.p[k][sl & 7]) & 7) % kWaves] += cs[sl];
+    int64_t m = 0;
+    for (int w = 0; w < kWaves; ++w) m = load[w] > m ? load[w] : m;
+    if (best_max < 0 || m < best_max) {
+      best_max = m;
+      best = k;
+    }
+  }
+  for (int u = 0; u < 8; ++u)
+    for (int col = 0; col < 8; ++col) {
+      const int64_t sl = (int64_t)t * 64 + u * 8 + col;
+      const int32_t ob = tile_out[sl];
+      if (ob < 0) {
+        owner[sl] = 0xff;
+        continue;
+      }
+      const int k = (u + perms.p[best][col]) & 7;
+      const int o = (k % kWaves) * kSlots + (8 / kWaves) * u + k / kWaves;
+      owner[sl] = (uint8_t)o;
+      wave_out[(int64_t)t * 64 + o] = ob;
+    }
+}
+
+// The kLatinTries column permutations (the first is the identity; the others
+// a fixed pseudo-random sequence, so plans are reproducible).
+LatinPerms latin_perms() {
+  LatinPerms lp;
   uint64_t lcg = 0x9e3779b97f4a7c15ull;
   for (int k = 0; k < kLatinTries; ++k) {
-    for (int i = 0; i < 8; ++i) perms[k][i] = i;
-    for (int i = 7; k > 0 && i > 0; --i) {  // k = 0: identity
+    int q[8];
+    for (int i = 0; i < 8; ++i) q[i] = i;
+    for (int i = 7; k > 0 && i > 0; --i) {
       lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
       const int j = (int)((lcg >> 33) % (uint64_t)(i + 1));
-      std::swap(perms[k][i], perms[k][j]);
+      std::swap(q[i], q[j]);
     }
+    for (int i = 0; i < 8; ++i) lp.p[k][i] = (int8_t)q[i];
   }
-  owner.assign((size_t)in.n_tiles * 64, 0xff);
-  for (int64_t t = 0; t < in.n_tiles; ++t) {
-    const int32_t *c = cnt.data() + t * 64;
-    int best = 0;
-    int64_t best_max = -1;
-    for (int k = 0; k < kLatinTries; ++k) {
-      int64_t load[kWaves] = {};
-      for (int sl = 0; sl < 64; ++sl) load[(((sl >> 3) + perms[k][sl & 7]) & 7) % kWaves] += c[sl];
-      int64_t m = 0;
-      for (int w = 0; w < kWaves; ++w) m = std::max(m, load[w]);
-      if (best_max < 0 || m < best_max) {
-        best_max = m;
-        best = k;
-      }
-    }
-    for (int u = 0; u < 8; ++u)
-      for (int col = 0; col < 8; ++col) {
-        if (in.tile_out[t * 64 + u * 8 + col] < 0) continue;
-        const int k = (u + perms[best][col]) & 7;
-        owner[t * 64 + u * 8 + col] = (uint8_t)((k % kWaves) * kSlots + (8 / kWaves) * u + k / kWaves);
-      }
-  }
-  return CQMM_OK;
+  return lp;
+}
+
+// Device owner table and wave_out (tile_out32: the tiles' output blocks on
+// the device) from the products per output slot.
+int assign_slots(const TileBuildInput &in, const int32_t *tile_out32, DevBuf &d_owner, DevBuf &wave_out,
+                 hipStream_t s) {
+  const int64_t n = in.n_trip;
+  const size_t ns = (size_t)in.n_tiles * 64;
+  int st;
+  DevBuf d_cnt;
+  if ((st = d_cnt.alloc(ns * 4)) || (st = d_owner.alloc(ns)) || (st = wave_out.alloc(ns * 4))) return st;
+  if ((st = hip_check(hipMemsetAsync(d_cnt.p, 0, ns * 4, s), "memset slot counts"))) return st;
+  hipLaunchKernelGGL(k_slot_counts, dim3(grid1(n, 256)), dim3(256), 0, s, in.key, in.slot, n, in.nx, d_cnt.as<int32_t>());
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1((int64_t)ns, 256)), dim3(256), 0, s, wave_out.as<int32_t>(), (int64_t)ns, -1);
+  hipLaunchKernelGGL(k_assign_slots, dim3(grid1(in.n_tiles, 128)), dim3(128), 0, s, d_cnt.as<int32_t>(), tile_out32,
+                     latin_perms(), in.n_tiles, d_owner.as<uint8_t>(), wave_out.as<int32_t>());
+  if ((st = hip_check(hipGetLastError(), "slot assignment launch"))) return st;
+  // d_cnt is freed on return: wait for the kernels that read it.
+  return hip_check(hipStreamSynchronize(s), "slot assignment");
 }
 
 int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s);
 
 int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if (in.nsf == 0) return build_tiles_mixed(in, tp, s);
+  cq::PhaseTimer pt;
   const int64_t n = in.n_trip;
   const int64_t ns2 = (int64_t)in.nsf * in.nsf;
   const int32_t cap = tile_cap(in.nsf);
@@ -1210,7 +1241,9 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   tp.n_tiles = in.n_tiles;
   // Output blocks by block index (element offset / NSF^2).
   std::vector<int32_t> tout32((size_t)in.n_tiles * 64);
-  for (size_t k = 0; k < tout32.size(); ++k) tout32[k] = in.tile_out[k] < 0 ? -1 : (int32_t)(in.tile_out[k] / ns2);
+  parallel_for((int64_t)tout32.size(), [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; ++k) tout32[k] = in.tile_out[k] < 0 ? -1 : (int32_t)(in.tile_out[k] / ns2);
+  });
   if ((st = tp.tile_out.alloc((size_t)in.n_tiles * 64 * 4))) return st;
   if ((st = hip_check(hipMemcpyAsync(tp.tile_out.p, tout32.data(), (size_t)in.n_tiles * 64 * 4, hipMemcpyHostToDevice, s),
                       "H2D tile_out")) ||
@@ -1232,22 +1265,11 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
         (st = hip_check(hipStreamSynchronize(s), "wave_out")))
       return st;
   } else {
-    std::vector<uint8_t> owner;
-    if ((st = assign_slots(in, owner, s))) return st;
-    std::vector<int32_t> wave_out((size_t)in.n_tiles * 64, -1);
-    for (int64_t t = 0; t < in.n_tiles; ++t)
-      for (int sl = 0; sl < 64; ++sl) {
-        const uint8_t o = owner[t * 64 + sl];
-        if (o != 0xff) wave_out[t * 64 + o] = tout32[t * 64 + sl];
-      }
-    if ((st = d_owner.alloc(owner.size())) || (st = tp.wave_out.alloc(wave_out.size() * 4))) return st;
-    if ((st = hip_check(hipMemcpyAsync(d_owner.p, owner.data(), owner.size(), hipMemcpyHostToDevice, s), "H2D owner")) ||
-        (st = hip_check(hipMemcpyAsync(tp.wave_out.p, wave_out.data(), wave_out.size() * 4, hipMemcpyHostToDevice, s),
-                        "H2D wave_out")) ||
-        (st = hip_check(hipStreamSynchronize(s), "owner")))
-      return st;
+    pt.mark("  tiles: tile_out");
+    if ((st = assign_slots(in, tp.tile_out.as<int32_t>(), d_owner, tp.wave_out, s))) return st;
   }
   const int32_t max_entries = tp.dense ? kDenseEntries : kChunkEntries;
+  pt.mark("  tiles: owners");
   // Sort the products by (tile, contraction atom).
   unsigned end_bit = 1;
   const uint64_t kmax = (uint64_t)in.n_tiles * (uint64_t)in.nx;
@@ -1266,6 +1288,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if ((st = hip_check(hipStreamSynchronize(s), "tile sort"))) return st;
   d_tmp.release();
   d_perm.release();
+  pt.mark("  tiles: sort");
   // Entries = distinct keys.
   DevBuf d_flag, d_eid;
   if ((st = d_flag.alloc(n * 4)) || (st = d_eid.alloc(n * 4))) return st;
@@ -1334,6 +1357,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if ((st = d_ecoff.alloc((size_t)ne * 4))) return st;
   hipLaunchKernelGGL(k_entry_coff, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, d_centry.as<int32_t>(),
                      d_elbeg.as<int64_t>(), (int64_t)nch, d_ecoff.as<int32_t>());
+  pt.mark("  tiles: entries + chunks");
   if (tp.dense) {
     // The dense kernel reads the entries themselves; no product lists.
     tp.chunk_entry = std::move(d_centry);
@@ -1380,7 +1404,9 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
                      d_owner.as<uint8_t>(), (int32_t)ns2, nullptr, d_pbase.as<int64_t>(), tp.pstart.as<int64_t>(),
                      tp.pcnt.as<uint64_t>(), tp.prod.as<uint32_t>());
   if ((st = hip_check(hipGetLastError(), "tile builder launch"))) return st;
-  return hip_check(hipStreamSynchronize(s), "tile build");
+  st = hip_check(hipStreamSynchronize(s), "tile build");
+  pt.mark("  tiles: product lists");
+  return st;
 }
 
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
@@ -1402,10 +1428,26 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   tp.n_tiles = in.n_tiles;
   DevBuf d_owner, d_rdim, d_cdim, d_xdim;
   {
-    std::vector<uint8_t> owner;
-    if ((st = assign_slots(in, owner, s))) return st;
-    std::vector<int64_t> wave_out((size_t)in.n_tiles * 64, -1);
-    std::vector<uint8_t> wave_dim((size_t)in.n_tiles * 64, 0);
+    // Slot owners on the device (from which output blocks exist), then the
+    // per-wavefront element offsets and block shapes on the host.
+    const size_t ns = (size_t)in.n_tiles * 64;
+    std::vector<int32_t> exists(ns);
+    parallel_for((int64_t)ns, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; ++k) exists[k] = in.tile_out[k] < 0 ? -1 : 0;
+    });
+    std::vector<uint8_t> owner(ns);
+    {
+      DevBuf d_exists, d_wo;
+      if ((st = d_exists.alloc(ns * 4))) return st;
+      if ((st = hip_check(hipMemcpyAsync(d_exists.p, exists.data(), ns * 4, hipMemcpyHostToDevice, s), "H2D exists")))
+        return st;
+      if ((st = assign_slots(in, d_exists.as<int32_t>(), d_owner, d_wo, s))) return st;
+      if ((st = hip_check(hipMemcpyAsync(owner.data(), d_owner.p, ns, hipMemcpyDeviceToHost, s), "D2H owner")) ||
+          (st = hip_check(hipStreamSynchronize(s), "owner")))
+        return st;
+    }
+    std::vector<int64_t> wave_out(ns, -1);
+    std::vector<uint8_t> wave_dim(ns, 0);
     for (int64_t t = 0; t < in.n_tiles; ++t)
       for (int sl = 0; sl < 64; ++sl) {
         const uint8_t o = owner[t * 64 + sl];
@@ -1417,12 +1459,11 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
           return set_errorf(CQMM_ERR_STATE, "tile plan: output block %d x %d outside 1..16", nr, nc);
         wave_dim[k] = (uint8_t)((nr - 1) | (nc - 1) << 4);
       }
-    if ((st = d_owner.alloc(owner.size())) || (st = tp.wave_out64.alloc(wave_out.size() * 8)) ||
+    if ((st = tp.wave_out64.alloc(wave_out.size() * 8)) ||
         (st = tp.wave_dim.alloc(wave_dim.size())) || (st = d_rdim.alloc((size_t)in.n_tiles * 8)) ||
         (st = d_cdim.alloc((size_t)in.n_tiles * 8)) || (st = d_xdim.alloc((size_t)std::max<int64_t>(in.nx, 1))))
       return st;
-    if ((st = hip_check(hipMemcpyAsync(d_owner.p, owner.data(), owner.size(), hipMemcpyHostToDevice, s), "H2D owner")) ||
-        (st = hip_check(hipMemcpyAsync(tp.wave_out64.p, wave_out.data(), wave_out.size() * 8, hipMemcpyHostToDevice, s),
+    if ((st = hip_check(hipMemcpyAsync(tp.wave_out64.p, wave_out.data(), wave_out.size() * 8, hipMemcpyHostToDevice, s),
                         "H2D wave_out")) ||
         (st = hip_check(hipMemcpyAsync(tp.wave_dim.p, wave_dim.data(), wave_dim.size(), hipMemcpyHostToDevice, s),
                         "H2D wave_dim")) ||
