@@ -56,7 +56,11 @@ void parallel_for(int64_t n, F f, int64_t serial_below = 65536) {
 struct PhaseTimer {
   bool on;
   std::chrono::steady_clock::time_point t;
-  PhaseTimer() : on(std::getenv("CQMM_PLAN_TIMING") != nullptr), t(std::chrono::steady_clock::now()) {}
+  PhaseTimer() : on(enabled()), t(std::chrono::steady_clock::now()) {}
+  static bool enabled() {
+    static const bool v = std::getenv("CQMM_PLAN_TIMING") != nullptr;
+    return v;
+  }
   void mark(const char *what) {
     if (!on) return;
     const auto now = std::chrono::steady_clock::now();
