@@ -224,6 +224,7 @@ struct ListReader {
     std::vector<std::string> t;
     if (!items(t, n)) return false;
     for (size_t k = 0; k < n; ++k) {
+      if (t[k].empty()) return false;  // a null value (r*) sets nothing here
       char *end = nullptr;
       const long x = strtol(t[k].c_str(), &end, 10);
       if (!end || *end) return false;
