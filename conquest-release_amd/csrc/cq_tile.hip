@@ -228,9 +228,10 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
         const int n = (int)((cnt >> (4 * u)) & 15u);
         for (int i = 0; i < n; ++i, ++p) {
           const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
-          if constexpr (MODE == 0 && REM == 1) {
+          if constexpr (REM == 1) {
             // NSF = 4m + 1: the last contraction index k = NSF - 1 is a
-            // rank-1 update on the VALU, D[g+4e][r] += A[g+4e][k] * B[k][r],
+            // rank-1 update on the VALU, D[row][r] += A[row][k] * B[k][r]
+            // (one fused multiply-add per accumulator element, FP64 or FP32),
             // instead of an MFMA step with three zeroed k (a quarter fewer
             // MFMAs; accumulators in VGPRs, built with the VGPR MFMA form).
             // Rows g+4e >= NSF read past the block (inside the stage and its
@@ -243,16 +244,31 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
               a[s4] = la[SA * s4];
               bb[s4] = lq[SB * s4];
             }
-            const T *ca = cur + (int)(v & 0xffffu) + NSF * (NSF - 1) + g;
+            // Accumulator rows: FP64 g+4e, FP32 4g+e.
+            const T *ca = cur + (int)(v & 0xffffu) + NSF * (NSF - 1) + (MODE == 0 ? g : 4 * g);
             const T bl = QT ? cur[(int)(v >> 16) + NSF * (NSF - 1) + r] : cur[(int)(v >> 16) + (NSF - 1) + NSF * r];
             T ce[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) ce[e] = ca[4 * e];
+            for (int e = 0; e < 4; ++e) ce[e] = ca[MODE == 0 ? 4 * e : e];
+            if constexpr (MODE == 0) {
 #pragma unroll
-            for (int s4 = 0; s4 < C::KS - 1; ++s4)
-              a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], a64[u], 0, 0, 0);
+              for (int s4 = 0; s4 < C::KS - 1; ++s4)
+                a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], bb[s4], a64[u], 0, 0, 0);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) a64[u][e] = __builtin_fma(ce[e], bl, a64[u][e]);
+              for (int e = 0; e < 4; ++e) a64[u][e] = __builtin_fma(ce[e], bl, a64[u][e]);
+            } else if constexpr (MODE == 1) {
+#pragma unroll
+              for (int s4 = 0; s4 < C::KS - 1; ++s4)
+                a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], a32[u], 0, 0, 0);
+#pragma unroll
+              for (int e = 0; e < 4; ++e) a32[u][e] = __builtin_fmaf(ce[e], bl, a32[u][e]);
+            } else {
+              flt4 t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+              for (int s4 = 0; s4 < C::KS - 1; ++s4) t = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], t, 0, 0, 0);
+#pragma unroll
+              for (int e = 0; e < 4; ++e) a64[u][e] += (double)__builtin_fmaf(ce[e], bl, t[e]);
+            }
             continue;
           }
           const T *la = cur + (int)(v & 0xffffu) + oA;
