@@ -686,14 +686,16 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     tp.n_seg = n_items;
     std::vector<int64_t> seg_out(n_items);
     std::vector<uint8_t> seg_mn(uniform ? 0 : 2 * n_items);
-    for (int64_t t = 0; t < n_items; ++t) {
-      const int32_t kh = item_k[t], i = T.i_prim[t];
-      seg_out[t] = (int64_t)A.i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1;
-      if (!uniform) {
-        seg_mn[2 * t] = (uint8_t)A.ndimi[i - 1];
-        seg_mn[2 * t + 1] = (uint8_t)A.ndimj[kh];
+    cq::parallel_for(n_items, [&](int64_t b, int64_t e) {
+      for (int64_t t = b; t < e; ++t) {
+        const int32_t kh = item_k[t], i = T.i_prim[t];
+        seg_out[t] = (int64_t)A.i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1;
+        if (!uniform) {
+          seg_mn[2 * t] = (uint8_t)A.ndimi[i - 1];
+          seg_mn[2 * t + 1] = (uint8_t)A.ndimj[kh];
+        }
       }
-    }
+    });
     if ((st = upload(tp.seg_beg, item_off.data(), item_off.size(), s))) return st;
     if ((st = upload(tp.seg_out, seg_out.data(), seg_out.size(), s))) return st;
     if (!uniform && (st = upload(tp.seg_mn, seg_mn.data(), seg_mn.size(), s))) return st;
@@ -707,25 +709,55 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     if (!uniform) tp.tk = std::move(d_k);
   } else {
     // Output blocks of C in storage order: nonzero chalo%i_h2d in row-major order.
-    std::vector<int32_t> qkey;
-    std::vector<int64_t> seg_out;
-    std::vector<uint8_t> seg_mn;
+    // Counted per row of C (checking every offset), then filled at the row's
+    // first segment, in parallel.
     const int64_t nc_all = (int64_t)d->n_prim * ni_c;
-    for (int64_t pos = 0; pos < nc_all; ++pos) {
-      const int64_t off = (int64_t)C.i_h2d[pos];
-      if (off == 0) continue;
-      const int32_t i = (int32_t)(pos / ni_c), jh = (int32_t)(pos % ni_c);
-      if (off < 1 || (int64_t)off - 1 + (int64_t)C.ndimi[i] * C.ndimj[jh] > d->lenc)
-        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: C block (%d,%d) offset %lld invalid", i + 1, jh + 1,
-                              (long long)off);
-      qkey.push_back((int32_t)pos);
-      seg_out.push_back(off - 1);
-      if (!uniform) {
-        seg_mn.push_back((uint8_t)C.ndimi[i]);
-        seg_mn.push_back((uint8_t)C.ndimj[jh]);
+    std::vector<int64_t> rbeg((size_t)d->n_prim + 1, 0);
+    std::atomic<int64_t> bad(INT64_MAX);  // first position with an invalid offset
+    cq::parallel_for(d->n_prim, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) {
+        int64_t c = 0;
+        for (int32_t jh = 0; jh < ni_c; ++jh) {
+          const int64_t off = (int64_t)C.i_h2d[i * ni_c + jh];
+          if (off == 0) continue;
+          ++c;
+          if (off < 1 || off - 1 + (int64_t)C.ndimi[i] * C.ndimj[jh] > d->lenc) {
+            int64_t cur = bad.load();
+            const int64_t pos = i * ni_c + jh;
+            while (pos < cur && !bad.compare_exchange_weak(cur, pos)) {
+            }
+          }
+        }
+        rbeg[i + 1] = c;
       }
+    }, 64);
+    if (bad.load() != INT64_MAX) {
+      const int64_t pos = bad.load();
+      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: C block (%d,%d) offset %lld invalid",
+                            (int)(pos / ni_c) + 1, (int)(pos % ni_c) + 1, (long long)C.i_h2d[pos]);
     }
-    tp.n_seg = (int64_t)qkey.size();
+    for (int32_t i = 0; i < d->n_prim; ++i) rbeg[i + 1] += rbeg[i];
+    const int64_t nseg = rbeg[d->n_prim];
+    std::vector<int32_t> qkey(nseg);
+    std::vector<int64_t> seg_out(nseg);
+    std::vector<uint8_t> seg_mn(uniform ? 0 : 2 * nseg);
+    cq::parallel_for(d->n_prim, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) {
+        int64_t k = rbeg[i];
+        for (int32_t jh = 0; jh < ni_c; ++jh) {
+          const int64_t off = (int64_t)C.i_h2d[i * ni_c + jh];
+          if (off == 0) continue;
+          qkey[k] = (int32_t)(i * ni_c + jh);
+          seg_out[k] = off - 1;
+          if (!uniform) {
+            seg_mn[2 * k] = (uint8_t)C.ndimi[i];
+            seg_mn[2 * k + 1] = (uint8_t)C.ndimj[jh];
+          }
+          ++k;
+        }
+      }
+    }, 64);
+    tp.n_seg = nseg;
     if ((st = upload(tp.seg_out, seg_out.data(), seg_out.size(), s))) return st;
     if (!uniform && (st = upload(tp.seg_mn, seg_mn.data(), seg_mn.size(), s))) return st;
     // Stable radix sort of the products by output block keeps the k order.

@@ -143,6 +143,11 @@ def test_bad_descriptor_is_rejected_before_any_kernel():
     d.lenb = 10  # B blocks now point past lenb
     with pytest.raises(cqmm.CqmmError, match="lenb|B block"):
         cqmm.Plan(d)
+    case2 = Case(30, 4, 5.0, 6.0, 11.0, 1, seed=3)
+    d2 = case2.desc
+    d2.lenc = 10  # C blocks now point past lenc
+    with pytest.raises(cqmm.CqmmError, match="lenc|C block"):
+        cqmm.Plan(d2)
 
 
 TILE_CASES = [
