@@ -308,3 +308,39 @@ def test_plan_tune_keeps_the_faster_kernel_and_a_correct_result(nsf, mt):
     assert np.abs(out - ref).max() <= TOL * np.abs(ref).max()
     for x in (da, db, dc):
         x.free()
+
+
+def test_cfg2_full_size_properties():
+    """The DZP drop-in workload at full size (4096 atoms, 13x13 blocks,
+    X(16 A) = A(8 A).B(16 A)^T, 273,159,028 block products), checked through
+    properties that hold at any size: the tile kernel and the per-output-block
+    kernel (different schedules and summation structure) agree to 1e-13
+    relative; doubling an operand doubles the result bit for bit; every output
+    element is overwritten. The oracle checks the same shape on 128 atoms."""
+    xyz, edge = cqmm.random_structure(4096, 0.05, 2.0, 1)
+    print("cfg2 full size: building the system", flush=True)
+    sysm = cqmm.System(xyz, edge, 13, 32.0, cqmm.default_grid(edge, 4.0))
+    ids = {r: sysm.add_range(r) for r in (8.0, 16.0)}
+    desc = sysm.mult_desc(2, ids[16.0], ids[16.0], ids[8.0])
+    b = sysm.fill(ids[16.0], 0, 1, 2, (2.0,))
+    c = sysm.fill(ids[8.0], 0, 1, 3, (2.0,))
+    n_out = sysm.length(ids[16.0])
+    plan = cqmm.Plan(desc)
+    info = plan.info()
+    print(f"  products {info.n_triplets}, tiles {info.n_tiles}", flush=True)
+    assert info.n_triplets == 273159028 and info.n_tiles > 0
+    out = {}
+    for kernel in (cqmm.KERNEL_TILE, cqmm.KERNEL_TRIPLET):
+        plan.set_kernel(kernel)
+        a = np.full(n_out, np.nan)
+        plan.mat_mult(a, b, c)
+        assert np.isfinite(a).all()
+        out[kernel] = a
+    ref = out[cqmm.KERNEL_TRIPLET]
+    err = np.abs(out[cqmm.KERNEL_TILE] - ref).max() / np.abs(ref).max()
+    print(f"  tile vs per-block kernel: rel err {err:.3e}", flush=True)
+    assert err <= 1e-13
+    plan.set_kernel(cqmm.KERNEL_TILE)
+    a2 = np.full(n_out, np.nan)
+    plan.mat_mult(a2, b, 2.0 * c)
+    np.testing.assert_array_equal(a2, 2.0 * out[cqmm.KERNEL_TILE])
