@@ -100,7 +100,7 @@ int64_t cqmm_purify_hbm_estimate(int32_t natoms, int32_t nsf, const cqmm_purify_
 }
 
 int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_params *p, cqmm_purify_result *r) {
-  if (!p || !r || natoms <= 0 || nsf <= 0 || nsf > 16) return cq::set_error(CQMM_ERR_ARG, "cqmm_purify: bad argument");
+  if (!p || !r || natoms <= 0 || nsf <= 0 || nsf > 64) return cq::set_error(CQMM_ERR_ARG, "cqmm_purify: bad argument");
   std::memset(r, 0, sizeof(*r));
   r->natoms = natoms;
   r->n_orb = natoms * nsf;

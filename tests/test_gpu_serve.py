@@ -30,6 +30,17 @@ def test_native_solve_equals_python_solve():
     assert r["ne_error"] < 1e-8
 
 
+def test_large_blocks_solve_within_the_estimate():
+    """20 support functions per atom (blocks above 16x16: no tile schedule,
+    sub-block product kernel): the solve converges to the electron count and
+    its measured device peak stays within the admission estimate."""
+    params = sv.default_params(precision=0, **SMALL)
+    r = sv.purify(24, 20, 7, params)
+    assert r["status"] == 0 and r["ne_error"] < 1e-8
+    assert np.isfinite(r["band_energy"])
+    assert 0 < r["hbm_peak"] <= r["hbm_estimate"]
+
+
 def test_requests_alone_equal_requests_in_the_mix():
     params = sv.default_params(precision=1, **SMALL)
     reqs = [dict(natoms=n, nsf=f, seed=s, arrival_s=0.01 * k)
