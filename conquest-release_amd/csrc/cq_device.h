@@ -81,7 +81,9 @@ struct TilePlan {
   bool dense = false;
   DevBuf chunk_entry;  // int32 [n_chunks+1] first entry of each chunk
   DevBuf entry_mask;   // uint32 [n_entries] left-operand rows | right-operand columns << 8 present
-  DevBuf entry_coff;   // int32 [n_entries] first operand block of the entry in its chunk's stage
+  DevBuf entry_coff;   // int32 [n_entries] first operand block (mixed: element) of the entry in its chunk's stage
+  DevBuf tile_rdim;    // uint8 [n_tiles*8] rows of each tile row's blocks (mixed dense: blocks up to 4x4)
+  DevBuf tile_cdim;    // uint8 [n_tiles*8] columns of each tile column's blocks
   // Mixed block sizes (maxn > 0): operands and outputs by element offset.
   int maxn = 0;        // largest block dimension; 0: uniform schedule above
   DevBuf list64;       // uint64 [n_loads] operand element offset; bit 63 set: right operand

@@ -730,7 +730,8 @@ __global__ void k_entry_fill_mx(const uint64_t *__restrict__ key, const int32_t 
   }
 }
 
-// Mask, operand elements and operand count of each entry.
+// Mask (left rows | right columns << 8 | (contraction dimension - 1) << 16),
+// operand elements and operand count of each entry.
 __global__ void k_entry_mask_mx(const int64_t *__restrict__ ep, const int64_t *__restrict__ eq,
                                 const int32_t *__restrict__ etile, const int32_t *__restrict__ ex,
                                 const uint8_t *__restrict__ rdim, const uint8_t *__restrict__ cdim,
@@ -752,7 +753,7 @@ __global__ void k_entry_mask_mx(const int64_t *__restrict__ ep, const int64_t *_
       sz += nxd * cdim[t * 8 + u];
     }
   }
-  mask[e] = m;
+  mask[e] = m | (uint32_t)(nxd - 1) << 16;  // contraction dimension in bits 16-19
   esz[e] = sz;
   cnt[e] = __popc(m);
 }
@@ -781,34 +782,39 @@ __global__ void k_list_fill_mx(const int64_t *__restrict__ ep, const int64_t *__
     }
 }
 
-// Chunks of a tile by operand elements (at most cap), entries and
-// wavefront products (as k_chunks).
+// Chunks of a tile by operand elements (at most cap), entries, operand
+// blocks (at most kWaves * 64) and wavefront products (as k_chunks).
 template <bool FILL>
 __global__ void k_chunks_mx(const int32_t *__restrict__ tebeg, const uint32_t *__restrict__ mask,
                             const int32_t *__restrict__ esz, const uint8_t *__restrict__ owner, int32_t n_tiles,
-                            int32_t cap, const int32_t *__restrict__ tchunk, int32_t *__restrict__ out) {
+                            int32_t cap, int32_t max_entries, const int32_t *__restrict__ tchunk,
+                            int32_t *__restrict__ out) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tiles) return;
-  const uint8_t *own = owner + (int64_t)t * 64;
-  int32_t nch = 0, sum = 0, ne = 0;
+  int32_t nch = 0, sum = 0, ne = 0, nb = 0;
   int wc[kWaves] = {};
   for (int32_t e = tebeg[t]; e < tebeg[t + 1]; ++e) {
     const uint32_t m = mask[e];
     const int32_t c = esz[e];
-    int pw[kWaves];
-    entry_wave_counts(m, own, pw);
+    const int32_t b = __popc(m & 0xffffu);
+    int pw[kWaves] = {};
+    if (owner) entry_wave_counts(m, owner + (int64_t)t * 64, pw);  // no owner: no per-wavefront limit
     bool over = false;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) over |= wc[w] + pw[w] > kWaveProducts;
-    if (nch == 0 || sum + c > cap || ne == kChunkEntries || over) {
+    // A chunk's operand blocks are spread over the lanes of the wavefronts'
+    // operand lists: at most kWaves * 64 of them.
+    if (nch == 0 || sum + c > cap || ne == max_entries || nb + b > kWaves * 64 || over) {
       if (FILL) out[tchunk[t] + nch] = e;
       ++nch;
       sum = 0;
       ne = 0;
+      nb = 0;
 #pragma unroll
       for (int w = 0; w < kWaves; ++w) wc[w] = 0;
     }
     sum += c;
+    nb += b;
     ++ne;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) wc[w] += pw[w];
@@ -1092,6 +1098,149 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_mixed(
   }
 }
 
+// Mixed block sizes of at most 4x4 (e.g. C and H with single-zeta bases):
+// the dense-quadrant scheme of k_tile_dense4 over the mixed-size schedule.
+// Wavefront w owns quadrant (w / 2, w % 2) of the tile (4x4 blocks, each
+// inside a 4x4 corner of the quadrant's 16x16 MFMA tile) and issues one
+// 16x16x4 MFMA per entry: lane (r, g) supplies element (r % 4, g) of
+// P(row 4*(w/2) + r/4, x) and element (g, r % 4) of Q(column 4*(w%2) + r/4, x),
+// zero outside the blocks' shapes or where the entry lacks the block. A
+// block's LDS offset within its entry is the entry offset plus nx times the
+// rows (columns) of the entry's blocks before it, summed from the tile's
+// block-shape bit masks with three popcounts.
+template <int MODE, bool QT>
+__global__ void __launch_bounds__(kTileThreads) k_tile_dense_mx(
+    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
+    const uint64_t *__restrict__ list64, const uint32_t *__restrict__ listls,
+    const int32_t *__restrict__ chunk_entry, const uint32_t *__restrict__ emask, const int32_t *__restrict__ ecoff,
+    const uint8_t *__restrict__ rdim, const uint8_t *__restrict__ cdim, const int64_t *__restrict__ wave_out,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
+    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
+    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
+  constexpr int STAGE = kStgSmall + kLdsPad * (int)(8 / sizeof(T));
+  __shared__ T stage0[STAGE];
+  __shared__ T stage1[STAGE];
+  const int32_t per = (n_tiles + 7) >> 3;
+  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  if (tile >= n_tiles) return;  // the whole workgroup leaves together
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, r = lane & 15, g = lane >> 4, rr = r & 3;
+  const int qr = w >> 1, qc = w & 1;
+  const int ua = 4 * qr + (r >> 2), cb = 4 * qc + (r >> 2);
+  // Block shapes of the tile: this lane's row and column, and bit masks of the
+  // rows (columns) whose size has bit 0, 1 or 2 set.
+  const int rd = rdim[(int64_t)tile * 8 + ua], cd = cdim[(int64_t)tile * 8 + cb];
+  uint32_t rb0 = 0, rb1 = 0, rb2 = 0, cb0 = 0, cb1 = 0, cb2 = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t a = rdim[(int64_t)tile * 8 + k], c = cdim[(int64_t)tile * 8 + k];
+    rb0 |= (a & 1u) << k;
+    rb1 |= ((a >> 1) & 1u) << k;
+    rb2 |= ((a >> 2) & 1u) << k;
+    cb0 |= (c & 1u) << k;
+    cb1 |= ((c >> 1) & 1u) << k;
+    cb2 |= ((c >> 2) & 1u) << k;
+  }
+  auto rows_of = [&](uint32_t m) { return __popc(m & rb0) + 2 * __popc(m & rb1) + 4 * __popc(m & rb2); };
+  auto cols_of = [&](uint32_t m) { return __popc(m & cb0) + 2 * __popc(m & cb1) + 4 * __popc(m & cb2); };
+  const uint32_t below_a = (1u << ua) - 1u, below_b = (1u << cb) - 1u;
+  const uint32_t quad_rows = 0xfu << (4 * qr), quad_cols = 0xfu << (8 + 4 * qc);
+  dbl4 a64 = {0.0, 0.0, 0.0, 0.0};
+  flt4 a32 = {0.f, 0.f, 0.f, 0.f};
+  const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
+  if (c0 < c1) {
+    int32_t wb = c0;
+    int32_t cc = min(wb + lane, c1);
+    int64_t wcl = chunk_lbeg[cc];
+    int32_t wce = chunk_entry[cc];
+    auto cl_at = [&](int32_t x) { return readlane64(wcl, min(x, c1) - wb); };
+    auto ce_at = [&](int32_t x) { return __builtin_amdgcn_readlane(wce, min(x, c1) - wb); };
+    auto list_idx = [&](int64_t lbeg, int32_t nb) { return lbeg + min(w + kWaves * lane, nb - 1); };
+    auto entry_regs = [&](int32_t e0, int32_t e1, uint32_t &m, int32_t &o) {
+      const int32_t e = min(e0 + lane, e1 - 1);
+      m = emask[e];
+      o = ecoff[e];
+    };
+    uint32_t em, emn = 0;
+    int32_t eo, eon = 0;
+    uint64_t lvn = 0;
+    uint32_t lsn = 0;
+    {
+      const int64_t cl0 = cl_at(c0), cl1 = cl_at(c0 + 1);
+      const int64_t i0 = list_idx(cl0, (int32_t)(cl1 - cl0));
+      issue_stage_mx<T>(list64[i0], listls[i0], (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+      entry_regs(ce_at(c0), ce_at(c0 + 1), emn, eon);
+      if (c0 + 1 < c1) {
+        const int64_t i1 = list_idx(cl1, (int32_t)(cl_at(c0 + 2) - cl1));
+        lvn = list64[i1];
+        lsn = listls[i1];
+      }
+    }
+    for (int32_t c = c0; c < c1; ++c) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      const bool odd = ((c - c0) & 1) != 0;
+      const T *cur = odd ? stage1 : stage0;
+      em = emn;
+      eo = eon;
+      const int32_t ne = ce_at(c + 1) - ce_at(c);
+      if (c + 1 < c1) {
+        const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2);
+        issue_stage_mx<T>(lvn, lsn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
+        entry_regs(ce_at(c + 1), ce_at(c + 2), emn, eon);
+        if (c + 2 < c1) {
+          const int64_t i2 = list_idx(cl2, (int32_t)(cl_at(c + 3) - cl2));
+          lvn = list64[i2];
+          lsn = listls[i2];
+        }
+      }
+      for (int i = 0; i < ne; ++i) {
+        const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)em, i);
+        if ((m & quad_rows) == 0 || (m & quad_cols) == 0) continue;  // no block pair in this quadrant
+        const int32_t base = __builtin_amdgcn_readlane(eo, i);
+        const uint32_t ym = m & 0xffu, bm = (m >> 8) & 0xffu;
+        const int nx = (int)((m >> 16) & 15u) + 1;
+        const bool va = ((ym >> ua) & 1u) != 0 && rr < rd && g < nx;
+        const bool vb = ((bm >> cb) & 1u) != 0 && rr < cd && g < nx;
+        const int pa = base + nx * rows_of(ym & below_a);
+        const int pb = base + nx * (rows_of(ym) + cols_of(bm & below_b));
+        const int ia = va ? pa + rr + rd * g : 0;
+        const int ib = vb ? (QT ? pb + rr + cd * g : pb + g + nx * rr) : 0;
+        T a = cur[ia], b = cur[ib];
+        if (!va) a = (T)0;
+        if (!vb) b = (T)0;
+        if constexpr (MODE == 0) {
+          a64 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a64, 0, 0, 0);
+        } else if constexpr (MODE == 1) {
+          a32 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, a32, 0, 0, 0);
+        } else {
+          const flt4 t = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) a64[e] += (double)t[e];
+        }
+      }
+      if (c + 4 > wb + 63 && c + 1 < c1) {
+        wb = c + 1;
+        cc = min(wb + lane, c1);
+        wcl = chunk_lbeg[cc];
+        wce = chunk_entry[cc];
+      }
+    }
+  }
+  // Quadrant row R, column r: element (R % 4, r % 4) of block (R / 4, r / 4);
+  // written where it lies inside that block's shape.
+  const int64_t *wo = wave_out + ((int64_t)tile * kWaves + w) * 16;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int rq = MODE == 0 ? e : g, ri = MODE == 0 ? g : e;
+    const int64_t off = wo[rq * 4 + (r >> 2)];
+    const int nr = rdim[(int64_t)tile * 8 + 4 * qr + rq];
+    if (off >= 0 && ri < nr && rr < cd) O[off + ri + nr * rr] = MODE == 1 ? (T)a32[e] : (T)a64[e];
+  }
+}
+
 template <int MODE, bool QT, class T>
 int launch_mixed(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
   const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
@@ -1101,6 +1250,12 @@ int launch_mixed(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream
   const uint64_t *l64 = tp.list64.as<uint64_t>(), *pc = tp.pcnt.as<uint64_t>();
   const uint32_t *lls = tp.listls.as<uint32_t>(), *pr = tp.prod.as<uint32_t>();
   const uint8_t *wd = tp.wave_dim.as<uint8_t>();
+  if (tp.dense) {
+    hipLaunchKernelGGL((k_tile_dense_mx<MODE, QT>), dim3(gridx), blk, 0, s, tc, cl, l64, lls,
+                       tp.chunk_entry.as<int32_t>(), tp.entry_mask.as<uint32_t>(), tp.entry_coff.as<int32_t>(),
+                       tp.tile_rdim.as<uint8_t>(), tp.tile_cdim.as<uint8_t>(), wo, P, Q, O, tp.n_tiles);
+    return cq::hip_check(hipGetLastError(), "dense mixed tile-product kernel launch");
+  }
   if (stage_elems(tp.maxn) == kStgSmall)
     hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgSmall>), dim3(gridx), blk, 0, s, tc, cl, l64, lls, ps, pc, pr, wo, wd,
                        P, Q, O, tp.n_tiles);
@@ -1443,7 +1598,31 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   tp.maxn = in.maxn;
   tp.n_tiles = in.n_tiles;
   DevBuf d_owner, d_rdim, d_cdim, d_xdim;
-  {
+  tp.dense = in.maxn <= kDenseNsf;
+  if (tp.dense) {
+    // Blocks of at most 4x4: wavefront w computes quadrant (w / 2, w % 2) of
+    // the tile (k_tile_dense_mx); wave_out64 holds its blocks' element offsets.
+    const size_t ns = (size_t)in.n_tiles * 64;
+    std::vector<int64_t> wave_out(ns, -1);
+    parallel_for(in.n_tiles, [&](int64_t b, int64_t e) {
+      for (int64_t t = b; t < e; ++t)
+        for (int u = 0; u < 8; ++u)
+          for (int c = 0; c < 8; ++c)
+            wave_out[(t * kWaves + (u >> 2) * 2 + (c >> 2)) * 16 + (u & 3) * 4 + (c & 3)] = in.tile_out[t * 64 + u * 8 + c];
+    });
+    if ((st = tp.wave_out64.alloc(ns * 8)) || (st = d_rdim.alloc((size_t)in.n_tiles * 8)) ||
+        (st = d_cdim.alloc((size_t)in.n_tiles * 8)) || (st = d_xdim.alloc((size_t)std::max<int64_t>(in.nx, 1))))
+      return st;
+    if ((st = hip_check(hipMemcpyAsync(tp.wave_out64.p, wave_out.data(), ns * 8, hipMemcpyHostToDevice, s),
+                        "H2D wave_out")) ||
+        (st = hip_check(hipMemcpyAsync(d_rdim.p, in.tile_rdim, (size_t)in.n_tiles * 8, hipMemcpyHostToDevice, s),
+                        "H2D rdim")) ||
+        (st = hip_check(hipMemcpyAsync(d_cdim.p, in.tile_cdim, (size_t)in.n_tiles * 8, hipMemcpyHostToDevice, s),
+                        "H2D cdim")) ||
+        (st = hip_check(hipMemcpyAsync(d_xdim.p, in.xdim, (size_t)in.nx, hipMemcpyHostToDevice, s), "H2D xdim")) ||
+        (st = hip_check(hipStreamSynchronize(s), "dense tile data")))
+      return st;
+  } else {
     // Slot owners on the device (from which output blocks exist), then the
     // per-wavefront element offsets and block shapes on the host.
     const size_t ns = (size_t)in.n_tiles * 64;
@@ -1556,9 +1735,10 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   if ((st = d_tebeg.alloc(((size_t)in.n_tiles + 1) * 4)) || (st = d_nch.alloc((size_t)in.n_tiles * 4))) return st;
   hipLaunchKernelGGL(k_tile_ebeg, dim3(grid1((int64_t)in.n_tiles + 1, 256)), dim3(256), 0, s, d_etile.as<int32_t>(), ne,
                      in.n_tiles, d_tebeg.as<int32_t>());
+  const int32_t max_entries = tp.dense ? kDenseEntries : kChunkEntries;
   hipLaunchKernelGGL((k_chunks_mx<false>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
-                     d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, nullptr,
-                     d_nch.as<int32_t>());
+                     d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, max_entries,
+                     nullptr, d_nch.as<int32_t>());
   if ((st = tp.tile_chunk.alloc(((size_t)in.n_tiles + 1) * 4))) return st;
   if ((st = hip_check(hipMemsetAsync(tp.tile_chunk.p, 0, 4, s), "memset tile_chunk"))) return st;
   if ((st = inclusive_scan(d_nch.as<int32_t>(), tp.tile_chunk.as<int32_t>() + 1, (size_t)in.n_tiles, s))) return st;
@@ -1571,7 +1751,7 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   DevBuf d_centry;
   if ((st = d_centry.alloc(((size_t)nch + 1) * 4))) return st;
   hipLaunchKernelGGL((k_chunks_mx<true>), dim3(grid1(in.n_tiles, 256)), dim3(256), 0, s, d_tebeg.as<int32_t>(),
-                     d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap,
+                     d_emask.as<uint32_t>(), d_esz.as<int32_t>(), d_owner.as<uint8_t>(), in.n_tiles, cap, max_entries,
                      tp.tile_chunk.as<int32_t>(), d_centry.as<int32_t>());
   if ((st = hip_check(hipMemcpyAsync(d_centry.as<int32_t>() + nch, &ne32, 4, hipMemcpyHostToDevice, s),
                       "H2D chunk end")))
@@ -1584,6 +1764,17 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
                      d_esz.as<int32_t>(), (int64_t)nch, d_ecoff.as<int32_t>());
   hipLaunchKernelGGL(k_list_lds_mx, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, tp.chunk_lbeg.as<int64_t>(),
                      d_lsz.as<uint16_t>(), (int64_t)nch, tp.listls.as<uint32_t>());
+  if (tp.dense) {
+    // The dense kernel reads the entries and the tile block shapes; no product lists.
+    tp.chunk_entry = std::move(d_centry);
+    tp.entry_mask = std::move(d_emask);
+    tp.entry_coff = std::move(d_ecoff);
+    tp.tile_rdim = std::move(d_rdim);
+    tp.tile_cdim = std::move(d_cdim);
+    tp.n_prod = n;
+    if ((st = hip_check(hipGetLastError(), "tile builder launch"))) return st;
+    return hip_check(hipStreamSynchronize(s), "tile build");
+  }
   // Product lists.
   DevBuf d_ctile, d_np, d_pbase;
   if ((st = d_ctile.alloc((size_t)std::max(nch, 1) * 4)) || (st = d_np.alloc((size_t)std::max(nch, 1) * 8)) ||

@@ -208,6 +208,12 @@ MIXED_TILE_CASES = [
     (40, ((16, .3), (4, .4), (9, .3)), 6.0, 6.0, 6.0, 2),
     (40, 5, 6.0, 6.0, 12.0, 1),                              # a uniform size without a fixed-size instance
     (160, ((4, .5), (9, .25), (13, .25)), 10.0, 12.0, 8.0, 2),  # many tiles, multi-chunk entries
+    # Blocks of at most 4x4 (single-zeta C/H-like mixes, uniform 1): the dense-quadrant mixed kernel.
+    (60, ((4, .6), (1, .4)), 6.0, 7.0, 10.0, 1),
+    (60, ((4, .6), (1, .4)), 7.0, 9.0, 6.0, 2),
+    (40, ((2, .5), (3, .5)), 6.0, 7.0, 10.0, 1),
+    (50, 1, 6.0, 8.0, 12.0, 1),
+    (160, ((4, .6), (1, .4)), 10.0, 12.0, 8.0, 2),             # multi-chunk entries
 ]
 
 
@@ -253,6 +259,7 @@ FP32_CASES = [
     (40, 4, 6.0, 8.0, 6.0, 2),
     (30, ((4, .5), (22, .5)), 5.0, 6.0, 7.0, 1),               # blocks above 16x16: sub-block kernel
     (24, 20, 5.0, 6.0, 6.0, 2),
+    (60, ((4, .6), (1, .4)), 6.0, 7.0, 10.0, 1),               # dense-quadrant mixed kernel
 ]
 
 
