@@ -59,7 +59,9 @@ def kernel_name(info, nsf):
     """Name of the FP64 product kernel a plan runs (as rocprofv3 lists it, shortened)."""
     mixed = not isinstance(nsf, int)
     if info.kernel == cqmm.KERNEL_TILE:
-        return "k_tile_mixed_f64" if mixed else ("k_tile_dense4_f64" if nsf == 4 else "k_tile_products_f64")
+        if mixed:
+            return "k_tile_dense_mx_f64" if max(n for n, _ in nsf) <= 4 else "k_tile_mixed_f64"
+        return "k_tile_dense4_f64" if nsf == 4 else "k_tile_products_f64"
     return "k_block_products_f64" if mixed else "k_block_products_f64_u"
 
 
