@@ -214,6 +214,11 @@ MIXED_TILE_CASES = [
     (40, ((2, .5), (3, .5)), 6.0, 7.0, 10.0, 1),
     (50, 1, 6.0, 8.0, 12.0, 1),
     (160, ((4, .6), (1, .4)), 10.0, 12.0, 8.0, 2),             # multi-chunk entries
+    # Double-zeta C/H-like mixes (blocks up to 8x8): the general mixed kernel.
+    (50, ((8, .6), (2, .4)), 6.0, 7.0, 10.0, 1),
+    (50, ((8, .6), (2, .4)), 7.0, 9.0, 6.0, 2),
+    (40, ((6, .5), (7, .5)), 6.0, 7.0, 9.0, 1),
+    (120, ((8, .6), (2, .4)), 9.0, 11.0, 7.0, 2),
 ]
 
 
@@ -260,6 +265,7 @@ FP32_CASES = [
     (30, ((4, .5), (22, .5)), 5.0, 6.0, 7.0, 1),               # blocks above 16x16: sub-block kernel
     (24, 20, 5.0, 6.0, 6.0, 2),
     (60, ((4, .6), (1, .4)), 6.0, 7.0, 10.0, 1),               # dense-quadrant mixed kernel
+    (50, ((8, .6), (2, .4)), 6.0, 7.0, 10.0, 1),               # general mixed kernel, blocks up to 8x8
 ]
 
 
