@@ -2,6 +2,9 @@
 finishes in seconds). Lengths in Angstrom."""
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 
 import cqmm
@@ -22,8 +25,13 @@ class Case:
 
     mult_type 1: c(rc) = a(ra).b(rb); mult_type 2: a(ra) = c(rc).b(rb)^T."""
 
-    def __init__(self, natoms, nsf, ra, rb, rc, mult_type=1, seed=1, density=0.05, grid=None, decay=2.0):
-        self.xyz, self.edge = cqmm.random_structure(natoms, density, 2.0, seed)
+    def __init__(self, natoms, nsf, ra, rb, rc, mult_type=1, seed=1, density=0.05, grid=None, decay=2.0,
+                 structure=None):
+        if structure is None:
+            self.xyz, self.edge = cqmm.random_structure(natoms, density, 2.0, seed)
+        else:                                   # a fixed (xyz, cubic edge) in the ranges' length unit
+            self.xyz, self.edge = np.asarray(structure[0], np.float64), float(structure[1])
+            assert len(self.xyz) == natoms
         self.nsf = species_nsf(natoms, nsf, seed)
         rcover = max(ra + rb, rc, ra, rb)
         if grid is None:
@@ -64,3 +72,24 @@ def cluster_structure(natoms, radius=3.5, dmin=1.8, box=50.0, seed=1):
         if all(np.linalg.norm(p - q) >= dmin for q in pts):
             pts.append(p)
     return np.array(pts) + box / 2.0, box
+
+
+# The structure of the reference's Order-N test 002 (8 Si atoms, 10.36 bohr
+# cubic cell, 4 support functions per atom, S range 14.24 bohr, L range 16
+# bohr): tests/golden/si8_test002.json, written by tests/golden/make_si8.py.
+SI8 = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "si8_test002.json")))
+
+# Products of the LNV step on it: L.S (full), (L.S).L truncated to L's
+# pattern, and the type-2 form that builds a result on L's pattern from an
+# operand on the L+S range.
+SI8_PRODUCTS = [
+    (SI8["l_range_bohr"], SI8["s_range_bohr"], SI8["l_range_bohr"] + SI8["s_range_bohr"], 1),
+    (SI8["l_range_bohr"] + SI8["s_range_bohr"], SI8["l_range_bohr"], SI8["l_range_bohr"], 1),
+    (SI8["l_range_bohr"], SI8["s_range_bohr"], SI8["l_range_bohr"] + SI8["s_range_bohr"], 2),
+]
+
+
+def si8_case(ra, rb, rc, mt, seed=3):
+    edge = SI8["cell_bohr"][0][0]
+    xyz = np.array(SI8["frac"]) * edge
+    return Case(8, SI8["nsf"], ra, rb, rc, mt, seed=seed, grid=(2, 2, 2), structure=(xyz, edge))

@@ -15,7 +15,7 @@ import pytest
 
 import cqmm
 from cqmm.dm import Purifier
-from cases import cluster_structure
+from cases import SI8, cluster_structure
 from oracle import dmm as od
 from oracle import lnv
 from oracle import mcweeny as mo
@@ -87,6 +87,25 @@ def test_lnv_matches_sparse_oracle_periodic(nsf):
     only_phi = pu.lnv(8)
     assert abs(only_phi["electrons"] - got["electrons"]) < 1e-12 * abs(got["electrons"])
     np.testing.assert_array_equal(only_phi["phi"], got["phi"])
+
+
+def test_lnv_matches_sparse_oracle_on_si8_reference_structure():
+    """The structure and ranges of the reference's Order-N test 002 (bohr):
+    L.S.L reaches 46 bohr, over four cells, on 8 atoms."""
+    edge = SI8["cell_bohr"][0][0]
+    xyz = np.array(SI8["frac"]) * edge
+    pu = Purifier(xyz, edge, SI8["nsf"], r_S=SI8["s_range_bohr"], r_H=SI8["s_range_bohr"],
+                  r_L=SI8["l_range_bohr"], grid=(2, 2, 2), seed=5, lnv=True)
+    print("lnv si8 test-002 structure", flush=True)
+    sp = mo.Sparse(pu.sys, pu.rid)
+    L = random_L(pu, 7)
+    pu.ctx.set(pu.matL, L)
+    got = pu.lnv(15)
+    ref = lnv.lnv_sparse(sp, L, pu.S_vals, pu.H_vals)
+    for k in ("K", "M3", "phi"):
+        assert rel(got[k], ref[k]) < 1e-11, k
+    assert abs(got["energy"] - ref["energy"]) < 1e-11 * max(1.0, abs(ref["energy"]))
+    assert abs(got["electrons"] - ref["electrons"]) < 1e-11 * max(1.0, abs(ref["electrons"]))
 
 
 def test_lnv_matches_dense_closed_form_on_cluster():

@@ -14,7 +14,7 @@ import pytest
 
 import cqmm
 import oracle
-from cases import Case
+from cases import SI8_PRODUCTS, Case, si8_case
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +86,13 @@ SMALL = [
 @pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", SMALL)
 def test_small_parity(natoms, nsf, ra, rb, rc, mt):
     check_case(Case(natoms, nsf, ra, rb, rc, mt, seed=21))
+
+
+@pytest.mark.parametrize("ra,rb,rc,mt", SI8_PRODUCTS)
+def test_si8_reference_structure_parity(ra, rb, rc, mt):
+    """The structure of the reference's Order-N test 002 (diamond Si, every range
+    longer than the cell): many images of each atom per pattern."""
+    check_case(si8_case(ra, rb, rc, mt))
 
 
 def test_cfg1_shape_full_size_parity():
