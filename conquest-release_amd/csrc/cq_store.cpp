@@ -20,6 +20,8 @@
 // File name: <root>matrix2.i<index, 2 digits>.p<inode, 6 digits> (get_file_name_2rank,
 // src/io_module.f90:3134-3164); one process writes inode 1.
 #include <cctype>
+#include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -123,11 +125,21 @@ bool fortran_real(std::string t, double *v) {
 struct Reader {
   FILE *f = nullptr;
   bool binary = true;
+  int64_t fsize = 0;  // bytes in the file: no count read from it may ask for more
   std::vector<char> rec;
   size_t pos = 0;
+  bool open(const char *path) {
+    f = fopen(path, binary ? "rb" : "r");
+    if (!f) return false;
+    if (fseek(f, 0, SEEK_END) == 0) fsize = ftell(f);
+    rewind(f);
+    return fsize >= 0;
+  }
+  int64_t left() const { return fsize - (int64_t)ftell(f); }
   bool next_record() {
     uint32_t n = 0, n2 = 0;
     if (fread(&n, 4, 1, f) != 1) return false;
+    if ((int64_t)n + 4 > left()) return false;
     rec.resize(n);
     if (n && fread(rec.data(), n, 1, f) != 1) return false;
     if (fread(&n2, 4, 1, f) != 1 || n2 != n) return false;
@@ -154,7 +166,11 @@ struct Reader {
     std::string t;
     for (int64_t k = 0; k < n; ++k) {
       if (!token(t)) return false;
-      v[k] = (int32_t)strtol(t.c_str(), nullptr, 10);
+      char *end = nullptr;
+      errno = 0;
+      const long x = strtol(t.c_str(), &end, 10);
+      if (errno || *end != '\0' || x < INT32_MIN || x > INT32_MAX) return false;
+      v[k] = (int32_t)x;
     }
     return true;
   }
@@ -363,9 +379,9 @@ int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *
   const RangeIndex &r = s.ranges[range_id];
   Reader rd;
   rd.binary = binary != 0;
-  rd.f = fopen(path, rd.binary ? "rb" : "r");
-  if (!rd.f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: cannot open %s", path);
+  const bool opened = rd.open(path);
   FileCloser fc{rd.f};
+  if (!opened) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: cannot open %s", path);
   auto bad = [&](const char *what) {
     return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: %s: malformed or truncated (%s)", path, what);
   };
@@ -378,6 +394,12 @@ int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *
   if (!rd.ints(nsf_i.data(), nprim) || !rd.ints(jmax.data(), nprim) || !rd.ints(jbeta.data(), nprim) ||
       !rd.ints(&ns, 1))
     return bad("header");
+  // Counts that the file cannot hold (every value takes at least one byte of
+  // it) are rejected before anything is sized by them.
+  for (int32_t k = 0; k < nprim; ++k)
+    if (nsf_i[k] < 1 || nsf_i[k] > 64 || jmax[k] < 0 || jmax[k] > rd.fsize || jbeta[k] < 0 ||
+        jbeta[k] > 64 * (int64_t)jmax[k])
+      return bad("row sizes");
   if (ns < 1 || ns > nspin_max) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: %s holds %d spins, room for %d", path, ns, nspin_max);
   for (int32_t sp = 0; sp < ns; ++sp)
     if (!out[sp]) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: null output");
@@ -408,6 +430,7 @@ int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *
     for (int32_t jj = 0; jj < jmax[k]; ++jj)
       if (!rd.doubles(&vec[3 * (size_t)jj], 3)) return bad("vec_Rij");
     const int64_t len = (int64_t)jbeta[k] * nsf_i[k];
+    if (len * ns > rd.left()) return bad("values");
     vals.resize((size_t)len * ns);
     for (int32_t sp = 0; sp < ns; ++sp)
       for (int64_t q = 0; q < len; ++q)
@@ -420,6 +443,7 @@ int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *
       const int32_t jg = idj[jj] - 1;
       const int64_t nval = (int64_t)beta[jj] * nsf_i[k];
       if (jg < 0 || jg >= s.natoms) return bad("idglob_j out of range");
+      if (beta[jj] < 1 || beta[jj] > 64 || voff + nval > len) return bad("neighbour sizes exceed jbeta_max_i");
       const int32_t pj = prim_of[jg];
       // Image of j at R_i - vec * cell: shift = (R_i - R_j) / cell - vec.
       const double fx = (s.px[p] - s.px[pj]) / s.box[0] - vec[3 * (size_t)jj];

@@ -149,6 +149,57 @@ def test_truncated_file_is_an_error(tmp_path):
         s.load_matrix2(rid, path)
 
 
+def write_records(path, recs):
+    with open(path, "wb") as f:
+        for r in recs:
+            f.write(struct.pack("<i", len(r)) + r + struct.pack("<i", len(r)))
+
+
+def _edit_ints(rec, k, value):
+    a = np.frombuffer(rec, "<i4").copy()
+    a[k] = value
+    return a.tobytes()
+
+
+@pytest.mark.parametrize("edit,match", [
+    # jbeta_max of the first atom one less than its neighbours' sizes add up to
+    (lambda r: r.__setitem__(3, _edit_ints(r[3], 0, np.frombuffer(r[3], "<i4")[0] - 1)), "exceed jbeta_max"),
+    (lambda r: r.__setitem__(2, _edit_ints(r[2], 0, -1)), "row sizes"),             # negative jmax
+    (lambda r: r.__setitem__(1, _edit_ints(r[1], 0, 100)), "row sizes"),            # nsf above 64
+    (lambda r: r.__setitem__(3, _edit_ints(r[3], 0, 2**31 - 1)), "row sizes"),      # jbeta beyond any file
+])
+def test_corrupt_binary_counts_are_errors(tmp_path, edit, match):
+    """Counts read from the file are checked before they size a buffer or a copy."""
+    s, _, _ = system(natoms=12, nsf=4)
+    rid = s.add_range(6.0)
+    path = tmp_path / "Lmatrix2.i00.p000001"
+    s.dump_matrix2(rid, [s.fill(rid, 0, 1, 1, (2.0,))], path, binary=True)
+    recs = records(path)
+    edit(recs)
+    write_records(path, recs)
+    with pytest.raises(cqmm.CqmmError, match=match):
+        s.load_matrix2(rid, path, binary=True)
+
+
+def test_corrupt_record_marker_and_text_integer_are_errors(tmp_path):
+    s, _, _ = system(natoms=12, nsf=4)
+    rid = s.add_range(6.0)
+    v = s.fill(rid, 0, 1, 1, (2.0,))
+    path = tmp_path / "Lmatrix2.i00.p000001"
+    s.dump_matrix2(rid, [v], path, binary=True)
+    data = bytearray(open(path, "rb").read())
+    data[0:4] = struct.pack("<I", 0xFFFFFFF0)          # a record longer than the file
+    open(path, "wb").write(bytes(data))
+    with pytest.raises(cqmm.CqmmError, match="malformed or truncated"):
+        s.load_matrix2(rid, path, binary=True)
+    s.dump_matrix2(rid, [v], path, binary=False)
+    text = open(path).read().split("\n")
+    text[0] = text[0].replace("1", "x", 1)             # inode is not an integer
+    open(path, "w").write("\n".join(text))
+    with pytest.raises(cqmm.CqmmError, match="malformed or truncated"):
+        s.load_matrix2(rid, path, binary=False)
+
+
 # ---- InfoGlobal (dump_InfoMatGlobal, src/store_matrix_module.f90:700-781) ----
 
 def test_info_global_file_name():
