@@ -351,11 +351,16 @@ extern "C" {
 
 int cqmm_random_structure(int32_t natoms, double density, double dmin, uint64_t seed,
                           double *xyz, double *edge) {
-  if (natoms <= 0 || density <= 0.0 || !xyz) return cq::set_error(CQMM_ERR_ARG, "cqmm_random_structure: bad argument");
+  if (natoms <= 0 || density <= 0.0 || !(dmin >= 0.0) || !xyz)
+    return cq::set_error(CQMM_ERR_ARG, "cqmm_random_structure: bad argument");
   const double L = std::cbrt((double)natoms / density);
   const double d2min = dmin * dmin;
-  // Cell list over the box for the minimum-image rejection test.
-  const int32_t nc = std::max(1, (int32_t)std::floor(L / std::max(dmin, 1e-3)));
+  // Cell list over the box for the minimum-image rejection test. It only
+  // narrows which placed atoms are compared (every one within dmin is), so
+  // its size, capped near the atom count for small dmin, does not change the
+  // structure.
+  const int32_t nc_cap = 2 * (int32_t)std::ceil(std::cbrt((double)natoms)) + 1;
+  const int32_t nc = std::max(1, (int32_t)std::min<double>(nc_cap, std::floor(L / std::max(dmin, 1e-3))));
   const double cs = L / nc;
   std::vector<std::vector<int32_t>> cells((size_t)nc * nc * nc);
   uint64_t key = cq::hash_combine(cq::mix64(seed), 0x5eed5eedULL);

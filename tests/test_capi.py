@@ -4,6 +4,9 @@ import glob
 import os
 import re
 
+import numpy as np
+import pytest
+
 import cqmm
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -38,3 +41,11 @@ def test_errors_are_reported_not_raised_in_c():
     st = lib.cqmm_sys_create(0, None, None, None, None, 1.0, ctypes.byref(h))
     assert st != 0
     assert b"cqmm_sys_create" in lib.cqmm_last_error()
+
+
+def test_random_structure_small_dmin_and_bad_dmin():
+    # dmin 0 (no exclusion) must not size the rejection cell list by 1/dmin.
+    xyz, edge = cqmm.random_structure(100, 0.05, 0.0, 4)
+    assert xyz.shape == (100, 3) and np.all((xyz >= 0) & (xyz < edge))
+    with pytest.raises(cqmm.CqmmError, match="bad argument"):
+        cqmm.random_structure(10, 0.05, -1.0, 4)
