@@ -537,7 +537,9 @@ int cqmm_load_info_global(const char *path, int32_t index, int32_t natoms_max, c
   if (!rd.ints(counts, 2)) return bad("ni_in_cell, numprocs");
   g.natoms = counts[0];
   g.numprocs = counts[1];
-  if (g.natoms < 0) return bad("ni_in_cell");
+  // Each atom has a coordinate line, so a count above the file's lines is a
+  // corrupt or truncated file (checked before the count sizes anything).
+  if (g.natoms < 0 || (size_t)g.natoms > rd.lines.size()) return bad("ni_in_cell");
   if (g.natoms > natoms_max && (glob_to_node || coords || velocity))
     return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds %d atoms, room for %d", path, g.natoms,
                           natoms_max);

@@ -297,8 +297,14 @@ def test_info_global_errors(tmp_path):
         cqmm.load_info_global(p, index=1)
     with pytest.raises(cqmm.CqmmError, match="room"):
         cqmm.load_info_global(p, index=2, natoms_max=5)
-    p.write_text("".join(open(p).readlines()[:9]))
+    lines = open(p).readlines()
+    p.write_text("".join(lines[:9]))
     with pytest.raises(cqmm.CqmmError, match="truncated"):
+        cqmm.load_info_global(p, index=2)
+    # An atom count no file of this length can hold fails before it sizes anything.
+    assert lines[1].split()[0] == "10"
+    p.write_text("".join([lines[0], lines[1].replace("10", "2000000000", 1)] + lines[2:]))
+    with pytest.raises(cqmm.CqmmError, match="ni_in_cell"):
         cqmm.load_info_global(p, index=2)
 
 
