@@ -6,6 +6,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <exception>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -45,9 +48,45 @@ void parallel_for(int64_t n, F f, int64_t serial_below = 65536) {
     return;
   }
   const int64_t chunk = (n + nt - 1) / nt;
+  // An exception in a worker (host allocation failure) is passed to the
+  // caller after every started worker has joined.
+  std::vector<std::exception_ptr> err((size_t)nt);
   std::vector<std::thread> th;
-  for (int64_t b = 0; b < n; b += chunk) th.emplace_back(f, b, std::min(n, b + chunk));
+  try {
+    int64_t k = 0;
+    for (int64_t b = 0; b < n; b += chunk, ++k)
+      th.emplace_back([f, &err, k, b, e = std::min(n, b + chunk)]() mutable {
+        try {
+          f(b, e);
+        } catch (...) {
+          err[(size_t)k] = std::current_exception();
+        }
+      });
+  } catch (...) {
+    for (auto &t : th) t.join();
+    throw;
+  }
   for (auto &t : th) t.join();
+  for (auto &x : err)
+    if (x) std::rethrow_exception(x);
+}
+
+// Body of a C entry point: a C++ exception (host allocation failure, thread
+// creation failure) becomes a negative status and a message instead of
+// unwinding into the C or Fortran caller.
+template <class F>
+int guarded(const char *fn, F &&body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return set_errorf(-3, "%s: host allocation failed", fn);
+  } catch (const std::length_error &) {
+    return set_errorf(-3, "%s: host allocation failed (size)", fn);
+  } catch (const std::exception &e) {
+    return set_errorf(-5, "%s: %s", fn, e.what());
+  } catch (...) {
+    return set_errorf(-5, "%s: unexpected exception", fn);
+  }
 }
 
 // Wall-clock phases of plan construction, printed to stderr when the

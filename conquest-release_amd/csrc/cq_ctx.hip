@@ -178,279 +178,317 @@ int product_f32(cqmm_ctx *ctx, int32_t mult_id, const float *A, const float *B, 
 extern "C" {
 
 int cqmm_ctx_create(cqmm_ctx **ctx) {
-  if (!ctx) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_create: null argument");
-  int st = cq::ensure_init();
-  if (st) return st;
-  *ctx = new cqmm_ctx();
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_create: null argument");
+    int st = cq::ensure_init();
+    if (st) return st;
+    *ctx = new cqmm_ctx();
+    return CQMM_OK;
+  });
 }
 
 int cqmm_ctx_destroy(cqmm_ctx *ctx) {
-  if (ctx) {
-    (void)hipStreamSynchronize(cq::stream());
-    delete ctx;
-  }
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (ctx) {
+      (void)hipStreamSynchronize(cq::stream());
+      delete ctx;
+    }
+    return CQMM_OK;
+  });
 }
 
 int cqmm_ctx_add_range(cqmm_ctx *ctx, int32_t range, const cqmm_matrix_index *idx) {
-  if (!ctx || !idx || range < 0 || range >= kMaxRanges) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_range: bad argument");
-  if (idx->n_parts <= 0 || idx->n_prim <= 0 || !idx->n_atoms || !idx->nab_offset || !idx->nd_offset || !idx->n_nab ||
-      !idx->i_acc || !idx->i_nd_acc || !idx->ndimi || !idx->i_part || !idx->i_seq || !idx->ndimj || idx->length <= 0)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_range: incomplete matrix index");
-  RangeRec r;
-  r.n_prim = idx->n_prim;
-  r.length = idx->length;
-  r.n_nab.assign(idx->n_nab, idx->n_nab + idx->n_prim);
-  r.ndimi.assign(idx->ndimi, idx->ndimi + idx->n_prim);
-  r.onsite.assign(idx->n_prim, 0);
-  if (idx->onsite) r.onsite.assign(idx->onsite, idx->onsite + idx->n_prim);
-  r.i_part.assign(idx->i_part, idx->i_part + idx->n_nabs);
-  r.i_seq.assign(idx->i_seq, idx->i_seq + idx->n_nabs);
-  r.ndimj.assign(idx->ndimj, idx->ndimj + idx->n_nabs);
-  r.row_nab.resize(idx->n_prim);
-  r.row_off.resize(idx->n_prim);
-  int32_t p = 0;
-  for (int32_t nn = 0; nn < idx->n_parts; ++nn)
-    for (int32_t a = 0; a < idx->n_atoms[nn]; ++a, ++p) {
-      if (p >= idx->n_prim) return cq::set_error(CQMM_ERR_RANGE, "cqmm_ctx_add_range: n_atoms exceed n_prim");
-      r.row_nab[p] = (int64_t)idx->nab_offset[nn] + idx->i_acc[p] - 1;
-      r.row_off[p] = idx->nd_offset[nn] + idx->i_nd_acc[p] - 1;
-      if (r.row_nab[p] < 0 || r.row_nab[p] + r.n_nab[p] > idx->n_nabs)
-        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_add_range: row %d outside the neighbour arrays", p + 1);
-      int64_t sz = 0;
-      for (int32_t e = 0; e < r.n_nab[p]; ++e) sz += (int64_t)r.ndimi[p] * r.ndimj[r.row_nab[p] + e];
-      if (r.row_off[p] < 0 || r.row_off[p] + sz > r.length)
-        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_add_range: row %d data outside length", p + 1);
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !idx || range < 0 || range >= kMaxRanges) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_range: bad argument");
+    if (idx->n_parts <= 0 || idx->n_prim <= 0 || !idx->n_atoms || !idx->nab_offset || !idx->nd_offset || !idx->n_nab ||
+        !idx->i_acc || !idx->i_nd_acc || !idx->ndimi || !idx->i_part || !idx->i_seq || !idx->ndimj || idx->length <= 0)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_range: incomplete matrix index");
+    RangeRec r;
+    r.n_prim = idx->n_prim;
+    r.length = idx->length;
+    r.n_nab.assign(idx->n_nab, idx->n_nab + idx->n_prim);
+    r.ndimi.assign(idx->ndimi, idx->ndimi + idx->n_prim);
+    r.onsite.assign(idx->n_prim, 0);
+    if (idx->onsite) r.onsite.assign(idx->onsite, idx->onsite + idx->n_prim);
+    r.i_part.assign(idx->i_part, idx->i_part + idx->n_nabs);
+    r.i_seq.assign(idx->i_seq, idx->i_seq + idx->n_nabs);
+    r.ndimj.assign(idx->ndimj, idx->ndimj + idx->n_nabs);
+    r.row_nab.resize(idx->n_prim);
+    r.row_off.resize(idx->n_prim);
+    int32_t p = 0;
+    for (int32_t nn = 0; nn < idx->n_parts; ++nn)
+      for (int32_t a = 0; a < idx->n_atoms[nn]; ++a, ++p) {
+        if (p >= idx->n_prim) return cq::set_error(CQMM_ERR_RANGE, "cqmm_ctx_add_range: n_atoms exceed n_prim");
+        r.row_nab[p] = (int64_t)idx->nab_offset[nn] + idx->i_acc[p] - 1;
+        r.row_off[p] = idx->nd_offset[nn] + idx->i_nd_acc[p] - 1;
+        if (r.row_nab[p] < 0 || r.row_nab[p] + r.n_nab[p] > idx->n_nabs)
+          return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_add_range: row %d outside the neighbour arrays", p + 1);
+        int64_t sz = 0;
+        for (int32_t e = 0; e < r.n_nab[p]; ++e) sz += (int64_t)r.ndimi[p] * r.ndimj[r.row_nab[p] + e];
+        if (r.row_off[p] < 0 || r.row_off[p] + sz > r.length)
+          return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_add_range: row %d data outside length", p + 1);
+      }
+    if (p != idx->n_prim) return cq::set_error(CQMM_ERR_RANGE, "cqmm_ctx_add_range: n_atoms do not sum to n_prim");
+    // Onsite blocks for matrix_trace.
+    std::vector<int64_t> doff;
+    std::vector<int32_t> dn;
+    for (int32_t q = 0; q < r.n_prim; ++q)
+      if (r.onsite[q] > 0) {
+        if (r.onsite[q] - 1 + (int64_t)r.ndimi[q] * r.ndimi[q] > r.length)
+          return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_add_range: onsite(%d) outside length", q + 1);
+        doff.push_back(r.onsite[q] - 1);
+        dn.push_back(r.ndimi[q]);
+      }
+    int st;
+    hipStream_t s = cq::stream();
+    r.diag.nblk = (int64_t)doff.size();
+    if ((st = r.diag.off.alloc(std::max<size_t>(1, doff.size()) * 8))) return st;
+    if ((st = r.diag.nsf.alloc(std::max<size_t>(1, dn.size()) * 4))) return st;
+    if (!doff.empty()) {
+      if ((st = cq::hip_check(hipMemcpyAsync(r.diag.off.p, doff.data(), doff.size() * 8, hipMemcpyHostToDevice, s), "H2D"))) return st;
+      if ((st = cq::hip_check(hipMemcpyAsync(r.diag.nsf.p, dn.data(), dn.size() * 4, hipMemcpyHostToDevice, s), "H2D"))) return st;
     }
-  if (p != idx->n_prim) return cq::set_error(CQMM_ERR_RANGE, "cqmm_ctx_add_range: n_atoms do not sum to n_prim");
-  // Onsite blocks for matrix_trace.
-  std::vector<int64_t> doff;
-  std::vector<int32_t> dn;
-  for (int32_t q = 0; q < r.n_prim; ++q)
-    if (r.onsite[q] > 0) {
-      if (r.onsite[q] - 1 + (int64_t)r.ndimi[q] * r.ndimi[q] > r.length)
-        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_add_range: onsite(%d) outside length", q + 1);
-      doff.push_back(r.onsite[q] - 1);
-      dn.push_back(r.ndimi[q]);
-    }
-  int st;
-  hipStream_t s = cq::stream();
-  r.diag.nblk = (int64_t)doff.size();
-  if ((st = r.diag.off.alloc(std::max<size_t>(1, doff.size()) * 8))) return st;
-  if ((st = r.diag.nsf.alloc(std::max<size_t>(1, dn.size()) * 4))) return st;
-  if (!doff.empty()) {
-    if ((st = cq::hip_check(hipMemcpyAsync(r.diag.off.p, doff.data(), doff.size() * 8, hipMemcpyHostToDevice, s), "H2D"))) return st;
-    if ((st = cq::hip_check(hipMemcpyAsync(r.diag.nsf.p, dn.data(), dn.size() * 4, hipMemcpyHostToDevice, s), "H2D"))) return st;
-  }
-  if ((st = cq::hip_check(hipStreamSynchronize(s), "range upload"))) return st;
-  r.set = true;
-  // Replacing a range invalidates its cached sum maps.
-  for (auto it = ctx->summaps.begin(); it != ctx->summaps.end();)
-    it = (it->first.first == range || it->first.second == range) ? ctx->summaps.erase(it) : std::next(it);
-  ctx->ranges[range] = std::move(r);
-  return CQMM_OK;
+    if ((st = cq::hip_check(hipStreamSynchronize(s), "range upload"))) return st;
+    r.set = true;
+    // Replacing a range invalidates its cached sum maps.
+    for (auto it = ctx->summaps.begin(); it != ctx->summaps.end();)
+      it = (it->first.first == range || it->first.second == range) ? ctx->summaps.erase(it) : std::next(it);
+    ctx->ranges[range] = std::move(r);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_ctx_set_transpose(cqmm_ctx *ctx, int32_t range, int64_t npairs, const int64_t *p, const int64_t *q,
                            const int32_t *n1, const int32_t *n2) {
-  if (!ctx || range < 0 || range >= kMaxRanges || !ctx->ranges[range].set || npairs < 0 || (npairs && (!p || !q || !n1 || !n2)))
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_set_transpose: bad argument");
-  RangeRec &r = ctx->ranges[range];
-  std::vector<int64_t> pp(npairs), qq(npairs);
-  std::vector<uint8_t> a(npairs), b(npairs);
-  for (int64_t t = 0; t < npairs; ++t) {
-    if (p[t] < 1 || q[t] < 1 || n1[t] < 1 || n2[t] < 1 || n1[t] > 64 || n2[t] > 64 ||
-        p[t] - 1 + (int64_t)n1[t] * n2[t] > r.length || q[t] - 1 + (int64_t)n1[t] * n2[t] > r.length)
-      return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_set_transpose: pair %lld outside the matrix", (long long)t + 1);
-    if (p[t] == q[t] && n1[t] != n2[t]) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_set_transpose: non-square self pair");
-    pp[t] = p[t] - 1;
-    qq[t] = q[t] - 1;
-    a[t] = (uint8_t)n1[t];
-    b[t] = (uint8_t)n2[t];
-  }
-  int st;
-  hipStream_t s = cq::stream();
-  r.pairs.npairs = npairs;
-  if ((st = r.pairs.poff.alloc(std::max<int64_t>(1, npairs) * 8))) return st;
-  if ((st = r.pairs.qoff.alloc(std::max<int64_t>(1, npairs) * 8))) return st;
-  if ((st = r.pairs.n1.alloc(std::max<int64_t>(1, npairs)))) return st;
-  if ((st = r.pairs.n2.alloc(std::max<int64_t>(1, npairs)))) return st;
-  if (npairs) {
-    if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.poff.p, pp.data(), npairs * 8, hipMemcpyHostToDevice, s), "H2D"))) return st;
-    if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.qoff.p, qq.data(), npairs * 8, hipMemcpyHostToDevice, s), "H2D"))) return st;
-    if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.n1.p, a.data(), npairs, hipMemcpyHostToDevice, s), "H2D"))) return st;
-    if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.n2.p, b.data(), npairs, hipMemcpyHostToDevice, s), "H2D"))) return st;
-  }
-  r.has_pairs = true;
-  return cq::hip_check(hipStreamSynchronize(s), "transpose upload");
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || range < 0 || range >= kMaxRanges || !ctx->ranges[range].set || npairs < 0 || (npairs && (!p || !q || !n1 || !n2)))
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_set_transpose: bad argument");
+    RangeRec &r = ctx->ranges[range];
+    std::vector<int64_t> pp(npairs), qq(npairs);
+    std::vector<uint8_t> a(npairs), b(npairs);
+    for (int64_t t = 0; t < npairs; ++t) {
+      if (p[t] < 1 || q[t] < 1 || n1[t] < 1 || n2[t] < 1 || n1[t] > 64 || n2[t] > 64 ||
+          p[t] - 1 + (int64_t)n1[t] * n2[t] > r.length || q[t] - 1 + (int64_t)n1[t] * n2[t] > r.length)
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_ctx_set_transpose: pair %lld outside the matrix", (long long)t + 1);
+      if (p[t] == q[t] && n1[t] != n2[t]) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_set_transpose: non-square self pair");
+      pp[t] = p[t] - 1;
+      qq[t] = q[t] - 1;
+      a[t] = (uint8_t)n1[t];
+      b[t] = (uint8_t)n2[t];
+    }
+    int st;
+    hipStream_t s = cq::stream();
+    r.pairs.npairs = npairs;
+    if ((st = r.pairs.poff.alloc(std::max<int64_t>(1, npairs) * 8))) return st;
+    if ((st = r.pairs.qoff.alloc(std::max<int64_t>(1, npairs) * 8))) return st;
+    if ((st = r.pairs.n1.alloc(std::max<int64_t>(1, npairs)))) return st;
+    if ((st = r.pairs.n2.alloc(std::max<int64_t>(1, npairs)))) return st;
+    if (npairs) {
+      if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.poff.p, pp.data(), npairs * 8, hipMemcpyHostToDevice, s), "H2D"))) return st;
+      if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.qoff.p, qq.data(), npairs * 8, hipMemcpyHostToDevice, s), "H2D"))) return st;
+      if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.n1.p, a.data(), npairs, hipMemcpyHostToDevice, s), "H2D"))) return st;
+      if ((st = cq::hip_check(hipMemcpyAsync(r.pairs.n2.p, b.data(), npairs, hipMemcpyHostToDevice, s), "H2D"))) return st;
+    }
+    r.has_pairs = true;
+    return cq::hip_check(hipStreamSynchronize(s), "transpose upload");
+  });
 }
 
 int cqmm_ctx_add_mult(cqmm_ctx *ctx, int32_t mult_id, int32_t ra, int32_t rb, int32_t rc, cqmm_plan *plan) {
-  if (!ctx || !plan) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_mult: null argument");
-  for (int32_t r : {ra, rb, rc})
-    if (r < 0 || r >= kMaxRanges || !ctx->ranges[r].set)
-      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_ctx_add_mult: range %d not registered", r);
-  if (plan->lena != ctx->ranges[ra].length || plan->lenb != ctx->ranges[rb].length || plan->lenc != ctx->ranges[rc].length)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_mult: plan lengths differ from the ranges' lengths");
-  auto it = ctx->mults.find(mult_id);
-  if (it != ctx->mults.end()) cqmm_plan_destroy(it->second.plan);
-  MultRec m;
-  m.plan = plan;
-  m.ra = ra;
-  m.rb = rb;
-  m.rc = rc;
-  ctx->mults[mult_id] = m;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !plan) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_mult: null argument");
+    for (int32_t r : {ra, rb, rc})
+      if (r < 0 || r >= kMaxRanges || !ctx->ranges[r].set)
+        return cq::set_errorf(CQMM_ERR_ARG, "cqmm_ctx_add_mult: range %d not registered", r);
+    if (plan->lena != ctx->ranges[ra].length || plan->lenb != ctx->ranges[rb].length || plan->lenc != ctx->ranges[rc].length)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_mult: plan lengths differ from the ranges' lengths");
+    auto it = ctx->mults.find(mult_id);
+    if (it != ctx->mults.end()) cqmm_plan_destroy(it->second.plan);
+    MultRec m;
+    m.plan = plan;
+    m.ra = ra;
+    m.rb = rb;
+    m.rc = rc;
+    ctx->mults[mult_id] = m;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_allocate(cqmm_ctx *ctx, int32_t range, int32_t *handle) {
-  if (!ctx || !handle || range < 0 || range >= kMaxRanges || !ctx->ranges[range].set)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_allocate: range %d not registered", range);
-  const size_t bytes = (size_t)ctx->ranges[range].length * 8;
-  MatRec m;
-  m.range = range;
-  auto it = ctx->pool.find(bytes);
-  if (it != ctx->pool.end()) {
-    m.data = std::move(it->second);
-    ctx->pool.erase(it);
-  } else {
-    int st = m.data.alloc(bytes);
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !handle || range < 0 || range >= kMaxRanges || !ctx->ranges[range].set)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_allocate: range %d not registered", range);
+    const size_t bytes = (size_t)ctx->ranges[range].length * 8;
+    MatRec m;
+    m.range = range;
+    auto it = ctx->pool.find(bytes);
+    if (it != ctx->pool.end()) {
+      m.data = std::move(it->second);
+      ctx->pool.erase(it);
+    } else {
+      int st = m.data.alloc(bytes);
+      if (st) return st;
+    }
+    int st = cq::hip_check(hipMemsetAsync(m.data.p, 0, bytes, cq::stream()), "zero matrix");
     if (st) return st;
-  }
-  int st = cq::hip_check(hipMemsetAsync(m.data.p, 0, bytes, cq::stream()), "zero matrix");
-  if (st) return st;
-  m.live = true;
-  ctx->mats.push_back(std::move(m));
-  *handle = (int32_t)ctx->mats.size();
-  return CQMM_OK;
+    m.live = true;
+    ctx->mats.push_back(std::move(m));
+    *handle = (int32_t)ctx->mats.size();
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_free(cqmm_ctx *ctx, int32_t handle) {
-  int st = ctx_check(ctx, handle, "cqmm_matrix_free");
-  if (st) return st;
-  if (handle != (int32_t)ctx->mats.size())
-    return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_free: handle %d freed out of order (top is %d)", handle,
-                          (int)ctx->mats.size());
-  MatRec &m = ctx->mats.back();
-  const size_t bytes = m.data.bytes;
-  ctx->pool.emplace(bytes, std::move(m.data));
-  ctx->mats.pop_back();
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, handle, "cqmm_matrix_free");
+    if (st) return st;
+    if (handle != (int32_t)ctx->mats.size())
+      return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_free: handle %d freed out of order (top is %d)", handle,
+                            (int)ctx->mats.size());
+    MatRec &m = ctx->mats.back();
+    const size_t bytes = m.data.bytes;
+    ctx->pool.emplace(bytes, std::move(m.data));
+    ctx->mats.pop_back();
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_length(const cqmm_ctx *ctx, int32_t h, int64_t *len) {
-  int st = ctx_check(ctx, h, "cqmm_matrix_length");
-  if (st) return st;
-  *len = hlen(ctx, h);
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, h, "cqmm_matrix_length");
+    if (st) return st;
+    *len = hlen(ctx, h);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_range(const cqmm_ctx *ctx, int32_t h, int32_t *range) {
-  int st = ctx_check(ctx, h, "cqmm_matrix_range");
-  if (st) return st;
-  *range = hrange(ctx, h);
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, h, "cqmm_matrix_range");
+    if (st) return st;
+    *range = hrange(ctx, h);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_device_ptr(const cqmm_ctx *ctx, int32_t h, double **ptr) {
-  int st = ctx_check(ctx, h, "cqmm_matrix_device_ptr");
-  if (st) return st;
-  *ptr = ctx->mats[h - 1].data.as<double>();
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, h, "cqmm_matrix_device_ptr");
+    if (st) return st;
+    *ptr = ctx->mats[h - 1].data.as<double>();
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_set(cqmm_ctx *ctx, int32_t h, const double *host) {
-  int st = ctx_check(ctx, h, "cqmm_matrix_set");
-  if (st) return st;
-  hipStream_t s = cq::stream();
-  if ((st = cq::hip_check(hipMemcpyAsync(dptr(ctx, h), host, hlen(ctx, h) * 8, hipMemcpyHostToDevice, s), "H2D matrix"))) return st;
-  return cq::hip_check(hipStreamSynchronize(s), "cqmm_matrix_set");
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, h, "cqmm_matrix_set");
+    if (st) return st;
+    hipStream_t s = cq::stream();
+    if ((st = cq::hip_check(hipMemcpyAsync(dptr(ctx, h), host, hlen(ctx, h) * 8, hipMemcpyHostToDevice, s), "H2D matrix"))) return st;
+    return cq::hip_check(hipStreamSynchronize(s), "cqmm_matrix_set");
+  });
 }
 
 int cqmm_matrix_get(cqmm_ctx *ctx, int32_t h, double *host) {
-  int st = ctx_check(ctx, h, "cqmm_matrix_get");
-  if (st) return st;
-  hipStream_t s = cq::stream();
-  if ((st = cq::hip_check(hipMemcpyAsync(host, dptr(ctx, h), hlen(ctx, h) * 8, hipMemcpyDeviceToHost, s), "D2H matrix"))) return st;
-  return cq::hip_check(hipStreamSynchronize(s), "cqmm_matrix_get");
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, h, "cqmm_matrix_get");
+    if (st) return st;
+    hipStream_t s = cq::stream();
+    if ((st = cq::hip_check(hipMemcpyAsync(host, dptr(ctx, h), hlen(ctx, h) * 8, hipMemcpyDeviceToHost, s), "D2H matrix"))) return st;
+    return cq::hip_check(hipStreamSynchronize(s), "cqmm_matrix_get");
+  });
 }
 
 int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t mult_id) {
-  int st;
-  if ((st = ctx_check(ctx, A, "cqmm_matrix_product")) || (st = ctx_check(ctx, B, "cqmm_matrix_product")) ||
-      (st = ctx_check(ctx, C, "cqmm_matrix_product")))
-    return st;
-  auto it = ctx->mults.find(mult_id);
-  if (it == ctx->mults.end()) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_product: mult %d not registered", mult_id);
-  const MultRec &m = it->second;
-  const cq::TripletPlan &tp = m.plan->tp;
-  // matrix_product: type 1 -> mat_mult(A,B,C); type 2 -> mat_mult(C,B,A).
-  const int32_t a_role = tp.mult_type == 1 ? A : C, c_role = tp.mult_type == 1 ? C : A;
-  if (hrange(ctx, a_role) != m.ra || hrange(ctx, B) != m.rb || hrange(ctx, c_role) != m.rc)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_product: operand ranges (%d,%d,%d) do not match mult %d",
-                          hrange(ctx, A), hrange(ctx, B), hrange(ctx, C), mult_id);
-  return cq::launch_block_products(tp, dptr(ctx, A), dptr(ctx, B), dptr(ctx, C), false, cq::stream());
+  return cq::guarded(__func__, [&]() -> int {
+    int st;
+    if ((st = ctx_check(ctx, A, "cqmm_matrix_product")) || (st = ctx_check(ctx, B, "cqmm_matrix_product")) ||
+        (st = ctx_check(ctx, C, "cqmm_matrix_product")))
+      return st;
+    auto it = ctx->mults.find(mult_id);
+    if (it == ctx->mults.end()) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_product: mult %d not registered", mult_id);
+    const MultRec &m = it->second;
+    const cq::TripletPlan &tp = m.plan->tp;
+    // matrix_product: type 1 -> mat_mult(A,B,C); type 2 -> mat_mult(C,B,A).
+    const int32_t a_role = tp.mult_type == 1 ? A : C, c_role = tp.mult_type == 1 ? C : A;
+    if (hrange(ctx, a_role) != m.ra || hrange(ctx, B) != m.rb || hrange(ctx, c_role) != m.rc)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_product: operand ranges (%d,%d,%d) do not match mult %d",
+                            hrange(ctx, A), hrange(ctx, B), hrange(ctx, C), mult_id);
+    return cq::launch_block_products(tp, dptr(ctx, A), dptr(ctx, B), dptr(ctx, C), false, cq::stream());
+  });
 }
 
 int cqmm_matrix_sum(cqmm_ctx *ctx, double alpha, int32_t A, double beta, int32_t B) {
-  int st;
-  if ((st = ctx_check(ctx, A, "cqmm_matrix_sum")) || (st = ctx_check(ctx, B, "cqmm_matrix_sum"))) return st;
-  hipStream_t s = cq::stream();
-  const int32_t ra = hrange(ctx, A), rb = hrange(ctx, B);
-  if (ra == rb) return cq::op_axpby(hlen(ctx, A), alpha, dptr(ctx, A), beta, dptr(ctx, B), s);
-  const cq::RunMap *m = nullptr;
-  if ((st = summap(ctx, ra, rb, &m))) return st;
-  return cq::op_runs_axpby(*m, alpha, dptr(ctx, A), beta, dptr(ctx, B), s);
+  return cq::guarded(__func__, [&]() -> int {
+    int st;
+    if ((st = ctx_check(ctx, A, "cqmm_matrix_sum")) || (st = ctx_check(ctx, B, "cqmm_matrix_sum"))) return st;
+    hipStream_t s = cq::stream();
+    const int32_t ra = hrange(ctx, A), rb = hrange(ctx, B);
+    if (ra == rb) return cq::op_axpby(hlen(ctx, A), alpha, dptr(ctx, A), beta, dptr(ctx, B), s);
+    const cq::RunMap *m = nullptr;
+    if ((st = summap(ctx, ra, rb, &m))) return st;
+    return cq::op_runs_axpby(*m, alpha, dptr(ctx, A), beta, dptr(ctx, B), s);
+  });
 }
 
 int cqmm_matrix_scale(cqmm_ctx *ctx, double alpha, int32_t A) {
-  int st = ctx_check(ctx, A, "cqmm_matrix_scale");
-  if (st) return st;
-  return cq::op_scale(hlen(ctx, A), alpha, dptr(ctx, A), cq::stream());
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, A, "cqmm_matrix_scale");
+    if (st) return st;
+    return cq::op_scale(hlen(ctx, A), alpha, dptr(ctx, A), cq::stream());
+  });
 }
 
 int cqmm_matrix_product_trace(cqmm_ctx *ctx, int32_t A, int32_t B, double *result) {
-  int st;
-  if ((st = ctx_check(ctx, A, "cqmm_matrix_product_trace")) || (st = ctx_check(ctx, B, "cqmm_matrix_product_trace"))) return st;
-  if (!result) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_product_trace: null result");
-  if (hrange(ctx, A) != hrange(ctx, B))
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_product_trace: matrices of different ranges");
-  if ((st = ensure_red(ctx))) return st;
-  if ((st = cq::op_dot(hlen(ctx, A), dptr(ctx, A), dptr(ctx, B), ctx->red_part.as<double>(), ctx->red_out.as<double>(),
-                       cq::stream())))
-    return st;
-  return reduce_result(ctx, result);
+  return cq::guarded(__func__, [&]() -> int {
+    int st;
+    if ((st = ctx_check(ctx, A, "cqmm_matrix_product_trace")) || (st = ctx_check(ctx, B, "cqmm_matrix_product_trace"))) return st;
+    if (!result) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_product_trace: null result");
+    if (hrange(ctx, A) != hrange(ctx, B))
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_product_trace: matrices of different ranges");
+    if ((st = ensure_red(ctx))) return st;
+    if ((st = cq::op_dot(hlen(ctx, A), dptr(ctx, A), dptr(ctx, B), ctx->red_part.as<double>(), ctx->red_out.as<double>(),
+                         cq::stream())))
+      return st;
+    return reduce_result(ctx, result);
+  });
 }
 
 int cqmm_matrix_trace(cqmm_ctx *ctx, int32_t A, double *result) {
-  int st = ctx_check(ctx, A, "cqmm_matrix_trace");
-  if (st) return st;
-  if (!result) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_trace: null result");
-  if ((st = ensure_red(ctx))) return st;
-  const RangeRec &r = ctx->ranges[hrange(ctx, A)];
-  if ((st = cq::op_trace(r.diag, dptr(ctx, A), ctx->red_part.as<double>(), ctx->red_out.as<double>(), cq::stream())))
-    return st;
-  return reduce_result(ctx, result);
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, A, "cqmm_matrix_trace");
+    if (st) return st;
+    if (!result) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_trace: null result");
+    if ((st = ensure_red(ctx))) return st;
+    const RangeRec &r = ctx->ranges[hrange(ctx, A)];
+    if ((st = cq::op_trace(r.diag, dptr(ctx, A), ctx->red_part.as<double>(), ctx->red_out.as<double>(), cq::stream())))
+      return st;
+    return reduce_result(ctx, result);
+  });
 }
 
 int cqmm_matrix_symmetrise(cqmm_ctx *ctx, int32_t A) {
-  int st = ctx_check(ctx, A, "cqmm_matrix_symmetrise");
-  if (st) return st;
-  const RangeRec &r = ctx->ranges[hrange(ctx, A)];
-  if (!r.has_pairs) return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_symmetrise: range %d has no transpose pairs", hrange(ctx, A));
-  return cq::op_symmetrise(r.pairs, dptr(ctx, A), cq::stream());
+  return cq::guarded(__func__, [&]() -> int {
+    int st = ctx_check(ctx, A, "cqmm_matrix_symmetrise");
+    if (st) return st;
+    const RangeRec &r = ctx->ranges[hrange(ctx, A)];
+    if (!r.has_pairs) return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_symmetrise: range %d has no transpose pairs", hrange(ctx, A));
+    return cq::op_symmetrise(r.pairs, dptr(ctx, A), cq::stream());
+  });
 }
 
 int cqmm_matrix_transpose(cqmm_ctx *ctx, int32_t A, int32_t B) {
-  int st;
-  if ((st = ctx_check(ctx, A, "cqmm_matrix_transpose")) || (st = ctx_check(ctx, B, "cqmm_matrix_transpose"))) return st;
-  if (A == B) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_transpose: A and B must differ");
-  if (hrange(ctx, A) != hrange(ctx, B))
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_transpose: A and B must share a range");
-  const RangeRec &r = ctx->ranges[hrange(ctx, A)];
-  if (!r.has_pairs) return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_transpose: range %d has no transpose pairs", hrange(ctx, A));
-  return cq::op_transpose(r.pairs, dptr(ctx, A), dptr(ctx, B), cq::stream());
+  return cq::guarded(__func__, [&]() -> int {
+    int st;
+    if ((st = ctx_check(ctx, A, "cqmm_matrix_transpose")) || (st = ctx_check(ctx, B, "cqmm_matrix_transpose"))) return st;
+    if (A == B) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_transpose: A and B must differ");
+    if (hrange(ctx, A) != hrange(ctx, B))
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_transpose: A and B must share a range");
+    const RangeRec &r = ctx->ranges[hrange(ctx, A)];
+    if (!r.has_pairs) return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_transpose: range %d has no transpose pairs", hrange(ctx, A));
+    return cq::op_transpose(r.pairs, dptr(ctx, A), dptr(ctx, B), cq::stream());
+  });
 }
 
 }  // extern "C"
@@ -526,242 +564,248 @@ extern "C" {
 
 int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t matS, int32_t matL,
                  cqmm_mcw_result *res) {
-  if (!ctx || !su || !res) return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: null argument");
-  int st;
-  if ((st = ctx_check(ctx, matH, "cqmm_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_mcweeny")) ||
-      (st = ctx_check(ctx, matL, "cqmm_mcweeny")))
-    return st;
-  if (hrange(ctx, matL) != su->range_L || hrange(ctx, matS) != su->range_S || hrange(ctx, matH) != su->range_H)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: handle ranges differ from the setup");
-  std::memset(res, 0, sizeof(*res));
-  const auto t0 = std::chrono::steady_clock::now();
-  hipStream_t s = cq::stream();
-  // Temporaries (freed in reverse order at the end).
-  int32_t oldL, rho;
-  McwWork w;
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &oldL));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &rho));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_LS, &w.LS));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_LSL, &w.LSL));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &w.LSLSL));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &w.top));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &w.bottom));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_H, &w.botH));
-  w.L32 = w.S32 = w.LS32 = w.LSL32 = w.LSLSL32 = nullptr;
-  const bool mixed = su->precision == 1 || su->precision == 2;
-  if (mixed) {
-    void *p;
-    CQ_TRY(get_scratch(ctx, "L32", hlen(ctx, matL) * 4, &p)); w.L32 = (float *)p;
-    CQ_TRY(get_scratch(ctx, "S32", hlen(ctx, matS) * 4, &p)); w.S32 = (float *)p;
-    CQ_TRY(get_scratch(ctx, "LS32", hlen(ctx, w.LS) * 4, &p)); w.LS32 = (float *)p;
-    CQ_TRY(get_scratch(ctx, "LSL32", hlen(ctx, w.LSL) * 4, &p)); w.LSL32 = (float *)p;
-    CQ_TRY(get_scratch(ctx, "LSLSL32", hlen(ctx, w.LSLSL) * 4, &p)); w.LSLSL32 = (float *)p;
-    CQ_TRY(cq::op_to_f32(hlen(ctx, matS), dptr(ctx, matS), w.S32, s));
-  }
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, oldL, 1.0, matL));
-  double oldE = 1.0e30, omega = 0.0, cn = 0.0, ne = 0.0, lastE = 0.0, lastNe = 0.0, lastC = 0.0;
-  bool fp32_phase = mixed;
-  int fp64_done = 0, n = 0;
-  int stop = 4;
-  const int max_iter = su->max_iter > 0 ? su->max_iter : 50;
-  const int nfinal = su->n_final_fp64 > 0 ? su->n_final_fp64 : 2;
-  for (n = 1; n <= max_iter; ++n) {
-    CQ_TRY(mcw_step(ctx, su, matH, matS, matL, rho, w, fp32_phase, &omega, &cn, &ne));
-    if (n - 1 < CQMM_MCW_TRACE) res->energy_trace[n - 1] = omega;
-    if (fp32_phase) ++res->fp32_iterations;
-    else ++fp64_done;
-    // McWMin's stop test (src/McWeeny.f90:166-173).
-    int why = 0;
-    if (omega > oldE) why = 1;
-    else if (std::fabs(omega - oldE) < 1.0e-8) why = 2;
-    else if (cn > 1.0 || cn < 0.0) why = 3;
-    if (fp32_phase) {
-      // FP32 phase: also end when the energy change reaches FP32's resolution.
-      const bool flat = std::fabs(omega - oldE) < su->fp32_tol * std::max(1.0, std::fabs(omega));
-      if (why || flat) {
-        fp32_phase = false;
-        if (why == 1 || why == 3) {
-          // Reject the step; refine from the last accepted L in FP64.
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su || !res) return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: null argument");
+    int st;
+    if ((st = ctx_check(ctx, matH, "cqmm_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_mcweeny")) ||
+        (st = ctx_check(ctx, matL, "cqmm_mcweeny")))
+      return st;
+    if (hrange(ctx, matL) != su->range_L || hrange(ctx, matS) != su->range_S || hrange(ctx, matH) != su->range_H)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: handle ranges differ from the setup");
+    std::memset(res, 0, sizeof(*res));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = cq::stream();
+    // Temporaries (freed in reverse order at the end).
+    int32_t oldL, rho;
+    McwWork w;
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &oldL));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &rho));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_LS, &w.LS));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_LSL, &w.LSL));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &w.LSLSL));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &w.top));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &w.bottom));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_H, &w.botH));
+    w.L32 = w.S32 = w.LS32 = w.LSL32 = w.LSLSL32 = nullptr;
+    const bool mixed = su->precision == 1 || su->precision == 2;
+    if (mixed) {
+      void *p;
+      CQ_TRY(get_scratch(ctx, "L32", hlen(ctx, matL) * 4, &p)); w.L32 = (float *)p;
+      CQ_TRY(get_scratch(ctx, "S32", hlen(ctx, matS) * 4, &p)); w.S32 = (float *)p;
+      CQ_TRY(get_scratch(ctx, "LS32", hlen(ctx, w.LS) * 4, &p)); w.LS32 = (float *)p;
+      CQ_TRY(get_scratch(ctx, "LSL32", hlen(ctx, w.LSL) * 4, &p)); w.LSL32 = (float *)p;
+      CQ_TRY(get_scratch(ctx, "LSLSL32", hlen(ctx, w.LSLSL) * 4, &p)); w.LSLSL32 = (float *)p;
+      CQ_TRY(cq::op_to_f32(hlen(ctx, matS), dptr(ctx, matS), w.S32, s));
+    }
+    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, oldL, 1.0, matL));
+    double oldE = 1.0e30, omega = 0.0, cn = 0.0, ne = 0.0, lastE = 0.0, lastNe = 0.0, lastC = 0.0;
+    bool fp32_phase = mixed;
+    int fp64_done = 0, n = 0;
+    int stop = 4;
+    const int max_iter = su->max_iter > 0 ? su->max_iter : 50;
+    const int nfinal = su->n_final_fp64 > 0 ? su->n_final_fp64 : 2;
+    for (n = 1; n <= max_iter; ++n) {
+      CQ_TRY(mcw_step(ctx, su, matH, matS, matL, rho, w, fp32_phase, &omega, &cn, &ne));
+      if (n - 1 < CQMM_MCW_TRACE) res->energy_trace[n - 1] = omega;
+      if (fp32_phase) ++res->fp32_iterations;
+      else ++fp64_done;
+      // McWMin's stop test (src/McWeeny.f90:166-173).
+      int why = 0;
+      if (omega > oldE) why = 1;
+      else if (std::fabs(omega - oldE) < 1.0e-8) why = 2;
+      else if (cn > 1.0 || cn < 0.0) why = 3;
+      if (fp32_phase) {
+        // FP32 phase: also end when the energy change reaches FP32's resolution.
+        const bool flat = std::fabs(omega - oldE) < su->fp32_tol * std::max(1.0, std::fabs(omega));
+        if (why || flat) {
+          fp32_phase = false;
+          if (why == 1 || why == 3) {
+            // Reject the step; refine from the last accepted L in FP64.
+            oldE = 1.0e30;
+            continue;
+          }
+          // Accept the step, then continue in FP64 with a fresh energy reference.
+          why = 0;
+          CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, rho));
+          lastE = omega; lastNe = ne; lastC = cn;
+          CQ_TRY(cqmm_matrix_symmetrise(ctx, matL));
           oldE = 1.0e30;
           continue;
         }
-        // Accept the step, then continue in FP64 with a fresh energy reference.
+      } else if (why && fp64_done <= nfinal && mixed && why != 3) {
+        // The first FP64 refinements may see FP32-level energy noise: keep them.
         why = 0;
-        CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, rho));
-        lastE = omega; lastNe = ne; lastC = cn;
-        CQ_TRY(cqmm_matrix_symmetrise(ctx, matL));
-        oldE = 1.0e30;
-        continue;
       }
-    } else if (why && fp64_done <= nfinal && mixed && why != 3) {
-      // The first FP64 refinements may see FP32-level energy noise: keep them.
-      why = 0;
+      if (why) {
+        stop = why;
+        break;
+      }
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, rho));
+      oldE = omega;
+      lastE = omega; lastNe = ne; lastC = cn;
+      CQ_TRY(cqmm_matrix_symmetrise(ctx, matL));
     }
-    if (why) {
-      stop = why;
-      break;
+    res->iterations = std::min(n, max_iter);
+    res->stop_reason = stop;
+    if (n < 3) {
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, oldL));
+      res->restored = 1;
     }
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, rho));
-    oldE = omega;
-    lastE = omega; lastNe = ne; lastC = cn;
-    CQ_TRY(cqmm_matrix_symmetrise(ctx, matL));
-  }
-  res->iterations = std::min(n, max_iter);
-  res->stop_reason = stop;
-  if (n < 3) {
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, oldL));
-    res->restored = 1;
-  }
-  res->energy = lastE;
-  res->electrons = lastNe;
-  res->c = lastC;
-  CQ_TRY(cqmm_matrix_free(ctx, w.botH));
-  CQ_TRY(cqmm_matrix_free(ctx, w.bottom));
-  CQ_TRY(cqmm_matrix_free(ctx, w.top));
-  CQ_TRY(cqmm_matrix_free(ctx, w.LSLSL));
-  CQ_TRY(cqmm_matrix_free(ctx, w.LSL));
-  CQ_TRY(cqmm_matrix_free(ctx, w.LS));
-  CQ_TRY(cqmm_matrix_free(ctx, rho));
-  CQ_TRY(cqmm_matrix_free(ctx, oldL));
-  CQ_TRY(cq::hip_check(hipStreamSynchronize(s), "cqmm_mcweeny"));
-  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return CQMM_OK;
+    res->energy = lastE;
+    res->electrons = lastNe;
+    res->c = lastC;
+    CQ_TRY(cqmm_matrix_free(ctx, w.botH));
+    CQ_TRY(cqmm_matrix_free(ctx, w.bottom));
+    CQ_TRY(cqmm_matrix_free(ctx, w.top));
+    CQ_TRY(cqmm_matrix_free(ctx, w.LSLSL));
+    CQ_TRY(cqmm_matrix_free(ctx, w.LSL));
+    CQ_TRY(cqmm_matrix_free(ctx, w.LS));
+    CQ_TRY(cqmm_matrix_free(ctx, rho));
+    CQ_TRY(cqmm_matrix_free(ctx, oldL));
+    CQ_TRY(cq::hip_check(hipStreamSynchronize(s), "cqmm_mcweeny"));
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return CQMM_OK;
+  });
 }
 
 int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *su, int32_t matS, int32_t matT, int32_t *iters,
                    double *omega_out) {
-  if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_hotelling: null argument");
-  int st;
-  if ((st = ctx_check(ctx, matS, "cqmm_hotelling")) || (st = ctx_check(ctx, matT, "cqmm_hotelling"))) return st;
-  if (hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_hotelling: handle ranges differ from the setup");
-  hipStream_t s = cq::stream();
-  int32_t matI, matT1, matTold, matA, matTS, matGrad;
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matI));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matT1));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matTold));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matA));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matTS));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matGrad));
-  CQ_TRY(cq::op_set_diag(ctx->ranges[su->range_TS].diag, 1.0, dptr(ctx, matI), s));
-  // T0 = eps S with eps = 1 / sum S_jk^2 (Iter_Hott_InvS, src/S_matrix_module.f90:846-870).
-  double tot = 0.0;
-  CQ_TRY(cqmm_matrix_product_trace(ctx, matS, matS, &tot));
-  CQ_TRY(cqmm_matrix_scale(ctx, 0.0, matT));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT, 1.0 / tot, matS));
-  double deltaomega = 0.0, oldomega = 0.0, omega = 0.0;
-  int n = 0;
-  const int max_iter = su->max_iter > 0 ? su->max_iter : 50;
-  for (n = 1; n <= max_iter; ++n) {
-    deltaomega *= 0.5;
-    if (!(n < 3 || std::fabs(deltaomega) > su->tol)) break;
-    // HotInvS_mm (src/S_matrix_module.f90:1005-1034): TS = T.S, A = I - TS,
-    // Grad = TS.T, T1 = 2T - Grad, omega = Tr[A A].
-    CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matA, 1.0, matI));
-    CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matA, -1.0, matTS));
-    CQ_TRY(cqmm_matrix_product(ctx, matTS, matT, matGrad, su->mult_TS_T_T));
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT1, 2.0, matT));
-    CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matT1, -1.0, matGrad));
-    CQ_TRY(cqmm_matrix_product_trace(ctx, matA, matA, &omega));
-    deltaomega = omega - oldomega;
-    if (omega > oldomega && oldomega != 0.0) {
-      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT, 1.0, matTold));
-      omega = oldomega;
-      break;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_hotelling: null argument");
+    int st;
+    if ((st = ctx_check(ctx, matS, "cqmm_hotelling")) || (st = ctx_check(ctx, matT, "cqmm_hotelling"))) return st;
+    if (hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_hotelling: handle ranges differ from the setup");
+    hipStream_t s = cq::stream();
+    int32_t matI, matT1, matTold, matA, matTS, matGrad;
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matI));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matT1));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matTold));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matA));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matTS));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matGrad));
+    CQ_TRY(cq::op_set_diag(ctx->ranges[su->range_TS].diag, 1.0, dptr(ctx, matI), s));
+    // T0 = eps S with eps = 1 / sum S_jk^2 (Iter_Hott_InvS, src/S_matrix_module.f90:846-870).
+    double tot = 0.0;
+    CQ_TRY(cqmm_matrix_product_trace(ctx, matS, matS, &tot));
+    CQ_TRY(cqmm_matrix_scale(ctx, 0.0, matT));
+    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT, 1.0 / tot, matS));
+    double deltaomega = 0.0, oldomega = 0.0, omega = 0.0;
+    int n = 0;
+    const int max_iter = su->max_iter > 0 ? su->max_iter : 50;
+    for (n = 1; n <= max_iter; ++n) {
+      deltaomega *= 0.5;
+      if (!(n < 3 || std::fabs(deltaomega) > su->tol)) break;
+      // HotInvS_mm (src/S_matrix_module.f90:1005-1034): TS = T.S, A = I - TS,
+      // Grad = TS.T, T1 = 2T - Grad, omega = Tr[A A].
+      CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matA, 1.0, matI));
+      CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matA, -1.0, matTS));
+      CQ_TRY(cqmm_matrix_product(ctx, matTS, matT, matGrad, su->mult_TS_T_T));
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT1, 2.0, matT));
+      CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matT1, -1.0, matGrad));
+      CQ_TRY(cqmm_matrix_product_trace(ctx, matA, matA, &omega));
+      deltaomega = omega - oldomega;
+      if (omega > oldomega && oldomega != 0.0) {
+        CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT, 1.0, matTold));
+        omega = oldomega;
+        break;
+      }
+      oldomega = omega;
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTold, 1.0, matT));
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT, 1.0, matT1));
     }
-    oldomega = omega;
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTold, 1.0, matT));
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matT, 1.0, matT1));
-  }
-  if (iters) *iters = std::min(n, max_iter);
-  if (omega_out) *omega_out = omega;
-  CQ_TRY(cqmm_matrix_free(ctx, matGrad));
-  CQ_TRY(cqmm_matrix_free(ctx, matTS));
-  CQ_TRY(cqmm_matrix_free(ctx, matA));
-  CQ_TRY(cqmm_matrix_free(ctx, matTold));
-  CQ_TRY(cqmm_matrix_free(ctx, matT1));
-  CQ_TRY(cqmm_matrix_free(ctx, matI));
-  return cq::hip_check(hipStreamSynchronize(s), "cqmm_hotelling");
+    if (iters) *iters = std::min(n, max_iter);
+    if (omega_out) *omega_out = omega;
+    CQ_TRY(cqmm_matrix_free(ctx, matGrad));
+    CQ_TRY(cqmm_matrix_free(ctx, matTS));
+    CQ_TRY(cqmm_matrix_free(ctx, matA));
+    CQ_TRY(cqmm_matrix_free(ctx, matTold));
+    CQ_TRY(cqmm_matrix_free(ctx, matT1));
+    CQ_TRY(cqmm_matrix_free(ctx, matI));
+    return cq::hip_check(hipStreamSynchronize(s), "cqmm_hotelling");
+  });
 }
 
 int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH, int32_t matS, int32_t matT,
                       int32_t matL, double out_mu_lambda[2]) {
-  if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny: null argument");
-  int st;
-  if ((st = ctx_check(ctx, matH, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_init_mcweeny")) ||
-      (st = ctx_check(ctx, matT, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matL, "cqmm_init_mcweeny")))
-    return st;
-  const RangeRec &rh = ctx->ranges[su->range_H];
-  double n_o = 0.0;
-  for (int32_t p = 0; p < rh.n_prim; ++p) n_o += rh.ndimi[p];
-  const double ne = su->n_electrons;
-  // SX = Tr[T S]
-  int32_t matTS, matTH, matXHX, matTmp;
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matTS));
-  CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
-  double SX = 0.0;
-  CQ_TRY(cqmm_matrix_trace(ctx, matTS, &SX));
-  // McWXHX: XHX = T H T at L range, truncated copy at S range, SXHX = Tr[S (THT)_S].
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TH, &matTH));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &matXHX));
-  CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &matTmp));
-  CQ_TRY(cqmm_matrix_product(ctx, matT, matH, matTH, su->mult_T_H_TH));
-  CQ_TRY(cqmm_matrix_product(ctx, matTH, matT, matXHX, su->mult_TH_T_L));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTmp, 1.0, matXHX));
-  double SXHX = 0.0;
-  CQ_TRY(cqmm_matrix_product_trace(ctx, matS, matTmp, &SXHX));
-  // GetHLimits (src/McWeeny.f90:1020-1109) on a host copy of H.
-  std::vector<double> H(rh.length);
-  CQ_TRY(cqmm_matrix_get(ctx, matH, H.data()));
-  double hmin = 1e30, hmax = -1e30;
-  for (int32_t p = 0; p < rh.n_prim; ++p) {
-    const int32_t nsfi = rh.ndimi[p];
-    std::vector<double> lo(64, 0.0), hi(64, 0.0);
-    int64_t off = rh.row_off[p];
-    for (int32_t nb = 0; nb < rh.n_nab[p]; ++nb) {
-      const int32_t nsfj = rh.ndimj[rh.row_nab[p] + nb];
-      const bool onsite = rh.onsite[p] > 0 && off == rh.onsite[p] - 1;
-      for (int32_t i = 0; i < nsfj; ++i)
-        for (int32_t j = 0; j < nsfi; ++j) {
-          const double v = std::fabs(H[off + j + (int64_t)nsfi * i]);
-          lo[i] += onsite ? v : -v;
-          hi[i] += v;
-        }
-      off += (int64_t)nsfi * nsfj;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny: null argument");
+    int st;
+    if ((st = ctx_check(ctx, matH, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_init_mcweeny")) ||
+        (st = ctx_check(ctx, matT, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matL, "cqmm_init_mcweeny")))
+      return st;
+    const RangeRec &rh = ctx->ranges[su->range_H];
+    double n_o = 0.0;
+    for (int32_t p = 0; p < rh.n_prim; ++p) n_o += rh.ndimi[p];
+    const double ne = su->n_electrons;
+    // SX = Tr[T S]
+    int32_t matTS, matTH, matXHX, matTmp;
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matTS));
+    CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
+    double SX = 0.0;
+    CQ_TRY(cqmm_matrix_trace(ctx, matTS, &SX));
+    // McWXHX: XHX = T H T at L range, truncated copy at S range, SXHX = Tr[S (THT)_S].
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TH, &matTH));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &matXHX));
+    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &matTmp));
+    CQ_TRY(cqmm_matrix_product(ctx, matT, matH, matTH, su->mult_T_H_TH));
+    CQ_TRY(cqmm_matrix_product(ctx, matTH, matT, matXHX, su->mult_TH_T_L));
+    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTmp, 1.0, matXHX));
+    double SXHX = 0.0;
+    CQ_TRY(cqmm_matrix_product_trace(ctx, matS, matTmp, &SXHX));
+    // GetHLimits (src/McWeeny.f90:1020-1109) on a host copy of H.
+    std::vector<double> H(rh.length);
+    CQ_TRY(cqmm_matrix_get(ctx, matH, H.data()));
+    double hmin = 1e30, hmax = -1e30;
+    for (int32_t p = 0; p < rh.n_prim; ++p) {
+      const int32_t nsfi = rh.ndimi[p];
+      std::vector<double> lo(64, 0.0), hi(64, 0.0);
+      int64_t off = rh.row_off[p];
+      for (int32_t nb = 0; nb < rh.n_nab[p]; ++nb) {
+        const int32_t nsfj = rh.ndimj[rh.row_nab[p] + nb];
+        const bool onsite = rh.onsite[p] > 0 && off == rh.onsite[p] - 1;
+        for (int32_t i = 0; i < nsfj; ++i)
+          for (int32_t j = 0; j < nsfi; ++j) {
+            const double v = std::fabs(H[off + j + (int64_t)nsfi * i]);
+            lo[i] += onsite ? v : -v;
+            hi[i] += v;
+          }
+        off += (int64_t)nsfi * nsfj;
+      }
+      for (int32_t i = 0; i < nsfi; ++i) {
+        hmin = std::min(hmin, lo[i]);
+        hmax = std::max(hmax, hi[i]);
+      }
     }
-    for (int32_t i = 0; i < nsfi; ++i) {
-      hmin = std::min(hmin, lo[i]);
-      hmax = std::max(hmax, hi[i]);
+    const double A = 1.0 - SX / n_o;
+    double mubar = 0.0, lambda = 0.0;
+    if (n_o != ne) {
+      const double mu1 = SXHX / n_o + hmax * A;
+      const double mu2 = (hmin * A * ne / (n_o - ne) - SXHX / n_o) / (n_o * A / (n_o - ne) - 1.0);
+      if (ne / (hmax - mu1) < (n_o - ne) / (mu1 - hmin)) {
+        mubar = mu1;
+        lambda = ne / (hmax - mu1);
+      } else if (ne / (hmax - mu2) > (n_o - ne) / (mu2 - hmin)) {
+        mubar = mu2;
+        lambda = (n_o - ne) / (mu2 - hmin);
+      } else {
+        return cq::set_error(CQMM_ERR_NOTCONV, "cqmm_init_mcweeny: cannot find mubar");
+      }
     }
-  }
-  const double A = 1.0 - SX / n_o;
-  double mubar = 0.0, lambda = 0.0;
-  if (n_o != ne) {
-    const double mu1 = SXHX / n_o + hmax * A;
-    const double mu2 = (hmin * A * ne / (n_o - ne) - SXHX / n_o) / (n_o * A / (n_o - ne) - 1.0);
-    if (ne / (hmax - mu1) < (n_o - ne) / (mu1 - hmin)) {
-      mubar = mu1;
-      lambda = ne / (hmax - mu1);
-    } else if (ne / (hmax - mu2) > (n_o - ne) / (mu2 - hmin)) {
-      mubar = mu2;
-      lambda = (n_o - ne) / (mu2 - hmin);
-    } else {
-      return cq::set_error(CQMM_ERR_NOTCONV, "cqmm_init_mcweeny: cannot find mubar");
+    // McWRho0 (src/McWeeny.f90:961-1000): L = ((l m + Ne)/No) T - (l/No) XHX.
+    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, (lambda * mubar + ne) / n_o, matT));
+    CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matL, -lambda / n_o, matXHX));
+    if (out_mu_lambda) {
+      out_mu_lambda[0] = mubar;
+      out_mu_lambda[1] = lambda;
     }
-  }
-  // McWRho0 (src/McWeeny.f90:961-1000): L = ((l m + Ne)/No) T - (l/No) XHX.
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, (lambda * mubar + ne) / n_o, matT));
-  CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matL, -lambda / n_o, matXHX));
-  if (out_mu_lambda) {
-    out_mu_lambda[0] = mubar;
-    out_mu_lambda[1] = lambda;
-  }
-  CQ_TRY(cqmm_matrix_free(ctx, matTmp));
-  CQ_TRY(cqmm_matrix_free(ctx, matXHX));
-  CQ_TRY(cqmm_matrix_free(ctx, matTH));
-  CQ_TRY(cqmm_matrix_free(ctx, matTS));
-  return cq::hip_check(hipStreamSynchronize(cq::stream()), "cqmm_init_mcweeny");
+    CQ_TRY(cqmm_matrix_free(ctx, matTmp));
+    CQ_TRY(cqmm_matrix_free(ctx, matXHX));
+    CQ_TRY(cqmm_matrix_free(ctx, matTH));
+    CQ_TRY(cqmm_matrix_free(ctx, matTS));
+    return cq::hip_check(hipStreamSynchronize(cq::stream()), "cqmm_init_mcweeny");
+  });
 }
 
 }  // extern "C"

@@ -423,131 +423,137 @@ extern "C" {
 
 int cqmm_lnv(cqmm_ctx *ctx, const cqmm_lnv_setup *su, uint32_t flags, int32_t H, int32_t S, int32_t L, int32_t K,
              int32_t M3, int32_t phi, double *electrons, double *energy) {
-  if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: null argument");
-  const bool doE = flags & CQMM_LNV_E, doK = (flags & CQMM_LNV_K) || doE, doM3 = flags & CQMM_LNV_M3,
-             dophi = flags & CQMM_LNV_PHI;
-  if (doK && K < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: K or energy requested without a K handle");
-  if (doE && !energy) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: energy requested without an output");
-  if (doM3 && M3 < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: M3 requested without a handle");
-  if (dophi && (phi < 0 || !electrons)) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: phi requested without outputs");
-  Temps t(ctx);
-  int32_t LS, SL;
-  DTRY(t.alloc(su->range_LS, &LS));
-  DTRY(t.alloc(su->range_LS, &SL));
-  DTRY(cqmm_matrix_product(ctx, L, S, LS, su->mult_L_S_LS));
-  DTRY(cqmm_matrix_transpose(ctx, LS, SL));
-  int32_t LH = -1, HL = -1, HLS = -1, SLH = -1, A = -1, B = -1, C = -1;
-  if (doM3) {
-    DTRY(t.alloc(su->range_LH, &LH));
-    DTRY(t.alloc(su->range_LH, &HL));
-    DTRY(cqmm_matrix_product(ctx, L, H, LH, su->mult_L_H_LH));
-    DTRY(cqmm_matrix_transpose(ctx, LH, HL));
-    DTRY(t.alloc(su->range_LSL, &HLS));
-    DTRY(t.alloc(su->range_LSL, &SLH));
-    DTRY(t.alloc(su->range_LSL, &A));
-    DTRY(t.alloc(su->range_L, &B));
-    DTRY(t.alloc(su->range_L, &C));
-    DTRY(cqmm_matrix_product(ctx, HL, S, HLS, su->mult_HL_S_LSL));
-    DTRY(cqmm_matrix_transpose(ctx, HLS, SLH));
-    DTRY(cqmm_matrix_sum(ctx, 0.0, A, 2.0, HLS));
-    DTRY(cqmm_matrix_sum(ctx, 1.0, A, 1.0, SLH));
-    DTRY(cqmm_matrix_product(ctx, A, SL, B, su->mult_HLS_LS_L));
-    DTRY(cqmm_matrix_transpose(ctx, B, C));
-  }
-  int32_t SLS = -1, SLSLS = -1;
-  if (dophi) {
-    DTRY(t.alloc(su->range_SLS, &SLS));
-    DTRY(t.alloc(su->range_L, &SLSLS));
-    DTRY(cqmm_matrix_product(ctx, SL, S, SLS, su->mult_SL_S_SLS));
-    DTRY(cqmm_matrix_product(ctx, SLS, SL, SLSLS, su->mult_SLS_LS_L));
-  }
-  if (doK) {
-    int32_t LSL, LSLSL;
-    DTRY(t.alloc(su->range_LSL, &LSL));
-    DTRY(t.alloc(su->range_H, &LSLSL));
-    DTRY(cqmm_matrix_product(ctx, LS, L, LSL, su->mult_LS_L_LSL));
-    DTRY(cqmm_matrix_product(ctx, LSL, LS, LSLSL, su->mult_LSL_SL_H));
-    DTRY(cqmm_matrix_sum(ctx, 0.0, K, 3.0, LSL));
-    DTRY(cqmm_matrix_sum(ctx, 1.0, K, -2.0, LSLSL));
-  }
-  if (doE) DTRY(cqmm_matrix_product_trace(ctx, H, K, energy));
-  if (doM3) {
-    DTRY(cqmm_matrix_sum(ctx, 0.0, M3, 3.0, HLS));
-    DTRY(cqmm_matrix_sum(ctx, 1.0, M3, 3.0, SLH));
-    DTRY(cqmm_matrix_sum(ctx, 1.0, M3, -1.0, B));
-    DTRY(cqmm_matrix_sum(ctx, 1.0, M3, -1.0, C));
-  }
-  if (dophi) {
-    double e1 = 0.0, e2 = 0.0;
-    DTRY(cqmm_matrix_sum(ctx, 0.0, phi, 1.0, SLS));
-    DTRY(cqmm_matrix_product_trace(ctx, L, phi, &e1));
-    DTRY(cqmm_matrix_product_trace(ctx, L, SLSLS, &e2));
-    *electrons = 3.0 * e1 - 2.0 * e2;
-    DTRY(cqmm_matrix_sum(ctx, 6.0, phi, -6.0, SLSLS));
-  }
-  return t.release();
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: null argument");
+    const bool doE = flags & CQMM_LNV_E, doK = (flags & CQMM_LNV_K) || doE, doM3 = flags & CQMM_LNV_M3,
+               dophi = flags & CQMM_LNV_PHI;
+    if (doK && K < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: K or energy requested without a K handle");
+    if (doE && !energy) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: energy requested without an output");
+    if (doM3 && M3 < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: M3 requested without a handle");
+    if (dophi && (phi < 0 || !electrons)) return cq::set_error(CQMM_ERR_ARG, "cqmm_lnv: phi requested without outputs");
+    Temps t(ctx);
+    int32_t LS, SL;
+    DTRY(t.alloc(su->range_LS, &LS));
+    DTRY(t.alloc(su->range_LS, &SL));
+    DTRY(cqmm_matrix_product(ctx, L, S, LS, su->mult_L_S_LS));
+    DTRY(cqmm_matrix_transpose(ctx, LS, SL));
+    int32_t LH = -1, HL = -1, HLS = -1, SLH = -1, A = -1, B = -1, C = -1;
+    if (doM3) {
+      DTRY(t.alloc(su->range_LH, &LH));
+      DTRY(t.alloc(su->range_LH, &HL));
+      DTRY(cqmm_matrix_product(ctx, L, H, LH, su->mult_L_H_LH));
+      DTRY(cqmm_matrix_transpose(ctx, LH, HL));
+      DTRY(t.alloc(su->range_LSL, &HLS));
+      DTRY(t.alloc(su->range_LSL, &SLH));
+      DTRY(t.alloc(su->range_LSL, &A));
+      DTRY(t.alloc(su->range_L, &B));
+      DTRY(t.alloc(su->range_L, &C));
+      DTRY(cqmm_matrix_product(ctx, HL, S, HLS, su->mult_HL_S_LSL));
+      DTRY(cqmm_matrix_transpose(ctx, HLS, SLH));
+      DTRY(cqmm_matrix_sum(ctx, 0.0, A, 2.0, HLS));
+      DTRY(cqmm_matrix_sum(ctx, 1.0, A, 1.0, SLH));
+      DTRY(cqmm_matrix_product(ctx, A, SL, B, su->mult_HLS_LS_L));
+      DTRY(cqmm_matrix_transpose(ctx, B, C));
+    }
+    int32_t SLS = -1, SLSLS = -1;
+    if (dophi) {
+      DTRY(t.alloc(su->range_SLS, &SLS));
+      DTRY(t.alloc(su->range_L, &SLSLS));
+      DTRY(cqmm_matrix_product(ctx, SL, S, SLS, su->mult_SL_S_SLS));
+      DTRY(cqmm_matrix_product(ctx, SLS, SL, SLSLS, su->mult_SLS_LS_L));
+    }
+    if (doK) {
+      int32_t LSL, LSLSL;
+      DTRY(t.alloc(su->range_LSL, &LSL));
+      DTRY(t.alloc(su->range_H, &LSLSL));
+      DTRY(cqmm_matrix_product(ctx, LS, L, LSL, su->mult_LS_L_LSL));
+      DTRY(cqmm_matrix_product(ctx, LSL, LS, LSLSL, su->mult_LSL_SL_H));
+      DTRY(cqmm_matrix_sum(ctx, 0.0, K, 3.0, LSL));
+      DTRY(cqmm_matrix_sum(ctx, 1.0, K, -2.0, LSLSL));
+    }
+    if (doE) DTRY(cqmm_matrix_product_trace(ctx, H, K, energy));
+    if (doM3) {
+      DTRY(cqmm_matrix_sum(ctx, 0.0, M3, 3.0, HLS));
+      DTRY(cqmm_matrix_sum(ctx, 1.0, M3, 3.0, SLH));
+      DTRY(cqmm_matrix_sum(ctx, 1.0, M3, -1.0, B));
+      DTRY(cqmm_matrix_sum(ctx, 1.0, M3, -1.0, C));
+    }
+    if (dophi) {
+      double e1 = 0.0, e2 = 0.0;
+      DTRY(cqmm_matrix_sum(ctx, 0.0, phi, 1.0, SLS));
+      DTRY(cqmm_matrix_product_trace(ctx, L, phi, &e1));
+      DTRY(cqmm_matrix_product_trace(ctx, L, SLSLS, &e2));
+      *electrons = 3.0 * e1 - 2.0 * e2;
+      DTRY(cqmm_matrix_sum(ctx, 6.0, phi, -6.0, SLSLS));
+    }
+    return t.release();
+  });
 }
 
 int cqmm_correct_electrons(cqmm_ctx *ctx, const cqmm_dmm_setup *su, int32_t H, int32_t S, int32_t T, int32_t L,
                            double *electrons) {
-  if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_correct_electrons: null argument");
-  Temps t(ctx);
-  int32_t K, phi, Tt;
-  DTRY(t.alloc(su->lnv.range_H, &K));
-  DTRY(t.alloc(su->lnv.range_L, &phi));
-  DTRY(t.alloc(su->range_T, &Tt));
-  DTRY(cqmm_matrix_transpose(ctx, T, Tt));
-  Dmm d{ctx, su, H, S, T, Tt, L, K, phi};
-  DTRY(d.correct());
-  if (electrons) DTRY(cqmm_lnv(ctx, &su->lnv, CQMM_LNV_PHI, H, S, L, -1, -1, phi, electrons, nullptr));
-  return t.release();
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_correct_electrons: null argument");
+    Temps t(ctx);
+    int32_t K, phi, Tt;
+    DTRY(t.alloc(su->lnv.range_H, &K));
+    DTRY(t.alloc(su->lnv.range_L, &phi));
+    DTRY(t.alloc(su->range_T, &Tt));
+    DTRY(cqmm_matrix_transpose(ctx, T, Tt));
+    Dmm d{ctx, su, H, S, T, Tt, L, K, phi};
+    DTRY(d.correct());
+    if (electrons) DTRY(cqmm_lnv(ctx, &su->lnv, CQMM_LNV_PHI, H, S, L, -1, -1, phi, electrons, nullptr));
+    return t.release();
+  });
 }
 
 int cqmm_dmm(cqmm_ctx *ctx, const cqmm_dmm_setup *su, int32_t H, int32_t S, int32_t T, int32_t L, int32_t K,
              cqmm_dmm_result *res) {
-  if (!ctx || !su || !res) return cq::set_error(CQMM_ERR_ARG, "cqmm_dmm: null argument");
-  if (su->maxpulay < 1 || su->maxpulay > 16) return cq::set_error(CQMM_ERR_ARG, "cqmm_dmm: maxpulay outside 1..16");
-  if (su->n_atoms <= 0 || su->n_L_iterations < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_dmm: bad sizes");
-  std::memset(res, 0, sizeof(*res));
-  const auto t0 = std::chrono::steady_clock::now();
-  Temps t(ctx);
-  int32_t phi, Tt;
-  DTRY(t.alloc(su->lnv.range_L, &phi));
-  DTRY(t.alloc(su->range_T, &Tt));
-  DTRY(cqmm_matrix_transpose(ctx, T, Tt));
-  Dmm d{ctx, su, H, S, T, Tt, L, K, phi};
-  d.delta_e = su->delta_e;
-  // FindMinDM: early stage (steepest descent with line minimisation) until the
-  // line search interpolation says the quadratic regime is reached, then Pulay.
-  // An inflexion restarts the early stage (the reference first resets L with
-  // InitMcW/McWMin, which callers of this entry point do before calling it).
-  bool done = false;
-  int rounds = 0;
-  double energy = 0.0, g = 0.0;
-  while (!done && rounds < 10) {
-    ++rounds;
-    bool inflex = false;
-    int ndone = 0;
-    DTRY(d.early(&ndone, &inflex, &energy, &g));
-    res->early_iterations += ndone;
-    if (inflex) {
-      ++res->inflexions;
-      continue;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su || !res) return cq::set_error(CQMM_ERR_ARG, "cqmm_dmm: null argument");
+    if (su->maxpulay < 1 || su->maxpulay > 16) return cq::set_error(CQMM_ERR_ARG, "cqmm_dmm: maxpulay outside 1..16");
+    if (su->n_atoms <= 0 || su->n_L_iterations < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_dmm: bad sizes");
+    std::memset(res, 0, sizeof(*res));
+    const auto t0 = std::chrono::steady_clock::now();
+    Temps t(ctx);
+    int32_t phi, Tt;
+    DTRY(t.alloc(su->lnv.range_L, &phi));
+    DTRY(t.alloc(su->range_T, &Tt));
+    DTRY(cqmm_matrix_transpose(ctx, T, Tt));
+    Dmm d{ctx, su, H, S, T, Tt, L, K, phi};
+    d.delta_e = su->delta_e;
+    // FindMinDM: early stage (steepest descent with line minimisation) until the
+    // line search interpolation says the quadratic regime is reached, then Pulay.
+    // An inflexion restarts the early stage (the reference first resets L with
+    // InitMcW/McWMin, which callers of this entry point do before calling it).
+    bool done = false;
+    int rounds = 0;
+    double energy = 0.0, g = 0.0;
+    while (!done && rounds < 10) {
+      ++rounds;
+      bool inflex = false;
+      int ndone = 0;
+      DTRY(d.early(&ndone, &inflex, &energy, &g));
+      res->early_iterations += ndone;
+      if (inflex) {
+        ++res->inflexions;
+        continue;
+      }
+      DTRY(d.late(&ndone, &done, &energy, &g));
+      res->late_iterations += ndone;
+      if (!done && ndone == su->n_L_iterations) break;
     }
-    DTRY(d.late(&ndone, &done, &energy, &g));
-    res->late_iterations += ndone;
-    if (!done && ndone == su->n_L_iterations) break;
-  }
-  double el = 0.0, en = 0.0;
-  DTRY(cqmm_lnv(ctx, &su->lnv, CQMM_LNV_K | CQMM_LNV_E | CQMM_LNV_PHI, H, S, L, K, -1, phi, &el, &en));
-  res->done = done ? 1 : 0;
-  res->energy = kSpinFactor * en;
-  res->electrons = kSpinFactor * el;
-  res->residual = g;
-  DTRY(t.release());
-  DTRY(cqmm_synchronize());
-  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return CQMM_OK;
+    double el = 0.0, en = 0.0;
+    DTRY(cqmm_lnv(ctx, &su->lnv, CQMM_LNV_K | CQMM_LNV_E | CQMM_LNV_PHI, H, S, L, K, -1, phi, &el, &en));
+    res->done = done ? 1 : 0;
+    res->energy = kSpinFactor * en;
+    res->electrons = kSpinFactor * el;
+    res->residual = g;
+    DTRY(t.release());
+    DTRY(cqmm_synchronize());
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return CQMM_OK;
+  });
 }
 
 }  // extern "C"

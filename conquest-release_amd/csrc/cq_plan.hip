@@ -827,205 +827,221 @@ void halo_view64(const cq::RangeIndex &r, HaloV<int64_t> &v) {
 extern "C" {
 
 int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
-  if (!d || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: null argument");
-  DescV<int32_t> v;
-  v.mult_type = d->mult_type; v.n_prim = d->n_prim; v.ncovery = d->ncovery; v.ncoverz = d->ncoverz;
-  halo_view(d->ahalo, v.ahalo);
-  halo_view(d->chalo, v.chalo);
-  v.n_hnab = d->ltrans.n_hnab; v.i_beg = d->ltrans.i_beg; v.i_prim = d->ltrans.i_prim; v.len_i_prim = d->ltrans.len_i_prim;
-  v.b_n_parts = d->bmat.n_parts; v.b_array_posn = d->bmat.array_posn; v.b_n_atoms = d->bmat.n_atoms;
-  v.b_part_nabs = d->bmat.part_nabs; v.b_bindex = d->bmat.bindex; v.b_nd_offset32 = d->bmat.nd_offset;
-  v.b_len_bindex = d->bmat.len_bindex;
-  v.i_cc2seq = d->i_cc2seq; v.n_cell_parts = d->n_cell_parts;
-  v.lena = d->lena; v.lenb = d->lenb; v.lenc = d->lenc;
-  return build_plan(&v, out);
+  return cq::guarded(__func__, [&]() -> int {
+    if (!d || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: null argument");
+    DescV<int32_t> v;
+    v.mult_type = d->mult_type; v.n_prim = d->n_prim; v.ncovery = d->ncovery; v.ncoverz = d->ncoverz;
+    halo_view(d->ahalo, v.ahalo);
+    halo_view(d->chalo, v.chalo);
+    v.n_hnab = d->ltrans.n_hnab; v.i_beg = d->ltrans.i_beg; v.i_prim = d->ltrans.i_prim; v.len_i_prim = d->ltrans.len_i_prim;
+    v.b_n_parts = d->bmat.n_parts; v.b_array_posn = d->bmat.array_posn; v.b_n_atoms = d->bmat.n_atoms;
+    v.b_part_nabs = d->bmat.part_nabs; v.b_bindex = d->bmat.bindex; v.b_nd_offset32 = d->bmat.nd_offset;
+    v.b_len_bindex = d->bmat.len_bindex;
+    v.i_cc2seq = d->i_cc2seq; v.n_cell_parts = d->n_cell_parts;
+    v.lena = d->lena; v.lenb = d->lenb; v.lenc = d->lenc;
+    return build_plan(&v, out);
+  });
 }
 
 int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
                          cqmm_plan **out) {
-  if (!sys || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: null argument");
-  const cq::System &s = sys->s;
-  const int32_t nr = (int32_t)s.ranges.size();
-  if (ra < 0 || ra >= nr || rb < 0 || rb >= nr || rc < 0 || rc >= nr)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: range id out of bounds");
-  const cq::RangeIndex &A = s.ranges[ra], &Bm = s.ranges[rb], &Cm = s.ranges[rc];
-  if (A.rcut + Bm.rcut > s.rcut_cover + 1e-9)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_plan_create: cover radius %.6g < r(A halo)+r(B) = %.6g",
-                          s.rcut_cover, A.rcut + Bm.rcut);
-  DescV<int64_t> v;
-  v.mult_type = mult_type; v.n_prim = s.natoms; v.ncovery = s.ncover[1]; v.ncoverz = s.ncover[2];
-  halo_view64(A, v.ahalo);
-  halo_view64(Cm, v.chalo);
-  v.n_hnab = A.n_hnab.data(); v.i_beg = A.i_beg.data(); v.i_prim = A.i_prim.data(); v.len_i_prim = (int64_t)A.i_prim.size();
-  v.b_n_parts = s.n_parts; v.b_array_posn = Bm.array_posn.data(); v.b_n_atoms = Bm.n_atoms_part.data();
-  v.b_part_nabs = Bm.part_nabs.data(); v.b_bindex = Bm.bindex.data(); v.b_nd_offset64 = Bm.nd_offset64.data();
-  v.b_len_bindex = (int64_t)Bm.bindex.size();
-  v.i_cc2seq = s.i_cc2seq.data(); v.n_cell_parts = s.n_parts;
-  v.lena = A.length; v.lenb = Bm.length; v.lenc = Cm.length;
-  return build_plan(&v, out);
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: null argument");
+    const cq::System &s = sys->s;
+    const int32_t nr = (int32_t)s.ranges.size();
+    if (ra < 0 || ra >= nr || rb < 0 || rb >= nr || rc < 0 || rc >= nr)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: range id out of bounds");
+    const cq::RangeIndex &A = s.ranges[ra], &Bm = s.ranges[rb], &Cm = s.ranges[rc];
+    if (A.rcut + Bm.rcut > s.rcut_cover + 1e-9)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_plan_create: cover radius %.6g < r(A halo)+r(B) = %.6g",
+                            s.rcut_cover, A.rcut + Bm.rcut);
+    DescV<int64_t> v;
+    v.mult_type = mult_type; v.n_prim = s.natoms; v.ncovery = s.ncover[1]; v.ncoverz = s.ncover[2];
+    halo_view64(A, v.ahalo);
+    halo_view64(Cm, v.chalo);
+    v.n_hnab = A.n_hnab.data(); v.i_beg = A.i_beg.data(); v.i_prim = A.i_prim.data(); v.len_i_prim = (int64_t)A.i_prim.size();
+    v.b_n_parts = s.n_parts; v.b_array_posn = Bm.array_posn.data(); v.b_n_atoms = Bm.n_atoms_part.data();
+    v.b_part_nabs = Bm.part_nabs.data(); v.b_bindex = Bm.bindex.data(); v.b_nd_offset64 = Bm.nd_offset64.data();
+    v.b_len_bindex = (int64_t)Bm.bindex.size();
+    v.i_cc2seq = s.i_cc2seq.data(); v.n_cell_parts = s.n_parts;
+    v.lena = A.length; v.lenb = Bm.length; v.lenc = Cm.length;
+    return build_plan(&v, out);
+  });
 }
 
 int cqmm_plan_destroy(cqmm_plan *plan) {
-  if (plan) {
-    (void)hipStreamSynchronize(cq::stream());
-    delete plan;
-  }
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (plan) {
+      (void)hipStreamSynchronize(cq::stream());
+      delete plan;
+    }
+    return CQMM_OK;
+  });
 }
 
 int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info) {
-  if (!plan || !info) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_info: null argument");
-  info->n_triplets = plan->tp.n_trip;
-  info->n_out_total = plan->tp.n_seg;
-  info->flops = plan->tp.flops;
-  info->uniform_nsf = plan->tp.uniform;
-  info->mult_type = plan->tp.mult_type;
-  info->n_tiles = plan->tp.tiles.n_tiles;
-  info->n_entries = plan->tp.tiles.n_entries;
-  info->n_loads = plan->tp.tiles.n_loads;
-  info->kernel = (plan->tp.tiles.n_tiles > 0 && plan->tp.kernel != CQMM_KERNEL_TRIPLET) ? CQMM_KERNEL_TILE
-                                                                                          : CQMM_KERNEL_TRIPLET;
-  // Output blocks with at least one product.
-  std::vector<int64_t> sb(plan->tp.n_seg + 1);
-  int st = cq::hip_check(hipMemcpy(sb.data(), plan->tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg");
-  if (st) return st;
-  int64_t nz = 0;
-  for (int64_t b = 0; b < plan->tp.n_seg; ++b) nz += sb[b + 1] > sb[b];
-  info->n_out_blocks = nz;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!plan || !info) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_info: null argument");
+    info->n_triplets = plan->tp.n_trip;
+    info->n_out_total = plan->tp.n_seg;
+    info->flops = plan->tp.flops;
+    info->uniform_nsf = plan->tp.uniform;
+    info->mult_type = plan->tp.mult_type;
+    info->n_tiles = plan->tp.tiles.n_tiles;
+    info->n_entries = plan->tp.tiles.n_entries;
+    info->n_loads = plan->tp.tiles.n_loads;
+    info->kernel = (plan->tp.tiles.n_tiles > 0 && plan->tp.kernel != CQMM_KERNEL_TRIPLET) ? CQMM_KERNEL_TILE
+                                                                                            : CQMM_KERNEL_TRIPLET;
+    // Output blocks with at least one product.
+    std::vector<int64_t> sb(plan->tp.n_seg + 1);
+    int st = cq::hip_check(hipMemcpy(sb.data(), plan->tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg");
+    if (st) return st;
+    int64_t nz = 0;
+    for (int64_t b = 0; b < plan->tp.n_seg; ++b) nz += sb[b + 1] > sb[b];
+    info->n_out_blocks = nz;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel) {
-  if (!plan) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_set_kernel: null plan");
-  if (kernel != CQMM_KERNEL_AUTO && kernel != CQMM_KERNEL_TRIPLET && kernel != CQMM_KERNEL_TILE)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_set_kernel: unknown kernel %d", kernel);
-  if (kernel == CQMM_KERNEL_TILE && plan->tp.tiles.n_tiles == 0)
-    return cq::set_error(CQMM_ERR_STATE, "cqmm_plan_set_kernel: plan has no tile schedule");
-  plan->tp.kernel = kernel;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!plan) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_set_kernel: null plan");
+    if (kernel != CQMM_KERNEL_AUTO && kernel != CQMM_KERNEL_TRIPLET && kernel != CQMM_KERNEL_TILE)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_plan_set_kernel: unknown kernel %d", kernel);
+    if (kernel == CQMM_KERNEL_TILE && plan->tp.tiles.n_tiles == 0)
+      return cq::set_error(CQMM_ERR_STATE, "cqmm_plan_set_kernel: plan has no tile schedule");
+    plan->tp.kernel = kernel;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_plan_tune(cqmm_plan *plan, double *a, int64_t lena, const double *b, int64_t lenb, double *c,
                    int64_t lenc, uint32_t flags, int32_t reps, float *ms_tile, float *ms_triplet) {
-  if (!plan || !ms_tile || !ms_triplet || reps < 1)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_tune: bad argument");
-  int st = cq::ensure_init();
-  if (st) return st;
-  hipStream_t s = cq::stream();
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if ((st = cq::hip_check(hipEventCreate(&e0), "event")) || (st = cq::hip_check(hipEventCreate(&e1), "event"))) {
-    if (e0) (void)hipEventDestroy(e0);
-    return st;
-  }
-  auto time_kernel = [&](int32_t kernel, float *ms) -> int {
-    plan->tp.kernel = kernel;
-    int r = cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
-    if (r) return r;
-    if ((r = cq::hip_check(hipEventRecord(e0, s), "event record"))) return r;
-    for (int32_t k = 0; k < reps; ++k)
-      if ((r = cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags | CQMM_NO_SYNC))) return r;
-    if ((r = cq::hip_check(hipEventRecord(e1, s), "event record")) ||
-        (r = cq::hip_check(hipEventSynchronize(e1), "event sync")) ||
-        (r = cq::hip_check(hipEventElapsedTime(ms, e0, e1), "event time")))
-      return r;
-    *ms /= (float)reps;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!plan || !ms_tile || !ms_triplet || reps < 1)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_tune: bad argument");
+    int st = cq::ensure_init();
+    if (st) return st;
+    hipStream_t s = cq::stream();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((st = cq::hip_check(hipEventCreate(&e0), "event")) || (st = cq::hip_check(hipEventCreate(&e1), "event"))) {
+      if (e0) (void)hipEventDestroy(e0);
+      return st;
+    }
+    auto time_kernel = [&](int32_t kernel, float *ms) -> int {
+      plan->tp.kernel = kernel;
+      int r = cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
+      if (r) return r;
+      if ((r = cq::hip_check(hipEventRecord(e0, s), "event record"))) return r;
+      for (int32_t k = 0; k < reps; ++k)
+        if ((r = cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags | CQMM_NO_SYNC))) return r;
+      if ((r = cq::hip_check(hipEventRecord(e1, s), "event record")) ||
+          (r = cq::hip_check(hipEventSynchronize(e1), "event sync")) ||
+          (r = cq::hip_check(hipEventElapsedTime(ms, e0, e1), "event time")))
+        return r;
+      *ms /= (float)reps;
+      return CQMM_OK;
+    };
+    *ms_tile = -1.0f;
+    *ms_triplet = -1.0f;
+    const int32_t before = plan->tp.kernel;
+    st = time_kernel(CQMM_KERNEL_TRIPLET, ms_triplet);
+    if (!st && plan->tp.tiles.n_tiles > 0) st = time_kernel(CQMM_KERNEL_TILE, ms_tile);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) {
+      plan->tp.kernel = before;
+      return st;
+    }
+    plan->tp.kernel = (*ms_tile >= 0.0f && *ms_tile <= *ms_triplet) ? CQMM_KERNEL_TILE : CQMM_KERNEL_TRIPLET;
+    // The output must hold a result: re-run when the last timed kernel is not the chosen one
+    // (both are correct; this keeps the call's result from the kernel that stays).
+    if (plan->tp.kernel == CQMM_KERNEL_TRIPLET && *ms_tile >= 0.0f)
+      return cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
     return CQMM_OK;
-  };
-  *ms_tile = -1.0f;
-  *ms_triplet = -1.0f;
-  const int32_t before = plan->tp.kernel;
-  st = time_kernel(CQMM_KERNEL_TRIPLET, ms_triplet);
-  if (!st && plan->tp.tiles.n_tiles > 0) st = time_kernel(CQMM_KERNEL_TILE, ms_tile);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st) {
-    plan->tp.kernel = before;
-    return st;
-  }
-  plan->tp.kernel = (*ms_tile >= 0.0f && *ms_tile <= *ms_triplet) ? CQMM_KERNEL_TILE : CQMM_KERNEL_TRIPLET;
-  // The output must hold a result: re-run when the last timed kernel is not the chosen one
-  // (both are correct; this keeps the call's result from the kernel that stays).
-  if (plan->tp.kernel == CQMM_KERNEL_TRIPLET && *ms_tile >= 0.0f)
-    return cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
-  return CQMM_OK;
+  });
 }
 
 int cqmm_plan_get_triplets(const cqmm_plan *plan, int64_t *out, int64_t *p, int64_t *q) {
-  if (!plan || !out || !p || !q) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_triplets: null argument");
-  const cq::TripletPlan &tp = plan->tp;
-  std::vector<int64_t> sb(tp.n_seg + 1), so(tp.n_seg);
-  int st;
-  if ((st = cq::hip_check(hipMemcpy(sb.data(), tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg"))) return st;
-  if ((st = cq::hip_check(hipMemcpy(so.data(), tp.seg_out.p, so.size() * 8, hipMemcpyDeviceToHost), "D2H out"))) return st;
-  if (tp.off64) {
-    if ((st = cq::hip_check(hipMemcpy(p, tp.tp.p, tp.n_trip * 8, hipMemcpyDeviceToHost), "D2H p"))) return st;
-    if ((st = cq::hip_check(hipMemcpy(q, tp.tq.p, tp.n_trip * 8, hipMemcpyDeviceToHost), "D2H q"))) return st;
-  } else {
-    std::vector<int32_t> pp(tp.n_trip), qq(tp.n_trip);
-    if ((st = cq::hip_check(hipMemcpy(pp.data(), tp.tp.p, tp.n_trip * 4, hipMemcpyDeviceToHost), "D2H p"))) return st;
-    if ((st = cq::hip_check(hipMemcpy(qq.data(), tp.tq.p, tp.n_trip * 4, hipMemcpyDeviceToHost), "D2H q"))) return st;
-    for (int64_t t = 0; t < tp.n_trip; ++t) { p[t] = pp[t]; q[t] = qq[t]; }
-  }
-  for (int64_t b = 0; b < tp.n_seg; ++b)
-    for (int64_t t = sb[b]; t < sb[b + 1]; ++t) out[t] = so[b];
-  for (int64_t t = 0; t < tp.n_trip; ++t) { out[t] += 1; p[t] += 1; q[t] += 1; }
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!plan || !out || !p || !q) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_get_triplets: null argument");
+    const cq::TripletPlan &tp = plan->tp;
+    std::vector<int64_t> sb(tp.n_seg + 1), so(tp.n_seg);
+    int st;
+    if ((st = cq::hip_check(hipMemcpy(sb.data(), tp.seg_beg.p, sb.size() * 8, hipMemcpyDeviceToHost), "D2H seg"))) return st;
+    if ((st = cq::hip_check(hipMemcpy(so.data(), tp.seg_out.p, so.size() * 8, hipMemcpyDeviceToHost), "D2H out"))) return st;
+    if (tp.off64) {
+      if ((st = cq::hip_check(hipMemcpy(p, tp.tp.p, tp.n_trip * 8, hipMemcpyDeviceToHost), "D2H p"))) return st;
+      if ((st = cq::hip_check(hipMemcpy(q, tp.tq.p, tp.n_trip * 8, hipMemcpyDeviceToHost), "D2H q"))) return st;
+    } else {
+      std::vector<int32_t> pp(tp.n_trip), qq(tp.n_trip);
+      if ((st = cq::hip_check(hipMemcpy(pp.data(), tp.tp.p, tp.n_trip * 4, hipMemcpyDeviceToHost), "D2H p"))) return st;
+      if ((st = cq::hip_check(hipMemcpy(qq.data(), tp.tq.p, tp.n_trip * 4, hipMemcpyDeviceToHost), "D2H q"))) return st;
+      for (int64_t t = 0; t < tp.n_trip; ++t) { p[t] = pp[t]; q[t] = qq[t]; }
+    }
+    for (int64_t b = 0; b < tp.n_seg; ++b)
+      for (int64_t t = sb[b]; t < sb[b + 1]; ++t) out[t] = so[b];
+    for (int64_t t = 0; t < tp.n_trip; ++t) { out[t] += 1; p[t] += 1; q[t] += 1; }
+    return CQMM_OK;
+  });
 }
 
 int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b, int64_t lenb, double *c,
                   int64_t lenc, uint32_t flags) {
-  if (!plan || !a || !b || !c) return cq::set_error(CQMM_ERR_ARG, "cqmm_mat_mult: null argument");
-  if (lena < plan->lena || lenb < plan->lenb || lenc < plan->lenc)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_mat_mult: lengths (%lld,%lld,%lld) below the plan's (%lld,%lld,%lld)",
-                          (long long)lena, (long long)lenb, (long long)lenc, (long long)plan->lena,
-                          (long long)plan->lenb, (long long)plan->lenc);
-  int st = cq::ensure_init();
-  if (st) return st;
-  hipStream_t s = cq::stream();
-  const bool dev = flags & CQMM_DEVICE_PTRS;
-  const bool fp32 = flags & CQMM_FP32;
-  double *da = a, *dc = c;
-  const double *db = b;
-  if (!dev) {
-    if (plan->sa.bytes < (size_t)plan->lena * 8 && (st = plan->sa.alloc(plan->lena * 8))) return st;
-    if (plan->sb.bytes < (size_t)plan->lenb * 8 && (st = plan->sb.alloc(plan->lenb * 8))) return st;
-    if (plan->sc.bytes < (size_t)plan->lenc * 8 && (st = plan->sc.alloc(plan->lenc * 8))) return st;
-    da = plan->sa.as<double>();
-    dc = plan->sc.as<double>();
-    db = plan->sb.as<double>();
-    if ((st = cq::hip_check(hipMemcpyAsync((void *)db, b, plan->lenb * 8, hipMemcpyHostToDevice, s), "H2D b"))) return st;
-    if (plan->tp.mult_type == 1) {
-      if ((st = cq::hip_check(hipMemcpyAsync(da, a, plan->lena * 8, hipMemcpyHostToDevice, s), "H2D a"))) return st;
-    } else {
-      if ((st = cq::hip_check(hipMemcpyAsync(dc, c, plan->lenc * 8, hipMemcpyHostToDevice, s), "H2D c"))) return st;
-    }
-  }
-  // Operands as the kernels see them: P (left), Q (right), O (result).
-  const double *dP = plan->tp.mult_type == 1 ? da : dc;
-  double *dO = plan->tp.mult_type == 1 ? dc : da;
-  if (!fp32) {
-    st = cq::launch_block_products(plan->tp, dP, db, dO, false, s);
-  } else {
-    // FP32 products on FP32 copies; the result is widened back into the FP64 array.
-    const cq::TripletPlan &tp = plan->tp;
-    if (plan->fp.bytes < (size_t)tp.len_p * 4 && (st = plan->fp.alloc(tp.len_p * 4))) return st;
-    if (plan->fq.bytes < (size_t)tp.len_q * 4 && (st = plan->fq.alloc(tp.len_q * 4))) return st;
-    if (plan->fo.bytes < (size_t)tp.len_out * 4 && (st = plan->fo.alloc(tp.len_out * 4))) return st;
-    if ((st = cq::op_to_f32(tp.len_p, dP, plan->fp.as<float>(), s))) return st;
-    if ((st = cq::op_to_f32(tp.len_q, db, plan->fq.as<float>(), s))) return st;
-    if ((st = cq::launch_block_products_f32(tp, plan->fp.as<float>(), plan->fq.as<float>(), plan->fo.as<float>(),
-                                            (flags & CQMM_FP32_ACC64) != 0, s)))
-      return st;
-    st = cq::op_to_f64(tp.len_out, plan->fo.as<float>(), dO, s);
-  }
-  if (st) return st;
-  if (!dev) {
-    if (plan->tp.mult_type == 1) st = cq::hip_check(hipMemcpyAsync(c, dc, plan->lenc * 8, hipMemcpyDeviceToHost, s), "D2H c");
-    else st = cq::hip_check(hipMemcpyAsync(a, da, plan->lena * 8, hipMemcpyDeviceToHost, s), "D2H a");
+  return cq::guarded(__func__, [&]() -> int {
+    if (!plan || !a || !b || !c) return cq::set_error(CQMM_ERR_ARG, "cqmm_mat_mult: null argument");
+    if (lena < plan->lena || lenb < plan->lenb || lenc < plan->lenc)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_mat_mult: lengths (%lld,%lld,%lld) below the plan's (%lld,%lld,%lld)",
+                            (long long)lena, (long long)lenb, (long long)lenc, (long long)plan->lena,
+                            (long long)plan->lenb, (long long)plan->lenc);
+    int st = cq::ensure_init();
     if (st) return st;
-  }
-  if (!dev || !(flags & CQMM_NO_SYNC)) return cq::hip_check(hipStreamSynchronize(s), "cqmm_mat_mult");
-  return CQMM_OK;
+    hipStream_t s = cq::stream();
+    const bool dev = flags & CQMM_DEVICE_PTRS;
+    const bool fp32 = flags & CQMM_FP32;
+    double *da = a, *dc = c;
+    const double *db = b;
+    if (!dev) {
+      if (plan->sa.bytes < (size_t)plan->lena * 8 && (st = plan->sa.alloc(plan->lena * 8))) return st;
+      if (plan->sb.bytes < (size_t)plan->lenb * 8 && (st = plan->sb.alloc(plan->lenb * 8))) return st;
+      if (plan->sc.bytes < (size_t)plan->lenc * 8 && (st = plan->sc.alloc(plan->lenc * 8))) return st;
+      da = plan->sa.as<double>();
+      dc = plan->sc.as<double>();
+      db = plan->sb.as<double>();
+      if ((st = cq::hip_check(hipMemcpyAsync((void *)db, b, plan->lenb * 8, hipMemcpyHostToDevice, s), "H2D b"))) return st;
+      if (plan->tp.mult_type == 1) {
+        if ((st = cq::hip_check(hipMemcpyAsync(da, a, plan->lena * 8, hipMemcpyHostToDevice, s), "H2D a"))) return st;
+      } else {
+        if ((st = cq::hip_check(hipMemcpyAsync(dc, c, plan->lenc * 8, hipMemcpyHostToDevice, s), "H2D c"))) return st;
+      }
+    }
+    // Operands as the kernels see them: P (left), Q (right), O (result).
+    const double *dP = plan->tp.mult_type == 1 ? da : dc;
+    double *dO = plan->tp.mult_type == 1 ? dc : da;
+    if (!fp32) {
+      st = cq::launch_block_products(plan->tp, dP, db, dO, false, s);
+    } else {
+      // FP32 products on FP32 copies; the result is widened back into the FP64 array.
+      const cq::TripletPlan &tp = plan->tp;
+      if (plan->fp.bytes < (size_t)tp.len_p * 4 && (st = plan->fp.alloc(tp.len_p * 4))) return st;
+      if (plan->fq.bytes < (size_t)tp.len_q * 4 && (st = plan->fq.alloc(tp.len_q * 4))) return st;
+      if (plan->fo.bytes < (size_t)tp.len_out * 4 && (st = plan->fo.alloc(tp.len_out * 4))) return st;
+      if ((st = cq::op_to_f32(tp.len_p, dP, plan->fp.as<float>(), s))) return st;
+      if ((st = cq::op_to_f32(tp.len_q, db, plan->fq.as<float>(), s))) return st;
+      if ((st = cq::launch_block_products_f32(tp, plan->fp.as<float>(), plan->fq.as<float>(), plan->fo.as<float>(),
+                                              (flags & CQMM_FP32_ACC64) != 0, s)))
+        return st;
+      st = cq::op_to_f64(tp.len_out, plan->fo.as<float>(), dO, s);
+    }
+    if (st) return st;
+    if (!dev) {
+      if (plan->tp.mult_type == 1) st = cq::hip_check(hipMemcpyAsync(c, dc, plan->lenc * 8, hipMemcpyDeviceToHost, s), "D2H c");
+      else st = cq::hip_check(hipMemcpyAsync(a, da, plan->lena * 8, hipMemcpyDeviceToHost, s), "D2H a");
+      if (st) return st;
+    }
+    if (!dev || !(flags & CQMM_NO_SYNC)) return cq::hip_check(hipStreamSynchronize(s), "cqmm_mat_mult");
+    return CQMM_OK;
+  });
 }
 
 }  // extern "C"

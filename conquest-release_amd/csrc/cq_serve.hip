@@ -100,206 +100,210 @@ int64_t cqmm_purify_hbm_estimate(int32_t natoms, int32_t nsf, const cqmm_purify_
 }
 
 int cqmm_purify(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_params *p, cqmm_purify_result *r) {
-  if (!p || !r || natoms <= 0 || nsf <= 0 || nsf > 64) return cq::set_error(CQMM_ERR_ARG, "cqmm_purify: bad argument");
-  std::memset(r, 0, sizeof(*r));
-  r->natoms = natoms;
-  r->n_orb = natoms * nsf;
-  r->hbm_estimate = cqmm_purify_hbm_estimate(natoms, nsf, p);
-  const double t0 = now_s();
-  PTRY(cq::ensure_init());
-  cq::hbm_peak_reset();
-  const int64_t hbm_base = cq::hbm_current();
-  Cleanup cl;
-  std::vector<double> xyz(3 * (size_t)natoms);
-  double edge = 0.0;
-  PTRY(cqmm_random_structure(natoms, p->density, p->dmin, seed, xyz.data(), &edge));
-  const double box[3] = {edge, edge, edge};
-  const int32_t ng = std::max<int32_t>(1, (int32_t)std::lround(edge / 5.0));
-  const int32_t grid[3] = {ng, ng, ng};
-  std::vector<int32_t> nsfv(natoms, nsf);
-  const double rS = p->r_S, rH = p->r_H, rL = p->r_L, rT = p->r_S;
-  const double rLS = rL + rS, rLSL = 2 * rL + rS, rTS = rT + rS, rTH = rT + rH;
-  const double rcover = std::max({rLSL, rL + rLS, rTH + rT, 2 * rT, rT + rS, rT + rH});
-  PTRY(cqmm_sys_create(natoms, xyz.data(), box, nsfv.data(), grid, rcover, &cl.sys));
-  // Ranges in the order S, H, L, T, LS, LSL, TS, TH; equal radii share one range.
-  const double radii[8] = {rS, rH, rL, rT, rLS, rLSL, rTS, rTH};
-  int32_t id[8];
-  std::map<double, int32_t> seen;
-  for (int k = 0; k < 8; ++k) {
-    auto it = seen.find(radii[k]);
-    if (it != seen.end()) {
-      id[k] = it->second;
-    } else {
-      PTRY(cqmm_sys_add_range(cl.sys, radii[k], &id[k]));
-      seen[radii[k]] = id[k];
+  return cq::guarded(__func__, [&]() -> int {
+    if (!p || !r || natoms <= 0 || nsf <= 0 || nsf > 64) return cq::set_error(CQMM_ERR_ARG, "cqmm_purify: bad argument");
+    std::memset(r, 0, sizeof(*r));
+    r->natoms = natoms;
+    r->n_orb = natoms * nsf;
+    r->hbm_estimate = cqmm_purify_hbm_estimate(natoms, nsf, p);
+    const double t0 = now_s();
+    PTRY(cq::ensure_init());
+    cq::hbm_peak_reset();
+    const int64_t hbm_base = cq::hbm_current();
+    Cleanup cl;
+    std::vector<double> xyz(3 * (size_t)natoms);
+    double edge = 0.0;
+    PTRY(cqmm_random_structure(natoms, p->density, p->dmin, seed, xyz.data(), &edge));
+    const double box[3] = {edge, edge, edge};
+    const int32_t ng = std::max<int32_t>(1, (int32_t)std::lround(edge / 5.0));
+    const int32_t grid[3] = {ng, ng, ng};
+    std::vector<int32_t> nsfv(natoms, nsf);
+    const double rS = p->r_S, rH = p->r_H, rL = p->r_L, rT = p->r_S;
+    const double rLS = rL + rS, rLSL = 2 * rL + rS, rTS = rT + rS, rTH = rT + rH;
+    const double rcover = std::max({rLSL, rL + rLS, rTH + rT, 2 * rT, rT + rS, rT + rH});
+    PTRY(cqmm_sys_create(natoms, xyz.data(), box, nsfv.data(), grid, rcover, &cl.sys));
+    // Ranges in the order S, H, L, T, LS, LSL, TS, TH; equal radii share one range.
+    const double radii[8] = {rS, rH, rL, rT, rLS, rLSL, rTS, rTH};
+    int32_t id[8];
+    std::map<double, int32_t> seen;
+    for (int k = 0; k < 8; ++k) {
+      auto it = seen.find(radii[k]);
+      if (it != seen.end()) {
+        id[k] = it->second;
+      } else {
+        PTRY(cqmm_sys_add_range(cl.sys, radii[k], &id[k]));
+        seen[radii[k]] = id[k];
+      }
     }
-  }
-  enum { S = 0, H, Lr, T, LS, LSL, TS, TH };
-  PTRY(cqmm_ctx_create(&cl.ctx));
-  for (auto &kv : seen) {
-    cqmm_matrix_index idx;
-    PTRY(cqmm_sys_matrix_index(cl.sys, kv.second, &idx));
-    PTRY(cqmm_ctx_add_range(cl.ctx, kv.second, &idx));
-  }
-  {
-    int64_t np = 0;
-    PTRY(cqmm_sys_transpose_pairs(cl.sys, id[Lr], &np, nullptr, nullptr, nullptr, nullptr));
-    std::vector<int64_t> pp(np), qq(np);
-    std::vector<int32_t> a(np), b(np);
-    PTRY(cqmm_sys_transpose_pairs(cl.sys, id[Lr], &np, pp.data(), qq.data(), a.data(), b.data()));
-    PTRY(cqmm_ctx_set_transpose(cl.ctx, id[Lr], np, pp.data(), qq.data(), a.data(), b.data()));
-  }
-  struct M { int mid, type, ra, rb, rc; };
-  const M mults[7] = {{L_S_LS, 1, id[Lr], id[S], id[LS]},   {LS_L_LSL, 1, id[LS], id[Lr], id[LSL]},
-                      {LSL_SL_L, 2, id[Lr], id[LS], id[LSL]}, {T_S_TS, 1, id[T], id[S], id[TS]},
-                      {TS_T_T, 2, id[T], id[T], id[TS]},     {T_H_TH, 1, id[T], id[H], id[TH]},
-                      {TH_T_L, 1, id[TH], id[T], id[Lr]}};
-  for (const M &m : mults) {
-    cqmm_plan *plan = nullptr;
-    PTRY(cqmm_sys_plan_create(cl.sys, m.type, m.ra, m.rb, m.rc, &plan));
-    PTRY(cqmm_ctx_add_mult(cl.ctx, m.mid, m.ra, m.rb, m.rc, plan));
-  }
-  cqmm_range_info ri;
-  PTRY(cqmm_sys_range_info(cl.sys, id[S], &ri));
-  std::vector<double> Sv(ri.length);
-  const double ps[2] = {p->s_amp, p->s_decay};
-  PTRY(cqmm_sys_fill(cl.sys, id[S], 1, seed, 11, ps, 2, Sv.data()));
-  PTRY(cqmm_sys_range_info(cl.sys, id[H], &ri));
-  std::vector<double> Hv(ri.length);
-  const double ph[4] = {p->h_amp, p->h_decay, p->h_onsite, p->h_split};
-  PTRY(cqmm_sys_fill(cl.sys, id[H], 2, seed, 12, ph, 4, Hv.data()));
-  if (p->h_shift != 0.0) {
-    if (id[S] != id[H]) PTRY(cq::set_error(CQMM_ERR_ARG, "cqmm_purify: h_shift needs r_S == r_H"));
-    for (size_t t = 0; t < Hv.size(); ++t) Hv[t] += p->h_shift * Sv[t];
-  }
-  int32_t mS, mH, mT, mL;
-  PTRY(cqmm_matrix_allocate(cl.ctx, id[S], &mS));
-  PTRY(cqmm_matrix_allocate(cl.ctx, id[H], &mH));
-  PTRY(cqmm_matrix_allocate(cl.ctx, id[T], &mT));
-  PTRY(cqmm_matrix_allocate(cl.ctx, id[Lr], &mL));
-  PTRY(cqmm_matrix_set(cl.ctx, mS, Sv.data()));
-  PTRY(cqmm_matrix_set(cl.ctx, mH, Hv.data()));
-  const double t1 = now_s();
-  cqmm_hotelling_setup hs = {id[T], id[S], id[TS], T_S_TS, TS_T_T, 50, 1e-10};
-  int32_t hit = 0;
-  double om = 0.0;
-  PTRY(cqmm_hotelling(cl.ctx, &hs, mS, mT, &hit, &om));
-  const double ne = (double)(r->n_orb / 2);
-  cqmm_initmcw_setup is = {id[T], id[S], id[H], id[Lr], id[TS], id[TH], T_S_TS, T_H_TH, TH_T_L, ne};
-  double ml[2];
-  PTRY(cqmm_init_mcweeny(cl.ctx, &is, mH, mS, mT, mL, ml));
-  cqmm_mcw_setup ms = {id[Lr], id[S], id[H], id[LS], id[LSL], L_S_LS, LS_L_LSL, LSL_SL_L,
-                       p->max_iter, p->precision, 2, p->fp32_tol};
-  cqmm_mcw_result mr;
-  PTRY(cqmm_mcweeny(cl.ctx, &ms, mH, mS, mL, &mr));
-  const double t2 = now_s();
-  r->iterations = mr.iterations;
-  r->fp32_iterations = mr.fp32_iterations;
-  r->stop_reason = mr.stop_reason;
-  r->hotelling_iterations = hit;
-  r->band_energy = 2.0 * mr.energy;
-  r->n_electrons = 2.0 * mr.electrons;
-  r->ne_error = std::fabs(r->n_electrons - 2.0 * ne);
-  r->setup_s = t1 - t0;
-  r->solve_s = t2 - t1;
-  r->hbm_peak = cq::hbm_peak() - hbm_base;
-  return CQMM_OK;
+    enum { S = 0, H, Lr, T, LS, LSL, TS, TH };
+    PTRY(cqmm_ctx_create(&cl.ctx));
+    for (auto &kv : seen) {
+      cqmm_matrix_index idx;
+      PTRY(cqmm_sys_matrix_index(cl.sys, kv.second, &idx));
+      PTRY(cqmm_ctx_add_range(cl.ctx, kv.second, &idx));
+    }
+    {
+      int64_t np = 0;
+      PTRY(cqmm_sys_transpose_pairs(cl.sys, id[Lr], &np, nullptr, nullptr, nullptr, nullptr));
+      std::vector<int64_t> pp(np), qq(np);
+      std::vector<int32_t> a(np), b(np);
+      PTRY(cqmm_sys_transpose_pairs(cl.sys, id[Lr], &np, pp.data(), qq.data(), a.data(), b.data()));
+      PTRY(cqmm_ctx_set_transpose(cl.ctx, id[Lr], np, pp.data(), qq.data(), a.data(), b.data()));
+    }
+    struct M { int mid, type, ra, rb, rc; };
+    const M mults[7] = {{L_S_LS, 1, id[Lr], id[S], id[LS]},   {LS_L_LSL, 1, id[LS], id[Lr], id[LSL]},
+                        {LSL_SL_L, 2, id[Lr], id[LS], id[LSL]}, {T_S_TS, 1, id[T], id[S], id[TS]},
+                        {TS_T_T, 2, id[T], id[T], id[TS]},     {T_H_TH, 1, id[T], id[H], id[TH]},
+                        {TH_T_L, 1, id[TH], id[T], id[Lr]}};
+    for (const M &m : mults) {
+      cqmm_plan *plan = nullptr;
+      PTRY(cqmm_sys_plan_create(cl.sys, m.type, m.ra, m.rb, m.rc, &plan));
+      PTRY(cqmm_ctx_add_mult(cl.ctx, m.mid, m.ra, m.rb, m.rc, plan));
+    }
+    cqmm_range_info ri;
+    PTRY(cqmm_sys_range_info(cl.sys, id[S], &ri));
+    std::vector<double> Sv(ri.length);
+    const double ps[2] = {p->s_amp, p->s_decay};
+    PTRY(cqmm_sys_fill(cl.sys, id[S], 1, seed, 11, ps, 2, Sv.data()));
+    PTRY(cqmm_sys_range_info(cl.sys, id[H], &ri));
+    std::vector<double> Hv(ri.length);
+    const double ph[4] = {p->h_amp, p->h_decay, p->h_onsite, p->h_split};
+    PTRY(cqmm_sys_fill(cl.sys, id[H], 2, seed, 12, ph, 4, Hv.data()));
+    if (p->h_shift != 0.0) {
+      if (id[S] != id[H]) PTRY(cq::set_error(CQMM_ERR_ARG, "cqmm_purify: h_shift needs r_S == r_H"));
+      for (size_t t = 0; t < Hv.size(); ++t) Hv[t] += p->h_shift * Sv[t];
+    }
+    int32_t mS, mH, mT, mL;
+    PTRY(cqmm_matrix_allocate(cl.ctx, id[S], &mS));
+    PTRY(cqmm_matrix_allocate(cl.ctx, id[H], &mH));
+    PTRY(cqmm_matrix_allocate(cl.ctx, id[T], &mT));
+    PTRY(cqmm_matrix_allocate(cl.ctx, id[Lr], &mL));
+    PTRY(cqmm_matrix_set(cl.ctx, mS, Sv.data()));
+    PTRY(cqmm_matrix_set(cl.ctx, mH, Hv.data()));
+    const double t1 = now_s();
+    cqmm_hotelling_setup hs = {id[T], id[S], id[TS], T_S_TS, TS_T_T, 50, 1e-10};
+    int32_t hit = 0;
+    double om = 0.0;
+    PTRY(cqmm_hotelling(cl.ctx, &hs, mS, mT, &hit, &om));
+    const double ne = (double)(r->n_orb / 2);
+    cqmm_initmcw_setup is = {id[T], id[S], id[H], id[Lr], id[TS], id[TH], T_S_TS, T_H_TH, TH_T_L, ne};
+    double ml[2];
+    PTRY(cqmm_init_mcweeny(cl.ctx, &is, mH, mS, mT, mL, ml));
+    cqmm_mcw_setup ms = {id[Lr], id[S], id[H], id[LS], id[LSL], L_S_LS, LS_L_LSL, LSL_SL_L,
+                         p->max_iter, p->precision, 2, p->fp32_tol};
+    cqmm_mcw_result mr;
+    PTRY(cqmm_mcweeny(cl.ctx, &ms, mH, mS, mL, &mr));
+    const double t2 = now_s();
+    r->iterations = mr.iterations;
+    r->fp32_iterations = mr.fp32_iterations;
+    r->stop_reason = mr.stop_reason;
+    r->hotelling_iterations = hit;
+    r->band_energy = 2.0 * mr.energy;
+    r->n_electrons = 2.0 * mr.electrons;
+    r->ne_error = std::fabs(r->n_electrons - 2.0 * ne);
+    r->setup_s = t1 - t0;
+    r->solve_s = t2 - t1;
+    r->hbm_peak = cq::hbm_peak() - hbm_base;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_config *cfg, cqmm_serve_result *res,
                double *wall_s) {
-  if (!reqs || !cfg || !res || n < 0 || cfg->slots <= 0 || cfg->hbm_budget <= 0)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_serve: bad argument");
-  int st = cq::ensure_init();
-  if (st) return st;
-  std::vector<int32_t> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
-  std::vector<int64_t> est(n);
-  for (int32_t i = 0; i < n; ++i) {
-    std::memset(&res[i], 0, sizeof(res[i]));
-    est[i] = cqmm_purify_hbm_estimate(reqs[i].natoms, reqs[i].nsf, &cfg->params);
-    res[i].arrival_s = reqs[i].arrival_s;
-    res[i].slot = -1;
-  }
-  std::mutex mu;
-  std::condition_variable cv;
-  std::deque<int32_t> pending;
-  bool all_released = false;
-  int64_t in_use = 0;
-  const double t_start = now_s();
-  const int device = cq::runtime().device;
-  auto worker = [&](int slot) {
-    (void)hipSetDevice(device);
-    hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
-    cq::set_thread_stream(s);
-    for (;;) {
-      int32_t i = -1;
-      {
-        std::unique_lock<std::mutex> lk(mu);
-        for (;;) {
-          if (!pending.empty()) {
-            const int32_t h = pending.front();
-            if (est[h] > cfg->hbm_budget) {
-              // Can never fit: fail it without blocking the queue.
-              pending.pop_front();
-              res[h].r.status = CQMM_ERR_NOMEM;
-              res[h].admit_s = res[h].done_s = now_s() - t_start;
-              cv.notify_all();
-              continue;
-            }
-            if (in_use + est[h] <= cfg->hbm_budget) {
-              i = h;
-              pending.pop_front();
-              in_use += est[h];
+  return cq::guarded(__func__, [&]() -> int {
+    if (!reqs || !cfg || !res || n < 0 || cfg->slots <= 0 || cfg->hbm_budget <= 0)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_serve: bad argument");
+    int st = cq::ensure_init();
+    if (st) return st;
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    std::vector<int64_t> est(n);
+    for (int32_t i = 0; i < n; ++i) {
+      std::memset(&res[i], 0, sizeof(res[i]));
+      est[i] = cqmm_purify_hbm_estimate(reqs[i].natoms, reqs[i].nsf, &cfg->params);
+      res[i].arrival_s = reqs[i].arrival_s;
+      res[i].slot = -1;
+    }
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int32_t> pending;
+    bool all_released = false;
+    int64_t in_use = 0;
+    const double t_start = now_s();
+    const int device = cq::runtime().device;
+    auto worker = [&](int slot) {
+      (void)hipSetDevice(device);
+      hipStream_t s = nullptr;
+      if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
+      cq::set_thread_stream(s);
+      for (;;) {
+        int32_t i = -1;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          for (;;) {
+            if (!pending.empty()) {
+              const int32_t h = pending.front();
+              if (est[h] > cfg->hbm_budget) {
+                // Can never fit: fail it without blocking the queue.
+                pending.pop_front();
+                res[h].r.status = CQMM_ERR_NOMEM;
+                res[h].admit_s = res[h].done_s = now_s() - t_start;
+                cv.notify_all();
+                continue;
+              }
+              if (in_use + est[h] <= cfg->hbm_budget) {
+                i = h;
+                pending.pop_front();
+                in_use += est[h];
+                break;
+              }
+            } else if (all_released) {
               break;
             }
-          } else if (all_released) {
-            break;
+            cv.wait(lk);
           }
-          cv.wait(lk);
         }
+        if (i < 0) break;
+        res[i].admit_s = now_s() - t_start;
+        res[i].slot = slot;
+        cqmm_purify(reqs[i].natoms, reqs[i].nsf, reqs[i].seed, &cfg->params, &res[i].r);
+        (void)hipStreamSynchronize(s);
+        res[i].done_s = now_s() - t_start;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          in_use -= est[i];
+          if (cfg->log_progress)
+            std::fprintf(stderr, "[serve] request %d (%d atoms, %d SF) slot %d: status %d, %.2f s latency, %d iterations\n",
+                         i, reqs[i].natoms, reqs[i].nsf, slot, res[i].r.status, res[i].done_s - res[i].arrival_s,
+                         res[i].r.iterations);
+        }
+        cv.notify_all();
       }
-      if (i < 0) break;
-      res[i].admit_s = now_s() - t_start;
-      res[i].slot = slot;
-      cqmm_purify(reqs[i].natoms, reqs[i].nsf, reqs[i].seed, &cfg->params, &res[i].r);
-      (void)hipStreamSynchronize(s);
-      res[i].done_s = now_s() - t_start;
+      cq::set_thread_stream(nullptr);
+      (void)hipStreamDestroy(s);
+    };
+    std::vector<std::thread> pool;
+    for (int32_t k = 0; k < cfg->slots; ++k) pool.emplace_back(worker, k);
+    for (int32_t t = 0; t < n; ++t) {
+      const int32_t i = order[t];
+      const double wait = reqs[i].arrival_s - (now_s() - t_start);
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
       {
         std::lock_guard<std::mutex> lk(mu);
-        in_use -= est[i];
-        if (cfg->log_progress)
-          std::fprintf(stderr, "[serve] request %d (%d atoms, %d SF) slot %d: status %d, %.2f s latency, %d iterations\n",
-                       i, reqs[i].natoms, reqs[i].nsf, slot, res[i].r.status, res[i].done_s - res[i].arrival_s,
-                       res[i].r.iterations);
+        pending.push_back(i);
       }
       cv.notify_all();
     }
-    cq::set_thread_stream(nullptr);
-    (void)hipStreamDestroy(s);
-  };
-  std::vector<std::thread> pool;
-  for (int32_t k = 0; k < cfg->slots; ++k) pool.emplace_back(worker, k);
-  for (int32_t t = 0; t < n; ++t) {
-    const int32_t i = order[t];
-    const double wait = reqs[i].arrival_s - (now_s() - t_start);
-    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
     {
       std::lock_guard<std::mutex> lk(mu);
-      pending.push_back(i);
+      all_released = true;
     }
     cv.notify_all();
-  }
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    all_released = true;
-  }
-  cv.notify_all();
-  for (auto &th : pool) th.join();
-  if (wall_s) *wall_s = now_s() - t_start;
-  return CQMM_OK;
+    for (auto &th : pool) th.join();
+    if (wall_s) *wall_s = now_s() - t_start;
+    return CQMM_OK;
+  });
 }
 
 }  // extern "C"

@@ -295,318 +295,336 @@ struct FileCloser {
 extern "C" {
 
 int cqmm_matrix2_file_name(const char *root, int32_t index, int32_t inode, char *out, int32_t out_len) {
-  if (!root || !out || out_len <= 0 || index < 0 || index > 99 || inode < 0 || inode > 999999)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix2_file_name: bad argument");
-  const int n = snprintf(out, (size_t)out_len, "%smatrix2.i%02d.p%06d", root, index, inode);
-  if (n < 0 || n >= out_len) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix2_file_name: buffer too short");
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!root || !out || out_len <= 0 || index < 0 || index > 99 || inode < 0 || inode > 999999)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix2_file_name: bad argument");
+    const int n = snprintf(out, (size_t)out_len, "%smatrix2.i%02d.p%06d", root, index, inode);
+    if (n < 0 || n >= out_len) return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix2_file_name: buffer too short");
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_dump_matrix2(const cqmm_system *sys, int32_t range_id, int32_t nspin, const double *const *values,
                           const char *path, int32_t binary) {
-  if (!sys || !values || !path || nspin < 1 || range_id < 0 || range_id >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: bad argument");
-  for (int32_t sp = 0; sp < nspin; ++sp)
-    if (!values[sp]) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: null values");
-  const System &s = sys->s;
-  const RangeIndex &r = s.ranges[range_id];
-  Writer w;
-  w.binary = binary != 0;
-  w.f = fopen(path, w.binary ? "wb" : "w");
-  if (!w.f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: cannot open %s", path);
-  FileCloser fc{w.f};
-  const int32_t nprim = s.natoms;
-  std::vector<int32_t> nsf_i(nprim), jmax(nprim), jbeta(nprim, 0);
-  for (int32_t p = 0; p < nprim; ++p) {
-    nsf_i[p] = s.pnsf[p];
-    jmax[p] = (int32_t)(r.row_beg[p + 1] - r.row_beg[p]);
-    for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) jbeta[p] += r.nb_ndimj[e];
-  }
-  bool ok = true;
-  const int32_t head[2] = {1, nprim};
-  w.put(head, 8);
-  ok = ok && w.end_record(ints_line(head, 2));
-  w.put(nsf_i.data(), nsf_i.size() * 4);
-  ok = ok && w.end_record(ints_line(nsf_i.data(), nprim));
-  w.put(jmax.data(), jmax.size() * 4);
-  ok = ok && w.end_record(ints_line(jmax.data(), nprim));
-  w.put(jbeta.data(), jbeta.size() * 4);
-  ok = ok && w.end_record(ints_line(jbeta.data(), nprim));
-  w.put(nspin);
-  ok = ok && w.end_record(ints_line(&nspin, 1));
-  std::vector<int32_t> beta, idj;
-  for (int32_t p = 0; p < nprim && ok; ++p) {
-    const int32_t idi = s.prim_gid[p] + 1;
-    w.put(idi);
-    ok = ok && w.end_record(ints_line(&idi, 1));
-    beta.clear();
-    idj.clear();
-    for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
-      beta.push_back(r.nb_ndimj[e]);
-      idj.push_back(s.prim_gid[s.c_prim[r.nb_cover[e]]] + 1);
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !values || !path || nspin < 1 || range_id < 0 || range_id >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: bad argument");
+    for (int32_t sp = 0; sp < nspin; ++sp)
+      if (!values[sp]) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: null values");
+    const System &s = sys->s;
+    const RangeIndex &r = s.ranges[range_id];
+    Writer w;
+    w.binary = binary != 0;
+    w.f = fopen(path, w.binary ? "wb" : "w");
+    if (!w.f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: cannot open %s", path);
+    FileCloser fc{w.f};
+    const int32_t nprim = s.natoms;
+    std::vector<int32_t> nsf_i(nprim), jmax(nprim), jbeta(nprim, 0);
+    for (int32_t p = 0; p < nprim; ++p) {
+      nsf_i[p] = s.pnsf[p];
+      jmax[p] = (int32_t)(r.row_beg[p + 1] - r.row_beg[p]);
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) jbeta[p] += r.nb_ndimj[e];
     }
-    w.put(beta.data(), beta.size() * 4);
-    ok = ok && w.end_record(ints_line(beta.data(), (int64_t)beta.size()));
-    w.put(idj.data(), idj.size() * 4);
-    ok = ok && w.end_record(ints_line(idj.data(), (int64_t)idj.size()));
-    for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1] && ok; ++e) {
-      const int32_t m = r.nb_cover[e];
-      const double v[3] = {(s.px[p] - s.cx[m]) / s.box[0], (s.py[p] - s.cy[m]) / s.box[1],
-                           (s.pz[p] - s.cz[m]) / s.box[2]};
-      w.put(v, 24);
-      char line[128];
-      snprintf(line, sizeof line, " %.17E %.17E %.17E\n", v[0], v[1], v[2]);
-      ok = ok && w.end_record(line);
-    }
-    const int64_t len = (int64_t)jbeta[p] * nsf_i[p];
-    for (int32_t sp = 0; sp < nspin && ok; ++sp)
-      for (int64_t k = 0; k < len && ok; ++k) {
-        const double x = values[sp][r.row_off[p] + k];
-        w.put(x);
-        ok = w.end_record(e30_20(x));
+    bool ok = true;
+    const int32_t head[2] = {1, nprim};
+    w.put(head, 8);
+    ok = ok && w.end_record(ints_line(head, 2));
+    w.put(nsf_i.data(), nsf_i.size() * 4);
+    ok = ok && w.end_record(ints_line(nsf_i.data(), nprim));
+    w.put(jmax.data(), jmax.size() * 4);
+    ok = ok && w.end_record(ints_line(jmax.data(), nprim));
+    w.put(jbeta.data(), jbeta.size() * 4);
+    ok = ok && w.end_record(ints_line(jbeta.data(), nprim));
+    w.put(nspin);
+    ok = ok && w.end_record(ints_line(&nspin, 1));
+    std::vector<int32_t> beta, idj;
+    for (int32_t p = 0; p < nprim && ok; ++p) {
+      const int32_t idi = s.prim_gid[p] + 1;
+      w.put(idi);
+      ok = ok && w.end_record(ints_line(&idi, 1));
+      beta.clear();
+      idj.clear();
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
+        beta.push_back(r.nb_ndimj[e]);
+        idj.push_back(s.prim_gid[s.c_prim[r.nb_cover[e]]] + 1);
       }
-  }
-  if (!ok) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: write to %s failed", path);
-  return CQMM_OK;
+      w.put(beta.data(), beta.size() * 4);
+      ok = ok && w.end_record(ints_line(beta.data(), (int64_t)beta.size()));
+      w.put(idj.data(), idj.size() * 4);
+      ok = ok && w.end_record(ints_line(idj.data(), (int64_t)idj.size()));
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1] && ok; ++e) {
+        const int32_t m = r.nb_cover[e];
+        const double v[3] = {(s.px[p] - s.cx[m]) / s.box[0], (s.py[p] - s.cy[m]) / s.box[1],
+                             (s.pz[p] - s.cz[m]) / s.box[2]};
+        w.put(v, 24);
+        char line[128];
+        snprintf(line, sizeof line, " %.17E %.17E %.17E\n", v[0], v[1], v[2]);
+        ok = ok && w.end_record(line);
+      }
+      const int64_t len = (int64_t)jbeta[p] * nsf_i[p];
+      for (int32_t sp = 0; sp < nspin && ok; ++sp)
+        for (int64_t k = 0; k < len && ok; ++k) {
+          const double x = values[sp][r.row_off[p] + k];
+          w.put(x);
+          ok = w.end_record(e30_20(x));
+        }
+    }
+    if (!ok) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_matrix2: write to %s failed", path);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_load_matrix2(const cqmm_system *sys, int32_t range_id, const char *path, int32_t binary,
                           int32_t nspin_max, double *const *out, int32_t *nspin, int64_t *blocks_loaded,
                           int64_t *blocks_skipped) {
-  if (!sys || !path || !out || nspin_max < 1 || range_id < 0 || range_id >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: bad argument");
-  const System &s = sys->s;
-  const RangeIndex &r = s.ranges[range_id];
-  Reader rd;
-  rd.binary = binary != 0;
-  const bool opened = rd.open(path);
-  FileCloser fc{rd.f};
-  if (!opened) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: cannot open %s", path);
-  auto bad = [&](const char *what) {
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: %s: malformed or truncated (%s)", path, what);
-  };
-  int32_t head[2];
-  if (!rd.ints(head, 2)) return bad("inode, nprim");
-  const int32_t nprim = head[1];
-  if (nprim < 0 || nprim > s.natoms) return bad("nprim");
-  std::vector<int32_t> nsf_i(nprim), jmax(nprim), jbeta(nprim);
-  int32_t ns = 0;
-  if (!rd.ints(nsf_i.data(), nprim) || !rd.ints(jmax.data(), nprim) || !rd.ints(jbeta.data(), nprim) ||
-      !rd.ints(&ns, 1))
-    return bad("header");
-  // Counts that the file cannot hold (every value takes at least one byte of
-  // it) are rejected before anything is sized by them.
-  for (int32_t k = 0; k < nprim; ++k)
-    if (nsf_i[k] < 1 || nsf_i[k] > 64 || jmax[k] < 0 || jmax[k] > rd.fsize || jbeta[k] < 0 ||
-        jbeta[k] > 64 * (int64_t)jmax[k])
-      return bad("row sizes");
-  if (ns < 1 || ns > nspin_max) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: %s holds %d spins, room for %d", path, ns, nspin_max);
-  for (int32_t sp = 0; sp < ns; ++sp)
-    if (!out[sp]) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: null output");
-  // Target blocks by (global i, global j, image shift).
-  std::vector<int32_t> prim_of(s.natoms, -1);
-  for (int32_t p = 0; p < s.natoms; ++p) prim_of[s.prim_gid[p]] = p;
-  auto key = [&](int64_t gi, int64_t gj, int32_t sx, int32_t sy, int32_t sz) {
-    return (((gi * s.natoms + gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32);
-  };
-  std::unordered_map<int64_t, int64_t> block;
-  block.reserve((size_t)(r.row_beg[s.natoms] * 2));
-  for (int32_t p = 0; p < s.natoms; ++p)
-    for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
-      const int32_t m = r.nb_cover[e], ic = s.c_part[m];
-      block[key(s.prim_gid[p], s.prim_gid[s.c_prim[m]], s.cs_shift[3 * ic], s.cs_shift[3 * ic + 1],
-                s.cs_shift[3 * ic + 2])] = e;
-    }
-  int64_t loaded = 0, skipped = 0;
-  std::vector<int32_t> beta, idj;
-  std::vector<double> vec, vals;
-  for (int32_t k = 0; k < nprim; ++k) {
-    int32_t idi = 0;
-    if (!rd.ints(&idi, 1)) return bad("idglob_i");
-    beta.resize(jmax[k]);
-    idj.resize(jmax[k]);
-    vec.resize(3 * (size_t)jmax[k]);
-    if (!rd.ints(beta.data(), jmax[k]) || !rd.ints(idj.data(), jmax[k])) return bad("neighbours");
-    for (int32_t jj = 0; jj < jmax[k]; ++jj)
-      if (!rd.doubles(&vec[3 * (size_t)jj], 3)) return bad("vec_Rij");
-    const int64_t len = (int64_t)jbeta[k] * nsf_i[k];
-    if (len * ns > rd.left()) return bad("values");
-    vals.resize((size_t)len * ns);
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !path || !out || nspin_max < 1 || range_id < 0 || range_id >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: bad argument");
+    const System &s = sys->s;
+    const RangeIndex &r = s.ranges[range_id];
+    Reader rd;
+    rd.binary = binary != 0;
+    const bool opened = rd.open(path);
+    FileCloser fc{rd.f};
+    if (!opened) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: cannot open %s", path);
+    auto bad = [&](const char *what) {
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: %s: malformed or truncated (%s)", path, what);
+    };
+    int32_t head[2];
+    if (!rd.ints(head, 2)) return bad("inode, nprim");
+    const int32_t nprim = head[1];
+    if (nprim < 0 || nprim > s.natoms) return bad("nprim");
+    std::vector<int32_t> nsf_i(nprim), jmax(nprim), jbeta(nprim);
+    int32_t ns = 0;
+    if (!rd.ints(nsf_i.data(), nprim) || !rd.ints(jmax.data(), nprim) || !rd.ints(jbeta.data(), nprim) ||
+        !rd.ints(&ns, 1))
+      return bad("header");
+    // Counts that the file cannot hold (every value takes at least one byte of
+    // it) are rejected before anything is sized by them.
+    for (int32_t k = 0; k < nprim; ++k)
+      if (nsf_i[k] < 1 || nsf_i[k] > 64 || jmax[k] < 0 || jmax[k] > rd.fsize || jbeta[k] < 0 ||
+          jbeta[k] > 64 * (int64_t)jmax[k])
+        return bad("row sizes");
+    if (ns < 1 || ns > nspin_max) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: %s holds %d spins, room for %d", path, ns, nspin_max);
     for (int32_t sp = 0; sp < ns; ++sp)
-      for (int64_t q = 0; q < len; ++q)
-        if (!rd.doubles(&vals[(size_t)sp * len + q], 1)) return bad("values");
-    if (idi < 1 || idi > s.natoms) return bad("idglob_i out of range");
-    const int32_t p = prim_of[idi - 1];
-    if (nsf_i[k] != s.pnsf[p]) return bad("support functions of atom i differ");
-    int64_t voff = 0;
-    for (int32_t jj = 0; jj < jmax[k]; ++jj) {
-      const int32_t jg = idj[jj] - 1;
-      const int64_t nval = (int64_t)beta[jj] * nsf_i[k];
-      if (jg < 0 || jg >= s.natoms) return bad("idglob_j out of range");
-      if (beta[jj] < 1 || beta[jj] > 64 || voff + nval > len) return bad("neighbour sizes exceed jbeta_max_i");
-      const int32_t pj = prim_of[jg];
-      // Image of j at R_i - vec * cell: shift = (R_i - R_j) / cell - vec.
-      const double fx = (s.px[p] - s.px[pj]) / s.box[0] - vec[3 * (size_t)jj];
-      const double fy = (s.py[p] - s.py[pj]) / s.box[1] - vec[3 * (size_t)jj + 1];
-      const double fz = (s.pz[p] - s.pz[pj]) / s.box[2] - vec[3 * (size_t)jj + 2];
-      const int32_t sx = (int32_t)std::lround(fx), sy = (int32_t)std::lround(fy), sz = (int32_t)std::lround(fz);
-      auto it = (std::abs(sx) < 32 && std::abs(sy) < 32 && std::abs(sz) < 32) ? block.find(key(idi - 1, jg, sx, sy, sz))
-                                                                              : block.end();
-      if (it == block.end() || r.nb_ndimj[it->second] != beta[jj]) {
-        ++skipped;
-      } else {
-        const int64_t off = r.nb_off[it->second];
-        for (int32_t sp = 0; sp < ns; ++sp)
-          memcpy(out[sp] + off, &vals[(size_t)sp * len + voff], (size_t)nval * 8);
-        ++loaded;
+      if (!out[sp]) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_load_matrix2: null output");
+    // Target blocks by (global i, global j, image shift).
+    std::vector<int32_t> prim_of(s.natoms, -1);
+    for (int32_t p = 0; p < s.natoms; ++p) prim_of[s.prim_gid[p]] = p;
+    auto key = [&](int64_t gi, int64_t gj, int32_t sx, int32_t sy, int32_t sz) {
+      return (((gi * s.natoms + gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32);
+    };
+    std::unordered_map<int64_t, int64_t> block;
+    block.reserve((size_t)(r.row_beg[s.natoms] * 2));
+    for (int32_t p = 0; p < s.natoms; ++p)
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
+        const int32_t m = r.nb_cover[e], ic = s.c_part[m];
+        block[key(s.prim_gid[p], s.prim_gid[s.c_prim[m]], s.cs_shift[3 * ic], s.cs_shift[3 * ic + 1],
+                  s.cs_shift[3 * ic + 2])] = e;
       }
-      voff += nval;
+    int64_t loaded = 0, skipped = 0;
+    std::vector<int32_t> beta, idj;
+    std::vector<double> vec, vals;
+    for (int32_t k = 0; k < nprim; ++k) {
+      int32_t idi = 0;
+      if (!rd.ints(&idi, 1)) return bad("idglob_i");
+      beta.resize(jmax[k]);
+      idj.resize(jmax[k]);
+      vec.resize(3 * (size_t)jmax[k]);
+      if (!rd.ints(beta.data(), jmax[k]) || !rd.ints(idj.data(), jmax[k])) return bad("neighbours");
+      for (int32_t jj = 0; jj < jmax[k]; ++jj)
+        if (!rd.doubles(&vec[3 * (size_t)jj], 3)) return bad("vec_Rij");
+      const int64_t len = (int64_t)jbeta[k] * nsf_i[k];
+      if (len * ns > rd.left()) return bad("values");
+      vals.resize((size_t)len * ns);
+      for (int32_t sp = 0; sp < ns; ++sp)
+        for (int64_t q = 0; q < len; ++q)
+          if (!rd.doubles(&vals[(size_t)sp * len + q], 1)) return bad("values");
+      if (idi < 1 || idi > s.natoms) return bad("idglob_i out of range");
+      const int32_t p = prim_of[idi - 1];
+      if (nsf_i[k] != s.pnsf[p]) return bad("support functions of atom i differ");
+      int64_t voff = 0;
+      for (int32_t jj = 0; jj < jmax[k]; ++jj) {
+        const int32_t jg = idj[jj] - 1;
+        const int64_t nval = (int64_t)beta[jj] * nsf_i[k];
+        if (jg < 0 || jg >= s.natoms) return bad("idglob_j out of range");
+        if (beta[jj] < 1 || beta[jj] > 64 || voff + nval > len) return bad("neighbour sizes exceed jbeta_max_i");
+        const int32_t pj = prim_of[jg];
+        // Image of j at R_i - vec * cell: shift = (R_i - R_j) / cell - vec.
+        const double fx = (s.px[p] - s.px[pj]) / s.box[0] - vec[3 * (size_t)jj];
+        const double fy = (s.py[p] - s.py[pj]) / s.box[1] - vec[3 * (size_t)jj + 1];
+        const double fz = (s.pz[p] - s.pz[pj]) / s.box[2] - vec[3 * (size_t)jj + 2];
+        const int32_t sx = (int32_t)std::lround(fx), sy = (int32_t)std::lround(fy), sz = (int32_t)std::lround(fz);
+        auto it = (std::abs(sx) < 32 && std::abs(sy) < 32 && std::abs(sz) < 32) ? block.find(key(idi - 1, jg, sx, sy, sz))
+                                                                                : block.end();
+        if (it == block.end() || r.nb_ndimj[it->second] != beta[jj]) {
+          ++skipped;
+        } else {
+          const int64_t off = r.nb_off[it->second];
+          for (int32_t sp = 0; sp < ns; ++sp)
+            memcpy(out[sp] + off, &vals[(size_t)sp * len + voff], (size_t)nval * 8);
+          ++loaded;
+        }
+        voff += nval;
+      }
+      if (voff != len) return bad("neighbour sizes do not add up to jbeta_max_i");
     }
-    if (voff != len) return bad("neighbour sizes do not add up to jbeta_max_i");
-  }
-  if (nspin) *nspin = ns;
-  if (blocks_loaded) *blocks_loaded = loaded;
-  if (blocks_skipped) *blocks_skipped = skipped;
-  return CQMM_OK;
+    if (nspin) *nspin = ns;
+    if (blocks_loaded) *blocks_loaded = loaded;
+    if (blocks_skipped) *blocks_skipped = skipped;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_info_global_file_name(int32_t index, char *out, int32_t out_len) {
-  if (!out || out_len <= 0 || index < 0 || index > 99)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_info_global_file_name: bad argument");
-  const int n = snprintf(out, (size_t)out_len, "InfoGlobal.i%02d", index);
-  if (n < 0 || n >= out_len) return cq::set_error(CQMM_ERR_ARG, "cqmm_info_global_file_name: buffer too short");
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!out || out_len <= 0 || index < 0 || index > 99)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_info_global_file_name: bad argument");
+    const int n = snprintf(out, (size_t)out_len, "InfoGlobal.i%02d", index);
+    if (n < 0 || n >= out_len) return cq::set_error(CQMM_ERR_ARG, "cqmm_info_global_file_name: buffer too short");
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_dump_info_global(const cqmm_system *sys, const char *path, int32_t index, int32_t has_md_step,
                               int32_t md_step, const double *velocity) {
-  if (!sys || !path || index < 0 || index > 99)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: bad argument");
-  const System &s = sys->s;
-  FILE *f = fopen(path, "w");
-  if (!f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: cannot open %s", path);
-  FileCloser fc{f};
-  // Coordinates by global id (atom_coord(1:3, id_glob)); one process owns every atom.
-  std::vector<double> xyz(3 * (size_t)s.natoms);
-  for (int32_t p = 0; p < s.natoms; ++p) {
-    const size_t g = (size_t)s.prim_gid[p];
-    xyz[3 * g] = s.px[p];
-    xyz[3 * g + 1] = s.py[p];
-    xyz[3 * g + 2] = s.pz[p];
-  }
-  const int32_t step = has_md_step ? md_step : 0;
-  bool ok = fprintf(f, " %c %c  = flag_velocity, flag_MDstep\n", velocity ? 'T' : 'F', has_md_step ? 'T' : 'F') > 0;
-  ok = ok && fprintf(f, " %11d %11d  # of atoms, # of process \n", s.natoms, 1) > 0;
-  ok = ok && fprintf(f, " %11d %11d %11d  npcellx,y,z\n", s.ng[0], s.ng[1], s.ng[2]) > 0;
-  ok = ok && fprintf(f, "%s%s%s rcellx,y,z\n", f_format(s.box[0], 25, 15).c_str(), f_format(s.box[1], 25, 15).c_str(),
-                     f_format(s.box[2], 25, 15).c_str()) > 0;
-  std::string nodes;
-  for (int32_t g = 0; g < s.natoms && ok; ++g) nodes += "           1";
-  ok = ok && fprintf(f, "%s\n", nodes.c_str()) >= 0;
-  ok = ok && fprintf(f, " %11d  MD step\n", step) > 0;
-  auto atom_lines = [&](const double *v) {
-    for (int32_t g = 0; g < s.natoms && ok; ++g)
-      ok = fprintf(f, "     %8d   %s%s%s\n", g + 1, e_format(v[3 * (size_t)g], 22, 13).c_str(),
-                   e_format(v[3 * (size_t)g + 1], 22, 13).c_str(), e_format(v[3 * (size_t)g + 2], 22, 13).c_str()) > 0;
-  };
-  atom_lines(xyz.data());
-  if (velocity) {
-    ok = ok && fprintf(f, " velocity\n") > 0;
-    atom_lines(velocity);
-  }
-  ok = ok && fprintf(f, " %11d  = index_local in dump_InfoMatGlobal\n", index) > 0;
-  if (!ok) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: write to %s failed", path);
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !path || index < 0 || index > 99)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: bad argument");
+    const System &s = sys->s;
+    FILE *f = fopen(path, "w");
+    if (!f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: cannot open %s", path);
+    FileCloser fc{f};
+    // Coordinates by global id (atom_coord(1:3, id_glob)); one process owns every atom.
+    std::vector<double> xyz(3 * (size_t)s.natoms);
+    for (int32_t p = 0; p < s.natoms; ++p) {
+      const size_t g = (size_t)s.prim_gid[p];
+      xyz[3 * g] = s.px[p];
+      xyz[3 * g + 1] = s.py[p];
+      xyz[3 * g + 2] = s.pz[p];
+    }
+    const int32_t step = has_md_step ? md_step : 0;
+    bool ok = fprintf(f, " %c %c  = flag_velocity, flag_MDstep\n", velocity ? 'T' : 'F', has_md_step ? 'T' : 'F') > 0;
+    ok = ok && fprintf(f, " %11d %11d  # of atoms, # of process \n", s.natoms, 1) > 0;
+    ok = ok && fprintf(f, " %11d %11d %11d  npcellx,y,z\n", s.ng[0], s.ng[1], s.ng[2]) > 0;
+    ok = ok && fprintf(f, "%s%s%s rcellx,y,z\n", f_format(s.box[0], 25, 15).c_str(), f_format(s.box[1], 25, 15).c_str(),
+                       f_format(s.box[2], 25, 15).c_str()) > 0;
+    std::string nodes;
+    for (int32_t g = 0; g < s.natoms && ok; ++g) nodes += "           1";
+    ok = ok && fprintf(f, "%s\n", nodes.c_str()) >= 0;
+    ok = ok && fprintf(f, " %11d  MD step\n", step) > 0;
+    auto atom_lines = [&](const double *v) {
+      for (int32_t g = 0; g < s.natoms && ok; ++g)
+        ok = fprintf(f, "     %8d   %s%s%s\n", g + 1, e_format(v[3 * (size_t)g], 22, 13).c_str(),
+                     e_format(v[3 * (size_t)g + 1], 22, 13).c_str(), e_format(v[3 * (size_t)g + 2], 22, 13).c_str()) > 0;
+    };
+    atom_lines(xyz.data());
+    if (velocity) {
+      ok = ok && fprintf(f, " velocity\n") > 0;
+      atom_lines(velocity);
+    }
+    ok = ok && fprintf(f, " %11d  = index_local in dump_InfoMatGlobal\n", index) > 0;
+    if (!ok) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_dump_info_global: write to %s failed", path);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_load_info_global(const char *path, int32_t index, int32_t natoms_max, cqmm_info_global *info,
                           int32_t *glob_to_node, double *coords, double *velocity) {
-  if (!path || !info || natoms_max < 0 || index < 0)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_load_info_global: bad argument");
-  ListReader rd;
-  if (!rd.open(path)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: cannot open %s", path);
-  auto bad = [&](const char *what) {
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s: malformed or truncated (%s)", path, what);
-  };
-  cqmm_info_global g;
-  memset(&g, 0, sizeof g);
-  int32_t flags[2], counts[2];
-  if (!rd.logicals(flags, 2)) return bad("flag_velocity, flag_MDstep");
-  g.has_velocity = flags[0];
-  g.has_md_step = flags[1];
-  if (!rd.ints(counts, 2)) return bad("ni_in_cell, numprocs");
-  g.natoms = counts[0];
-  g.numprocs = counts[1];
-  // Each atom has a coordinate line, so a count above the file's lines is a
-  // corrupt or truncated file (checked before the count sizes anything).
-  if (g.natoms < 0 || (size_t)g.natoms > rd.lines.size()) return bad("ni_in_cell");
-  if (g.natoms > natoms_max && (glob_to_node || coords || velocity))
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds %d atoms, room for %d", path, g.natoms,
-                          natoms_max);
-  if (!rd.ints(g.npcell, 3)) return bad("npcellx,y,z");
-  if (!rd.reals(g.rcell, 3)) return bad("rcellx,y,z");
-  std::vector<int32_t> nodes((size_t)g.natoms);
-  if (g.natoms && !rd.ints(nodes.data(), nodes.size())) return bad("glob_to_node");
-  if (!rd.ints(&g.md_step, 1)) return bad("MD step");
-  // (5x, i8, 3x, 3e22.13) records: global id in columns 6-13, values from column 17.
-  auto atom_lines = [&](double *v) {
-    std::string rec;
-    for (int32_t a = 0; a < g.natoms; ++a) {
-      if (!rd.record(rec)) return false;
-      double x[3];
-      for (int c = 0; c < 3; ++c)
-        if (!fortran_real(field(rec, 16 + 22 * (size_t)c, 22), &x[c])) return false;
-      if (v)
-        for (int c = 0; c < 3; ++c) v[3 * (size_t)a + c] = x[c];
+  return cq::guarded(__func__, [&]() -> int {
+    if (!path || !info || natoms_max < 0 || index < 0)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_load_info_global: bad argument");
+    ListReader rd;
+    if (!rd.open(path)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: cannot open %s", path);
+    auto bad = [&](const char *what) {
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s: malformed or truncated (%s)", path, what);
+    };
+    cqmm_info_global g;
+    memset(&g, 0, sizeof g);
+    int32_t flags[2], counts[2];
+    if (!rd.logicals(flags, 2)) return bad("flag_velocity, flag_MDstep");
+    g.has_velocity = flags[0];
+    g.has_md_step = flags[1];
+    if (!rd.ints(counts, 2)) return bad("ni_in_cell, numprocs");
+    g.natoms = counts[0];
+    g.numprocs = counts[1];
+    // Each atom has a coordinate line, so a count above the file's lines is a
+    // corrupt or truncated file (checked before the count sizes anything).
+    if (g.natoms < 0 || (size_t)g.natoms > rd.lines.size()) return bad("ni_in_cell");
+    if (g.natoms > natoms_max && (glob_to_node || coords || velocity))
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds %d atoms, room for %d", path, g.natoms,
+                            natoms_max);
+    if (!rd.ints(g.npcell, 3)) return bad("npcellx,y,z");
+    if (!rd.reals(g.rcell, 3)) return bad("rcellx,y,z");
+    std::vector<int32_t> nodes((size_t)g.natoms);
+    if (g.natoms && !rd.ints(nodes.data(), nodes.size())) return bad("glob_to_node");
+    if (!rd.ints(&g.md_step, 1)) return bad("MD step");
+    // (5x, i8, 3x, 3e22.13) records: global id in columns 6-13, values from column 17.
+    auto atom_lines = [&](double *v) {
+      std::string rec;
+      for (int32_t a = 0; a < g.natoms; ++a) {
+        if (!rd.record(rec)) return false;
+        double x[3];
+        for (int c = 0; c < 3; ++c)
+          if (!fortran_real(field(rec, 16 + 22 * (size_t)c, 22), &x[c])) return false;
+        if (v)
+          for (int c = 0; c < 3; ++c) v[3 * (size_t)a + c] = x[c];
+      }
+      return true;
+    };
+    if (!atom_lines(coords)) return bad("atom_coord");
+    if (g.has_velocity) {
+      std::string skip;
+      if (!rd.record(skip) || !atom_lines(velocity)) return bad("atom_veloc");
     }
-    return true;
-  };
-  if (!atom_lines(coords)) return bad("atom_coord");
-  if (g.has_velocity) {
-    std::string skip;
-    if (!rd.record(skip) || !atom_lines(velocity)) return bad("atom_veloc");
-  }
-  if (!rd.ints(&g.index, 1)) return bad("index");
-  if (g.index != index)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds index %d, expected %d", path, g.index, index);
-  if (glob_to_node && g.natoms) memcpy(glob_to_node, nodes.data(), nodes.size() * 4);
-  *info = g;
-  return CQMM_OK;
+    if (!rd.ints(&g.index, 1)) return bad("index");
+    if (g.index != index)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_load_info_global: %s holds index %d, expected %d", path, g.index, index);
+    if (glob_to_node && g.natoms) memcpy(glob_to_node, nodes.data(), nodes.size() * 4);
+    *info = g;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_matrix_file_name(const char *stub, int32_t numprocs, int32_t inode, char *out, int32_t out_len) {
-  if (!stub || !out || out_len <= 0 || numprocs < 1 || inode < 1)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_file_name: bad argument");
-  int digits = 0;
-  for (int32_t n = numprocs; n > 0; n /= 10) ++digits;
-  const int width = digits > 3 ? digits : 3;
-  const int n = snprintf(out, (size_t)out_len, "%smatrix.%0*d", stub, width, inode);
-  if (n < 0 || n >= out_len || strlen(stub) + 1 > 80 || n > 80)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_file_name: name too long");
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!stub || !out || out_len <= 0 || numprocs < 1 || inode < 1)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_file_name: bad argument");
+    int digits = 0;
+    for (int32_t n = numprocs; n > 0; n /= 10) ++digits;
+    const int width = digits > 3 ? digits : 3;
+    const int n = snprintf(out, (size_t)out_len, "%smatrix.%0*d", stub, width, inode);
+    if (n < 0 || n >= out_len || strlen(stub) + 1 > 80 || n > 80)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_matrix_file_name: name too long");
+    return CQMM_OK;
+  });
 }
 
 int cqmm_dump_matrix_text(const char *path, const double *values, int64_t len) {
-  if (!path || len < 0 || (len && !values)) return cq::set_error(CQMM_ERR_ARG, "cqmm_dump_matrix_text: bad argument");
-  FILE *f = fopen(path, "w");
-  if (!f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_dump_matrix_text: cannot open %s", path);
-  FileCloser fc{f};
-  for (int64_t k = 0; k < len; ++k)
-    if (fprintf(f, "%s\n", f_format(values[k], 25, 18).c_str()) < 0)
-      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_dump_matrix_text: write to %s failed", path);
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!path || len < 0 || (len && !values)) return cq::set_error(CQMM_ERR_ARG, "cqmm_dump_matrix_text: bad argument");
+    FILE *f = fopen(path, "w");
+    if (!f) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_dump_matrix_text: cannot open %s", path);
+    FileCloser fc{f};
+    for (int64_t k = 0; k < len; ++k)
+      if (fprintf(f, "%s\n", f_format(values[k], 25, 18).c_str()) < 0)
+        return cq::set_errorf(CQMM_ERR_ARG, "cqmm_dump_matrix_text: write to %s failed", path);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_grab_matrix_text(const char *path, double *values, int64_t len) {
-  if (!path || len < 0 || (len && !values)) return cq::set_error(CQMM_ERR_ARG, "cqmm_grab_matrix_text: bad argument");
-  ListReader rd;
-  if (!rd.open(path)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_grab_matrix_text: failed to open input file %s", path);
-  std::string rec;
-  for (int64_t k = 0; k < len; ++k)
-    if (!rd.record(rec) || !fortran_real(field(rec, 0, 25), &values[k]))
-      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_grab_matrix_text: %s: value %lld missing or unreadable", path,
-                            (long long)k + 1);
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!path || len < 0 || (len && !values)) return cq::set_error(CQMM_ERR_ARG, "cqmm_grab_matrix_text: bad argument");
+    ListReader rd;
+    if (!rd.open(path)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_grab_matrix_text: failed to open input file %s", path);
+    std::string rec;
+    for (int64_t k = 0; k < len; ++k)
+      if (!rd.record(rec) || !fortran_real(field(rec, 0, 25), &values[k]))
+        return cq::set_errorf(CQMM_ERR_ARG, "cqmm_grab_matrix_text: %s: value %lld missing or unreadable", path,
+                              (long long)k + 1);
+    return CQMM_OK;
+  });
 }
 
 }  // extern "C"

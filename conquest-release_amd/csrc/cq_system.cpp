@@ -351,165 +351,177 @@ extern "C" {
 
 int cqmm_random_structure(int32_t natoms, double density, double dmin, uint64_t seed,
                           double *xyz, double *edge) {
-  if (natoms <= 0 || density <= 0.0 || !(dmin >= 0.0) || !xyz)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_random_structure: bad argument");
-  const double L = std::cbrt((double)natoms / density);
-  const double d2min = dmin * dmin;
-  // Cell list over the box for the minimum-image rejection test. It only
-  // narrows which placed atoms are compared (every one within dmin is), so
-  // its size, capped near the atom count for small dmin, does not change the
-  // structure.
-  const int32_t nc_cap = 2 * (int32_t)std::ceil(std::cbrt((double)natoms)) + 1;
-  const int32_t nc = std::max(1, (int32_t)std::min<double>(nc_cap, std::floor(L / std::max(dmin, 1e-3))));
-  const double cs = L / nc;
-  std::vector<std::vector<int32_t>> cells((size_t)nc * nc * nc);
-  uint64_t key = cq::hash_combine(cq::mix64(seed), 0x5eed5eedULL);
-  uint64_t ctr = 0;
-  for (int32_t i = 0; i < natoms; ++i) {
-    int64_t tries = 0;
-    for (;;) {
-      if (++tries > 100000000LL) return cq::set_error(CQMM_ERR_ARG, "cqmm_random_structure: cannot place atoms at this density");
-      const double x = cq::u01(key, ctr++) * L, y = cq::u01(key, ctr++) * L, z = cq::u01(key, ctr++) * L;
-      const int32_t ix = std::min(nc - 1, (int32_t)(x / cs)), iy = std::min(nc - 1, (int32_t)(y / cs)),
-                    iz = std::min(nc - 1, (int32_t)(z / cs));
-      bool ok = true;
-      const int32_t span = (int32_t)std::ceil(dmin / cs);
-      for (int32_t ax = -span; ax <= span && ok; ++ax)
-        for (int32_t ay = -span; ay <= span && ok; ++ay)
-          for (int32_t az = -span; az <= span && ok; ++az) {
-            const int32_t cx = ((ix + ax) % nc + nc) % nc, cy = ((iy + ay) % nc + nc) % nc,
-                          cz = ((iz + az) % nc + nc) % nc;
-            for (int32_t j : cells[((size_t)cx * nc + cy) * nc + cz]) {
-              double dx = xyz[3 * j] - x, dy = xyz[3 * j + 1] - y, dz = xyz[3 * j + 2] - z;
-              dx -= L * std::nearbyint(dx / L);
-              dy -= L * std::nearbyint(dy / L);
-              dz -= L * std::nearbyint(dz / L);
-              if (dx * dx + dy * dy + dz * dz < d2min) { ok = false; break; }
+  return cq::guarded(__func__, [&]() -> int {
+    if (natoms <= 0 || density <= 0.0 || !(dmin >= 0.0) || !xyz)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_random_structure: bad argument");
+    const double L = std::cbrt((double)natoms / density);
+    const double d2min = dmin * dmin;
+    // Cell list over the box for the minimum-image rejection test. It only
+    // narrows which placed atoms are compared (every one within dmin is), so
+    // its size, capped near the atom count for small dmin, does not change the
+    // structure.
+    const int32_t nc_cap = 2 * (int32_t)std::ceil(std::cbrt((double)natoms)) + 1;
+    const int32_t nc = std::max(1, (int32_t)std::min<double>(nc_cap, std::floor(L / std::max(dmin, 1e-3))));
+    const double cs = L / nc;
+    std::vector<std::vector<int32_t>> cells((size_t)nc * nc * nc);
+    uint64_t key = cq::hash_combine(cq::mix64(seed), 0x5eed5eedULL);
+    uint64_t ctr = 0;
+    for (int32_t i = 0; i < natoms; ++i) {
+      int64_t tries = 0;
+      for (;;) {
+        if (++tries > 100000000LL) return cq::set_error(CQMM_ERR_ARG, "cqmm_random_structure: cannot place atoms at this density");
+        const double x = cq::u01(key, ctr++) * L, y = cq::u01(key, ctr++) * L, z = cq::u01(key, ctr++) * L;
+        const int32_t ix = std::min(nc - 1, (int32_t)(x / cs)), iy = std::min(nc - 1, (int32_t)(y / cs)),
+                      iz = std::min(nc - 1, (int32_t)(z / cs));
+        bool ok = true;
+        const int32_t span = (int32_t)std::ceil(dmin / cs);
+        for (int32_t ax = -span; ax <= span && ok; ++ax)
+          for (int32_t ay = -span; ay <= span && ok; ++ay)
+            for (int32_t az = -span; az <= span && ok; ++az) {
+              const int32_t cx = ((ix + ax) % nc + nc) % nc, cy = ((iy + ay) % nc + nc) % nc,
+                            cz = ((iz + az) % nc + nc) % nc;
+              for (int32_t j : cells[((size_t)cx * nc + cy) * nc + cz]) {
+                double dx = xyz[3 * j] - x, dy = xyz[3 * j + 1] - y, dz = xyz[3 * j + 2] - z;
+                dx -= L * std::nearbyint(dx / L);
+                dy -= L * std::nearbyint(dy / L);
+                dz -= L * std::nearbyint(dz / L);
+                if (dx * dx + dy * dy + dz * dz < d2min) { ok = false; break; }
+              }
             }
-          }
-      if (ok) {
-        xyz[3 * i] = x; xyz[3 * i + 1] = y; xyz[3 * i + 2] = z;
-        cells[((size_t)ix * nc + iy) * nc + iz].push_back(i);
-        break;
+        if (ok) {
+          xyz[3 * i] = x; xyz[3 * i + 1] = y; xyz[3 * i + 2] = z;
+          cells[((size_t)ix * nc + iy) * nc + iz].push_back(i);
+          break;
+        }
       }
     }
-  }
-  if (edge) *edge = L;
-  return CQMM_OK;
+    if (edge) *edge = L;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_create(int32_t natoms, const double *xyz, const double box[3], const int32_t *nsf,
                     const int32_t ngrid[3], double rcut_cover, cqmm_system **out) {
-  if (!out || natoms <= 0 || !xyz || !box || !nsf || !ngrid)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_create: null or empty argument");
-  for (int d = 0; d < 3; ++d)
-    if (ngrid[d] <= 0 || ngrid[d] > 4096 || !(box[d] > 0.0))
-      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_create: bad box or partition grid");
-  if (!(rcut_cover > 0.0)) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_create: cover radius must be positive");
-  for (int32_t i = 0; i < natoms; ++i)
-    if (nsf[i] <= 0 || nsf[i] > 64) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_create: nsf[%d]=%d out of range", i, nsf[i]);
-  cqmm_system *h = new cqmm_system();
-  System &s = h->s;
-  s.natoms = natoms;
-  s.rcut_cover = rcut_cover;
-  for (int d = 0; d < 3; ++d) {
-    s.box[d] = box[d];
-    s.ng[d] = ngrid[d];
-    s.dcell[d] = box[d] / ngrid[d];
-  }
-  s.n_parts = s.ng[0] * s.ng[1] * s.ng[2];
-  // Partition of each atom and primary order: CC label, then global id.
-  std::vector<int32_t> part(natoms);
-  for (int32_t i = 0; i < natoms; ++i) {
-    int32_t q[3];
+  return cq::guarded(__func__, [&]() -> int {
+    if (!out || natoms <= 0 || !xyz || !box || !nsf || !ngrid)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_create: null or empty argument");
+    for (int d = 0; d < 3; ++d)
+      if (ngrid[d] <= 0 || ngrid[d] > 4096 || !(box[d] > 0.0))
+        return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_create: bad box or partition grid");
+    if (!(rcut_cover > 0.0)) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_create: cover radius must be positive");
+    for (int32_t i = 0; i < natoms; ++i)
+      if (nsf[i] <= 0 || nsf[i] > 64) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_create: nsf[%d]=%d out of range", i, nsf[i]);
+    cqmm_system *h = new cqmm_system();
+    System &s = h->s;
+    s.natoms = natoms;
+    s.rcut_cover = rcut_cover;
     for (int d = 0; d < 3; ++d) {
-      double x = xyz[3 * i + d];
-      if (!(x >= 0.0 && x < box[d])) {
-        delete h;
-        return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_create: atom %d outside the cell", i);
-      }
-      q[d] = std::min(s.ng[d] - 1, (int32_t)(x / s.dcell[d])) + 1;
+      s.box[d] = box[d];
+      s.ng[d] = ngrid[d];
+      s.dcell[d] = box[d] / ngrid[d];
     }
-    part[i] = lab3(q[0], q[1], q[2], s.ng[1], s.ng[2]);
-  }
-  std::vector<int32_t> order(natoms);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return part[a] < part[b]; });
-  s.prim_gid = order;
-  s.px.resize(natoms); s.py.resize(natoms); s.pz.resize(natoms);
-  s.pnsf.resize(natoms); s.prim_part.resize(natoms);
-  for (int32_t p = 0; p < natoms; ++p) {
-    const int32_t g = order[p];
-    s.px[p] = xyz[3 * g]; s.py[p] = xyz[3 * g + 1]; s.pz[p] = xyz[3 * g + 2];
-    s.pnsf[p] = nsf[g];
-    s.prim_part[p] = part[g];
-  }
-  s.nm_group.assign(s.n_parts, 0);
-  s.icell_beg.assign(s.n_parts, 0);
-  for (int32_t p = 0; p < natoms; ++p) ++s.nm_group[s.prim_part[p] - 1];
-  for (int32_t q = 1; q < s.n_parts; ++q) s.icell_beg[q] = s.icell_beg[q - 1] + s.nm_group[q - 1];
-  for (int d = 0; d < 3; ++d) s.idisp[d].assign(s.n_parts, 0);
-  for (int32_t q = 1; q <= s.n_parts; ++q) {
-    const int32_t nx = 1 + (q - 1) / (s.ng[1] * s.ng[2]);
-    const int32_t ny = 1 + (q - 1 - (nx - 1) * s.ng[1] * s.ng[2]) / s.ng[2];
-    const int32_t nz = q - (nx - 1) * s.ng[1] * s.ng[2] - (ny - 1) * s.ng[2];
-    s.idisp[0][q - 1] = nx - 1; s.idisp[1][q - 1] = ny - 1; s.idisp[2][q - 1] = nz - 1;
-  }
-  s.i_cc2seq.resize(s.n_parts);
-  std::iota(s.i_cc2seq.begin(), s.i_cc2seq.end(), 1);
-  build_cover(s);
-  *out = h;
-  return CQMM_OK;
+    s.n_parts = s.ng[0] * s.ng[1] * s.ng[2];
+    // Partition of each atom and primary order: CC label, then global id.
+    std::vector<int32_t> part(natoms);
+    for (int32_t i = 0; i < natoms; ++i) {
+      int32_t q[3];
+      for (int d = 0; d < 3; ++d) {
+        double x = xyz[3 * i + d];
+        if (!(x >= 0.0 && x < box[d])) {
+          delete h;
+          return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_create: atom %d outside the cell", i);
+        }
+        q[d] = std::min(s.ng[d] - 1, (int32_t)(x / s.dcell[d])) + 1;
+      }
+      part[i] = lab3(q[0], q[1], q[2], s.ng[1], s.ng[2]);
+    }
+    std::vector<int32_t> order(natoms);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return part[a] < part[b]; });
+    s.prim_gid = order;
+    s.px.resize(natoms); s.py.resize(natoms); s.pz.resize(natoms);
+    s.pnsf.resize(natoms); s.prim_part.resize(natoms);
+    for (int32_t p = 0; p < natoms; ++p) {
+      const int32_t g = order[p];
+      s.px[p] = xyz[3 * g]; s.py[p] = xyz[3 * g + 1]; s.pz[p] = xyz[3 * g + 2];
+      s.pnsf[p] = nsf[g];
+      s.prim_part[p] = part[g];
+    }
+    s.nm_group.assign(s.n_parts, 0);
+    s.icell_beg.assign(s.n_parts, 0);
+    for (int32_t p = 0; p < natoms; ++p) ++s.nm_group[s.prim_part[p] - 1];
+    for (int32_t q = 1; q < s.n_parts; ++q) s.icell_beg[q] = s.icell_beg[q - 1] + s.nm_group[q - 1];
+    for (int d = 0; d < 3; ++d) s.idisp[d].assign(s.n_parts, 0);
+    for (int32_t q = 1; q <= s.n_parts; ++q) {
+      const int32_t nx = 1 + (q - 1) / (s.ng[1] * s.ng[2]);
+      const int32_t ny = 1 + (q - 1 - (nx - 1) * s.ng[1] * s.ng[2]) / s.ng[2];
+      const int32_t nz = q - (nx - 1) * s.ng[1] * s.ng[2] - (ny - 1) * s.ng[2];
+      s.idisp[0][q - 1] = nx - 1; s.idisp[1][q - 1] = ny - 1; s.idisp[2][q - 1] = nz - 1;
+    }
+    s.i_cc2seq.resize(s.n_parts);
+    std::iota(s.i_cc2seq.begin(), s.i_cc2seq.end(), 1);
+    build_cover(s);
+    *out = h;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_destroy(cqmm_system *sys) {
-  delete sys;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    delete sys;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_add_range(cqmm_system *sys, double rcut, int32_t *range_id) {
-  if (!sys || !range_id || !(rcut > 0.0)) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_add_range: bad argument");
-  System &s = sys->s;
-  if (rcut > s.rcut_cover + 1e-12)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_add_range: range %.6g exceeds the cover radius %.6g", rcut, s.rcut_cover);
-  RangeIndex r;
-  r.rcut = rcut;
-  build_neighbours(s, r);
-  build_bindex(s, r);
-  build_halo(s, r);
-  s.ranges.push_back(std::move(r));
-  *range_id = (int32_t)s.ranges.size() - 1;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !range_id || !(rcut > 0.0)) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_add_range: bad argument");
+    System &s = sys->s;
+    if (rcut > s.rcut_cover + 1e-12)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_add_range: range %.6g exceeds the cover radius %.6g", rcut, s.rcut_cover);
+    RangeIndex r;
+    r.rcut = rcut;
+    build_neighbours(s, r);
+    build_bindex(s, r);
+    build_halo(s, r);
+    s.ranges.push_back(std::move(r));
+    *range_id = (int32_t)s.ranges.size() - 1;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_range_info(const cqmm_system *sys, int32_t rid, cqmm_range_info *info) {
-  if (!sys || !info || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_range_info: bad argument");
-  const RangeIndex &r = sys->s.ranges[rid];
-  info->rcut = r.rcut;
-  info->length = r.length;
-  info->n_blocks = r.row_beg.back();
-  info->ni_in_halo = r.ni_in_halo;
-  info->np_in_halo = r.np_in_halo;
-  info->max_nab = r.max_nab;
-  info->fits_int32 = r.fits32 ? 1 : 0;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !info || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_range_info: bad argument");
+    const RangeIndex &r = sys->s.ranges[rid];
+    info->rcut = r.rcut;
+    info->length = r.length;
+    info->n_blocks = r.row_beg.back();
+    info->ni_in_halo = r.ni_in_halo;
+    info->np_in_halo = r.np_in_halo;
+    info->max_nab = r.max_nab;
+    info->fits_int32 = r.fits32 ? 1 : 0;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_get_info(const cqmm_system *sys, cqmm_sys_info *info) {
-  if (!sys || !info) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_get_info: bad argument");
-  const System &s = sys->s;
-  info->natoms = s.natoms;
-  info->n_parts = s.n_parts;
-  info->ng_cover = s.ng_cover;
-  info->mx_mcover = s.mx_mcover;
-  for (int d = 0; d < 3; ++d) {
-    info->ncover[d] = s.ncover[d];
-    info->nspanl[d] = s.nspanl[d];
-    info->ngrid[d] = s.ng[d];
-    info->box[d] = s.box[d];
-  }
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !info) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_get_info: bad argument");
+    const System &s = sys->s;
+    info->natoms = s.natoms;
+    info->n_parts = s.n_parts;
+    info->ng_cover = s.ng_cover;
+    info->mx_mcover = s.mx_mcover;
+    for (int d = 0; d < 3; ++d) {
+      info->ncover[d] = s.ncover[d];
+      info->nspanl[d] = s.nspanl[d];
+      info->ngrid[d] = s.ng[d];
+      info->box[d] = s.box[d];
+    }
+    return CQMM_OK;
+  });
 }
 
 static void fill_halo(const RangeIndex &r, cqmm_halo &h) {
@@ -531,224 +543,236 @@ static void fill_halo(const RangeIndex &r, cqmm_halo &h) {
 
 int cqmm_sys_mult_desc(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
                        cqmm_mult_desc *d) {
-  if (!sys || !d) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_mult_desc: null argument");
-  const System &s = sys->s;
-  const int32_t nr = (int32_t)s.ranges.size();
-  if (ra < 0 || ra >= nr || rb < 0 || rb >= nr || rc < 0 || rc >= nr)
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_mult_desc: range id out of bounds");
-  if (mult_type != 1 && mult_type != 2) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_mult_desc: mult_type must be 1 or 2");
-  const RangeIndex &A = s.ranges[ra], &B = s.ranges[rb], &C = s.ranges[rc];
-  if (!A.fits32 || !B.fits32 || !C.fits32)
-    return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_mult_desc: a matrix is too long for the reference's int32 index arrays");
-  if (A.rcut + B.rcut > s.rcut_cover + 1e-9)
-    return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_mult_desc: cover radius %.6g < r(A halo)+r(B) = %.6g",
-                          s.rcut_cover, A.rcut + B.rcut);
-  std::memset(d, 0, sizeof(*d));
-  d->mult_type = mult_type;
-  d->n_prim = s.natoms;
-  d->ncovery = s.ncover[1];
-  d->ncoverz = s.ncover[2];
-  fill_halo(A, d->ahalo);
-  fill_halo(C, d->chalo);
-  d->ltrans.n_hnab = A.n_hnab.data();
-  d->ltrans.i_beg = A.i_beg.data();
-  d->ltrans.i_nd_beg = A.i_nd_beg32.data();
-  d->ltrans.i_prim = A.i_prim.data();
-  d->ltrans.len_i_prim = (int64_t)A.i_prim.size();
-  d->bmat.n_parts = s.n_parts;
-  d->bmat.array_posn = B.array_posn.data();
-  d->bmat.n_atoms = B.n_atoms_part.data();
-  d->bmat.part_nabs = B.part_nabs.data();
-  d->bmat.nd_offset = B.nd_offset32.data();
-  d->bmat.bindex = B.bindex.data();
-  d->bmat.len_bindex = (int64_t)B.bindex.size();
-  d->i_cc2seq = s.i_cc2seq.data();
-  d->n_cell_parts = s.n_parts;
-  d->lena = A.length;
-  d->lenb = B.length;
-  d->lenc = C.length;
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !d) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_mult_desc: null argument");
+    const System &s = sys->s;
+    const int32_t nr = (int32_t)s.ranges.size();
+    if (ra < 0 || ra >= nr || rb < 0 || rb >= nr || rc < 0 || rc >= nr)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_mult_desc: range id out of bounds");
+    if (mult_type != 1 && mult_type != 2) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_mult_desc: mult_type must be 1 or 2");
+    const RangeIndex &A = s.ranges[ra], &B = s.ranges[rb], &C = s.ranges[rc];
+    if (!A.fits32 || !B.fits32 || !C.fits32)
+      return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_mult_desc: a matrix is too long for the reference's int32 index arrays");
+    if (A.rcut + B.rcut > s.rcut_cover + 1e-9)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_mult_desc: cover radius %.6g < r(A halo)+r(B) = %.6g",
+                            s.rcut_cover, A.rcut + B.rcut);
+    std::memset(d, 0, sizeof(*d));
+    d->mult_type = mult_type;
+    d->n_prim = s.natoms;
+    d->ncovery = s.ncover[1];
+    d->ncoverz = s.ncover[2];
+    fill_halo(A, d->ahalo);
+    fill_halo(C, d->chalo);
+    d->ltrans.n_hnab = A.n_hnab.data();
+    d->ltrans.i_beg = A.i_beg.data();
+    d->ltrans.i_nd_beg = A.i_nd_beg32.data();
+    d->ltrans.i_prim = A.i_prim.data();
+    d->ltrans.len_i_prim = (int64_t)A.i_prim.size();
+    d->bmat.n_parts = s.n_parts;
+    d->bmat.array_posn = B.array_posn.data();
+    d->bmat.n_atoms = B.n_atoms_part.data();
+    d->bmat.part_nabs = B.part_nabs.data();
+    d->bmat.nd_offset = B.nd_offset32.data();
+    d->bmat.bindex = B.bindex.data();
+    d->bmat.len_bindex = (int64_t)B.bindex.size();
+    d->i_cc2seq = s.i_cc2seq.data();
+    d->n_cell_parts = s.n_parts;
+    d->lena = A.length;
+    d->lenb = B.length;
+    d->lenc = C.length;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_blocks(const cqmm_system *sys, int32_t rid, int32_t *prim, int32_t *gid_i, int32_t *gid_j,
                     int32_t *shift, int64_t *offset, int32_t *nd1, int32_t *nd2) {
-  if (!sys || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_blocks: bad argument");
-  const System &s = sys->s;
-  const RangeIndex &r = s.ranges[rid];
-  for (int32_t p = 0; p < s.natoms; ++p)
-    for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
-      const int32_t m = r.nb_cover[e];
-      const int32_t ic = s.c_part[m];
-      if (prim) prim[e] = p;
-      if (gid_i) gid_i[e] = s.prim_gid[p];
-      if (gid_j) gid_j[e] = s.prim_gid[s.c_prim[m]];
-      if (shift) for (int d = 0; d < 3; ++d) shift[3 * e + d] = s.cs_shift[3 * ic + d];
-      if (offset) offset[e] = r.nb_off[e];
-      if (nd1) nd1[e] = s.pnsf[p];
-      if (nd2) nd2[e] = r.nb_ndimj[e];
-    }
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_blocks: bad argument");
+    const System &s = sys->s;
+    const RangeIndex &r = s.ranges[rid];
+    for (int32_t p = 0; p < s.natoms; ++p)
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
+        const int32_t m = r.nb_cover[e];
+        const int32_t ic = s.c_part[m];
+        if (prim) prim[e] = p;
+        if (gid_i) gid_i[e] = s.prim_gid[p];
+        if (gid_j) gid_j[e] = s.prim_gid[s.c_prim[m]];
+        if (shift) for (int d = 0; d < 3; ++d) shift[3 * e + d] = s.cs_shift[3 * ic + d];
+        if (offset) offset[e] = r.nb_off[e];
+        if (nd1) nd1[e] = s.pnsf[p];
+        if (nd2) nd2[e] = r.nb_ndimj[e];
+      }
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_matrix_index(const cqmm_system *sys, int32_t rid, cqmm_matrix_index *idx) {
-  if (!sys || !idx || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_matrix_index: bad argument");
-  const System &s = sys->s;
-  RangeIndex &r = const_cast<RangeIndex &>(s.ranges[rid]);
-  if (r.nab_offset.empty()) {
-    r.nab_offset.assign(s.n_parts, 0);
-    int64_t acc = 0;
-    for (int32_t nn = 0; nn < s.n_parts; ++nn) {
-      r.nab_offset[nn] = (int32_t)acc;
-      acc += r.part_nabs[nn];
-    }
-    r.onsite.assign(s.natoms, 0);
-    for (int32_t p = 0; p < s.natoms; ++p)
-      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
-        const int32_t m = r.nb_cover[e], ic = s.c_part[m];
-        if (s.c_prim[m] == p && s.cs_shift[3 * ic] == 0 && s.cs_shift[3 * ic + 1] == 0 && s.cs_shift[3 * ic + 2] == 0)
-          r.onsite[p] = r.nb_off[e] + 1;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !idx || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_matrix_index: bad argument");
+    const System &s = sys->s;
+    RangeIndex &r = const_cast<RangeIndex &>(s.ranges[rid]);
+    if (r.nab_offset.empty()) {
+      r.nab_offset.assign(s.n_parts, 0);
+      int64_t acc = 0;
+      for (int32_t nn = 0; nn < s.n_parts; ++nn) {
+        r.nab_offset[nn] = (int32_t)acc;
+        acc += r.part_nabs[nn];
       }
-  }
-  idx->n_parts = s.n_parts;
-  idx->n_atoms = r.n_atoms_part.data();
-  idx->nab_offset = r.nab_offset.data();
-  idx->nd_offset = r.nd_offset64.data();
-  idx->n_prim = s.natoms;
-  idx->n_nab = r.n_nab.data();
-  idx->i_acc = r.i_acc.data();
-  idx->i_nd_acc = r.i_nd_acc.data();
-  idx->ndimi = r.ndimi.data();
-  idx->onsite = r.onsite.data();
-  idx->i_part = r.nb_part.data();
-  idx->i_seq = r.nb_seq.data();
-  idx->ndimj = r.nb_ndimj.data();
-  idx->n_nabs = (int64_t)r.nb_part.size();
-  idx->length = r.length;
-  return CQMM_OK;
+      r.onsite.assign(s.natoms, 0);
+      for (int32_t p = 0; p < s.natoms; ++p)
+        for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
+          const int32_t m = r.nb_cover[e], ic = s.c_part[m];
+          if (s.c_prim[m] == p && s.cs_shift[3 * ic] == 0 && s.cs_shift[3 * ic + 1] == 0 && s.cs_shift[3 * ic + 2] == 0)
+            r.onsite[p] = r.nb_off[e] + 1;
+        }
+    }
+    idx->n_parts = s.n_parts;
+    idx->n_atoms = r.n_atoms_part.data();
+    idx->nab_offset = r.nab_offset.data();
+    idx->nd_offset = r.nd_offset64.data();
+    idx->n_prim = s.natoms;
+    idx->n_nab = r.n_nab.data();
+    idx->i_acc = r.i_acc.data();
+    idx->i_nd_acc = r.i_nd_acc.data();
+    idx->ndimi = r.ndimi.data();
+    idx->onsite = r.onsite.data();
+    idx->i_part = r.nb_part.data();
+    idx->i_seq = r.nb_seq.data();
+    idx->ndimj = r.nb_ndimj.data();
+    idx->n_nabs = (int64_t)r.nb_part.size();
+    idx->length = r.length;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_transpose_pairs(const cqmm_system *sys, int32_t rid, int64_t *npairs, int64_t *p, int64_t *q,
                              int32_t *n1, int32_t *n2) {
-  if (!sys || !npairs || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_transpose_pairs: bad argument");
-  const System &s = sys->s;
-  const RangeIndex &r = s.ranges[rid];
-  auto key = [&](int32_t gi, int32_t gj, int32_t sx, int32_t sy, int32_t sz) {
-    return (((((uint64_t)(uint32_t)gi * (uint64_t)s.natoms + (uint32_t)gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32));
-  };
-  std::unordered_map<uint64_t, int64_t> where;
-  where.reserve((size_t)r.row_beg[s.natoms] * 2);
-  std::vector<uint64_t> kt(r.row_beg[s.natoms]);
-  for (int32_t pp = 0; pp < s.natoms; ++pp)
-    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
-      const int32_t m = r.nb_cover[e], ic = s.c_part[m];
-      const int32_t *sh = &s.cs_shift[3 * ic];
-      const int32_t gi = s.prim_gid[pp], gj = s.prim_gid[s.c_prim[m]];
-      where[key(gi, gj, sh[0], sh[1], sh[2])] = e;
-      kt[e] = key(gj, gi, -sh[0], -sh[1], -sh[2]);
-    }
-  int64_t n = 0;
-  for (int32_t pp = 0; pp < s.natoms; ++pp)
-    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
-      auto it = where.find(kt[e]);
-      if (it == where.end()) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
-      const int64_t f = it->second;
-      if (f < e) continue;
-      if (p) {
-        p[n] = r.nb_off[e] + 1;
-        q[n] = r.nb_off[f] + 1;
-        n1[n] = s.pnsf[pp];
-        n2[n] = r.nb_ndimj[e];
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !npairs || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_transpose_pairs: bad argument");
+    const System &s = sys->s;
+    const RangeIndex &r = s.ranges[rid];
+    auto key = [&](int32_t gi, int32_t gj, int32_t sx, int32_t sy, int32_t sz) {
+      return (((((uint64_t)(uint32_t)gi * (uint64_t)s.natoms + (uint32_t)gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32));
+    };
+    std::unordered_map<uint64_t, int64_t> where;
+    where.reserve((size_t)r.row_beg[s.natoms] * 2);
+    std::vector<uint64_t> kt(r.row_beg[s.natoms]);
+    for (int32_t pp = 0; pp < s.natoms; ++pp)
+      for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+        const int32_t m = r.nb_cover[e], ic = s.c_part[m];
+        const int32_t *sh = &s.cs_shift[3 * ic];
+        const int32_t gi = s.prim_gid[pp], gj = s.prim_gid[s.c_prim[m]];
+        where[key(gi, gj, sh[0], sh[1], sh[2])] = e;
+        kt[e] = key(gj, gi, -sh[0], -sh[1], -sh[2]);
       }
-      ++n;
-    }
-  *npairs = n;
-  return CQMM_OK;
+    int64_t n = 0;
+    for (int32_t pp = 0; pp < s.natoms; ++pp)
+      for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+        auto it = where.find(kt[e]);
+        if (it == where.end()) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
+        const int64_t f = it->second;
+        if (f < e) continue;
+        if (p) {
+          p[n] = r.nb_off[e] + 1;
+          q[n] = r.nb_off[f] + 1;
+          n1[n] = s.pnsf[pp];
+          n2[n] = r.nb_ndimj[e];
+        }
+        ++n;
+      }
+    *npairs = n;
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_primary_order(const cqmm_system *sys, int32_t *prim_gid) {
-  if (!sys || !prim_gid) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_primary_order: bad argument");
-  std::copy(sys->s.prim_gid.begin(), sys->s.prim_gid.end(), prim_gid);
-  return CQMM_OK;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !prim_gid) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_primary_order: bad argument");
+    std::copy(sys->s.prim_gid.begin(), sys->s.prim_gid.end(), prim_gid);
+    return CQMM_OK;
+  });
 }
 
 int cqmm_sys_fill(const cqmm_system *sys, int32_t rid, int32_t kind, uint64_t seed, uint64_t tag,
                   const double *prm, int32_t np, double *values) {
-  if (!sys || !values || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
-    return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_fill: bad argument");
-  const int32_t need = kind == 0 ? 1 : kind == 1 ? 2 : kind == 2 ? 4 : 0;
-  if (kind < 0 || kind > 3) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_fill: unknown kind");
-  if (np < need || (need > 0 && !prm)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_fill: kind %d needs %d parameters", kind, need);
-  const System &s = sys->s;
-  const RangeIndex &r = s.ranges[rid];
-  // Onsite energies and the occupied half (kind 2).
-  std::vector<double> eps;
-  double eps_cut = 0.0;
-  std::vector<int64_t> orb0;
-  if (kind == 2) {
-    orb0.assign(s.natoms + 1, 0);
-    for (int32_t p = 0; p < s.natoms; ++p) orb0[p + 1] = orb0[p] + s.pnsf[p];
-    eps.resize(orb0[s.natoms]);
-    for (int32_t p = 0; p < s.natoms; ++p) {
-      const int32_t g = s.prim_gid[p];
-      const int32_t z[3] = {0, 0, 0};
-      const uint64_t key = block_key(seed, tag ^ 0x0e5e0e5eULL, g, g, z);
-      for (int32_t a = 0; a < s.pnsf[p]; ++a) eps[orb0[p] + a] = prm[2] * (2.0 * cq::u01(key, (uint64_t)a) - 1.0);
-    }
-    std::vector<double> sorted(eps);
-    std::sort(sorted.begin(), sorted.end());
-    const size_t nocc = sorted.size() / 2;
-    eps_cut = nocc < sorted.size() ? sorted[nocc] : 0.0;  // orbitals with eps < eps_cut are occupied
-  }
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int32_t p = 0; p < s.natoms; ++p) {
-    const int32_t gi = s.prim_gid[p];
-    const int32_t nd1 = s.pnsf[p];
-    for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
-      const int32_t m = r.nb_cover[e];
-      const int32_t ic = s.c_part[m];
-      const int32_t pj = s.c_prim[m];
-      const int32_t gj = s.prim_gid[pj];
-      const int32_t *sh = &s.cs_shift[3 * ic];
-      const int32_t nsh[3] = {-sh[0], -sh[1], -sh[2]};
-      const int32_t nd2 = r.nb_ndimj[e];
-      double *blk = values + r.nb_off[e];
-      const bool onsite = (gi == gj && sh[0] == 0 && sh[1] == 0 && sh[2] == 0);
-      // Canonical separation: antisymmetric under (i,j,s) -> (j,i,-s) bit for bit.
-      double rr = 0.0;
-      const double pi[3] = {s.px[p], s.py[p], s.pz[p]}, pj3[3] = {s.px[pj], s.py[pj], s.pz[pj]};
-      for (int d = 0; d < 3; ++d) {
-        const double dd = (pj3[d] - pi[d]) + sh[d] * s.box[d];
-        rr += dd * dd;
+  return cq::guarded(__func__, [&]() -> int {
+    if (!sys || !values || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_fill: bad argument");
+    const int32_t need = kind == 0 ? 1 : kind == 1 ? 2 : kind == 2 ? 4 : 0;
+    if (kind < 0 || kind > 3) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_fill: unknown kind");
+    if (np < need || (need > 0 && !prm)) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_sys_fill: kind %d needs %d parameters", kind, need);
+    const System &s = sys->s;
+    const RangeIndex &r = s.ranges[rid];
+    // Onsite energies and the occupied half (kind 2).
+    std::vector<double> eps;
+    double eps_cut = 0.0;
+    std::vector<int64_t> orb0;
+    if (kind == 2) {
+      orb0.assign(s.natoms + 1, 0);
+      for (int32_t p = 0; p < s.natoms; ++p) orb0[p + 1] = orb0[p] + s.pnsf[p];
+      eps.resize(orb0[s.natoms]);
+      for (int32_t p = 0; p < s.natoms; ++p) {
+        const int32_t g = s.prim_gid[p];
+        const int32_t z[3] = {0, 0, 0};
+        const uint64_t key = block_key(seed, tag ^ 0x0e5e0e5eULL, g, g, z);
+        for (int32_t a = 0; a < s.pnsf[p]; ++a) eps[orb0[p] + a] = prm[2] * (2.0 * cq::u01(key, (uint64_t)a) - 1.0);
       }
-      rr = std::sqrt(rr);
-      const uint64_t kf = block_key(seed, tag, gi, gj, sh);
-      const uint64_t kb = block_key(seed, tag, gj, gi, nsh);
-      for (int32_t b = 0; b < nd2; ++b)
-        for (int32_t a = 0; a < nd1; ++a) {
-          double v = 0.0;
-          if (kind == 0) {
-            v = normal01(kf, (uint64_t)a * 1024 + b) * std::exp(-rr / prm[0]);
-          } else if (kind == 3) {
-            v = (onsite && a == b) ? 1.0 : 0.0;
-          } else if (onsite && kind == 1) {
-            v = (a == b) ? 1.0 : 0.0;
-          } else if (onsite && kind == 2) {
-            if (a == b) {
-              const double ea = eps[orb0[p] + a];
-              v = ea + (ea < eps_cut ? -prm[3] : prm[3]);
-            }
-          } else {
-            const double g = 0.5 * (normal01(kf, (uint64_t)a * 1024 + b) + normal01(kb, (uint64_t)b * 1024 + a));
-            v = prm[0] * g * std::exp(-rr / prm[1]);
-          }
-          blk[a + (int64_t)nd1 * b] = v;
-        }
+      std::vector<double> sorted(eps);
+      std::sort(sorted.begin(), sorted.end());
+      const size_t nocc = sorted.size() / 2;
+      eps_cut = nocc < sorted.size() ? sorted[nocc] : 0.0;  // orbitals with eps < eps_cut are occupied
     }
-  }
-  return CQMM_OK;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int32_t p = 0; p < s.natoms; ++p) {
+      const int32_t gi = s.prim_gid[p];
+      const int32_t nd1 = s.pnsf[p];
+      for (int64_t e = r.row_beg[p]; e < r.row_beg[p + 1]; ++e) {
+        const int32_t m = r.nb_cover[e];
+        const int32_t ic = s.c_part[m];
+        const int32_t pj = s.c_prim[m];
+        const int32_t gj = s.prim_gid[pj];
+        const int32_t *sh = &s.cs_shift[3 * ic];
+        const int32_t nsh[3] = {-sh[0], -sh[1], -sh[2]};
+        const int32_t nd2 = r.nb_ndimj[e];
+        double *blk = values + r.nb_off[e];
+        const bool onsite = (gi == gj && sh[0] == 0 && sh[1] == 0 && sh[2] == 0);
+        // Canonical separation: antisymmetric under (i,j,s) -> (j,i,-s) bit for bit.
+        double rr = 0.0;
+        const double pi[3] = {s.px[p], s.py[p], s.pz[p]}, pj3[3] = {s.px[pj], s.py[pj], s.pz[pj]};
+        for (int d = 0; d < 3; ++d) {
+          const double dd = (pj3[d] - pi[d]) + sh[d] * s.box[d];
+          rr += dd * dd;
+        }
+        rr = std::sqrt(rr);
+        const uint64_t kf = block_key(seed, tag, gi, gj, sh);
+        const uint64_t kb = block_key(seed, tag, gj, gi, nsh);
+        for (int32_t b = 0; b < nd2; ++b)
+          for (int32_t a = 0; a < nd1; ++a) {
+            double v = 0.0;
+            if (kind == 0) {
+              v = normal01(kf, (uint64_t)a * 1024 + b) * std::exp(-rr / prm[0]);
+            } else if (kind == 3) {
+              v = (onsite && a == b) ? 1.0 : 0.0;
+            } else if (onsite && kind == 1) {
+              v = (a == b) ? 1.0 : 0.0;
+            } else if (onsite && kind == 2) {
+              if (a == b) {
+                const double ea = eps[orb0[p] + a];
+                v = ea + (ea < eps_cut ? -prm[3] : prm[3]);
+              }
+            } else {
+              const double g = 0.5 * (normal01(kf, (uint64_t)a * 1024 + b) + normal01(kb, (uint64_t)b * 1024 + a));
+              v = prm[0] * g * std::exp(-rr / prm[1]);
+            }
+            blk[a + (int64_t)nd1 * b] = v;
+          }
+      }
+    }
+    return CQMM_OK;
+  });
 }
 
 }  // extern "C"

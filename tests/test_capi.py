@@ -49,3 +49,24 @@ def test_random_structure_small_dmin_and_bad_dmin():
     assert xyz.shape == (100, 3) and np.all((xyz >= 0) & (xyz < edge))
     with pytest.raises(cqmm.CqmmError, match="bad argument"):
         cqmm.random_structure(10, 0.05, -1.0, 4)
+
+
+def _overcommit_refuses_huge():
+    try:
+        return open("/proc/sys/vm/overcommit_memory").read().strip() in ("0", "2")
+    except OSError:
+        return False
+
+
+@pytest.mark.skipif(not _overcommit_refuses_huge(), reason="needs an allocator that refuses a 400 GB request")
+def test_host_allocation_failure_is_a_status_not_an_abort():
+    # 2^31-1 atoms: the rejection cell list alone asks for ~400 GB. The entry
+    # point must return CQMM_ERR_NOMEM (the output buffer is never written:
+    # the failure comes before any atom is placed).
+    lib = cqmm.lib()
+    xyz = (ctypes.c_double * 3)()
+    edge = ctypes.c_double()
+    st = lib.cqmm_random_structure(2**31 - 1, ctypes.c_double(0.05), ctypes.c_double(0.0), ctypes.c_uint64(1),
+                                   xyz, ctypes.byref(edge))
+    assert st == -3
+    assert b"cqmm_random_structure: host allocation failed" in lib.cqmm_last_error()
