@@ -12,7 +12,7 @@ import pytest
 
 import cqmm
 from cqmm.dm import Context, Purifier
-from cases import Case, cluster_structure
+from cases import SI8, Case, cluster_structure
 from oracle import mcweeny as mo
 
 pytestmark = pytest.mark.gpu
@@ -104,9 +104,17 @@ def test_cluster_hotelling_init_and_mcweeny_match_dense_oracle(nsf):
     assert abs(res["electrons"] - ne) < 1e-6
 
 
-def test_periodic_mcweeny_fp64_matches_sparse_oracle():
-    xyz, edge = cqmm.random_structure(24, 0.05, 2.0, 5)
-    pu = Purifier(xyz, edge, 4, r_S=4.5, r_H=4.5, r_L=6.0, grid=(2, 2, 2), seed=5)
+@pytest.mark.parametrize("structure", ["random24", "si8_test002"])
+def test_periodic_mcweeny_fp64_matches_sparse_oracle(structure):
+    if structure == "random24":
+        xyz, edge = cqmm.random_structure(24, 0.05, 2.0, 5)
+        pu = Purifier(xyz, edge, 4, r_S=4.5, r_H=4.5, r_L=6.0, grid=(2, 2, 2), seed=5)
+    else:  # the reference's Order-N test 002 structure and ranges (bohr): L.S.L spans four cells
+        edge = SI8["cell_bohr"][0][0]
+        xyz = np.array(SI8["frac"]) * edge
+        pu = Purifier(xyz, edge, SI8["nsf"], r_S=SI8["s_range_bohr"], r_H=SI8["s_range_bohr"],
+                      r_L=SI8["l_range_bohr"], grid=(2, 2, 2), seed=5)
+    print(f"mcweeny periodic {structure}", flush=True)
     sp = mo.Sparse(pu.sys, pu.rid)
     pu.hotelling()
     pu.init_mcweeny()
