@@ -18,6 +18,7 @@
 #include "cq_common.h"
 #include "cq_device.h"
 #include "cq_ops.h"
+#include "cq_temps.h"
 
 namespace {
 
@@ -203,6 +204,16 @@ int cqmm_ctx_add_range(cqmm_ctx *ctx, int32_t range, const cqmm_matrix_index *id
     if (idx->n_parts <= 0 || idx->n_prim <= 0 || !idx->n_atoms || !idx->nab_offset || !idx->nd_offset || !idx->n_nab ||
         !idx->i_acc || !idx->i_nd_acc || !idx->ndimi || !idx->i_part || !idx->i_seq || !idx->ndimj || idx->length <= 0)
       return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_add_range: incomplete matrix index");
+    if (ctx->ranges[range].set) {
+      // A live matrix or a registered product sized by the old index would
+      // read or write past its buffers under the new one.
+      for (const MatRec &m : ctx->mats)
+        if (m.live && m.range == range)
+          return cq::set_errorf(CQMM_ERR_STATE, "cqmm_ctx_add_range: range %d has live matrices", range);
+      for (const auto &kv : ctx->mults)
+        if (kv.second.ra == range || kv.second.rb == range || kv.second.rc == range)
+          return cq::set_errorf(CQMM_ERR_STATE, "cqmm_ctx_add_range: range %d is used by mult %d", range, kv.first);
+    }
     RangeRec r;
     r.n_prim = idx->n_prim;
     r.length = idx->length;
@@ -575,17 +586,18 @@ int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t 
     std::memset(res, 0, sizeof(*res));
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = cq::stream();
-    // Temporaries (freed in reverse order at the end).
+    // Temporaries, freed in reverse order on every return.
+    cq::Temps tmp(ctx);
     int32_t oldL, rho;
     McwWork w;
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &oldL));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &rho));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_LS, &w.LS));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_LSL, &w.LSL));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &w.LSLSL));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &w.top));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &w.bottom));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_H, &w.botH));
+    CQ_TRY(tmp.alloc(su->range_L, &oldL));
+    CQ_TRY(tmp.alloc(su->range_L, &rho));
+    CQ_TRY(tmp.alloc(su->range_LS, &w.LS));
+    CQ_TRY(tmp.alloc(su->range_LSL, &w.LSL));
+    CQ_TRY(tmp.alloc(su->range_L, &w.LSLSL));
+    CQ_TRY(tmp.alloc(su->range_S, &w.top));
+    CQ_TRY(tmp.alloc(su->range_S, &w.bottom));
+    CQ_TRY(tmp.alloc(su->range_H, &w.botH));
     w.L32 = w.S32 = w.LS32 = w.LSL32 = w.LSLSL32 = nullptr;
     const bool mixed = su->precision == 1 || su->precision == 2;
     if (mixed) {
@@ -654,14 +666,7 @@ int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t 
     res->energy = lastE;
     res->electrons = lastNe;
     res->c = lastC;
-    CQ_TRY(cqmm_matrix_free(ctx, w.botH));
-    CQ_TRY(cqmm_matrix_free(ctx, w.bottom));
-    CQ_TRY(cqmm_matrix_free(ctx, w.top));
-    CQ_TRY(cqmm_matrix_free(ctx, w.LSLSL));
-    CQ_TRY(cqmm_matrix_free(ctx, w.LSL));
-    CQ_TRY(cqmm_matrix_free(ctx, w.LS));
-    CQ_TRY(cqmm_matrix_free(ctx, rho));
-    CQ_TRY(cqmm_matrix_free(ctx, oldL));
+    CQ_TRY(tmp.release());
     CQ_TRY(cq::hip_check(hipStreamSynchronize(s), "cqmm_mcweeny"));
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return CQMM_OK;
@@ -677,13 +682,14 @@ int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *su, int32_t matS, 
     if (hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T)
       return cq::set_error(CQMM_ERR_ARG, "cqmm_hotelling: handle ranges differ from the setup");
     hipStream_t s = cq::stream();
+    cq::Temps tmp(ctx);
     int32_t matI, matT1, matTold, matA, matTS, matGrad;
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matI));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matT1));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matTold));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matA));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matTS));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_T, &matGrad));
+    CQ_TRY(tmp.alloc(su->range_TS, &matI));
+    CQ_TRY(tmp.alloc(su->range_T, &matT1));
+    CQ_TRY(tmp.alloc(su->range_T, &matTold));
+    CQ_TRY(tmp.alloc(su->range_TS, &matA));
+    CQ_TRY(tmp.alloc(su->range_TS, &matTS));
+    CQ_TRY(tmp.alloc(su->range_T, &matGrad));
     CQ_TRY(cq::op_set_diag(ctx->ranges[su->range_TS].diag, 1.0, dptr(ctx, matI), s));
     // T0 = eps S with eps = 1 / sum S_jk^2 (Iter_Hott_InvS, src/S_matrix_module.f90:846-870).
     double tot = 0.0;
@@ -717,12 +723,7 @@ int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *su, int32_t matS, 
     }
     if (iters) *iters = std::min(n, max_iter);
     if (omega_out) *omega_out = omega;
-    CQ_TRY(cqmm_matrix_free(ctx, matGrad));
-    CQ_TRY(cqmm_matrix_free(ctx, matTS));
-    CQ_TRY(cqmm_matrix_free(ctx, matA));
-    CQ_TRY(cqmm_matrix_free(ctx, matTold));
-    CQ_TRY(cqmm_matrix_free(ctx, matT1));
-    CQ_TRY(cqmm_matrix_free(ctx, matI));
+    CQ_TRY(tmp.release());
     return cq::hip_check(hipStreamSynchronize(s), "cqmm_hotelling");
   });
 }
@@ -735,20 +736,27 @@ int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH,
     if ((st = ctx_check(ctx, matH, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_init_mcweeny")) ||
         (st = ctx_check(ctx, matT, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matL, "cqmm_init_mcweeny")))
       return st;
+    for (int32_t r : {su->range_T, su->range_S, su->range_H, su->range_L, su->range_TS, su->range_TH})
+      if (r < 0 || r >= kMaxRanges || !ctx->ranges[r].set)
+        return cq::set_errorf(CQMM_ERR_ARG, "cqmm_init_mcweeny: range %d not registered", r);
+    if (hrange(ctx, matH) != su->range_H || hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T ||
+        hrange(ctx, matL) != su->range_L)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny: handle ranges differ from the setup");
     const RangeRec &rh = ctx->ranges[su->range_H];
     double n_o = 0.0;
     for (int32_t p = 0; p < rh.n_prim; ++p) n_o += rh.ndimi[p];
     const double ne = su->n_electrons;
     // SX = Tr[T S]
+    cq::Temps tmp(ctx);
     int32_t matTS, matTH, matXHX, matTmp;
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TS, &matTS));
+    CQ_TRY(tmp.alloc(su->range_TS, &matTS));
     CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
     double SX = 0.0;
     CQ_TRY(cqmm_matrix_trace(ctx, matTS, &SX));
     // McWXHX: XHX = T H T at L range, truncated copy at S range, SXHX = Tr[S (THT)_S].
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_TH, &matTH));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_L, &matXHX));
-    CQ_TRY(cqmm_matrix_allocate(ctx, su->range_S, &matTmp));
+    CQ_TRY(tmp.alloc(su->range_TH, &matTH));
+    CQ_TRY(tmp.alloc(su->range_L, &matXHX));
+    CQ_TRY(tmp.alloc(su->range_S, &matTmp));
     CQ_TRY(cqmm_matrix_product(ctx, matT, matH, matTH, su->mult_T_H_TH));
     CQ_TRY(cqmm_matrix_product(ctx, matTH, matT, matXHX, su->mult_TH_T_L));
     CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTmp, 1.0, matXHX));
@@ -800,10 +808,7 @@ int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH,
       out_mu_lambda[0] = mubar;
       out_mu_lambda[1] = lambda;
     }
-    CQ_TRY(cqmm_matrix_free(ctx, matTmp));
-    CQ_TRY(cqmm_matrix_free(ctx, matXHX));
-    CQ_TRY(cqmm_matrix_free(ctx, matTH));
-    CQ_TRY(cqmm_matrix_free(ctx, matTS));
+    CQ_TRY(tmp.release());
     return cq::hip_check(hipStreamSynchronize(cq::stream()), "cqmm_init_mcweeny");
   });
 }

@@ -18,6 +18,7 @@
 #include "../../include/cqmm.h"
 #include "../../include/cqmm_matrix.h"
 #include "cq_common.h"
+#include "cq_temps.h"
 
 namespace {
 
@@ -31,28 +32,7 @@ constexpr double kTwoPi = 6.28318530717958647692;
     if (st_) return st_;       \
   } while (0)
 
-// Temporary matrices, freed in reverse order of allocation (the context
-// enforces free_temp_matrix's LIFO rule); the destructor frees what an early
-// error return left.
-struct Temps {
-  cqmm_ctx *ctx;
-  std::vector<int32_t> h;
-  explicit Temps(cqmm_ctx *c) : ctx(c) {}
-  int alloc(int32_t range, int32_t *out) {
-    const int st = cqmm_matrix_allocate(ctx, range, out);
-    if (!st) h.push_back(*out);
-    return st;
-  }
-  int release() {
-    while (!h.empty()) {
-      const int st = cqmm_matrix_free(ctx, h.back());
-      h.pop_back();
-      if (st) return st;
-    }
-    return CQMM_OK;
-  }
-  ~Temps() { release(); }
-};
+using cq::Temps;
 
 int trace2(cqmm_ctx *ctx, int32_t a, int32_t b, double *out) { return cqmm_matrix_product_trace(ctx, a, b, out); }
 

@@ -49,7 +49,9 @@ typedef struct cqmm_ctx cqmm_ctx;
 
 int cqmm_ctx_create(cqmm_ctx **ctx);
 int cqmm_ctx_destroy(cqmm_ctx *ctx);
-/* Register range `range` (0 <= range < 64) with its index (copied). */
+/* Register range `range` (0 <= range < 64) with its index (copied). Replacing a
+ * registered range is refused (CQMM_ERR_STATE) while a live matrix or a
+ * registered mult uses it; after atoms move, build a new context. */
 int cqmm_ctx_add_range(cqmm_ctx *ctx, int32_t range, const cqmm_matrix_index *idx);
 /* Transpose pairs of a range for symmetrise_L / matrix_transpose
  * (src/mult_module.f90:1856-1893, 2609-2673): 1-based offsets p, q of blocks

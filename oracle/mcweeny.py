@@ -17,6 +17,11 @@ iteration (src/S_matrix_module.f90:746-1034):
   * hlimits_dense_blocks (GetHLimits) — pinned by a two-atom known answer
     worked by hand from the reference loop (non-symmetric blocks, mixed sizes)
     and by the diagonal case (min/max of |H_ii|).
+  * init_mcw_dense (InitMcW, McWRho0) — pinned by hand-worked three-level
+    known answers on both mubar branches (with T = S^-1 and with T != S^-1),
+    the no-mubar case, and, with T = S^-1, Tr[L0 S] = Ne and the S-metric
+    occupancies of L0 being the linear map (lambda (mubar - e) + Ne)/No of the
+    generalised eigenvalues (tests/test_oracle_initmcw.py).
 """
 from __future__ import annotations
 
@@ -175,7 +180,15 @@ def sparsify(M, blocks, nsf, length):
 class Sparse:
     """mult_module operations on host arrays of a cqmm System (oracle products)."""
 
-    def __init__(self, system, rid):
+    def __init__(self, system, rid, ompgemm=False):
+        """ompgemm: products through the ompGemm restatement (OpenMP + numpy's
+        OpenBLAS dgemm), which equals the default kernels' to 1e-13
+        (tests/test_oracle.py); for workloads the single-threaded default
+        restatement would take minutes on."""
+        self.ompgemm = ompgemm
+        if ompgemm:
+            from . import use_numpy_openblas
+            use_numpy_openblas()
         self.sys = system
         self.rid = dict(rid)
         self.blocks = {r: system.blocks(r) for r in set(rid.values())}
@@ -214,7 +227,11 @@ class Sparse:
         key = (mult_type, self.rid[ra], self.rid[rb], self.rid[rc])
         if key not in self._desc:
             self._desc[key] = self.sys.mult_desc(mult_type, *key[1:])
-        oracle_mat_mult(self._desc[key], a, b, c)
+        if self.ompgemm:
+            from . import mat_mult_ompgemm
+            mat_mult_ompgemm(self._desc[key], a, b, c)
+        else:
+            oracle_mat_mult(self._desc[key], a, b, c)
 
     def symmetrise(self, a, name):
         r = self.rid[name]
