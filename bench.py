@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark of the drop-in block-sparse multiply on one MI355X.
+"""Benchmark of the drop-in block-sparse multiply and the purification loops on one MI355X.
 
-Default workload (the one the headline metric is quoted on): plain FP64 GPU
-multiply at DZP scale — 4096 atoms at 0.05 atoms/A^3 (seeded, minimum
-separation 2 A), 13 support functions per atom (13x13 blocks), result range
-16 A = (A range 8 A) . (B range 16 A)^T in CONQUEST's reduction form
-(m_kern_min), entries N(0,1)*exp(-r/2 A), FP64.
+Default run (no --workload): the headline line is the plain FP64 GPU multiply at
+DZP scale — 4096 atoms at 0.05 atoms/A^3 (seeded, minimum separation 2 A), 13
+support functions per atom (13x13 blocks), result range 16 A = (A range 8 A) .
+(B range 16 A)^T in CONQUEST's reduction form (m_kern_min), entries
+N(0,1)*exp(-r/2 A), FP64 — with, in `config`, the other workloads of the metric:
+  * multiply_4x4: 512 atoms, 4x4 blocks, C(20 A) = A(8 A).B(12 A), full product
+    (m_kern_max), FP64, against the HBM roofline (8 (lenA + lenB + lenC) bytes);
+  * multiply_mixed_species: 8192 atoms, 50% 4 / 25% 9 / 25% 13 SF, C(16 A) =
+    A(8 A).B(12 A), FP64 and FP32 accumulation with their relative errors;
+  * purification: McWeeny (Hotelling S^-1 -> InitMcW -> McWMin) of 2048 atoms x
+    13 SF at S/H/L 8/8/12 A, one all-FP64 solve and one mixed-precision solve
+    (FP32 MFMA products with FP64 block sums, FP64 traces/mu/Ne and final steps):
+    solves/s, band-energy error per atom and electron-count error against FP64.
 
-A "step" is one cqmm_mat_mult over device-resident operands (the plan and the
-HBM copies of a, b, c are built before the timed region). value = useful
-GFLOP/s = (sum over products of 2*n_i*n_k*n_j) / wall time per step.
-The roofline object uses the mean kernel time from HIP events on the library
-stream; cpu_baseline times the oracle's restatement of multiply_kernel_ompGemm
-on a bounded sample of halo partitions on the host cores.
+A multiply "step" is one cqmm_mat_mult over device-resident operands (plan and
+HBM copies built before the timed region). value = useful GFLOP/s = (sum over
+products of 2*n_i*n_k*n_j) / wall time per step. Each roofline object uses the
+mean kernel time from HIP events on the library stream; cpu_baseline times the
+oracle's restatement of multiply_kernel_ompGemm on a bounded sample of halo
+partitions on the host cores. --workload cfg5 runs the served request stream.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|cfg1|cfg2|cfg3|cfg4|cfg5]
 """
 import argparse
 import json
@@ -30,16 +38,18 @@ import numpy as np  # noqa: E402
 import cqmm  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 
 WORKLOADS = {
     # name: natoms, nsf, mult_type, (r_a, r_b, r_c) as mat_mult's a, b, c
-    "cfg1": dict(natoms=512, nsf=4, mult_type=1, ra=8.0, rb=12.0, rc=20.0,
+    "cfg1": dict(natoms=512, nsf=4, mult_type=1, ra=8.0, rb=12.0, rc=20.0, bound="hbm",
                  desc="plain FP64, 512 atoms, 4x4 blocks, C(20 A) = A(8 A).B(12 A), full product (m_kern_max)"),
-    "cfg2": dict(natoms=4096, nsf=13, mult_type=2, ra=16.0, rb=16.0, rc=8.0,
+    "cfg2": dict(natoms=4096, nsf=13, mult_type=2, ra=16.0, rb=16.0, rc=8.0, bound="mfma",
                  desc="plain FP64 drop-in at DZP scale, 4096 atoms, 13x13 blocks, "
                       "X(16 A) = A(8 A).B(16 A)^T, C truncated to 16 A (m_kern_min)"),
     "cfg3": dict(natoms=8192, nsf=((4, .5), (9, .25), (13, .25)), mult_type=1, ra=8.0, rb=12.0, rc=16.0,
+                 bound="mfma",
                  desc="batched uneven blocks, 8192 atoms, species 50% 4 / 25% 9 / 25% 13 SF (rectangular "
                       "blocks, one launch), C(16 A) = A(8 A).B(12 A), FP64 and FP32-accumulate"),
 }
@@ -106,106 +116,10 @@ def cpu_baseline(desc, a, b, c, target_s):
     }
 
 
-def purification_bench(args):
-    """cfg4: one structure, FP64 vs mixed solves; cfg5: the served request stream."""
-    from cqmm import serve as sv
-    cqmm.init(0)
-    if args.workload == "cfg4":
-        natoms, nsf = 2048, 13
-        out = {}
-        for prec in (0, 1):
-            params = sv.default_params(precision=prec)
-            runs = []
-            for k in range(max(1, args.steps)):
-                r = sv.purify(natoms, nsf, args.seed, params)
-                if r["status"] != 0:
-                    raise SystemExit(f"purify failed: {r}")
-                runs.append(r)
-                log(f"precision {prec} solve {k}: {r['solve_s']:.2f} s, {r['iterations']} it "
-                    f"({r['fp32_iterations']} FP32), E_band {r['band_energy']:.10f} Ha")
-            out[prec] = runs
-        t64 = np.median([r["solve_s"] for r in out[0]])
-        t32 = np.median([r["solve_s"] for r in out[1]])
-        e64, e32 = out[0][-1], out[1][-1]
-        line = {
-            "metric": "purification solves/s", "value": round(1.0 / t32, 5), "unit": "solves/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": 0, "ms_per_step": round(t32 * 1e3, 2), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-            "data": "synthetic (seeded random periodic structure; S/H recipe of the workload)",
-            "config": {"workload": "mixed-precision McWeeny, 2048 atoms, 13 SF/atom, S/H/L ranges 8/8/12 A",
-                       "fp64_solves_per_s": round(1.0 / t64, 5), "speedup_vs_fp64": round(t64 / t32, 3),
-                       "band_energy_fp64": e64["band_energy"], "band_energy_mixed": e32["band_energy"],
-                       "band_energy_err_per_atom": abs(e32["band_energy"] - e64["band_energy"]) / natoms,
-                       "n_electrons_err": abs(e32["n_electrons"] - e64["n_electrons"]),
-                       "iterations_fp64": e64["iterations"], "iterations_mixed": e32["iterations"],
-                       "fp32_iterations": e32["fp32_iterations"], "setup_s": round(e32["setup_s"], 2)},
-            "roofline": None, "cpu_baseline": None,
-        }
-        print(json.dumps(line), flush=True)
-        return
-    # cfg5: served stream
-    n = args.requests
-    reqs = sv.poisson_stream(n, args.rate, args.seed)
-    lines = {}
-    for prec in (1, 0):
-        params = sv.default_params(precision=prec)
-        log(f"serving {n} requests at {args.rate}/s, precision {prec}, {args.slots} slots")
-        res, wall = sv.serve(reqs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), params=params,
-                             log_progress=True)
-        ok = [r for r in res if r["status"] == 0]
-        lat = np.array([r["latency_s"] for r in ok])
-        lines[prec] = dict(solves_per_s=len(ok) / wall, p50_s=float(np.percentile(lat, 50)),
-                           p99_s=float(np.percentile(lat, 99)), wall_s=wall, failed=n - len(ok),
-                           max_ne_error=max(r["ne_error"] for r in ok),
-                           energies=[r["band_energy"] for r in res])
-        log(f"precision {prec}: {len(ok)}/{n} solved in {wall:.1f} s, p50 {lines[prec]['p50_s']:.2f} s, "
-            f"p99 {lines[prec]['p99_s']:.2f} s")
-    e1 = np.array(lines[1].pop("energies"))
-    e0 = np.array(lines[0].pop("energies"))
-    na = np.array([q["natoms"] for q in reqs])
-    err = float(np.max(np.abs(e1 - e0) / na))
-    m = lines[1]
-    line = {
-        "metric": "purification solves/s", "value": round(m["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
-        "steps": n, "warmup": 0, "ms_per_step": round(1e3 / max(m["solves_per_s"], 1e-9), 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic (seeded Poisson stream; atoms log-uniform [64, 2048]; 4 or 13 SF/atom)",
-        "config": {"workload": f"served stream of {n} purification requests, Poisson {args.rate}/s, "
-                               f"{args.slots} slots, HBM budget {args.hbm_gb} GiB",
-                   "mixed": m, "fp64": lines[0], "max_band_energy_err_per_atom": err,
-                   "speedup_vs_fp64": round(m["solves_per_s"] / lines[0]["solves_per_s"], 3)},
-        "roofline": None, "cpu_baseline": None,
-    }
-    print(json.dumps(line), flush=True)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4", "cfg5"])
-    ap.add_argument("--requests", type=int, default=256)
-    ap.add_argument("--rate", type=float, default=2.0)
-    ap.add_argument("--slots", type=int, default=4)
-    ap.add_argument("--hbm-gb", type=float, default=200.0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="run the streaming copy kernel once over this many MiB before the timed region "
-                         "(reference kernel for scaling HBM traffic counters)")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "triplet", "tile"],
-                    help="FP64 kernel: auto (tile schedule when present), triplet, tile")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)
-        sys.exit(2)
-    if args.workload in ("cfg4", "cfg5"):
-        purification_bench(args)
-        return
-    w = WORKLOADS[args.workload]
+def multiply_workload(name, args, fp32_modes=True, cpu=False):
+    """Build, tune and time one multiply workload; returns a result dict (and the
+    host operands when the CPU baseline is wanted)."""
+    w = WORKLOADS[name]
     t_setup = time.perf_counter()
     xyz, edge = cqmm.random_structure(w["natoms"], 0.05, 2.0, args.seed)
     ra, rb, rc = w["ra"], w["rb"], w["rc"]
@@ -215,7 +129,8 @@ def main():
     ids = {}
     for r in sorted({ra, rb, rc}):
         ids[r] = sysm.add_range(r)
-    log(f"system: {w['natoms']} atoms, box {edge:.2f} A, ranges {sorted(ids)} built in {time.perf_counter() - t_setup:.1f} s")
+    log(f"{name}: {w['natoms']} atoms, box {edge:.2f} A, ranges {sorted(ids)} built in "
+        f"{time.perf_counter() - t_setup:.1f} s")
     desc = sysm.mult_desc(w["mult_type"], ids[ra], ids[rb], ids[rc])
     # Operands as mat_mult receives them: type 1 reads a, b -> c; type 2 reads c, b -> a.
     b = sysm.fill(ids[rb], 0, args.seed, 2, (2.0,))
@@ -225,7 +140,7 @@ def main():
     else:
         c = sysm.fill(ids[rc], 0, args.seed, 3, (2.0,))
         a = np.zeros(sysm.length(ids[ra]))
-    log(f"operands: lena={a.size} lenb={b.size} lenc={c.size} doubles")
+    log(f"{name}: operands lena={a.size} lenb={b.size} lenc={c.size} doubles")
     cqmm.init(0)
     t0 = time.perf_counter()
     plan = cqmm.Plan(desc)
@@ -234,8 +149,8 @@ def main():
     info = plan.info()
     flops = info.flops
     kname = kernel_name(info, w["nsf"])
-    log(f"plan: {info.n_triplets} products, {flops / 1e12:.3f} TFLOP/step, built in {t_plan:.2f} s; "
-        f"kernel {kname}, {info.n_tiles} tiles, {info.n_entries} entries, "
+    log(f"{name}: {info.n_triplets} products, {flops / 1e12:.4f} TFLOP/step, plan built in {t_plan:.2f} s; "
+        f"{info.n_tiles} tiles, {info.n_entries} entries, "
         f"{info.n_loads / max(info.n_triplets, 1):.3f} operand blocks per product")
     da, db, dc = (cqmm.DeviceArray.from_host(x) for x in (a, b, c))
     setup_s = time.perf_counter() - t_setup
@@ -255,29 +170,42 @@ def main():
         kname = kernel_name(info, w["nsf"])
         tune = {"tile_ms": None if ms_tile is None else round(ms_tile, 3), "per_block_ms": round(ms_trip, 3),
                 "chosen": kname}
-        log(f"autotune: tile {ms_tile} ms, per-block {ms_trip:.3f} ms -> {kname}")
+        log(f"{name}: autotune tile {ms_tile} ms, per-block {ms_trip:.3f} ms -> {kname}")
+    flags = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC
     for _ in range(args.warmup):
-        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC)
+        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, flags)
     cqmm.synchronize()
-    log(f"warmup {args.warmup} done; timing {args.steps} steps")
     t0 = time.perf_counter()
     cqmm.timer_start()
     for _ in range(args.steps):
-        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC)
+        plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, flags)
     kern_ms = cqmm.timer_stop()  # waits for the stream
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     kern_avg_ms = kern_ms / args.steps
     value = flops / (ms_step / 1e3) / 1e9
-    achieved = flops / (kern_avg_ms / 1e3) / 1e12
     out = (dc if w["mult_type"] == 1 else da).to_host()
-    log(f"step {ms_step:.3f} ms (kernel {kern_avg_ms:.3f} ms), {value:.1f} GFLOP/s; checksum {float(np.abs(out).sum()):.6e}")
+    log(f"{name}: step {ms_step:.3f} ms (kernel {kern_avg_ms:.3f} ms), {value:.1f} GFLOP/s; "
+        f"checksum {float(np.abs(out).sum()):.6e}")
+    algo_bytes = 8.0 * (a.size + b.size + c.size)
+    if w["bound"] == "hbm":
+        achieved = algo_bytes / (kern_avg_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4)}
+    else:
+        achieved = flops / (kern_avg_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4)}
+    roof.update(traffic=traffic_for(name, kname), kernel=kname, kernel_avg_ms=round(kern_avg_ms, 4),
+                algorithmic_bytes=algo_bytes, tflops_fp64=round(flops / (kern_avg_ms / 1e3) / 1e12, 3))
     # The same products with FP32 MFMA on FP32 copies of the operands (per call:
     # FP64->FP32 operand copies, products, FP32->FP64 result): FP32 accumulation
-    # (CQMM_FP32) and each product's FP32 result summed in FP64 (CQMM_FP32_ACC64,
-    # the mixed-precision McWeeny's kernel); errors against the FP64 result.
+    # (CQMM_FP32), each product's FP32 result summed in FP64 (CQMM_FP32_ACC64), and
+    # a block's products summed in FP32 within a chunk, chunks in FP64
+    # (CQMM_FP32_RUN64); errors against the FP64 result.
     fp32 = {}
-    for key, extra in (("fp32_accumulate", 0), ("fp32_acc64", cqmm.CQMM_FP32_ACC64)):
+    modes = (("fp32_accumulate", 0), ("fp32_acc64", cqmm.CQMM_FP32_ACC64), ("fp32_run64", cqmm.CQMM_FP32_RUN64))
+    for key, extra in modes if fp32_modes else ():
         f32 = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_FP32 | cqmm.CQMM_NO_SYNC | extra
         plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, f32)
         cqmm.synchronize()
@@ -292,47 +220,197 @@ def main():
                      "speedup_vs_fp64": round(ms_step / ms32, 3),
                      "max_rel_err": float(np.abs(d).max() / np.abs(out).max()),
                      "frobenius_rel_err": float(np.linalg.norm(d) / np.linalg.norm(out))}
-        log(f"{key}: {ms32:.3f} ms, {fp32[key]['gflops']:.1f} GFLOP/s, max rel err {fp32[key]['max_rel_err']:.3e}, "
-            f"Frobenius rel err {fp32[key]['frobenius_rel_err']:.3e}")
+        log(f"{name} {key}: {ms32:.3f} ms, {fp32[key]['gflops']:.1f} GFLOP/s, max rel err "
+            f"{fp32[key]['max_rel_err']:.3e}, Frobenius rel err {fp32[key]['frobenius_rel_err']:.3e}")
+    res = {
+        "workload": w["desc"], "natoms": w["natoms"],
+        "nsf": w["nsf"] if isinstance(w["nsf"], int) else "4/9/13 mix 0.5/0.25/0.25",
+        "mult_type": w["mult_type"], "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
+        "flop_per_step": flops, "output_blocks": int(info.n_out_total), "plan_build_s": round(t_plan, 3),
+        "setup_s": round(setup_s, 2), "kernel": kname, "tiles": int(info.n_tiles), "autotune": tune,
+        "ms_per_step": round(ms_step, 4), "gflops": round(value, 3), "roofline": roof,
+        "operand_blocks_per_product": round(info.n_loads / max(info.n_triplets, 1), 4)
+        if info.kernel == cqmm.KERNEL_TILE else 2.0,
+    }
+    if fp32:
+        res["fp32"] = fp32
+    for x in (da, db, dc):
+        x.free()
+    plan.close()
+    host = (desc, a, b, c) if cpu else None
+    return res, host
+
+
+def purification_pair(args, natoms=2048, nsf=13):
+    """One FP64 and one mixed-precision solve of the same structure (the
+    mixed-precision McWeeny workload)."""
+    from cqmm import serve as sv
+    cqmm.init(0)
+    out = {}
+    for prec in (0, 1):
+        r = sv.purify(natoms, nsf, args.seed, sv.default_params(precision=prec))
+        if r["status"] != 0:
+            raise SystemExit(f"purify failed: {r}")
+        out[prec] = r
+        log(f"purification precision {prec}: solve {r['solve_s']:.2f} s (McWeeny {r['mcw_s']:.2f} s), "
+            f"{r['iterations']} it ({r['fp32_iterations']} FP32), E_band {r['band_energy']:.10f} Ha, "
+            f"Ne error {r['ne_error']:.2e}")
+    e64, e32 = out[0], out[1]
+    peak32 = PEAK_FP32_TFLOPS
+    res = {
+        "workload": f"mixed-precision McWeeny, {natoms} atoms x {nsf} SF, S/H/L ranges 8/8/12 A (Hotelling -> "
+                    "InitMcW -> McWMin; S = I + 0.1 N(0,1) exp(-r/1.5), H onsite U(-0.5,0.5) split -/+1.5 Ha, "
+                    "off-diagonal -0.2 N(0,1) exp(-r/2), + 3 S)",
+        "fp64_solves_per_s": round(1.0 / e64["solve_s"], 5), "mixed_solves_per_s": round(1.0 / e32["solve_s"], 5),
+        "speedup_vs_fp64": round(e64["solve_s"] / e32["solve_s"], 3),
+        "band_energy_fp64": e64["band_energy"], "band_energy_mixed": e32["band_energy"],
+        "band_energy_err_per_atom": abs(e32["band_energy"] - e64["band_energy"]) / natoms,
+        "n_electrons_err_vs_fp64": abs(e32["n_electrons"] - e64["n_electrons"]),
+        "ne_error_fp64": e64["ne_error"], "ne_error_mixed": e32["ne_error"],
+        "iterations_fp64": e64["iterations"], "iterations_mixed": e32["iterations"],
+        "fp32_iterations": e32["fp32_iterations"], "setup_s": round(e32["setup_s"], 2),
+        "solve_s_fp64": round(e64["solve_s"], 3), "solve_s_mixed": round(e32["solve_s"], 3),
+        "roofline": {
+            "bound": "mfma", "unit": "TFLOP/s",
+            "fp64": {"achieved": round(e64["mcw_flops"] / e64["mcw_s"] / 1e12, 3), "peak": PEAK_FP64_TFLOPS,
+                     "frac": round(e64["mcw_flops"] / e64["mcw_s"] / 1e12 / PEAK_FP64_TFLOPS, 4)},
+            "mixed": {"achieved": round(e32["mcw_flops"] / e32["mcw_s"] / 1e12, 3), "peak": peak32,
+                      "frac": round(e32["mcw_flops"] / e32["mcw_s"] / 1e12 / peak32, 4)},
+            "note": "useful FLOP of McWMin's products / McWMin wall time (FP32 peak for the mixed solve)",
+        },
+    }
+    return res
+
+
+def served_stream(args):
+    """The served request stream: plain FP64 (requests one at a time, no
+    batching) against batched FP64 and batched mixed precision."""
+    from cqmm import serve as sv
+    cqmm.init(0)
+    n = args.requests
+    reqs = sv.poisson_stream(n, args.rate, args.seed)
+    runs = {
+        "plain_fp64": dict(precision=0, batch=0, slots=1),
+        "batched_fp64": dict(precision=0, batch=1, slots=args.slots),
+        "batched_mixed": dict(precision=1, batch=1, slots=args.slots),
+    }
+    only = args.serve_modes.split(",") if args.serve_modes else list(runs)
+    lines = {}
+    energies = {}
+    for key in only:
+        m = runs[key]
+        params = sv.default_params(precision=m["precision"])
+        log(f"serving {n} requests at {args.rate}/s: {key} ({m['slots']} slots, batch {m['batch']})")
+        res, wall = sv.serve(reqs, slots=m["slots"], hbm_budget=int(args.hbm_gb * (1 << 30)), params=params,
+                             log_progress=True, batch=m["batch"], max_batch=args.max_batch)
+        ok = [r for r in res if r["status"] == 0]
+        lat = np.array([r["latency_s"] for r in ok])
+        svc = np.array([r["done_s"] - r["admit_s"] for r in ok])
+        lines[key] = dict(solves_per_s=len(ok) / wall, wall_s=wall, failed=n - len(ok),
+                          p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                          p50_service_s=float(np.percentile(svc, 50)), p99_service_s=float(np.percentile(svc, 99)),
+                          max_ne_error=max(r["ne_error"] for r in ok))
+        energies[key] = np.array([r["band_energy"] for r in res])
+        log(f"{key}: {len(ok)}/{n} solved in {wall:.1f} s ({lines[key]['solves_per_s']:.4f}/s), "
+            f"latency p50 {lines[key]['p50_latency_s']:.2f} s p99 {lines[key]['p99_latency_s']:.2f} s")
+    na = np.array([q["natoms"] for q in reqs])
+    cfg = {"workload": f"served stream of {n} purification requests (atoms log-uniform [64, 2048], 4 or 13 SF, "
+                       f"S/H/L 8/8/12 A), Poisson {args.rate}/s, HBM budget {args.hbm_gb} GiB, "
+                       f"max batch {args.max_batch}",
+           **lines}
+    if "batched_mixed" in energies and "plain_fp64" in energies:
+        cfg["max_band_energy_err_per_atom"] = float(np.max(np.abs(energies["batched_mixed"] -
+                                                                   energies["plain_fp64"]) / na))
+        cfg["speedup_vs_plain_fp64"] = round(lines["batched_mixed"]["solves_per_s"] /
+                                             lines["plain_fp64"]["solves_per_s"], 3)
+    if "batched_fp64" in energies and "plain_fp64" in energies:
+        cfg["batched_fp64_equals_plain"] = bool(np.array_equal(energies["batched_fp64"], energies["plain_fp64"]))
+    head = lines.get("batched_mixed", next(iter(lines.values())))
+    line = {
+        "metric": "purification solves/s", "value": round(head["solves_per_s"], 5), "unit": "solves/s",
+        "n_gpus": 1, "steps": n, "warmup": 0, "ms_per_step": round(1e3 / max(head["solves_per_s"], 1e-9), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded Poisson stream; seeded random periodic structures; S/H recipe of the workload)",
+        "config": cfg, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS) + ["cfg4", "cfg5"])
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=0.05)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--max-batch", type=int, default=16)
+    ap.add_argument("--serve-modes", default="", help="comma list of plain_fp64,batched_fp64,batched_mixed")
+    ap.add_argument("--hbm-gb", type=float, default=240.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-purification", action="store_true", help="default run: skip the purification pair")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run the streaming copy kernel once over this many MiB before the timed region "
+                         "(reference kernel for scaling HBM traffic counters)")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "triplet", "tile"],
+                    help="FP64 kernel: auto (tile schedule when present), triplet, tile")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "cfg5":
+        served_stream(args)
+        return
+    if args.workload == "cfg4":
+        res = purification_pair(args)
+        line = {"metric": "purification solves/s", "value": res["mixed_solves_per_s"], "unit": "solves/s",
+                "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(res["solve_s_mixed"] * 1e3, 2),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+                "data": "synthetic (seeded random periodic structure; S/H recipe of the workload)",
+                "config": res, "roofline": None, "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
+        return
+    head = "cfg2" if args.workload == "all" else args.workload
+    res, host = multiply_workload(head, args, fp32_modes=True, cpu=not args.no_cpu)
+    cpu = None
+    if host is not None:
+        desc, a, b, c = host
+        cpu = cpu_baseline(desc, a.copy() if WORKLOADS[head]["mult_type"] == 2 else a, b,
+                           c if WORKLOADS[head]["mult_type"] == 2 else c.copy(), args.cpu_seconds)
+        del desc, a, b, c, host
+    cfg = dict(res)
+    roof = cfg.pop("roofline")
+    cfg.pop("gflops")
+    cfg.pop("ms_per_step")
+    if args.workload == "all":
+        sub1, _ = multiply_workload("cfg1", args, fp32_modes=False)
+        sub3, _ = multiply_workload("cfg3", args, fp32_modes=True)
+        cfg["multiply_4x4"] = sub1
+        cfg["multiply_mixed_species"] = sub3
+        if not args.no_purification:
+            cfg["purification"] = purification_pair(args)
     line = {
         "metric": "block-sparse mult GFLOP/s",
-        "value": round(value, 3),
+        "value": res["gflops"],
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random periodic structure; N(0,1)*exp(-r/2) block entries)",
-        "config": {
-            "workload": w["desc"],
-            "natoms": w["natoms"], "nsf": w["nsf"] if isinstance(w["nsf"], int) else "4/9/13 mix 0.5/0.25/0.25",
-            "mult_type": w["mult_type"],
-            "ranges_a_b_c": [ra, rb, rc], "products": int(info.n_triplets),
-            "flop_per_step": flops, "output_blocks": int(info.n_out_total),
-            "plan_build_s": round(t_plan, 3), "setup_s": round(setup_s, 2),
-            "kernel": kname, "tiles": int(info.n_tiles), "autotune": tune, "fp32": fp32,
-            "operand_blocks_per_product": round(info.n_loads / max(info.n_triplets, 1), 4)
-            if info.kernel == cqmm.KERNEL_TILE else 2.0,
-        },
-        "roofline": {
-            "bound": "mfma",
-            "achieved": round(achieved, 3),
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": traffic_for(args.workload, kname),
-            "kernel": kname,
-            "kernel_avg_ms": round(kern_avg_ms, 4),
-        },
-        "cpu_baseline": None,
+        "data": "synthetic (seeded random periodic structures; N(0,1)*exp(-r/2) block entries; purification S/H "
+                "recipe)",
+        "config": cfg,
+        "roofline": {k: v for k, v in roof.items() if k not in ("tflops_fp64",)},
+        "cpu_baseline": cpu,
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(desc, a.copy() if w["mult_type"] == 2 else a, b,
-                                            c if w["mult_type"] == 2 else c.copy(), args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

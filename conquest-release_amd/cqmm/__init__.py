@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libcqmm.so")
 CQMM_DEVICE_PTRS = 1
 CQMM_FP32 = 2
 CQMM_FP32_ACC64 = 8
+CQMM_FP32_RUN64 = 16
 CQMM_NO_SYNC = 4
 
 i32p = C.POINTER(C.c_int32)
@@ -160,7 +161,7 @@ EXPORTS = [
     "cqmm_matrix_allocate", "cqmm_matrix_free", "cqmm_matrix_length", "cqmm_matrix_range",
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
     "cqmm_matrix_scale", "cqmm_matrix_product_trace", "cqmm_matrix_trace", "cqmm_matrix_symmetrise",
-    "cqmm_matrix_transpose", "cqmm_mcweeny", "cqmm_init_mcweeny", "cqmm_hotelling",
+    "cqmm_matrix_transpose", "cqmm_mcweeny", "cqmm_mcweeny_batch", "cqmm_init_mcweeny", "cqmm_hotelling",
     "cqmm_lnv", "cqmm_correct_electrons", "cqmm_dmm",
     "cqmm_purify_default_params", "cqmm_purify_hbm_estimate", "cqmm_purify", "cqmm_serve",
 ]
@@ -240,6 +241,8 @@ def lib():
     L.cqmm_matrix_trace.argtypes = [vp, C.c_int32, f64p]
     L.cqmm_matrix_symmetrise.argtypes = [vp, C.c_int32]
     L.cqmm_mcweeny.argtypes = [vp, C.POINTER(McwSetup), C.c_int32, C.c_int32, C.c_int32, C.POINTER(McwResult)]
+    L.cqmm_mcweeny_batch.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(McwSetup), i32p, i32p, i32p, C.c_uint32,
+                                     C.POINTER(McwResult), i32p]
     L.cqmm_init_mcweeny.argtypes = [vp, C.POINTER(InitMcwSetup), C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]
     L.cqmm_hotelling.argtypes = [vp, C.POINTER(HotellingSetup), C.c_int32, C.c_int32, i32p, f64p]
     _lib = L

@@ -118,6 +118,30 @@ class Context:
         check(lib().cqmm_matrix_transpose(self.h, A, B), "cqmm_matrix_transpose")
 
 
+def mcw_result_dict(res):
+    return dict(iterations=res.iterations, fp32_iterations=res.fp32_iterations, stop_reason=res.stop_reason,
+                restored=res.restored, energy=res.energy, electrons=res.electrons, c=res.c,
+                seconds=res.seconds, energy_trace=list(res.energy_trace[:min(res.iterations, 256)]))
+
+
+BATCH_NO_GRAPH, BATCH_NO_GROUPING = 1, 2
+
+
+def mcweeny_batch(purifiers, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6, flags=0):
+    """cqmm_mcweeny_batch over several Purifiers (one context each), in lockstep.
+    Returns a list of (status, result dict)."""
+    n = len(purifiers)
+    ctxs = (C.c_void_p * n)(*[p.ctx.h.value for p in purifiers])
+    setups = (McwSetup * n)(*[p.mcw_setup(precision, max_iter, n_final_fp64, fp32_tol) for p in purifiers])
+    H = (C.c_int32 * n)(*[p.matH for p in purifiers])
+    S = (C.c_int32 * n)(*[p.matS for p in purifiers])
+    Lm = (C.c_int32 * n)(*[p.matL for p in purifiers])
+    res = (McwResult * n)()
+    status = (C.c_int32 * n)()
+    check(lib().cqmm_mcweeny_batch(n, ctxs, setups, H, S, Lm, flags, res, status), "cqmm_mcweeny_batch")
+    return [(status[i], mcw_result_dict(res[i])) for i in range(n)]
+
+
 class Purifier:
     """McWeeny purification workload on one System.
 
@@ -231,15 +255,17 @@ class Purifier:
               "cqmm_init_mcweeny")
         return ml[0], ml[1]
 
+    def mcw_setup(self, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6):
+        return McwSetup(self.rid["L"], self.rid["S"], self.rid["H"], self.rid["LS"], self.rid["LSL"], L_S_LS,
+                        LS_L_LSL, LSL_SL_L, max_iter, precision, n_final_fp64, fp32_tol)
+
     def mcweeny(self, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6):
         su = McwSetup(self.rid["L"], self.rid["S"], self.rid["H"], self.rid["LS"], self.rid["LSL"], L_S_LS, LS_L_LSL,
                       LSL_SL_L, max_iter, precision, n_final_fp64, fp32_tol)
         res = McwResult()
         check(lib().cqmm_mcweeny(self.ctx.h, C.byref(su), self.matH, self.matS, self.matL, C.byref(res)),
               "cqmm_mcweeny")
-        return dict(iterations=res.iterations, fp32_iterations=res.fp32_iterations, stop_reason=res.stop_reason,
-                    restored=res.restored, energy=res.energy, electrons=res.electrons, c=res.c,
-                    seconds=res.seconds, energy_trace=list(res.energy_trace[:min(res.iterations, 256)]))
+        return mcw_result_dict(res)
 
     # -- LNV / DMM (src/mult_module.f90:1478-1823, src/DMMinModule.f90) ----
     def lnv_setup(self):

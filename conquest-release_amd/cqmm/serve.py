@@ -20,7 +20,7 @@ class PurifyResult(C.Structure):
                 ("stop_reason", C.c_int32), ("hotelling_iterations", C.c_int32), ("band_energy", C.c_double),
                 ("n_electrons", C.c_double), ("ne_error", C.c_double), ("setup_s", C.c_double),
                 ("solve_s", C.c_double), ("hbm_estimate", C.c_int64), ("natoms", C.c_int32), ("n_orb", C.c_int32),
-                ("hbm_peak", C.c_int64)]
+                ("hbm_peak", C.c_int64), ("mcw_s", C.c_double), ("mcw_flops", C.c_double)]
 
 
 class ServeRequest(C.Structure):
@@ -28,7 +28,8 @@ class ServeRequest(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("params", PurifyParams), ("log_progress", C.c_int32)]
+    _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("params", PurifyParams), ("log_progress", C.c_int32),
+                ("batch", C.c_int32), ("max_batch", C.c_int32)]
 
 
 class ServeResult(C.Structure):
@@ -88,15 +89,17 @@ def poisson_stream(n, rate, seed, nmin=64, nmax=2048, nsf_choices=(4, 13)):
             for a, b, s, t in zip(natoms, nsf, seeds, arrival)]
 
 
-def serve(requests, slots=4, hbm_budget=200 << 30, params=None, log_progress=False):
-    """Run the request loop; returns (list of per-request dicts, wall seconds)."""
+def serve(requests, slots=4, hbm_budget=200 << 30, params=None, log_progress=False, batch=0, max_batch=16):
+    """Run the request loop; returns (list of per-request dicts, wall seconds).
+    batch 0: each slot solves its request alone; 1: McWeeny steps of all
+    prepared requests batched (grouped launches, hipGraph); 2: as 1 without graphs."""
     init()
     params = params or default_params()
     n = len(requests)
     reqs = (ServeRequest * max(n, 1))()
     for i, q in enumerate(requests):
         reqs[i] = ServeRequest(q["natoms"], q["nsf"], q["seed"], q["arrival_s"])
-    cfg = ServeConfig(slots, int(hbm_budget), params, int(bool(log_progress)))
+    cfg = ServeConfig(slots, int(hbm_budget), params, int(bool(log_progress)), int(batch), int(max_batch))
     out = (ServeResult * max(n, 1))()
     wall = C.c_double()
     check(_bind().cqmm_serve(reqs, n, C.byref(cfg), out, C.byref(wall)), "cqmm_serve")

@@ -19,65 +19,16 @@
 #include "cq_device.h"
 #include "cq_ops.h"
 #include "cq_temps.h"
+#include "cq_ctx.h"
 
-namespace {
-
-using cq::DevBuf;
-
-constexpr int kMaxRanges = 64;
-
-// Host copy of one range's type(matrix) index plus its device maps.
-struct RangeRec {
-  bool set = false;
-  int64_t length = 0;
-  int32_t n_prim = 0;
-  std::vector<int32_t> n_nab, ndimi;           // per primary atom
-  std::vector<int64_t> onsite;                 // 1-based onsite block offset, 0 if none
-  std::vector<int64_t> row_nab, row_off;       // first neighbour / 0-based data offset of each row
-  std::vector<int32_t> i_part, i_seq, ndimj;   // per neighbour
-  cq::DiagMap diag;
-  cq::PairMap pairs;
-  bool has_pairs = false;
-};
-
-struct MultRec {
-  cqmm_plan *plan = nullptr;
-  int32_t ra = 0, rb = 0, rc = 0;
-};
-
-struct MatRec {
-  int32_t range = -1;
-  bool live = false;
-  DevBuf data;
-};
-
-}  // namespace
-
-struct cqmm_ctx {
-  RangeRec ranges[kMaxRanges];
-  std::map<int32_t, MultRec> mults;
-  std::vector<MatRec> mats;                         // handle h -> mats[h-1]
-  std::map<std::pair<int32_t, int32_t>, std::unique_ptr<cq::RunMap>> summaps;
-  std::multimap<size_t, DevBuf> pool;               // freed matrix buffers by size
-  DevBuf red_part, red_out;
-  std::map<std::string, DevBuf> scratch;            // FP32 operand copies
-  ~cqmm_ctx() {
-    for (auto &m : mults) cqmm_plan_destroy(m.second.plan);
-  }
-};
-
-namespace {
+namespace cq {
 
 int ctx_check(const cqmm_ctx *ctx, int32_t h, const char *who) {
-  if (!ctx) return cq::set_errorf(CQMM_ERR_ARG, "%s: null context", who);
+  if (!ctx) return set_errorf(CQMM_ERR_ARG, "%s: null context", who);
   if (h < 1 || h > (int32_t)ctx->mats.size() || !ctx->mats[h - 1].live)
-    return cq::set_errorf(CQMM_ERR_ARG, "%s: invalid matrix handle %d", who, h);
+    return set_errorf(CQMM_ERR_ARG, "%s: invalid matrix handle %d", who, h);
   return CQMM_OK;
 }
-
-double *dptr(cqmm_ctx *ctx, int32_t h) { return ctx->mats[h - 1].data.as<double>(); }
-int32_t hrange(const cqmm_ctx *ctx, int32_t h) { return ctx->mats[h - 1].range; }
-int64_t hlen(const cqmm_ctx *ctx, int32_t h) { return ctx->ranges[hrange(ctx, h)].length; }
 
 int get_scratch(cqmm_ctx *ctx, const std::string &key, size_t bytes, void **p) {
   DevBuf &b = ctx->scratch[key];
@@ -91,7 +42,7 @@ int get_scratch(cqmm_ctx *ctx, const std::string &key, size_t bytes, void **p) {
 
 // addscan's merge (src/multiply_module.f90:849-943) of two neighbour lists sorted
 // by (i_part, i_seq): element runs of the blocks common to both patterns.
-int build_summap(const RangeRec &A, const RangeRec &B, cq::RunMap &m) {
+static int build_summap(const RangeRec &A, const RangeRec &B, cq::RunMap &m) {
   if (A.n_prim != B.n_prim) return cq::set_error(CQMM_ERR_ARG, "matrix_sum: ranges with different primary sets");
   std::vector<int64_t> dst, src;
   std::vector<int32_t> len;
@@ -153,7 +104,7 @@ int summap(cqmm_ctx *ctx, int32_t ra, int32_t rb, const cq::RunMap **out) {
   return CQMM_OK;
 }
 
-int reduce_result(cqmm_ctx *ctx, double *result) {
+static int reduce_result(cqmm_ctx *ctx, double *result) {
   hipStream_t s = cq::stream();
   int st = cq::hip_check(hipMemcpyAsync(result, ctx->red_out.p, 8, hipMemcpyDeviceToHost, s), "D2H reduction");
   if (st) return st;
@@ -167,14 +118,10 @@ int ensure_red(cqmm_ctx *ctx) {
   return CQMM_OK;
 }
 
-// FP32 product over the plan of `mult_id` (operands as matrix_product passes them).
-int product_f32(cqmm_ctx *ctx, int32_t mult_id, const float *A, const float *B, float *C, bool acc64) {
-  auto it = ctx->mults.find(mult_id);
-  if (it == ctx->mults.end()) return cq::set_errorf(CQMM_ERR_ARG, "matrix_product: mult %d not registered", mult_id);
-  return cq::launch_block_products_f32(it->second.plan->tp, A, B, C, acc64, cq::stream());
-}
 
-}  // namespace
+}  // namespace cq
+
+using namespace cq;
 
 extern "C" {
 
@@ -515,163 +462,9 @@ namespace {
     if (st_) return st_;     \
   } while (0)
 
-// One McWeeny step (McW_matrix_multiply_nonds, src/McWeeny.f90:620-740).
-// The products run in FP32 when `fp32`; traces, c, rho' and E stay FP64.
-struct McwWork {
-  int32_t LS, LSL, LSLSL, top, bottom, botH;
-  float *L32, *S32, *LS32, *LSL32, *LSLSL32;
-};
-
-int mcw_step(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t matS, int32_t matL, int32_t matRho,
-             const McwWork &w, bool fp32, double *energy, double *cn_out, double *ne_out) {
-  hipStream_t s = cq::stream();
-  if (fp32) {
-    CQ_TRY(cq::op_to_f32(hlen(ctx, matL), dptr(ctx, matL), w.L32, s));
-    const bool acc64 = su->precision == 1;
-    CQ_TRY(product_f32(ctx, su->mult_L_S_LS, w.L32, w.S32, w.LS32, acc64));
-    CQ_TRY(product_f32(ctx, su->mult_LS_L_LSL, w.LS32, w.L32, w.LSL32, acc64));
-    // Type 2: matrix_product(LSL, LS, LSLSL) -> mat_mult(LSLSL, LS, LSL).
-    CQ_TRY(product_f32(ctx, su->mult_LSL_SL_L, w.LSL32, w.LS32, w.LSLSL32, acc64));
-    CQ_TRY(cq::op_to_f64(hlen(ctx, w.LSL), w.LSL32, dptr(ctx, w.LSL), s));
-    CQ_TRY(cq::op_to_f64(hlen(ctx, w.LSLSL), w.LSLSL32, dptr(ctx, w.LSLSL), s));
-  } else {
-    CQ_TRY(cqmm_matrix_product(ctx, matL, matS, w.LS, su->mult_L_S_LS));
-    CQ_TRY(cqmm_matrix_product(ctx, w.LS, matL, w.LSL, su->mult_LS_L_LSL));
-    CQ_TRY(cqmm_matrix_product(ctx, w.LSL, w.LS, w.LSLSL, su->mult_LSL_SL_L));
-  }
-  double c1 = 0, c2 = 0;
-  CQ_TRY(cqmm_matrix_scale(ctx, 0.0, w.top));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, w.top, 1.0, w.LSL));
-  CQ_TRY(cqmm_matrix_sum(ctx, 1.0, w.top, -1.0, w.LSLSL));
-  CQ_TRY(cqmm_matrix_product_trace(ctx, matS, w.top, &c1));
-  CQ_TRY(cqmm_matrix_scale(ctx, 0.0, w.bottom));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, w.bottom, 1.0, matL));
-  CQ_TRY(cqmm_matrix_sum(ctx, 1.0, w.bottom, -1.0, w.LSL));
-  CQ_TRY(cqmm_matrix_product_trace(ctx, matS, w.bottom, &c2));
-  if (!(std::fabs(c2) > 1e-15)) return cq::set_errorf(CQMM_ERR_NOTCONV, "McWeeny: c2 is near zero (%g)", c2);
-  const double cn = c1 / c2;
-  // rho' = ((1+c)LSL - LSLSL + (1-2c)L)/(1-c) for c < 1/2, else ((1+c)LSL - LSLSL)/c.
-  CQ_TRY(cqmm_matrix_scale(ctx, 0.0, matRho));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matRho, -1.0, w.LSLSL));
-  CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matRho, 1.0 + cn, w.LSL));
-  if (cn < 0.5) {
-    CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matRho, 1.0 - 2.0 * cn, matL));
-    CQ_TRY(cqmm_matrix_scale(ctx, 1.0 / (1.0 - cn), matRho));
-  } else {
-    CQ_TRY(cqmm_matrix_scale(ctx, 1.0 / cn, matRho));
-  }
-  CQ_TRY(cqmm_matrix_scale(ctx, 0.0, w.botH));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, w.botH, 1.0, matRho));
-  CQ_TRY(cqmm_matrix_product_trace(ctx, w.botH, matH, energy));
-  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, w.top, 1.0, matRho));
-  CQ_TRY(cqmm_matrix_product_trace(ctx, w.top, matS, ne_out));
-  *cn_out = (std::fabs(c2) < 1e-6) ? -0.01 : cn;
-  return CQMM_OK;
-}
-
 }  // namespace
 
 extern "C" {
-
-int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t matS, int32_t matL,
-                 cqmm_mcw_result *res) {
-  return cq::guarded(__func__, [&]() -> int {
-    if (!ctx || !su || !res) return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: null argument");
-    int st;
-    if ((st = ctx_check(ctx, matH, "cqmm_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_mcweeny")) ||
-        (st = ctx_check(ctx, matL, "cqmm_mcweeny")))
-      return st;
-    if (hrange(ctx, matL) != su->range_L || hrange(ctx, matS) != su->range_S || hrange(ctx, matH) != su->range_H)
-      return cq::set_error(CQMM_ERR_ARG, "cqmm_mcweeny: handle ranges differ from the setup");
-    std::memset(res, 0, sizeof(*res));
-    const auto t0 = std::chrono::steady_clock::now();
-    hipStream_t s = cq::stream();
-    // Temporaries, freed in reverse order on every return.
-    cq::Temps tmp(ctx);
-    int32_t oldL, rho;
-    McwWork w;
-    CQ_TRY(tmp.alloc(su->range_L, &oldL));
-    CQ_TRY(tmp.alloc(su->range_L, &rho));
-    CQ_TRY(tmp.alloc(su->range_LS, &w.LS));
-    CQ_TRY(tmp.alloc(su->range_LSL, &w.LSL));
-    CQ_TRY(tmp.alloc(su->range_L, &w.LSLSL));
-    CQ_TRY(tmp.alloc(su->range_S, &w.top));
-    CQ_TRY(tmp.alloc(su->range_S, &w.bottom));
-    CQ_TRY(tmp.alloc(su->range_H, &w.botH));
-    w.L32 = w.S32 = w.LS32 = w.LSL32 = w.LSLSL32 = nullptr;
-    const bool mixed = su->precision == 1 || su->precision == 2;
-    if (mixed) {
-      void *p;
-      CQ_TRY(get_scratch(ctx, "L32", hlen(ctx, matL) * 4, &p)); w.L32 = (float *)p;
-      CQ_TRY(get_scratch(ctx, "S32", hlen(ctx, matS) * 4, &p)); w.S32 = (float *)p;
-      CQ_TRY(get_scratch(ctx, "LS32", hlen(ctx, w.LS) * 4, &p)); w.LS32 = (float *)p;
-      CQ_TRY(get_scratch(ctx, "LSL32", hlen(ctx, w.LSL) * 4, &p)); w.LSL32 = (float *)p;
-      CQ_TRY(get_scratch(ctx, "LSLSL32", hlen(ctx, w.LSLSL) * 4, &p)); w.LSLSL32 = (float *)p;
-      CQ_TRY(cq::op_to_f32(hlen(ctx, matS), dptr(ctx, matS), w.S32, s));
-    }
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, oldL, 1.0, matL));
-    double oldE = 1.0e30, omega = 0.0, cn = 0.0, ne = 0.0, lastE = 0.0, lastNe = 0.0, lastC = 0.0;
-    bool fp32_phase = mixed;
-    int fp64_done = 0, n = 0;
-    int stop = 4;
-    const int max_iter = su->max_iter > 0 ? su->max_iter : 50;
-    const int nfinal = su->n_final_fp64 > 0 ? su->n_final_fp64 : 2;
-    for (n = 1; n <= max_iter; ++n) {
-      CQ_TRY(mcw_step(ctx, su, matH, matS, matL, rho, w, fp32_phase, &omega, &cn, &ne));
-      if (n - 1 < CQMM_MCW_TRACE) res->energy_trace[n - 1] = omega;
-      if (fp32_phase) ++res->fp32_iterations;
-      else ++fp64_done;
-      // McWMin's stop test (src/McWeeny.f90:166-173).
-      int why = 0;
-      if (omega > oldE) why = 1;
-      else if (std::fabs(omega - oldE) < 1.0e-8) why = 2;
-      else if (cn > 1.0 || cn < 0.0) why = 3;
-      if (fp32_phase) {
-        // FP32 phase: also end when the energy change reaches FP32's resolution.
-        const bool flat = std::fabs(omega - oldE) < su->fp32_tol * std::max(1.0, std::fabs(omega));
-        if (why || flat) {
-          fp32_phase = false;
-          if (why == 1 || why == 3) {
-            // Reject the step; refine from the last accepted L in FP64.
-            oldE = 1.0e30;
-            continue;
-          }
-          // Accept the step, then continue in FP64 with a fresh energy reference.
-          why = 0;
-          CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, rho));
-          lastE = omega; lastNe = ne; lastC = cn;
-          CQ_TRY(cqmm_matrix_symmetrise(ctx, matL));
-          oldE = 1.0e30;
-          continue;
-        }
-      } else if (why && fp64_done <= nfinal && mixed && why != 3) {
-        // The first FP64 refinements may see FP32-level energy noise: keep them.
-        why = 0;
-      }
-      if (why) {
-        stop = why;
-        break;
-      }
-      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, rho));
-      oldE = omega;
-      lastE = omega; lastNe = ne; lastC = cn;
-      CQ_TRY(cqmm_matrix_symmetrise(ctx, matL));
-    }
-    res->iterations = std::min(n, max_iter);
-    res->stop_reason = stop;
-    if (n < 3) {
-      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, 1.0, oldL));
-      res->restored = 1;
-    }
-    res->energy = lastE;
-    res->electrons = lastNe;
-    res->c = lastC;
-    CQ_TRY(tmp.release());
-    CQ_TRY(cq::hip_check(hipStreamSynchronize(s), "cqmm_mcweeny"));
-    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    return CQMM_OK;
-  });
-}
 
 int cqmm_hotelling(cqmm_ctx *ctx, const cqmm_hotelling_setup *su, int32_t matS, int32_t matT, int32_t *iters,
                    double *omega_out) {

@@ -92,6 +92,37 @@ struct TilePlan {
   DevBuf wave_dim;     // uint8 [n_tiles*4*16] (rows - 1) | (columns - 1) << 4 of that output block
 };
 
+// Arguments of one tile-kernel launch over one product's schedule (the fields
+// a kernel family does not use stay null). tile_base: first tile of this
+// product in a grouped launch.
+struct TileArgs {
+  const int32_t *tile_chunk;
+  const int64_t *chunk_lbeg;
+  const int32_t *list;          // uniform sizes: operand block indices
+  const uint64_t *list64;       // mixed sizes: operand element offsets
+  const uint32_t *listls;       // mixed sizes: LDS offsets and sizes
+  const int64_t *pstart;
+  const uint64_t *pcnt;
+  const uint32_t *prod;
+  const int32_t *wave_out;      // uniform sizes
+  const int64_t *wave_out64;    // mixed sizes
+  const uint8_t *wave_dim;
+  const int32_t *chunk_entry;   // dense-quadrant kernels
+  const uint32_t *emask;
+  const int32_t *ecoff;
+  const uint8_t *rdim, *cdim;
+  const void *P, *Q;
+  void *O;
+  int32_t n_tiles, tile_base;
+};
+// Up to kMaxGroup products in one grouped launch, passed by value (kernel
+// argument memory, 2.7 KB).
+constexpr int kMaxGroup = 16;
+struct TileGroup {
+  TileArgs a[kMaxGroup];
+  int32_t ngrp, total;
+};
+
 // Triplet list of one product, grouped by output block (segment), each group in
 // the reference's summation order. Offsets are 0-based element offsets.
 struct TripletPlan {
@@ -137,15 +168,33 @@ std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part
 int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s);
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
                          hipStream_t s);
-int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, bool acc64,
+// FP32 precision modes of the product kernels (operands FP32, FP32 MFMA):
+// 1 accumulates each output block in FP32; 2 adds each product's FP32 result
+// into FP64 registers; 3 sums a block's products within one chunk of the tile
+// schedule in FP32 (at most 15 products, consecutive in the block's list) and
+// adds that partial sum into FP64 (the per-output-block kernels treat 3 as 2).
+int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, int mode,
                              hipStream_t s);
 
+// Grouped launches: the tile schedules of up to kMaxGroup products whose
+// tile_group_key agrees run in one launch (a[j].tile_base = product j's first
+// tile in the group's numbering, total = all tiles). Each product's tiles are computed exactly as in its own
+// launch (same summation order). mode: 0 FP64, else an FP32 mode (1, 2, 3).
+TileArgs tile_launch_args(const TilePlan &tp, const void *P, const void *Q, void *O);
+int tile_group_key(const TilePlan &tp, bool q_trans, int mode);
+int launch_tile_group(const TilePlan &tp, bool q_trans, int mode, const TileGroup &g, hipStream_t s);
+// One product's tile launch from its TileArgs (the launch cqmm_mat_mult makes).
+int launch_tile_args(const TilePlan &tp, bool q_trans, int mode, const TileArgs &a, hipStream_t s);
+
+// True when a plan's products run through its tile schedule (else the
+// per-output-block kernels).
+bool uses_tiles(const TripletPlan &tp);
 // Launch the FP64 (or FP32) block-product kernel over a plan.
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O,
                           bool fp32, hipStream_t stream);
-// Same products on FP32 copies of the operands with FP32 MFMA; acc64 sums the
-// per-product results of each block in FP64 registers, else in the FP32 accumulator.
-int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, bool acc64,
+// Same products on FP32 copies of the operands with FP32 MFMA in precision mode
+// `mode` (1, 2 or 3, as for launch_tile_products_f32).
+int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, int mode,
                               hipStream_t stream);
 
 }  // namespace cq

@@ -353,11 +353,17 @@ int launch_f32_t(const cq::TripletPlan &tp, const float *P, const float *Q, floa
 
 namespace cq {
 
-int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, bool acc64,
+bool uses_tiles(const TripletPlan &tp) {
+  return tp.tiles.n_tiles > 0 && tp.kernel != CQMM_KERNEL_TRIPLET && !force_general();
+}
+
+int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, int mode,
                               hipStream_t s) {
   if (tp.n_seg == 0) return CQMM_OK;
-  if (tp.tiles.n_tiles > 0 && tp.kernel != CQMM_KERNEL_TRIPLET && !force_general())
-    return launch_tile_products_f32(tp.tiles, tp.q_trans, P, Q, O, acc64, s);
+  if (mode < 1 || mode > 3) return set_errorf(CQMM_ERR_ARG, "block products: FP32 mode %d", mode);
+  if (uses_tiles(tp))
+    return launch_tile_products_f32(tp.tiles, tp.q_trans, P, Q, O, mode, s);
+  const bool acc64 = mode != 1;
   if (acc64) {
     if (tp.off64) return tp.q_trans ? launch_f32_t<int64_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int64_t, false, true>(tp, P, Q, O, s);
     return tp.q_trans ? launch_f32_t<int32_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int32_t, false, true>(tp, P, Q, O, s);
@@ -374,7 +380,7 @@ int launch_block_products(const TripletPlan &tp, const double *P, const double *
     // Other uniform sizes run through the general instance with per-block sizes.
     return set_errorf(CQMM_ERR_ARG, "block products: uniform block size %d unsupported", tp.uniform);
   }
-  if (tp.tiles.n_tiles > 0 && tp.kernel != CQMM_KERNEL_TRIPLET && !force_general())
+  if (uses_tiles(tp))
     return launch_tile_products(tp.tiles, tp.q_trans, P, Q, O, s);
   if (tp.off64) return tp.q_trans ? launch_t<int64_t, true>(tp, P, Q, O, s) : launch_t<int64_t, false>(tp, P, Q, O, s);
   return tp.q_trans ? launch_t<int32_t, true>(tp, P, Q, O, s) : launch_t<int32_t, false>(tp, P, Q, O, s);

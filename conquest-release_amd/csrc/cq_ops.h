@@ -32,6 +32,13 @@ int op_axpby(int64_t n, double alpha, double *y, double beta, const double *x, h
 int op_scale(int64_t n, double alpha, double *y, hipStream_t s);
 int op_copy(int64_t n, double *y, const double *x, hipStream_t s);
 int op_runs_axpby(const RunMap &m, double alpha, double *y, double beta, const double *x, hipStream_t s);
+// The same with coefficients read on the device where the pointer is non-null
+// (the value argument is used otherwise).
+int op_axpby_dev(int64_t n, double alpha, const double *alpha_p, double *y, double beta, const double *beta_p,
+                 const double *x, hipStream_t s);
+int op_scale_dev(int64_t n, double alpha, const double *alpha_p, double *y, hipStream_t s);
+int op_runs_axpby_dev(const RunMap &m, double alpha, const double *alpha_p, double *y, double beta,
+                      const double *beta_p, const double *x, hipStream_t s);
 int op_dot(int64_t n, const double *x, const double *y, double *d_partials, double *d_out, hipStream_t s);
 int op_trace(const DiagMap &m, const double *a, double *d_partials, double *d_out, hipStream_t s);
 int op_symmetrise(const PairMap &m, double *a, hipStream_t s);

@@ -22,13 +22,20 @@ inline unsigned grid_for(int64_t n, int per_thread = 1) {
   return (unsigned)g;
 }
 
+// A coefficient given by value, or (pointer non-null) read from device memory
+// where an earlier kernel on the stream computed it (no host round trip).
+__device__ inline double coef(double v, const double *p) { return p ? *p : v; }
+
 // y = alpha*y + beta*x (alpha == 0 stores beta*x without reading y).
-__global__ void k_axpby(int64_t n, double alpha, double *__restrict__ y, double beta, const double *__restrict__ x) {
+__global__ void k_axpby(int64_t n, double av, const double *__restrict__ ap, double *__restrict__ y, double bv,
+                        const double *__restrict__ bp, const double *__restrict__ x) {
+  const double alpha = coef(av, ap), beta = coef(bv, bp);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
     y[t] = alpha == 0.0 ? beta * x[t] : alpha * y[t] + beta * x[t];
 }
 
-__global__ void k_scale(int64_t n, double alpha, double *__restrict__ y) {
+__global__ void k_scale(int64_t n, double av, const double *__restrict__ ap, double *__restrict__ y) {
+  const double alpha = coef(av, ap);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
     y[t] = alpha == 0.0 ? 0.0 : alpha * y[t];
 }
@@ -40,8 +47,10 @@ __global__ void k_copy(int64_t n, double *__restrict__ y, const double *__restri
 
 // Range-mapped sum: runs of matched elements (dst_off, src_off, len).
 __global__ void k_runs_axpby(int64_t nruns, const int64_t *__restrict__ dofs, const int64_t *__restrict__ sofs,
-                             const int32_t *__restrict__ lens, double alpha, double *__restrict__ y, double beta,
+                             const int32_t *__restrict__ lens, double av, const double *__restrict__ ap,
+                             double *__restrict__ y, double bv, const double *__restrict__ bp,
                              const double *__restrict__ x) {
+  const double alpha = coef(av, ap), beta = coef(bv, bp);
   const int lane = threadIdx.x & 63;
   for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < nruns;
        r += ((int64_t)gridDim.x * blockDim.x) >> 6) {
@@ -164,14 +173,21 @@ __global__ void k_to_f64(int64_t n, const float *__restrict__ x, double *__restr
 namespace cq {
 
 int op_axpby(int64_t n, double alpha, double *y, double beta, const double *x, hipStream_t s) {
+  return op_axpby_dev(n, alpha, nullptr, y, beta, nullptr, x, s);
+}
+
+int op_axpby_dev(int64_t n, double alpha, const double *alpha_p, double *y, double beta, const double *beta_p,
+                 const double *x, hipStream_t s) {
   if (n <= 0) return CQMM_OK;
-  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, alpha, y, beta, x);
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, alpha, alpha_p, y, beta, beta_p, x);
   return hip_check(hipGetLastError(), "axpby launch");
 }
 
-int op_scale(int64_t n, double alpha, double *y, hipStream_t s) {
+int op_scale(int64_t n, double alpha, double *y, hipStream_t s) { return op_scale_dev(n, alpha, nullptr, y, s); }
+
+int op_scale_dev(int64_t n, double alpha, const double *alpha_p, double *y, hipStream_t s) {
   if (n <= 0) return CQMM_OK;
-  hipLaunchKernelGGL(k_scale, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, alpha, y);
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, alpha, alpha_p, y);
   return hip_check(hipGetLastError(), "scale launch");
 }
 
@@ -182,10 +198,15 @@ int op_copy(int64_t n, double *y, const double *x, hipStream_t s) {
 }
 
 int op_runs_axpby(const RunMap &m, double alpha, double *y, double beta, const double *x, hipStream_t s) {
+  return op_runs_axpby_dev(m, alpha, nullptr, y, beta, nullptr, x, s);
+}
+
+int op_runs_axpby_dev(const RunMap &m, double alpha, const double *alpha_p, double *y, double beta,
+                      const double *beta_p, const double *x, hipStream_t s) {
   if (m.nruns <= 0) return CQMM_OK;
   const unsigned g = grid_for(m.nruns * 64, 1);
   hipLaunchKernelGGL(k_runs_axpby, dim3(g), dim3(kBS), 0, s, m.nruns, m.dst.as<int64_t>(), m.src.as<int64_t>(),
-                     m.len.as<int32_t>(), alpha, y, beta, x);
+                     m.len.as<int32_t>(), alpha, alpha_p, y, beta, beta_p, x);
   return hip_check(hipGetLastError(), "mapped sum launch");
 }
 

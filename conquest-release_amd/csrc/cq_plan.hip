@@ -1028,8 +1028,9 @@ int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b, int
       if (plan->fo.bytes < (size_t)tp.len_out * 4 && (st = plan->fo.alloc(tp.len_out * 4))) return st;
       if ((st = cq::op_to_f32(tp.len_p, dP, plan->fp.as<float>(), s))) return st;
       if ((st = cq::op_to_f32(tp.len_q, db, plan->fq.as<float>(), s))) return st;
+      const int mode = (flags & CQMM_FP32_RUN64) ? 3 : (flags & CQMM_FP32_ACC64) ? 2 : 1;
       if ((st = cq::launch_block_products_f32(tp, plan->fp.as<float>(), plan->fq.as<float>(), plan->fo.as<float>(),
-                                              (flags & CQMM_FP32_ACC64) != 0, s)))
+                                              mode, s)))
         return st;
       st = cq::op_to_f64(tp.len_out, plan->fo.as<float>(), dO, s);
     }

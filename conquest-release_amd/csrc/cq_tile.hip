@@ -131,6 +131,37 @@ __device__ inline void issue_stage(int32_t lv, int32_t nb, const T *__restrict__
   }
 }
 
+using cq::TileArgs;
+using cq::TileGroup;
+using cq::kMaxGroup;
+
+// Tile of this workgroup, XCD-aware: workgroup b runs on XCD b % 8 and takes
+// tile (b % 8) * ceil(T/8) + b / 8, so neighbouring tiles (same rows) share an
+// L2. Returns -1 past the last tile.
+__device__ inline int32_t xcd_tile(int32_t n) {
+  const int32_t per = (n + 7) >> 3;
+  const int32_t t = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  return t < n ? t : -1;
+}
+
+__device__ inline int32_t tile_select(const TileArgs &arg, TileArgs &a) {
+  a = arg;
+  return xcd_tile(a.n_tiles);
+}
+
+// A grouped launch numbers the tiles of its products consecutively (product j
+// from g.a[j].tile_base); the arguments (kernel-argument memory, read with
+// scalar loads) are those of the product the tile belongs to.
+__device__ inline int32_t tile_select(const TileGroup &g, TileArgs &a) {
+  const int32_t t = xcd_tile(g.total);
+  if (t < 0) return -1;
+  int32_t j = 0;
+#pragma unroll
+  for (int k = 1; k < kMaxGroup; ++k) j += (k < g.ngrp && g.a[k].tile_base <= t) ? 1 : 0;
+  a = g.a[j];
+  return t - a.tile_base;
+}
+
 __device__ inline int64_t readlane64(int64_t v, int k) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), k);
@@ -141,28 +172,34 @@ __device__ inline int64_t readlane64(int64_t v, int k) {
 // MODE 1: FP32 operands, v_mfma_f32_16x16x4, FP32 accumulation.
 // MODE 2: FP32 operands and MFMA, each product's result added into FP64
 //         registers (accumulator D[4(l/16)+e][l%16]).
+// MODE 3: FP32 operands and MFMA; the products of one output block within a
+//         chunk (consecutive in the wavefront's list, at most 15) chain in one
+//         FP32 accumulator, which is then added into the block's FP64 registers.
 // Pipeline per chunk c (one barrier): wait for chunk c's copies and the
 // prefetched registers, issue chunk c+1's copies into the other stage and the
 // register loads of chunk c+1's product list and chunk c+2's operand list,
 // then compute chunk c from LDS.
-template <int MODE, bool QT, int NSF>
+template <int MODE, bool QT, int NSF, class ARG>
 __global__ void __launch_bounds__(kTileThreads) k_tile_products(
-    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
-    const int32_t *__restrict__ list, const int64_t *__restrict__ pstart, const uint64_t *__restrict__ pcnt,
-    const uint32_t *__restrict__ prod, const int32_t *__restrict__ wave_out,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
-    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+    const ARG arg) {
   using T = typename std::conditional<MODE == 0, double, float>::type;
   using C = TileCfg<NSF>;
   constexpr int STAGE = C::CAP * C::NS2 + kLdsPad * (int)(8 / sizeof(T));
   __shared__ T stage0[STAGE];
   __shared__ T stage1[STAGE];
-  // XCD-aware order: workgroup b runs on XCD b % 8 and takes tile
-  // (b % 8) * ceil(T/8) + b / 8, so neighbouring tiles (same rows) share an L2.
-  const int32_t per = (n_tiles + 7) >> 3;
-  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
-  if (tile >= n_tiles) return;  // the whole workgroup leaves together
+  TileArgs a;
+  const int32_t tile = tile_select(arg, a);
+  if (tile < 0) return;  // the whole workgroup leaves together
+  const int32_t *__restrict__ tile_chunk = a.tile_chunk;
+  const int64_t *__restrict__ chunk_lbeg = a.chunk_lbeg;
+  const int32_t *__restrict__ list = a.list;
+  const int64_t *__restrict__ pstart = a.pstart;
+  const uint64_t *__restrict__ pcnt = a.pcnt;
+  const uint32_t *__restrict__ prod = a.prod;
+  const int32_t *__restrict__ wave_out = a.wave_out;
+  const T *__restrict__ P = static_cast<const T *>(a.P);
+  const T *__restrict__ Q = static_cast<const T *>(a.Q);
+  T *__restrict__ O = static_cast<T *>(a.O);
   const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, r = lane & 15, g = lane >> 4;
@@ -226,6 +263,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int n = (int)((cnt >> (4 * u)) & 15u);
+        flt4 run = {0.f, 0.f, 0.f, 0.f};  // MODE 3: this block's products of the chunk
         for (int i = 0; i < n; ++i, ++p) {
           const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
           if constexpr (REM == 1) {
@@ -262,6 +300,11 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
                 a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], a32[u], 0, 0, 0);
 #pragma unroll
               for (int e = 0; e < 4; ++e) a32[u][e] = __builtin_fmaf(ce[e], bl, a32[u][e]);
+            } else if constexpr (MODE == 3) {
+#pragma unroll
+              for (int s4 = 0; s4 < C::KS - 1; ++s4) run = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], run, 0, 0, 0);
+#pragma unroll
+              for (int e = 0; e < 4; ++e) run[e] = __builtin_fmaf(ce[e], bl, run[e]);
             } else {
               flt4 t = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -291,12 +334,21 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
 #pragma unroll
             for (int s4 = 0; s4 < C::KS; ++s4)
               a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], a32[u], 0, 0, 0);
+          } else if constexpr (MODE == 3) {
+#pragma unroll
+            for (int s4 = 0; s4 < C::KS; ++s4) run = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], run, 0, 0, 0);
           } else {
             flt4 t = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s4 = 0; s4 < C::KS; ++s4) t = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], bb[s4], t, 0, 0, 0);
 #pragma unroll
             for (int e = 0; e < 4; ++e) a64[u][e] += (double)t[e];
+          }
+        }
+        if constexpr (MODE == 3) {
+          if (n > 0) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) a64[u][e] += (double)run[e];
           }
         }
       }
@@ -333,22 +385,27 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
 // products in entry (x) order, one fused step per product, as the triplet
 // kernel does. Same chunks, stages and copies as k_tile_products; the chunk's
 // entries (at most 64, one per lane) replace the product lists.
-template <int MODE, bool QT>
+template <int MODE, bool QT, class ARG>
 __global__ void __launch_bounds__(kTileThreads) k_tile_dense4(
-    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
-    const int32_t *__restrict__ list, const int32_t *__restrict__ chunk_entry, const uint32_t *__restrict__ emask,
-    const int32_t *__restrict__ ecoff, const int32_t *__restrict__ wave_out,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
-    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+    const ARG arg) {
   using T = typename std::conditional<MODE == 0, double, float>::type;
   constexpr int NS2 = kDenseNsf * kDenseNsf;
   constexpr int STAGE = stage_cap(kDenseNsf) * NS2 + kLdsPad * (int)(8 / sizeof(T));
   __shared__ T stage0[STAGE];
   __shared__ T stage1[STAGE];
-  const int32_t per = (n_tiles + 7) >> 3;
-  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
-  if (tile >= n_tiles) return;  // the whole workgroup leaves together
+  TileArgs a;
+  const int32_t tile = tile_select(arg, a);
+  if (tile < 0) return;  // the whole workgroup leaves together
+  const int32_t *__restrict__ tile_chunk = a.tile_chunk;
+  const int64_t *__restrict__ chunk_lbeg = a.chunk_lbeg;
+  const int32_t *__restrict__ list = a.list;
+  const int32_t *__restrict__ chunk_entry = a.chunk_entry;
+  const uint32_t *__restrict__ emask = a.emask;
+  const int32_t *__restrict__ ecoff = a.ecoff;
+  const int32_t *__restrict__ wave_out = a.wave_out;
+  const T *__restrict__ P = static_cast<const T *>(a.P);
+  const T *__restrict__ Q = static_cast<const T *>(a.Q);
+  T *__restrict__ O = static_cast<T *>(a.O);
   const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, r = lane & 15, g = lane >> 4;
@@ -676,28 +733,10 @@ int inclusive_scan(const T *in, T *out, size_t n, hipStream_t s) {
   return cq::hip_check(hipStreamSynchronize(s), "inclusive_scan");
 }
 
-template <int MODE, bool QT, class T>
-int launch_mode(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
-  const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
-  const dim3 blk(kTileThreads);
-  const int32_t *tc = tp.tile_chunk.as<int32_t>(), *li = tp.list.as<int32_t>(), *wo = tp.wave_out.as<int32_t>();
-  const int64_t *cl = tp.chunk_lbeg.as<int64_t>(), *ps = tp.pstart.as<int64_t>();
-  const uint64_t *pc = tp.pcnt.as<uint64_t>();
-  const uint32_t *pr = tp.prod.as<uint32_t>();
-  if (tp.dense) {
-    hipLaunchKernelGGL((k_tile_dense4<MODE, QT>), dim3(gridx), blk, 0, s, tc, cl, li, tp.chunk_entry.as<int32_t>(),
-                       tp.entry_mask.as<uint32_t>(), tp.entry_coff.as<int32_t>(), wo, P, Q, O, tp.n_tiles);
-    return cq::hip_check(hipGetLastError(), "dense tile-product kernel launch");
-  }
-  switch (tp.nsf) {
-    case 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
-    case 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
-    case 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
-    case 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16>), dim3(gridx), blk, 0, s, tc, cl, li, ps, pc, pr, wo, P, Q, O, tp.n_tiles); break;
-    default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: block size %d has no kernel instance", tp.nsf);
-  }
-  return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
-}
+// Kernel family of a tile schedule: schedules of one family (with the same
+// precision mode and operand transposition) can share a grouped launch.
+enum TileKind { kKindDense4 = 1, kKindDenseMx = 2, kKindMixedSmall = 3, kKindMixedLarge = 4, kKindUniform = 16 };
+int tile_kind(const cq::TilePlan &tp);
 
 // ------------------------------------------------ mixed block sizes
 // Same schedule with per-atom block dimensions (up to 16): operands are
@@ -945,22 +984,28 @@ __device__ inline void issue_stage_mx(uint64_t lv, uint32_t lls, int32_t nb, con
 
 // Mixed-size tile kernel: as k_tile_products with per-slot output dimensions
 // (wave_dim) and per-product contraction dimension (product word bits 24-27).
-template <int MODE, bool QT, int STG>
+template <int MODE, bool QT, int STG, class ARG>
 __global__ void __launch_bounds__(kTileThreads) k_tile_mixed(
-    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
-    const uint64_t *__restrict__ list64, const uint32_t *__restrict__ listls, const int64_t *__restrict__ pstart,
-    const uint64_t *__restrict__ pcnt, const uint32_t *__restrict__ prod, const int64_t *__restrict__ wave_out,
-    const uint8_t *__restrict__ wave_dim,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
-    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+    const ARG arg) {
   using T = typename std::conditional<MODE == 0, double, float>::type;
   constexpr int STAGE = STG + kLdsPad * (int)(8 / sizeof(T));
   __shared__ T stage0[STAGE];
   __shared__ T stage1[STAGE];
-  const int32_t per = (n_tiles + 7) >> 3;
-  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
-  if (tile >= n_tiles) return;
+  TileArgs a;
+  const int32_t tile = tile_select(arg, a);
+  if (tile < 0) return;  // the whole workgroup leaves together
+  const int32_t *__restrict__ tile_chunk = a.tile_chunk;
+  const int64_t *__restrict__ chunk_lbeg = a.chunk_lbeg;
+  const uint64_t *__restrict__ list64 = a.list64;
+  const uint32_t *__restrict__ listls = a.listls;
+  const int64_t *__restrict__ pstart = a.pstart;
+  const uint64_t *__restrict__ pcnt = a.pcnt;
+  const uint32_t *__restrict__ prod = a.prod;
+  const int64_t *__restrict__ wave_out = a.wave_out64;
+  const uint8_t *__restrict__ wave_dim = a.wave_dim;
+  const T *__restrict__ P = static_cast<const T *>(a.P);
+  const T *__restrict__ Q = static_cast<const T *>(a.Q);
+  T *__restrict__ O = static_cast<T *>(a.O);
   const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, r = lane & 15, g = lane >> 4;
@@ -1108,22 +1153,29 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_mixed(
 // block's LDS offset within its entry is the entry offset plus nx times the
 // rows (columns) of the entry's blocks before it, summed from the tile's
 // block-shape bit masks with three popcounts.
-template <int MODE, bool QT>
+template <int MODE, bool QT, class ARG>
 __global__ void __launch_bounds__(kTileThreads) k_tile_dense_mx(
-    const int32_t *__restrict__ tile_chunk, const int64_t *__restrict__ chunk_lbeg,
-    const uint64_t *__restrict__ list64, const uint32_t *__restrict__ listls,
-    const int32_t *__restrict__ chunk_entry, const uint32_t *__restrict__ emask, const int32_t *__restrict__ ecoff,
-    const uint8_t *__restrict__ rdim, const uint8_t *__restrict__ cdim, const int64_t *__restrict__ wave_out,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ P,
-    const typename std::conditional<MODE == 0, double, float>::type *__restrict__ Q,
-    typename std::conditional<MODE == 0, double, float>::type *__restrict__ O, int32_t n_tiles) {
+    const ARG arg) {
   using T = typename std::conditional<MODE == 0, double, float>::type;
   constexpr int STAGE = kStgSmall + kLdsPad * (int)(8 / sizeof(T));
   __shared__ T stage0[STAGE];
   __shared__ T stage1[STAGE];
-  const int32_t per = (n_tiles + 7) >> 3;
-  const int32_t tile = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
-  if (tile >= n_tiles) return;  // the whole workgroup leaves together
+  TileArgs a;
+  const int32_t tile = tile_select(arg, a);
+  if (tile < 0) return;  // the whole workgroup leaves together
+  const int32_t *__restrict__ tile_chunk = a.tile_chunk;
+  const int64_t *__restrict__ chunk_lbeg = a.chunk_lbeg;
+  const uint64_t *__restrict__ list64 = a.list64;
+  const uint32_t *__restrict__ listls = a.listls;
+  const int32_t *__restrict__ chunk_entry = a.chunk_entry;
+  const uint32_t *__restrict__ emask = a.emask;
+  const int32_t *__restrict__ ecoff = a.ecoff;
+  const uint8_t *__restrict__ rdim = a.rdim;
+  const uint8_t *__restrict__ cdim = a.cdim;
+  const int64_t *__restrict__ wave_out = a.wave_out64;
+  const T *__restrict__ P = static_cast<const T *>(a.P);
+  const T *__restrict__ Q = static_cast<const T *>(a.Q);
+  T *__restrict__ O = static_cast<T *>(a.O);
   const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, r = lane & 15, g = lane >> 4, rr = r & 3;
@@ -1241,43 +1293,99 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_dense_mx(
   }
 }
 
-template <int MODE, bool QT, class T>
-int launch_mixed(const cq::TilePlan &tp, const T *P, const T *Q, T *O, hipStream_t s) {
-  const unsigned gridx = 8u * (unsigned)((tp.n_tiles + 7) / 8);
+template <int MODE, bool QT, class ARG>
+int launch_kind(int kind, const ARG &arg, int32_t n, hipStream_t s) {
+  const unsigned gridx = 8u * (unsigned)((n + 7) / 8);
   const dim3 blk(kTileThreads);
-  const int32_t *tc = tp.tile_chunk.as<int32_t>();
-  const int64_t *cl = tp.chunk_lbeg.as<int64_t>(), *ps = tp.pstart.as<int64_t>(), *wo = tp.wave_out64.as<int64_t>();
-  const uint64_t *l64 = tp.list64.as<uint64_t>(), *pc = tp.pcnt.as<uint64_t>();
-  const uint32_t *lls = tp.listls.as<uint32_t>(), *pr = tp.prod.as<uint32_t>();
-  const uint8_t *wd = tp.wave_dim.as<uint8_t>();
-  if (tp.dense) {
-    hipLaunchKernelGGL((k_tile_dense_mx<MODE, QT>), dim3(gridx), blk, 0, s, tc, cl, l64, lls,
-                       tp.chunk_entry.as<int32_t>(), tp.entry_mask.as<uint32_t>(), tp.entry_coff.as<int32_t>(),
-                       tp.tile_rdim.as<uint8_t>(), tp.tile_cdim.as<uint8_t>(), wo, P, Q, O, tp.n_tiles);
-    return cq::hip_check(hipGetLastError(), "dense mixed tile-product kernel launch");
+  switch (kind) {
+    case kKindDense4: hipLaunchKernelGGL((k_tile_dense4<MODE, QT, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindDenseMx: hipLaunchKernelGGL((k_tile_dense_mx<MODE, QT, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindMixedSmall: hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgSmall, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindMixedLarge: hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgLarge, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindUniform + 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindUniform + 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindUniform + 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindUniform + 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: kernel kind %d has no instance", kind);
   }
-  if (stage_elems(tp.maxn) == kStgSmall)
-    hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgSmall>), dim3(gridx), blk, 0, s, tc, cl, l64, lls, ps, pc, pr, wo, wd,
-                       P, Q, O, tp.n_tiles);
-  else
-    hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgLarge>), dim3(gridx), blk, 0, s, tc, cl, l64, lls, ps, pc, pr, wo, wd,
-                       P, Q, O, tp.n_tiles);
-  return cq::hip_check(hipGetLastError(), "mixed tile-product kernel launch");
+  return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
+}
+
+template <class ARG>
+int launch_any(int mode, bool qt, int kind, const ARG &arg, int32_t n, hipStream_t s) {
+  switch (mode * 2 + (qt ? 1 : 0)) {
+    case 0: return launch_kind<0, false, ARG>(kind, arg, n, s);
+    case 1: return launch_kind<0, true, ARG>(kind, arg, n, s);
+    case 2: return launch_kind<1, false, ARG>(kind, arg, n, s);
+    case 3: return launch_kind<1, true, ARG>(kind, arg, n, s);
+    case 4: return launch_kind<2, false, ARG>(kind, arg, n, s);
+    case 5: return launch_kind<2, true, ARG>(kind, arg, n, s);
+    // Mode 3 has its own instance in the uniform-size kernel only.
+    case 6: return kind >= kKindUniform ? launch_kind<3, false, ARG>(kind, arg, n, s) : launch_kind<2, false, ARG>(kind, arg, n, s);
+    case 7: return kind >= kKindUniform ? launch_kind<3, true, ARG>(kind, arg, n, s) : launch_kind<2, true, ARG>(kind, arg, n, s);
+    default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: precision mode %d", mode);
+  }
+}
+
+int tile_kind(const cq::TilePlan &tp) {
+  if (tp.maxn) {
+    if (tp.dense) return kKindDenseMx;
+    return stage_elems(tp.maxn) == kStgSmall ? kKindMixedSmall : kKindMixedLarge;
+  }
+  if (tp.dense) return kKindDense4;
+  return kKindUniform + tp.nsf;
 }
 
 }  // namespace
 
 namespace cq {
 
-int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, bool acc64,
+int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, int mode,
                              hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
-  if (tp.maxn) {
-    if (acc64) return q_trans ? launch_mixed<2, true>(tp, P, Q, O, s) : launch_mixed<2, false>(tp, P, Q, O, s);
-    return q_trans ? launch_mixed<1, true>(tp, P, Q, O, s) : launch_mixed<1, false>(tp, P, Q, O, s);
-  }
-  if (acc64) return q_trans ? launch_mode<2, true>(tp, P, Q, O, s) : launch_mode<2, false>(tp, P, Q, O, s);
-  return q_trans ? launch_mode<1, true>(tp, P, Q, O, s) : launch_mode<1, false>(tp, P, Q, O, s);
+  return launch_any(mode, q_trans, tile_kind(tp), tile_launch_args(tp, P, Q, O), tp.n_tiles, s);
+}
+
+TileArgs tile_launch_args(const TilePlan &tp, const void *P, const void *Q, void *O) {
+  TileArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.tile_chunk = tp.tile_chunk.as<int32_t>();
+  a.chunk_lbeg = tp.chunk_lbeg.as<int64_t>();
+  a.list = tp.list.as<int32_t>();
+  a.list64 = tp.list64.as<uint64_t>();
+  a.listls = tp.listls.as<uint32_t>();
+  a.pstart = tp.pstart.as<int64_t>();
+  a.pcnt = tp.pcnt.as<uint64_t>();
+  a.prod = tp.prod.as<uint32_t>();
+  a.wave_out = tp.wave_out.as<int32_t>();
+  a.wave_out64 = tp.wave_out64.as<int64_t>();
+  a.wave_dim = tp.wave_dim.as<uint8_t>();
+  a.chunk_entry = tp.chunk_entry.as<int32_t>();
+  a.emask = tp.entry_mask.as<uint32_t>();
+  a.ecoff = tp.entry_coff.as<int32_t>();
+  a.rdim = tp.tile_rdim.as<uint8_t>();
+  a.cdim = tp.tile_cdim.as<uint8_t>();
+  a.P = P;
+  a.Q = Q;
+  a.O = O;
+  a.n_tiles = tp.n_tiles;
+  a.tile_base = 0;
+  return a;
+}
+
+int tile_group_key(const TilePlan &tp, bool q_trans, int mode) {
+  return (tile_kind(tp) * 2 + (q_trans ? 1 : 0)) * 4 + mode;
+}
+
+int launch_tile_args(const TilePlan &tp, bool q_trans, int mode, const TileArgs &a, hipStream_t s) {
+  if (a.n_tiles <= 0) return CQMM_OK;
+  return launch_any(mode, q_trans, tile_kind(tp), a, a.n_tiles, s);
+}
+
+int launch_tile_group(const TilePlan &tp, bool q_trans, int mode, const TileGroup &g, hipStream_t s) {
+  if (g.ngrp <= 0 || g.total <= 0) return CQMM_OK;
+  if (g.ngrp > kMaxGroup) return set_errorf(CQMM_ERR_ARG, "grouped tile launch: %d products (at most %d)", g.ngrp, kMaxGroup);
+  return launch_any(mode, q_trans, tile_kind(tp), g, g.total, s);
 }
 
 // CQMM_TILES=0 builds plans without the tile schedule (triplet kernels only).
@@ -1583,8 +1691,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
                          hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
-  if (tp.maxn) return q_trans ? launch_mixed<0, true>(tp, P, Q, O, s) : launch_mixed<0, false>(tp, P, Q, O, s);
-  return q_trans ? launch_mode<0, true>(tp, P, Q, O, s) : launch_mode<0, false>(tp, P, Q, O, s);
+  return launch_any(0, q_trans, tile_kind(tp), tile_launch_args(tp, P, Q, O), tp.n_tiles, s);
 }
 
 // Mixed block sizes: the uniform builder's steps with element offsets and

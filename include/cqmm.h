@@ -39,6 +39,8 @@ extern "C" {
 #define CQMM_FP32 2u         /* run the block products in FP32 (MFMA, FP32 accumulate) */
 #define CQMM_NO_SYNC 4u      /* return without synchronising the plan stream */
 #define CQMM_FP32_ACC64 8u   /* with CQMM_FP32: sum each block's products in FP64 */
+#define CQMM_FP32_RUN64 16u  /* with CQMM_FP32: sum a block's products within one chunk of the
+                                tile schedule (at most 15) in FP32, the chunks in FP64 */
 
 /* type(matrix_halo), src/matrix_module.f90:121-136, built by make_halo
  * (src/matrix_elements_module.f90:560-660). */

@@ -102,7 +102,9 @@ typedef struct cqmm_mcw_setup {
   int32_t max_iter;        /* n_L_iterations                                          */
   int32_t precision;       /* 0: all FP64; bulk products with FP32 MFMA on FP32 operands,
                               then FP64 refinement: 1 sums each block's products in FP64
-                              registers, 2 sums them in the FP32 MFMA accumulator     */
+                              registers, 2 sums them in the FP32 MFMA accumulator, 3 sums
+                              them in FP32 within each chunk of the tile schedule (at most
+                              15 products) and the chunks in FP64                     */
   int32_t n_final_fp64;    /* FP64 iterations after the FP32 phase (>= 2)              */
   double fp32_tol;         /* FP32 phase ends when |dE| < fp32_tol (Ha, per spin)      */
 } cqmm_mcw_setup;
@@ -126,6 +128,24 @@ typedef struct cqmm_mcw_result {
  * return matL holds the purified density matrix. */
 int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *setup, int32_t matH, int32_t matS, int32_t matL,
                  cqmm_mcw_result *result);
+
+/* McWMin for n independent systems at once (no reference counterpart: CONQUEST
+ * purifies one system per run). System i is (ctxs[i], setups[i], matH[i],
+ * matS[i], matL[i]) exactly as in cqmm_mcweeny; contexts must differ. The
+ * systems step in lockstep on the calling thread's stream: each product kind
+ * of all systems runs as one grouped tile launch, the step's traces and
+ * coefficients stay on the device, one copy per step brings every system's
+ * scalars to the host, and a step whose set of systems and precision phases
+ * is unchanged replays a captured hipGraph. A system leaves the batch when
+ * McWMin stops it; its result and L are what cqmm_mcweeny gives alone (the
+ * same kernels and summation order; bitwise). status[i] receives system i's
+ * status (a failing system does not stop the others); the return value is
+ * nonzero only for argument or device errors of the batch itself. n <= 64. */
+#define CQMM_BATCH_NO_GRAPH 1u     /* enqueue every step directly (no hipGraph) */
+#define CQMM_BATCH_NO_GROUPING 2u  /* one product launch per system            */
+int cqmm_mcweeny_batch(int32_t n, cqmm_ctx *const *ctxs, const cqmm_mcw_setup *setups, const int32_t *matH,
+                       const int32_t *matS, const int32_t *matL, uint32_t flags, cqmm_mcw_result *results,
+                       int32_t *status);
 
 /* InitMcW (src/McWeeny.f90:340-570): L0 = ((lambda*mubar + Ne)/No) T - (lambda/No) THT
  * from T ~ S^-1, with Gershgorin-type limits of H (GetHLimits, :1020-1109).

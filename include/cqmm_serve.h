@@ -39,6 +39,9 @@ typedef struct cqmm_purify_result {
   int64_t hbm_estimate;     /* bytes reserved for admission                     */
   int32_t natoms, n_orb;
   int64_t hbm_peak;         /* device bytes the solve held at most (measured)   */
+  double mcw_s;             /* McWMin's wall time                               */
+  double mcw_flops;         /* useful FLOP of McWMin's products (sum over steps of
+                               2 n_i n_k n_j over the triplets of L.S, LS.L, LSL.(LS)^T) */
 } cqmm_purify_result;
 
 /* Defaults of the served workload: S/H/L 8/8/12 A, 0.05 atoms/A^3, 2 A. */
@@ -55,10 +58,16 @@ typedef struct cqmm_serve_request {
 } cqmm_serve_request;
 
 typedef struct cqmm_serve_config {
-  int32_t slots;            /* requests in flight at most (worker threads, one HIP stream each) */
+  int32_t slots;            /* worker threads, one HIP stream each: whole solves (batch 0)
+                               or set-ups up to InitMcW (batch 1, 2)              */
   int64_t hbm_budget;       /* bytes; a request is admitted when its estimate fits */
   cqmm_purify_params params;
   int32_t log_progress;     /* nonzero: one line on stderr per finished request */
+  int32_t batch;            /* 0: each slot runs its request's whole solve alone;
+                               1: McWMin of all prepared requests in lockstep with
+                               grouped product launches and hipGraph replay
+                               (cqmm_mcweeny_batch's engine); 2: as 1 without graphs */
+  int32_t max_batch;        /* batch modes: requests in the McWeeny batch at most (1-64) */
 } cqmm_serve_config;
 
 typedef struct cqmm_serve_result {
@@ -69,8 +78,17 @@ typedef struct cqmm_serve_result {
 
 /* Serve `n` requests: each is released at its arrival time, waits (FIFO) until
  * a slot is free and its HBM estimate fits the budget next to the requests in
- * flight, then runs on that slot's stream, overlapping on the device with the
- * others. A failing request records its status and does not stop the others.
+ * flight (every admitted request holds its estimate until it finishes), then
+ * is set up on that slot's stream. Batch 0: the slot also runs the request's
+ * McWMin, overlapping on the device with the other slots. Batch 1/2 (continuous
+ * batching): the prepared request joins the McWeeny batch at the next step
+ * boundary once fewer than max_batch requests are in it; all requests in the
+ * batch advance one step per iteration with one grouped product launch per
+ * product kind, and each leaves when McWMin stops it — its result is the one
+ * it gets alone (bitwise). Set-ups on the slots overlap with the batch's steps.
+ * A failing request records its status and does not stop the others; a
+ * request's status stays CQMM_ERR_STATE until it is finished. In batch modes
+ * hbm_peak is the set-up's measured peak plus the McWeeny working set.
  * wall_s receives the run's duration. */
 int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_config *cfg,
                cqmm_serve_result *results, double *wall_s);

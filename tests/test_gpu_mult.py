@@ -281,7 +281,10 @@ def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
     """CQMM_FP32: FP32 MFMA products on FP32 copies, FP32 accumulation
     (bar 2e-5 relative to the FP64 oracle: inputs rounded to FP32, sums of a few
     hundred terms in FP32) or, with CQMM_FP32_ACC64, each block's products summed
-    in FP64 (bar 2e-6: only the inputs and each 13-term product are FP32). The tile
+    in FP64 (bar 2e-6: only the inputs and each 13-term product are FP32) or, with
+    CQMM_FP32_RUN64, a block's products of one chunk (at most 15) summed in FP32
+    and the chunks in FP64 (bar 5e-6; the per-output-block kernel sums each product
+    in FP64 there). The tile
     and triplet kernels sum type-1 blocks of uniform size in the same order and agree bit
     for bit."""
     case = Case(natoms, nsf, ra, rb, rc, mt, seed=41)
@@ -292,7 +295,8 @@ def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
     oracle.mat_mult(case.desc, a_r, b_r, c_r)
     ref = c_r if mt == 1 else a_r
     kernels = (cqmm.KERNEL_TILE, cqmm.KERNEL_TRIPLET) if plan.info().n_tiles else (cqmm.KERNEL_TRIPLET,)
-    for flags, bar in ((cqmm.CQMM_FP32, 2e-5), (cqmm.CQMM_FP32 | cqmm.CQMM_FP32_ACC64, 2e-6)):
+    for flags, bar in ((cqmm.CQMM_FP32, 2e-5), (cqmm.CQMM_FP32 | cqmm.CQMM_FP32_ACC64, 2e-6),
+                       (cqmm.CQMM_FP32 | cqmm.CQMM_FP32_RUN64, 5e-6)):
         got = {}
         for kernel in kernels:
             plan.set_kernel(kernel)
@@ -306,7 +310,7 @@ def test_fp32_products(natoms, nsf, ra, rb, rc, mt):
             assert err <= bar
             assert np.all(out[ref == 0.0] == 0.0)
             got[kernel] = out.copy()
-        if mt == 1 and len(got) == 2 and np.ndim(nsf) == 0:
+        if mt == 1 and len(got) == 2 and np.ndim(nsf) == 0 and not flags & cqmm.CQMM_FP32_RUN64:
             np.testing.assert_array_equal(got[cqmm.KERNEL_TILE], got[cqmm.KERNEL_TRIPLET])
 
 

@@ -1,10 +1,15 @@
 """GPU tests of the one-call solve and the request loop (continuous batching).
 
 Bars: the native solve equals the Python-driven solve of the same structure
-bit for bit; in the request loop every result equals the same request solved
-alone (bitwise: each request's arithmetic is deterministic and independent of
-what else runs), requests overlap on the device, admission respects the HBM
-budget, and a request that cannot fit fails alone.
+bit for bit; in the request loop, with requests solved on their own slots or
+batched (McWeeny steps of all prepared requests in lockstep, grouped product
+launches, hipGraph replay), every result equals the same request solved alone
+(bitwise: each request's arithmetic is deterministic and independent of what
+else runs), requests overlap on the device, admission respects the HBM budget,
+and a request that cannot fit fails alone. A Poisson stream shaped like the
+served workload (log-uniform atom counts, 4 or 13 SF, S/H/L 8/8/12 A and its
+S/H recipe, mixed precision) holds every electron count to 1e-8, and its
+smallest request matches the oracle's FP64 solve to 1e-6 Ha/atom.
 """
 import numpy as np
 import pytest
@@ -41,12 +46,13 @@ def test_large_blocks_solve_within_the_estimate():
     assert 0 < r["hbm_peak"] <= r["hbm_estimate"]
 
 
-def test_requests_alone_equal_requests_in_the_mix():
+@pytest.mark.parametrize("batch", [0, 1, 2])
+def test_requests_alone_equal_requests_in_the_mix(batch):
     params = sv.default_params(precision=1, **SMALL)
     reqs = [dict(natoms=n, nsf=f, seed=s, arrival_s=0.01 * k)
             for k, (n, f, s) in enumerate([(40, 4, 1), (64, 13, 2), (48, 4, 3), (56, 13, 4), (72, 4, 5), (44, 13, 6)])]
     alone = [sv.purify(q["natoms"], q["nsf"], q["seed"], params) for q in reqs]
-    res, wall = sv.serve(reqs, slots=3, hbm_budget=64 << 30, params=params)
+    res, wall = sv.serve(reqs, slots=3, hbm_budget=64 << 30, params=params, batch=batch, max_batch=4)
     for a, m in zip(alone, res):
         assert m["status"] == 0 and a["status"] == 0
         assert m["band_energy"] == a["band_energy"]
@@ -55,6 +61,7 @@ def test_requests_alone_equal_requests_in_the_mix():
         assert m["ne_error"] < 1e-8
         # The admission estimate covers what the solve held at most.
         assert 0 < a["hbm_peak"] <= a["hbm_estimate"]
+        assert 0 < m["hbm_peak"] <= m["hbm_estimate"]
     # Work of different requests overlapped on the device.
     iv = sorted((m["admit_s"], m["done_s"]) for m in res)
     assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
@@ -74,3 +81,40 @@ def test_admission_respects_hbm_budget_and_isolates_failures():
     assert res[3]["status"] == -3  # CQMM_ERR_NOMEM: never admitted, the others unaffected
     iv = sorted((m["admit_s"], m["done_s"]) for m in res[:3])
     assert all(iv[k + 1][0] >= iv[k][1] - 1e-3 for k in range(2))
+
+
+def _workload_stream():
+    reqs = sv.poisson_stream(8, 50.0, 2, nmin=6, nmax=48)
+    return reqs
+
+
+def test_served_workload_stream():
+    params = sv.default_params(precision=1)  # S/H/L 8/8/12 A, the workload recipe
+    reqs = _workload_stream()
+    print("stream:", [(q["natoms"], q["nsf"]) for q in reqs], flush=True)
+    alone = [sv.purify(q["natoms"], q["nsf"], q["seed"], params) for q in reqs]
+    res, wall = sv.serve(reqs, slots=2, hbm_budget=200 << 30, params=params, batch=1, max_batch=8)
+    for q, a, m in zip(reqs, alone, res):
+        assert a["status"] == 0 and m["status"] == 0
+        assert m["band_energy"] == a["band_energy"] and m["iterations"] == a["iterations"]
+        assert m["ne_error"] <= 1e-8
+    lat = np.array([m["latency_s"] for m in res])
+    print(f"served {len(res)} in {wall:.2f} s, p50 {np.percentile(lat, 50):.2f} s, "
+          f"p99 {np.percentile(lat, 99):.2f} s", flush=True)
+    # The smallest 4-SF request against the oracle's all-FP64 McWMin from the
+    # device's L0 (Hotelling and InitMcW are pinned against the oracle elsewhere).
+    from oracle import mcweeny as mo
+    k = min((i for i, q in enumerate(reqs) if q["nsf"] == 4), key=lambda i: reqs[i]["natoms"])
+    q = reqs[k]
+    xyz, edge = cqmm.random_structure(q["natoms"], 0.05, 2.0, q["seed"])
+    pu = Purifier(xyz, edge, 4, r_S=8.0, r_H=8.0, r_L=12.0, seed=q["seed"])
+    pu.hotelling()
+    pu.init_mcweeny()
+    L0 = pu.ctx.get(pu.matL)
+    sp = mo.Sparse(pu.sys, pu.rid, ompgemm=True)
+    L_ref, E_ref, Ne_ref, n_ref = sp.mcwmin(L0, pu.S_vals, pu.H_vals, 100)
+    dE = abs(res[k]["band_energy"] - 2.0 * E_ref) / q["natoms"]
+    dN = abs(res[k]["n_electrons"] - 2.0 * Ne_ref)
+    print(f"request {k} ({q['natoms']} atoms x 4 SF): served {res[k]['band_energy']:.10f} Ha, "
+          f"oracle FP64 {2 * E_ref:.10f} Ha; dE/atom {dE:.2e}, dNe {dN:.2e}", flush=True)
+    assert dE <= 1e-6 and dN <= 1e-8
