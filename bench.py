@@ -65,13 +65,14 @@ def species_nsf(natoms, mix, seed):
     return vals[rng.choice(len(vals), size=natoms, p=probs / probs.sum())]
 
 
+TILE_KERNELS = {1: "k_tile_dense4_f64", 2: "k_tile_dense_mx_f64", 3: "k_tile_mixed_f64", 5: "k_tile_quad_f64"}
+
+
 def kernel_name(info, nsf):
     """Name of the FP64 product kernel a plan runs (as rocprofv3 lists it, shortened)."""
     mixed = not isinstance(nsf, int)
     if info.kernel == cqmm.KERNEL_TILE:
-        if mixed:
-            return "k_tile_dense_mx_f64" if max(n for n, _ in nsf) <= 4 else "k_tile_mixed_f64"
-        return "k_tile_dense4_f64" if nsf == 4 else "k_tile_products_f64"
+        return TILE_KERNELS.get(info.tile_family, "k_tile_products_f64")
     return "k_block_products_f64" if mixed else "k_block_products_f64_u"
 
 
@@ -169,7 +170,7 @@ def multiply_workload(name, args, fp32_modes=True, cpu=False):
         info = plan.info()
         kname = kernel_name(info, w["nsf"])
         tune = {"tile_ms": None if ms_tile is None else round(ms_tile, 3), "per_block_ms": round(ms_trip, 3),
-                "chosen": kname}
+                "chosen": kname, "persistent_workgroups": int(info.tile_wgs) or None}
         log(f"{name}: autotune tile {ms_tile} ms, per-block {ms_trip:.3f} ms -> {kname}")
     flags = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_NO_SYNC
     for _ in range(args.warmup):
@@ -205,6 +206,9 @@ def multiply_workload(name, args, fp32_modes=True, cpu=False):
     # (CQMM_FP32_RUN64); errors against the FP64 result.
     fp32 = {}
     modes = (("fp32_accumulate", 0), ("fp32_acc64", cqmm.CQMM_FP32_ACC64), ("fp32_run64", cqmm.CQMM_FP32_RUN64))
+    if info.kernel == cqmm.KERNEL_TILE and info.tile_family == cqmm.TILES_CLASS:
+        # split-BF16 x3 (hi.hi + hi.lo + lo.hi BF16 MFMA, FP64 sums) runs on class tiles only
+        modes += (("bf16x3", cqmm.CQMM_BF16X3),)
     for key, extra in modes if fp32_modes else ():
         f32 = cqmm.CQMM_DEVICE_PTRS | cqmm.CQMM_FP32 | cqmm.CQMM_NO_SYNC | extra
         plan.mat_mult_dev(da.ptr, da.n, db.ptr, db.n, dc.ptr, dc.n, f32)
@@ -357,7 +361,13 @@ def main():
                          "(reference kernel for scaling HBM traffic counters)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "triplet", "tile"],
                     help="FP64 kernel: auto (tile schedule when present), triplet, tile")
+    ap.add_argument("--tiles", default="default", choices=["default", "class"],
+                    help="tile schedule: default, or class tiles for every block size above 4 (needed by the "
+                         "split-BF16 x3 mode)")
     args = ap.parse_args()
+    if args.tiles == "class":
+        os.environ["CQMM_UNIFORM_TILES"] = "class"
+        os.environ["CQMM_MIXED_TILES"] = "class"
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described in DESIGN.md and is not built", file=sys.stderr)

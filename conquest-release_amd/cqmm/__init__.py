@@ -26,6 +26,7 @@ CQMM_DEVICE_PTRS = 1
 CQMM_FP32 = 2
 CQMM_FP32_ACC64 = 8
 CQMM_FP32_RUN64 = 16
+CQMM_BF16X3 = 32
 CQMM_NO_SYNC = 4
 
 i32p = C.POINTER(C.c_int32)
@@ -65,10 +66,12 @@ class MultDesc(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("n_triplets", C.c_int64), ("n_out_blocks", C.c_int64), ("n_out_total", C.c_int64),
                 ("flops", C.c_double), ("uniform_nsf", C.c_int32), ("mult_type", C.c_int32),
-                ("kernel", C.c_int32), ("n_tiles", C.c_int32), ("n_entries", C.c_int64), ("n_loads", C.c_int64)]
+                ("kernel", C.c_int32), ("n_tiles", C.c_int32), ("n_entries", C.c_int64), ("n_loads", C.c_int64),
+                ("tile_family", C.c_int32), ("tile_wgs", C.c_int32)]
 
 
 KERNEL_AUTO, KERNEL_TRIPLET, KERNEL_TILE = 0, 1, 2
+TILES_DENSE4, TILES_DENSE_MIXED, TILES_MIXED, TILES_CLASS, TILES_UNIFORM = 1, 2, 3, 5, 16
 
 
 class RangeInfo(C.Structure):
@@ -150,7 +153,7 @@ EXPORTS = [
     "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
     "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_device_copy", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
     "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_plan_get_info",
-    "cqmm_plan_get_triplets", "cqmm_plan_set_kernel", "cqmm_plan_tune",
+    "cqmm_plan_get_triplets", "cqmm_plan_set_kernel", "cqmm_plan_set_tile_workgroups", "cqmm_plan_tune",
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
     "cqmm_sys_fill", "cqmm_random_structure", "cqmm_sys_plan_create", "cqmm_sys_matrix_index",
@@ -158,6 +161,7 @@ EXPORTS = [
     "cqmm_info_global_file_name", "cqmm_sys_dump_info_global", "cqmm_load_info_global",
     "cqmm_matrix_file_name", "cqmm_dump_matrix_text", "cqmm_grab_matrix_text",
     "cqmm_ctx_create", "cqmm_ctx_destroy", "cqmm_ctx_add_range", "cqmm_ctx_set_transpose", "cqmm_ctx_add_mult",
+    "cqmm_ctx_set_product_precision", "cqmm_ctx_get_product_precision",
     "cqmm_matrix_allocate", "cqmm_matrix_free", "cqmm_matrix_length", "cqmm_matrix_range",
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
     "cqmm_matrix_scale", "cqmm_matrix_product_trace", "cqmm_matrix_trace", "cqmm_matrix_symmetrise",
@@ -208,6 +212,7 @@ def lib():
     L.cqmm_plan_get_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
     L.cqmm_plan_get_triplets.argtypes = [C.c_void_p, i64p, i64p, i64p]
     L.cqmm_plan_set_kernel.argtypes = [C.c_void_p, C.c_int32]
+    L.cqmm_plan_set_tile_workgroups.argtypes = [C.c_void_p, C.c_int32]
     L.cqmm_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                  C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.cqmm_mat_mult.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
@@ -227,6 +232,8 @@ def lib():
     L.cqmm_ctx_add_range.argtypes = [vp, C.c_int32, C.POINTER(MatrixIndex)]
     L.cqmm_ctx_set_transpose.argtypes = [vp, C.c_int32, C.c_int64, i64p, i64p, i32p, i32p]
     L.cqmm_ctx_add_mult.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]
+    L.cqmm_ctx_set_product_precision.argtypes = [vp, C.c_int32]
+    L.cqmm_ctx_get_product_precision.argtypes = [vp, i32p]
     L.cqmm_matrix_allocate.argtypes = [vp, C.c_int32, i32p]
     L.cqmm_matrix_free.argtypes = [vp, C.c_int32]
     L.cqmm_matrix_length.argtypes = [vp, C.c_int32, i64p]
@@ -529,6 +536,10 @@ class Plan:
     def set_kernel(self, kernel):
         """KERNEL_AUTO, KERNEL_TRIPLET or KERNEL_TILE for the FP64 products."""
         check(lib().cqmm_plan_set_kernel(self.h, int(kernel)), "cqmm_plan_set_kernel")
+
+    def set_tile_workgroups(self, wgs):
+        """Class-tile kernel grid: 0 one workgroup per tile, else a persistent grid of wgs workgroups."""
+        check(lib().cqmm_plan_set_tile_workgroups(self.h, int(wgs)), "cqmm_plan_set_tile_workgroups")
 
     def tune_dev(self, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags=CQMM_DEVICE_PTRS, reps=2):
         """Time the plan's kernels on these device operands and keep the faster;

@@ -117,6 +117,17 @@ class Context:
     def transpose(self, A, B):
         check(lib().cqmm_matrix_transpose(self.h, A, B), "cqmm_matrix_transpose")
 
+    @property
+    def product_precision(self):
+        p = C.c_int32()
+        check(lib().cqmm_ctx_get_product_precision(self.h, C.byref(p)), "cqmm_ctx_get_product_precision")
+        return p.value
+
+    @product_precision.setter
+    def product_precision(self, precision):
+        """0 FP64; 1, 2, 3 FP32 MFMA products (FP32 sums, FP64 block sums, FP32 chunk runs)."""
+        check(lib().cqmm_ctx_set_product_precision(self.h, int(precision)), "cqmm_ctx_set_product_precision")
+
 
 def mcw_result_dict(res):
     return dict(iterations=res.iterations, fp32_iterations=res.fp32_iterations, stop_reason=res.stop_reason,
@@ -300,13 +311,21 @@ class Purifier:
                                            C.byref(el)), "cqmm_correct_electrons")
         return el.value
 
-    def dmm(self, **kw):
-        """FindMinDM from the current L; K receives 3LSL - 2LSLSL."""
+    def dmm(self, product_precision=0, **kw):
+        """FindMinDM from the current L; K receives 3LSL - 2LSLSL. product_precision
+        (0 FP64; 1, 2, 3 FP32 modes) applies to the bulk products of the
+        minimisation; the electron-number correction and the final K, energy and
+        electron count use FP64 products."""
         su = self.dmm_setup(**kw)
         res = DmmResult()
         K = self.ctx.allocate(self.rid["H"])
-        check(lib().cqmm_dmm(self.ctx.h, C.byref(su), self.matH, self.matS, self.matT, self.matL, K, C.byref(res)),
-              "cqmm_dmm")
+        saved = self.ctx.product_precision
+        self.ctx.product_precision = product_precision
+        try:
+            check(lib().cqmm_dmm(self.ctx.h, C.byref(su), self.matH, self.matS, self.matT, self.matL, K,
+                                 C.byref(res)), "cqmm_dmm")
+        finally:
+            self.ctx.product_precision = saved
         out = dict(done=res.done, early_iterations=res.early_iterations, late_iterations=res.late_iterations,
                    inflexions=res.inflexions, band_energy=res.energy, n_electrons=res.electrons,
                    residual=res.residual, seconds=res.seconds, K=self.ctx.get(K))

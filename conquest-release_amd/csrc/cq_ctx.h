@@ -55,6 +55,7 @@ struct cqmm_ctx {
   std::multimap<size_t, cq::DevBuf> pool;               // freed matrix buffers by size
   cq::DevBuf red_part, red_out;
   std::map<std::string, cq::DevBuf> scratch;            // FP32 operand copies
+  int32_t product_precision = 0;                        // 0 FP64, else the FP32 mode of cqmm_matrix_product
   ~cqmm_ctx() {
     for (auto &m : mults) cqmm_plan_destroy(m.second.plan);
   }

@@ -373,7 +373,48 @@ int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t 
     if (hrange(ctx, a_role) != m.ra || hrange(ctx, B) != m.rb || hrange(ctx, c_role) != m.rc)
       return cq::set_errorf(CQMM_ERR_ARG, "cqmm_matrix_product: operand ranges (%d,%d,%d) do not match mult %d",
                             hrange(ctx, A), hrange(ctx, B), hrange(ctx, C), mult_id);
-    return cq::launch_block_products(tp, dptr(ctx, A), dptr(ctx, B), dptr(ctx, C), false, cq::stream());
+    hipStream_t s = cq::stream();
+    if (ctx->product_precision == 0) return cq::launch_block_products(tp, dptr(ctx, A), dptr(ctx, B), dptr(ctx, C), false, s);
+    // FP32 products (cqmm_ctx_set_product_precision): FP32 copies of both
+    // operands, the FP32 kernels of that mode, the result widened to FP64.
+    // The kernels take the left operand from A and write C for both types
+    // (type 2: A = C.B^T in mat_mult's naming is P = A, O = C here), as the
+    // FP64 launch above.
+    if (hlen(ctx, A) < tp.len_p || hlen(ctx, B) < tp.len_q || hlen(ctx, C) < tp.len_out)
+      return cq::set_errorf(CQMM_ERR_STATE, "cqmm_matrix_product: operand lengths below mult %d's", mult_id);
+    void *fp, *fq, *fo;
+    if ((st = get_scratch(ctx, "prodP", tp.len_p * 4, &fp)) || (st = get_scratch(ctx, "prodQ", tp.len_q * 4, &fq)) ||
+        (st = get_scratch(ctx, "prodO", tp.len_out * 4, &fo)))
+      return st;
+    if (ctx->product_precision == 4) {
+      if ((st = cq::op_to_bf16x2(tp.len_p, dptr(ctx, A), (uint32_t *)fp, s))) return st;
+      if ((st = cq::op_to_bf16x2(tp.len_q, dptr(ctx, B), (uint32_t *)fq, s))) return st;
+    } else {
+      if ((st = cq::op_to_f32(tp.len_p, dptr(ctx, A), (float *)fp, s))) return st;
+      if ((st = cq::op_to_f32(tp.len_q, dptr(ctx, B), (float *)fq, s))) return st;
+    }
+    if ((st = cq::launch_block_products_f32(tp, (const float *)fp, (const float *)fq, (float *)fo,
+                                            ctx->product_precision, s)))
+      return st;
+    return cq::op_to_f64(tp.len_out, (const float *)fo, dptr(ctx, C), s);
+  });
+}
+
+int cqmm_ctx_set_product_precision(cqmm_ctx *ctx, int32_t precision) {
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_set_product_precision: null context");
+    if (precision < 0 || precision > 4)
+      return cq::set_errorf(CQMM_ERR_ARG, "cqmm_ctx_set_product_precision: precision %d (0..4)", precision);
+    ctx->product_precision = precision;
+    return CQMM_OK;
+  });
+}
+
+int cqmm_ctx_get_product_precision(const cqmm_ctx *ctx, int32_t *precision) {
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !precision) return cq::set_error(CQMM_ERR_ARG, "cqmm_ctx_get_product_precision: null argument");
+    *precision = ctx->product_precision;
+    return CQMM_OK;
   });
 }
 

@@ -79,6 +79,8 @@ struct TilePlan {
   // tile per entry with one MFMA; no product lists. wave_out then holds the
   // quadrant's blocks (row * 4 + column within the quadrant).
   bool dense = false;
+  bool quad = false;   // class tiles (k_tile_quad): one row and one column block size per tile
+  int32_t wgs = 0;     // class tiles: workgroups of a persistent launch (0: one per tile)
   DevBuf chunk_entry;  // int32 [n_chunks+1] first entry of each chunk
   DevBuf entry_mask;   // uint32 [n_entries] left-operand rows | right-operand columns << 8 present
   DevBuf entry_coff;   // int32 [n_entries] first operand block (mixed: element) of the entry in its chunk's stage
@@ -158,8 +160,19 @@ struct TileBuildInput {
   const uint8_t *tile_rdim = nullptr;  // host [n_tiles*8]: block rows of each tile row (0: none)
   const uint8_t *tile_cdim = nullptr;  // host [n_tiles*8]: block columns of each tile column
   const uint8_t *xdim = nullptr;       // host [nx]: contraction dimension of each contraction atom
+  bool class_uniform = false;          // every tile has one row and one column block size
 };
 bool tiles_enabled();
+// Species mixes: tiles grouped by block size and run by the class-tile kernel
+// (CQMM_MIXED_TILES=class; default: primary/Morton order and the per-product
+// mixed-size kernel, which keeps tiles spatially compact).
+bool class_tiles_enabled();
+// Uniform 9, 13, 16 blocks as one class of the class-tile schedule
+// (CQMM_UNIFORM_TILES=class; default: the fixed-size tile kernel).
+bool uniform_class_tiles_enabled();
+// Ranks of n items in the order (size[i], rank[i]), each size group starting
+// at a multiple of 8.
+std::vector<int32_t> group_ranks_by_size(const std::vector<int32_t> &rank, const int32_t *size);
 // Column order of a halo for tiling: Morton order of the halo partitions'
 // cover-set coordinates (label = (x*ncovery + y)*ncoverz + z + 1), atoms of a
 // partition in halo order. Returns rank[halo atom].
@@ -189,6 +202,8 @@ int launch_tile_args(const TilePlan &tp, bool q_trans, int mode, const TileArgs 
 // True when a plan's products run through its tile schedule (else the
 // per-output-block kernels).
 bool uses_tiles(const TripletPlan &tp);
+// Tile kernel family of a schedule (CQMM_TILES_* of cqmm_plan_info).
+int tile_family(const TilePlan &tp);
 // Launch the FP64 (or FP32) block-product kernel over a plan.
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O,
                           bool fp32, hipStream_t stream);

@@ -36,6 +36,26 @@ using cq::Temps;
 
 int trace2(cqmm_ctx *ctx, int32_t a, int32_t b, double *out) { return cqmm_matrix_product_trace(ctx, a, b, out); }
 
+// FP64 products for a scope (the electron-number constraint and the final K,
+// energy and electron count), whatever the context's product precision; the
+// precision is restored when the scope ends.
+class Fp64Products {
+ public:
+  explicit Fp64Products(cqmm_ctx *ctx) : ctx_(ctx) {
+    if (cqmm_ctx_get_product_precision(ctx_, &saved_) != CQMM_OK) saved_ = 0;
+    if (saved_ != 0) (void)cqmm_ctx_set_product_precision(ctx_, 0);
+  }
+  ~Fp64Products() {
+    if (saved_ != 0) (void)cqmm_ctx_set_product_precision(ctx_, saved_);
+  }
+  Fp64Products(const Fp64Products &) = delete;
+  Fp64Products &operator=(const Fp64Products &) = delete;
+
+ private:
+  cqmm_ctx *ctx_;
+  int32_t saved_ = 0;
+};
+
 // X -> T X T^T at L range (the S^-1 metric of the gradients).
 int precondition(cqmm_ctx *ctx, const cqmm_dmm_setup *su, int32_t T, int32_t Tt, int32_t X, int32_t tmpTL,
                  int32_t out) {
@@ -135,8 +155,9 @@ struct Dmm {
     return cqmm_lnv(ctx, &su->lnv, flags, H, S, L, K, M3, ph, electrons, energy);
   }
 
-  // correct_electron_number_fixspin.
+  // correct_electron_number_fixspin, with FP64 products.
   int correct() {
+    Fp64Products fp64(ctx);
     Temps t(ctx);
     int32_t TL, phi2, Sphi, Sphi2;
     DTRY(t.alloc(su->range_TL, &TL));
@@ -524,7 +545,10 @@ int cqmm_dmm(cqmm_ctx *ctx, const cqmm_dmm_setup *su, int32_t H, int32_t S, int3
       if (!done && ndone == su->n_L_iterations) break;
     }
     double el = 0.0, en = 0.0;
-    DTRY(cqmm_lnv(ctx, &su->lnv, CQMM_LNV_K | CQMM_LNV_E | CQMM_LNV_PHI, H, S, L, K, -1, phi, &el, &en));
+    {
+      Fp64Products fp64(ctx);  // K, 2Tr[KH] and the electron count in FP64
+      DTRY(cqmm_lnv(ctx, &su->lnv, CQMM_LNV_K | CQMM_LNV_E | CQMM_LNV_PHI, H, S, L, K, -1, phi, &el, &en));
+    }
     res->done = done ? 1 : 0;
     res->energy = kSpinFactor * en;
     res->electrons = kSpinFactor * el;

@@ -62,6 +62,7 @@ class McwEngine {
   hipGraphExec_t exec_ = nullptr;
   std::string graph_key_, last_eager_key_;
   int64_t graph_steps_ = 0, captures_ = 0;
+  uint64_t serial_ = 0;  // admission count: names a system in the graph key
 };
 
 }  // namespace cq

@@ -83,6 +83,7 @@ struct McwJob {
   int fp64_done = 0, n = 0, stop = 4, max_iter = 50, nfinal = 2, status = 0;
   double oldE = 1.0e30, lastE = 0.0, lastNe = 0.0, lastC = 0.0, t0 = 0.0;
   int slot = -1;
+  uint64_t serial = 0;
 };
 
 McwEngine::McwEngine() = default;
@@ -121,7 +122,7 @@ int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S
     if (!ctx->mults.count(m)) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: mult %d not registered", m);
   if (!ctx->ranges[su.range_L].has_pairs)
     return set_errorf(CQMM_ERR_STATE, "cqmm_mcweeny: range %d has no transpose pairs", su.range_L);
-  if (su.precision < 0 || su.precision > 3) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: precision %d", su.precision);
+  if (su.precision < 0 || su.precision > 4) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: precision %d", su.precision);
   for (const auto &j : jobs_)
     if (j->ctx == ctx) return set_error(CQMM_ERR_ARG, "cqmm_mcweeny: two systems of one batch share a context");
   int slot = -1;
@@ -168,7 +169,8 @@ int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S
     j->LSL32 = (float *)p;
     MTRY(get_scratch(ctx, "LSLSL32" + k, hlen(ctx, j->LSLSL) * 4, &p));
     j->LSLSL32 = (float *)p;
-    MTRY(op_to_f32(hlen(ctx, S), dptr(ctx, S), j->S32, s_));
+    if (su.precision == 4) MTRY(op_to_bf16x2(hlen(ctx, S), dptr(ctx, S), (uint32_t *)j->S32, s_));
+    else MTRY(op_to_f32(hlen(ctx, S), dptr(ctx, S), j->S32, s_));
   }
   // Range maps of every sum of the step, built now (not during a capture).
   const int32_t rS = su.range_S, rH = su.range_H, rL = su.range_L, rLSL = su.range_LSL;
@@ -181,6 +183,7 @@ int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S
   MTRY(ensure_red(ctx));
   MTRY(cqmm_matrix_sum(ctx, 0.0, j->oldL, 1.0, L));
   j->slot = slot;
+  j->serial = ++serial_;
   slot_used_[slot] = true;
   jobs_.push_back(std::move(j));
   return CQMM_OK;
@@ -229,7 +232,7 @@ int McwEngine::products(int which) {
       P = A32;
       Q = B32;
       O = C32;
-      mode = j.su.precision == 1 ? 2 : j.su.precision == 3 ? 3 : 1;
+      mode = j.su.precision == 1 ? 2 : j.su.precision == 3 ? 3 : j.su.precision == 4 ? 4 : 1;
     } else {
       P = dptr(ctx, A);
       Q = dptr(ctx, B);
@@ -275,15 +278,35 @@ int McwEngine::products(int which) {
 int McwEngine::enqueue_step() {
   for (const auto &jp : jobs_) {
     McwJob &j = *jp;
-    if (j.fp32_phase) MTRY(op_to_f32(hlen(j.ctx, j.L), dptr(j.ctx, j.L), j.L32, s_));
+    if (!j.fp32_phase) continue;
+    if (j.su.precision == 4) MTRY(op_to_bf16x2(hlen(j.ctx, j.L), dptr(j.ctx, j.L), (uint32_t *)j.L32, s_));
+    else MTRY(op_to_f32(hlen(j.ctx, j.L), dptr(j.ctx, j.L), j.L32, s_));
   }
-  for (int which = 0; which < 3; ++which) MTRY(products(which));
+  // Split-BF16 systems take LS and LSL as operands in split-BF16 words: each
+  // FP32 result is widened (LSL, for the traces) and then split in place
+  // before the product that reads it.
+  auto split_bf16 = [&](int which) -> int {
+    for (const auto &jp : jobs_) {
+      McwJob &j = *jp;
+      if (!j.fp32_phase || j.su.precision != 4) continue;
+      if (which == 0) MTRY(op_f32_to_bf16x2(hlen(j.ctx, j.LS), (uint32_t *)j.LS32, s_));
+      if (which == 1) {
+        MTRY(op_to_f64(hlen(j.ctx, j.LSL), j.LSL32, dptr(j.ctx, j.LSL), s_));
+        MTRY(op_f32_to_bf16x2(hlen(j.ctx, j.LSL), (uint32_t *)j.LSL32, s_));
+      }
+    }
+    return CQMM_OK;
+  };
+  for (int which = 0; which < 3; ++which) {
+    MTRY(products(which));
+    if (which < 2) MTRY(split_bf16(which));
+  }
   for (const auto &jp : jobs_) {
     McwJob &j = *jp;
     cqmm_ctx *ctx = j.ctx;
     double *sc = dsc(j);
     if (j.fp32_phase) {
-      MTRY(op_to_f64(hlen(ctx, j.LSL), j.LSL32, dptr(ctx, j.LSL), s_));
+      if (j.su.precision != 4) MTRY(op_to_f64(hlen(ctx, j.LSL), j.LSL32, dptr(ctx, j.LSL), s_));
       MTRY(op_to_f64(hlen(ctx, j.LSLSL), j.LSLSL32, dptr(ctx, j.LSLSL), s_));
     }
     // c1 = Tr[S (LSL - LSLSL)] and c2 = Tr[S (L - LSL)] on the S range.
@@ -398,9 +421,12 @@ void McwEngine::drop_graph() {
   graph_key_.clear();
 }
 
+// A captured step holds the device addresses of its systems' matrices, so the
+// key names each system by its admission serial (a slot is reused by the next
+// system admitted after one finishes) and its precision phase.
 std::string McwEngine::key() const {
   std::string k;
-  for (const auto &jp : jobs_) k += std::to_string(jp->slot) + (jp->fp32_phase ? "f" : "d") + ",";
+  for (const auto &jp : jobs_) k += std::to_string(jp->serial) + (jp->fp32_phase ? "f" : "d") + ",";
   return k;
 }
 

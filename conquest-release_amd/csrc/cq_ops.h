@@ -45,6 +45,11 @@ int op_symmetrise(const PairMap &m, double *a, hipStream_t s);
 int op_transpose(const PairMap &m, const double *a, double *b, hipStream_t s);
 int op_to_f32(int64_t n, const double *x, float *y, hipStream_t s);
 int op_to_f64(int64_t n, const float *x, double *y, hipStream_t s);
+// Split-BF16 operand words for the BF16x3 products (bits 0-15: x rounded to
+// BF16, bits 16-31: the remainder rounded to BF16); the second form converts
+// FP32 values in place.
+int op_to_bf16x2(int64_t n, const double *x, uint32_t *y, hipStream_t s);
+int op_f32_to_bf16x2(int64_t n, uint32_t *y, hipStream_t s);
 int op_set_diag(const DiagMap &m, double v, double *a, hipStream_t s);
 int reduction_partials();
 

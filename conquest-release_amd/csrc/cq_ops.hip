@@ -168,6 +168,34 @@ __global__ void k_to_f64(int64_t n, const float *__restrict__ x, double *__restr
     y[t] = (double)x[t];
 }
 
+// Split-BF16 operand words: the high 16 bits of the float nearest to x
+// rounded to BF16 (round to nearest even) in bits 0-15, the BF16 rounding of
+// the remainder x - hi in bits 16-31, so that hi + lo carries 16 significant
+// bits of x. NaN and infinity keep their high half and a zero remainder.
+__device__ inline uint32_t bf16_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return u >> 16;
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+__device__ inline uint32_t split_bf16(double x) {
+  const uint32_t hi = bf16_rne((float)x);
+  if ((hi & 0x7f80u) == 0x7f80u) return hi;
+  const double rest = x - (double)__uint_as_float(hi << 16);
+  return hi | (bf16_rne((float)rest) << 16);
+}
+
+__global__ void k_to_bf16x2(int64_t n, const double *__restrict__ x, uint32_t *__restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    y[t] = split_bf16(x[t]);
+}
+
+// In place: FP32 values (a product's FP32 result) -> split-BF16 words.
+__global__ void k_f32_to_bf16x2(int64_t n, uint32_t *__restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    y[t] = split_bf16((double)__uint_as_float(y[t]));
+}
+
 }  // namespace
 
 namespace cq {
@@ -247,6 +275,18 @@ int op_to_f64(int64_t n, const float *x, double *y, hipStream_t s) {
   if (n <= 0) return CQMM_OK;
   hipLaunchKernelGGL(k_to_f64, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, x, y);
   return hip_check(hipGetLastError(), "to_f64 launch");
+}
+
+int op_to_bf16x2(int64_t n, const double *x, uint32_t *y, hipStream_t s) {
+  if (n <= 0) return CQMM_OK;
+  hipLaunchKernelGGL(k_to_bf16x2, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, x, y);
+  return hip_check(hipGetLastError(), "to_bf16x2 launch");
+}
+
+int op_f32_to_bf16x2(int64_t n, uint32_t *y, hipStream_t s) {
+  if (n <= 0) return CQMM_OK;
+  hipLaunchKernelGGL(k_f32_to_bf16x2, dim3(grid_for(n, 4)), dim3(kBS), 0, s, n, y);
+  return hip_check(hipGetLastError(), "f32_to_bf16x2 launch");
 }
 
 int op_set_diag(const DiagMap &m, double v, double *a, hipStream_t s) {

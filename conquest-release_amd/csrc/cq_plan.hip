@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_items(
     const int64_t *__restrict__ item_off, int32_t *__restrict__ key_out, int64_t *__restrict__ p_out,
     int64_t *__restrict__ q_out, uint8_t *__restrict__ k_out, DevErr *err, const int32_t *__restrict__ crank,
     const int32_t *__restrict__ tmap, int32_t ncg, int64_t nx, uint64_t *__restrict__ tkey,
-    uint8_t *__restrict__ tslot) {
+    uint8_t *__restrict__ tslot, const int32_t *__restrict__ rrank) {
   const int lane = threadIdx.x & 63;
   const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (item >= n_items) return;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_items(
       q_out[t] = boff_arr[m.jrow + j];
       if (tkey) {
         // Tile of the output block (row i, column c) and contraction atom x.
-        const int32_t r0 = i - 1;
+        const int32_t r0 = rrank ? rrank[i - 1] : i - 1;
         const int32_t c = mult_type == 1 ? jh - 1 : kh;
         const int64_t x = mult_type == 1 ? kh : jh - 1;
         const int32_t cr = crank[c];
@@ -484,7 +484,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
                      d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
                      d_ci_h2d.as<H>(), ni_c, d_ai_h2d.as<H>(), ni_a, d_ndimi.as<int32_t>(),
                      d->mult_type, len_p, d_counts.as<int32_t>(), d_flops.as<int64_t>(), nullptr, nullptr, nullptr,
-                     nullptr, nullptr, d_err.as<DevErr>(), nullptr, nullptr, 0, 0, nullptr, nullptr);
+                     nullptr, nullptr, d_err.as<DevErr>(), nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr);
   if ((st = cq::hip_check(hipGetLastError(), "builder kernel launch"))) return st;
   std::vector<int32_t> counts(n_items);
   std::vector<int64_t> iflops(n_items);
@@ -526,12 +526,16 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   // 16 (mixed species) the mixed-size one.
   bool tiles = (uniform != 0 || (maxn > 0 && maxn <= 16)) && n_trip > 0 && cq::tiles_enabled();
   const int64_t ns2 = (int64_t)uniform * uniform;
-  std::vector<int32_t> crank, tmap;
+  std::vector<int32_t> crank, tmap, rrank;  // rrank: tile row rank of each primary atom (empty: primary order)
   std::vector<int64_t> tout;
   std::vector<uint8_t> trdim, tcdim, xdim;
   int32_t ncg = 0, n_tiles = 0;
+  bool class_uniform = false;
   const int64_t nx = d->mult_type == 1 ? ni_a : ni_c;
-  if (tiles && uniform) {
+  // Uniform 9, 13 and 16 may run as one class of the class-tile schedule
+  // (CQMM_UNIFORM_TILES=class) instead of the fixed-size tile kernel.
+  const bool uclass = uniform != 0 && uniform != 4 && cq::uniform_class_tiles_enabled();
+  if (tiles && uniform && !uclass) {
     // Every operand offset must be a whole number of blocks.
     for (int32_t k = 0; k < ni_a && tiles; ++k) tiles = km[k].bdata % ns2 == 0;
     const H *ph2d = d->mult_type == 1 ? A.i_h2d : C.i_h2d;
@@ -551,8 +555,25 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     const HaloV<H> &CH = d->mult_type == 1 ? C : A;  // halo of the output's columns
     crank = cq::halo_morton_rank(CH.np_in_halo, CH.nh_part, CH.lab_hcover, d->ncovery, d->ncoverz);
     const int32_t ncol = CH.ni_in_halo;
-    ncg = (ncol + 7) / 8;
-    const int64_t nrg = (d->n_prim + 7) / 8;
+    const int32_t *cdims = d->mult_type == 1 ? C.ndimj : A.ndimj;
+    int64_t max_rrank = d->n_prim - 1;
+    if (!uniform && cq::class_tiles_enabled()) {
+      // Species mixes: tile rows (primary order) and columns (Morton order)
+      // grouped by block size, each group starting at a multiple of 8, so that
+      // every tile has one row block size and one column block size.
+      std::vector<int32_t> prim(d->n_prim);
+      for (int32_t p = 0; p < d->n_prim; ++p) prim[p] = p;
+      rrank = cq::group_ranks_by_size(prim, A.ndimi);
+      crank = cq::group_ranks_by_size(crank, cdims);
+      max_rrank = 0;
+      for (int32_t v : rrank) max_rrank = std::max<int64_t>(max_rrank, v);
+    }
+    int32_t max_crank = 0;
+    for (int32_t v : crank) max_crank = std::max(max_crank, v);
+    ncg = (max_crank + 8) / 8;
+    const int64_t nrg = (max_rrank + 8) / 8;
+    auto rowr = [&](int64_t p) -> int64_t { return rrank.empty() ? p : rrank[p]; };
+    (void)ncol;
     // Output blocks (row, column, offset) of the product's result.
     std::vector<int64_t> ob_r, ob_c, ob_o;
     if (d->mult_type == 2) {
@@ -600,7 +621,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     if (tiles && nrg * ncg > INT32_MAX) tiles = false;
     if (tiles) {
       tmap.assign((size_t)(nrg * ncg), -1);
-      for (size_t b = 0; b < ob_o.size(); ++b) tmap[(ob_r[b] >> 3) * ncg + (crank[ob_c[b]] >> 3)] = 0;
+      for (size_t b = 0; b < ob_o.size(); ++b) tmap[(rowr(ob_r[b]) >> 3) * ncg + (crank[ob_c[b]] >> 3)] = 0;
       for (auto &v : tmap)
         if (v == 0) v = n_tiles++;
       pt.mark("  tile map: numbering");
@@ -609,31 +630,46 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
       // atom (the mixed-size schedule packs blocks by their sizes).
       trdim.assign((size_t)n_tiles * 8, 0);
       tcdim.assign((size_t)n_tiles * 8, 0);
-      const int32_t *cdims = d->mult_type == 1 ? C.ndimj : A.ndimj;
       // Every output block has its own slot: the fill runs in parallel.
       cq::parallel_for((int64_t)ob_o.size(), [&](int64_t b0, int64_t b1) {
         for (int64_t b = b0; b < b1; ++b) {
           const int32_t cr = crank[ob_c[b]];
-          const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
-          tout[(size_t)tile * 64 + ((ob_r[b] & 7) << 3) + (cr & 7)] = ob_o[b];
+          const int64_t rr = rowr(ob_r[b]);
+          const int32_t tile = tmap[(rr >> 3) * ncg + (cr >> 3)];
+          tout[(size_t)tile * 64 + ((rr & 7) << 3) + (cr & 7)] = ob_o[b];
         }
       });
-      if (!uniform)
+      if (!uniform || uclass) {
         for (size_t b = 0; b < ob_o.size(); ++b) {
           const int32_t cr = crank[ob_c[b]];
-          const int32_t tile = tmap[(ob_r[b] >> 3) * ncg + (cr >> 3)];
-          trdim[(size_t)tile * 8 + (ob_r[b] & 7)] = (uint8_t)A.ndimi[ob_r[b]];
+          const int64_t rr = rowr(ob_r[b]);
+          const int32_t tile = tmap[(rr >> 3) * ncg + (cr >> 3)];
+          trdim[(size_t)tile * 8 + (rr & 7)] = (uint8_t)A.ndimi[ob_r[b]];
           tcdim[(size_t)tile * 8 + (cr & 7)] = (uint8_t)cdims[ob_c[b]];
         }
+        // One row size and one column size per tile (the class-tile kernel's condition).
+        class_uniform = uclass || !rrank.empty();
+        for (int32_t t = 0; t < n_tiles && class_uniform; ++t) {
+          int rsz = 0, csz = 0;
+          for (int k = 0; k < 8; ++k) {
+            const int a = trdim[(size_t)t * 8 + k], c = tcdim[(size_t)t * 8 + k];
+            if (a && rsz && a != rsz) class_uniform = false;
+            if (c && csz && c != csz) class_uniform = false;
+            if (a) rsz = a;
+            if (c) csz = c;
+          }
+        }
+      }
       const int32_t *xd = d->mult_type == 1 ? A.ndimj : C.ndimj;
       xdim.resize((size_t)nx);
       for (int64_t x = 0; x < nx; ++x) xdim[x] = (uint8_t)xd[x];
     }
   }
   pt.mark("host tile map");
-  DevBuf d_crank, d_tmap, d_tkey, d_tslot;
+  DevBuf d_crank, d_tmap, d_tkey, d_tslot, d_rrank;
   if (tiles) {
     if ((st = upload(d_crank, crank.data(), crank.size(), s))) return st;
+    if (!rrank.empty() && (st = upload(d_rrank, rrank.data(), rrank.size(), s))) return st;
     if ((st = upload(d_tmap, tmap.data(), tmap.size(), s))) return st;
     if ((st = d_tkey.alloc(n_trip * 8))) return st;
     if ((st = d_tslot.alloc(n_trip))) return st;
@@ -652,11 +688,12 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
                      d->mult_type, len_p, nullptr, nullptr, d_itemoff.as<int64_t>(), d_key.as<int32_t>(),
                      d_p.as<int64_t>(), d_q.as<int64_t>(), uniform ? nullptr : d_k.as<uint8_t>(), d_err.as<DevErr>(),
                      tiles ? d_crank.as<int32_t>() : nullptr, tiles ? d_tmap.as<int32_t>() : nullptr, ncg, nx,
-                     tiles ? d_tkey.as<uint64_t>() : nullptr, tiles ? d_tslot.as<uint8_t>() : nullptr);
+                     tiles ? d_tkey.as<uint64_t>() : nullptr, tiles ? d_tslot.as<uint8_t>() : nullptr,
+                     tiles && !rrank.empty() ? d_rrank.as<int32_t>() : nullptr);
   if ((st = cq::hip_check(hipGetLastError(), "builder fill launch"))) return st;
   if (tiles) {
     cq::TileBuildInput tin;
-    tin.nsf = uniform;
+    tin.nsf = uclass ? 0 : uniform;
     tin.maxn = maxn;
     tin.tile_rdim = trdim.data();
     tin.tile_cdim = tcdim.data();
@@ -669,6 +706,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     tin.nx = nx;
     tin.n_tiles = n_tiles;
     tin.tile_out = tout.data();
+    tin.class_uniform = class_uniform;
     pt.mark("triplet fill + tile keys");
     if ((st = cq::build_tiles(tin, tp.tiles, s))) return st;
     pt.mark("tile schedule");
@@ -676,6 +714,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     d_tslot.release();
     d_tmap.release();
     d_crank.release();
+    d_rrank.release();
   }
 
   const size_t osz = tp.off64 ? 8 : 4;
@@ -890,6 +929,8 @@ int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info) {
     info->n_tiles = plan->tp.tiles.n_tiles;
     info->n_entries = plan->tp.tiles.n_entries;
     info->n_loads = plan->tp.tiles.n_loads;
+    info->tile_family = plan->tp.tiles.n_tiles > 0 ? cq::tile_family(plan->tp.tiles) : 0;
+    info->tile_wgs = plan->tp.tiles.wgs;
     info->kernel = (plan->tp.tiles.n_tiles > 0 && plan->tp.kernel != CQMM_KERNEL_TRIPLET) ? CQMM_KERNEL_TILE
                                                                                             : CQMM_KERNEL_TRIPLET;
     // Output blocks with at least one product.
@@ -911,6 +952,14 @@ int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel) {
     if (kernel == CQMM_KERNEL_TILE && plan->tp.tiles.n_tiles == 0)
       return cq::set_error(CQMM_ERR_STATE, "cqmm_plan_set_kernel: plan has no tile schedule");
     plan->tp.kernel = kernel;
+    return CQMM_OK;
+  });
+}
+
+int cqmm_plan_set_tile_workgroups(cqmm_plan *plan, int32_t wgs) {
+  return cq::guarded(__func__, [&]() -> int {
+    if (!plan || wgs < 0) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_set_tile_workgroups: bad argument");
+    plan->tp.tiles.wgs = wgs;
     return CQMM_OK;
   });
 }
@@ -946,7 +995,30 @@ int cqmm_plan_tune(cqmm_plan *plan, double *a, int64_t lena, const double *b, in
     *ms_triplet = -1.0f;
     const int32_t before = plan->tp.kernel;
     st = time_kernel(CQMM_KERNEL_TRIPLET, ms_triplet);
-    if (!st && plan->tp.tiles.n_tiles > 0) st = time_kernel(CQMM_KERNEL_TILE, ms_tile);
+    if (!st && plan->tp.tiles.n_tiles > 0) {
+      cq::TilePlan &tl = plan->tp.tiles;
+      tl.wgs = 0;
+      st = time_kernel(CQMM_KERNEL_TILE, ms_tile);
+      // Class tiles: persistent grids of 2, 3 and 4 workgroups per CU against
+      // one workgroup per tile; the fastest grid stays.
+      if (!st && tl.quad) {
+        const int32_t n_cu = cq::runtime().n_cu > 0 ? cq::runtime().n_cu : 256;
+        int32_t best = 0;
+        float best_ms = *ms_tile;
+        for (int32_t per_cu = 2; per_cu <= 4 && !st; ++per_cu) {
+          if (per_cu * n_cu >= tl.n_tiles) break;
+          tl.wgs = per_cu * n_cu;
+          float ms = 0.0f;
+          st = time_kernel(CQMM_KERNEL_TILE, &ms);
+          if (!st && ms < best_ms) {
+            best_ms = ms;
+            best = tl.wgs;
+          }
+        }
+        tl.wgs = best;
+        *ms_tile = best_ms;
+      }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) {
@@ -955,8 +1027,8 @@ int cqmm_plan_tune(cqmm_plan *plan, double *a, int64_t lena, const double *b, in
     }
     plan->tp.kernel = (*ms_tile >= 0.0f && *ms_tile <= *ms_triplet) ? CQMM_KERNEL_TILE : CQMM_KERNEL_TRIPLET;
     // The output must hold a result: re-run when the last timed kernel is not the chosen one
-    // (both are correct; this keeps the call's result from the kernel that stays).
-    if (plan->tp.kernel == CQMM_KERNEL_TRIPLET && *ms_tile >= 0.0f)
+    // (all are correct; this keeps the call's result from the kernel and grid that stay).
+    if ((plan->tp.kernel == CQMM_KERNEL_TRIPLET && *ms_tile >= 0.0f) || plan->tp.tiles.quad)
       return cqmm_mat_mult(plan, a, lena, b, lenb, c, lenc, flags & ~(uint32_t)CQMM_NO_SYNC);
     return CQMM_OK;
   });
@@ -1026,9 +1098,14 @@ int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b, int
       if (plan->fp.bytes < (size_t)tp.len_p * 4 && (st = plan->fp.alloc(tp.len_p * 4))) return st;
       if (plan->fq.bytes < (size_t)tp.len_q * 4 && (st = plan->fq.alloc(tp.len_q * 4))) return st;
       if (plan->fo.bytes < (size_t)tp.len_out * 4 && (st = plan->fo.alloc(tp.len_out * 4))) return st;
-      if ((st = cq::op_to_f32(tp.len_p, dP, plan->fp.as<float>(), s))) return st;
-      if ((st = cq::op_to_f32(tp.len_q, db, plan->fq.as<float>(), s))) return st;
-      const int mode = (flags & CQMM_FP32_RUN64) ? 3 : (flags & CQMM_FP32_ACC64) ? 2 : 1;
+      const int mode = (flags & CQMM_BF16X3) ? 4 : (flags & CQMM_FP32_RUN64) ? 3 : (flags & CQMM_FP32_ACC64) ? 2 : 1;
+      if (mode == 4) {
+        if ((st = cq::op_to_bf16x2(tp.len_p, dP, plan->fp.as<uint32_t>(), s))) return st;
+        if ((st = cq::op_to_bf16x2(tp.len_q, db, plan->fq.as<uint32_t>(), s))) return st;
+      } else {
+        if ((st = cq::op_to_f32(tp.len_p, dP, plan->fp.as<float>(), s))) return st;
+        if ((st = cq::op_to_f32(tp.len_q, db, plan->fq.as<float>(), s))) return st;
+      }
       if ((st = cq::launch_block_products_f32(tp, plan->fp.as<float>(), plan->fq.as<float>(), plan->fo.as<float>(),
                                               mode, s)))
         return st;

@@ -218,7 +218,7 @@ int prepare(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_params
     len[k] = ri.length;
   }
   sv.mcw_bytes = 8 * (3 * len[Lr] + len[LS] + len[LSL] + 2 * len[S] + len[H]);
-  if (p->precision == 1 || p->precision == 2) sv.mcw_bytes += 4 * (2 * len[Lr] + len[S] + len[LS] + len[LSL]);
+  if (p->precision != 0) sv.mcw_bytes += 4 * (2 * len[Lr] + len[S] + len[LS] + len[LSL]);
   return CQMM_OK;
 }
 

@@ -49,6 +49,20 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// Four split-BF16 words -> their hi halves (bits 0-15) or lo halves (bits
+// 16-31) as the four BF16 elements of an MFMA operand.
+__device__ inline s16x4 pack_bf16_lo(const uint32_t w[4]) {
+  const uint32_t x = __builtin_amdgcn_perm(w[1], w[0], 0x05040100u);
+  const uint32_t y = __builtin_amdgcn_perm(w[3], w[2], 0x05040100u);
+  return __builtin_bit_cast(s16x4, (uint64_t)x | ((uint64_t)y << 32));
+}
+__device__ inline s16x4 pack_bf16_hi(const uint32_t w[4]) {
+  const uint32_t x = __builtin_amdgcn_perm(w[1], w[0], 0x07060302u);
+  const uint32_t y = __builtin_amdgcn_perm(w[3], w[2], 0x07060302u);
+  return __builtin_bit_cast(s16x4, (uint64_t)x | ((uint64_t)y << 32));
+}
 
 constexpr int kWaves = 4;                    // wavefronts per workgroup (one tile)
 constexpr int kTileThreads = 64 * kWaves;
@@ -735,7 +749,15 @@ int inclusive_scan(const T *in, T *out, size_t n, hipStream_t s) {
 
 // Kernel family of a tile schedule: schedules of one family (with the same
 // precision mode and operand transposition) can share a grouped launch.
-enum TileKind { kKindDense4 = 1, kKindDenseMx = 2, kKindMixedSmall = 3, kKindMixedLarge = 4, kKindUniform = 16 };
+enum TileKind {
+  kKindDense4 = 1,
+  kKindDenseMx = 2,
+  kKindMixedSmall = 3,
+  kKindMixedLarge = 4,
+  kKindQuadSmall = 5,
+  kKindQuadLarge = 6,
+  kKindUniform = 16
+};
 int tile_kind(const cq::TilePlan &tp);
 
 // ------------------------------------------------ mixed block sizes
@@ -1293,15 +1315,298 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_dense_mx(
   }
 }
 
+// Class tiles (species mixes, blocks up to 16x16): the tile's rows all have
+// one block size NI and its columns one size NJ (the plan groups tile rows
+// and columns by species), so the tile's blocks pack into 16x16-element MFMA
+// tiles of RP row blocks x CP column blocks (RP = 4, 2, 1 for NI <= 4, <= 8,
+// <= 16; CP likewise): sixteen 4x4 block pairs share one MFMA, four 4x13
+// pairs, one 13x13 pair. Wavefront w owns quadrant (w / 2, w % 2) of the tile
+// (4 block rows x 4 block columns, 16 / (RP CP) MFMA tiles). Per chunk entry
+// (contraction atom x with nx contraction indices; at most 64 entries per
+// chunk, one per lane) and k-step of 4 indices, each lane reads one element
+// of each of the wavefront's row units and column units from LDS (its block's
+// offset from the entry's offset and the popcount of the entry's blocks
+// before it) and every MFMA tile with a block pair present accumulates;
+// absent blocks, rows or columns past NI / NJ and indices past nx read as
+// zero. Each output block sums its products in entry (x) order. The entry
+// list replaces per-product lists: no product decode at run time.
+template <int MODE, bool QT>
+__device__ __forceinline__ void quad_tile(const TileArgs &a, const int32_t tile,
+                                          typename std::conditional<MODE == 0, double, float>::type *stage0,
+                                          typename std::conditional<MODE == 0, double, float>::type *stage1) {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
+  const int32_t *__restrict__ tile_chunk = a.tile_chunk;
+  const int64_t *__restrict__ chunk_lbeg = a.chunk_lbeg;
+  const uint64_t *__restrict__ list64 = a.list64;
+  const uint32_t *__restrict__ listls = a.listls;
+  const int32_t *__restrict__ chunk_entry = a.chunk_entry;
+  const uint32_t *__restrict__ emask = a.emask;
+  const int32_t *__restrict__ ecoff = a.ecoff;
+  const uint8_t *__restrict__ rdim = a.rdim;
+  const uint8_t *__restrict__ cdim = a.cdim;
+  const int64_t *__restrict__ wave_out = a.wave_out64;
+  const T *__restrict__ P = static_cast<const T *>(a.P);
+  const T *__restrict__ Q = static_cast<const T *>(a.Q);
+  T *__restrict__ O = static_cast<T *>(a.O);
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, r = lane & 15, g = lane >> 4;
+  const int qr = w >> 1, qc = w & 1;
+  // Block sizes of the tile (rows of absent tile rows read 0).
+  int NI = 0, NJ = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    NI = max(NI, (int)rdim[(int64_t)tile * 8 + k]);
+    NJ = max(NJ, (int)cdim[(int64_t)tile * 8 + k]);
+  }
+  NI = __builtin_amdgcn_readfirstlane(NI);
+  NJ = __builtin_amdgcn_readfirstlane(NJ);
+  // Packing: RP row blocks of RH element rows per MFMA tile (CP, CH for columns).
+  const int lrh = NI <= 4 ? 2 : NI <= 8 ? 3 : 4, lch = NJ <= 4 ? 2 : NJ <= 8 ? 3 : 4;
+  const int RP = 16 >> lrh, CP = 16 >> lch;
+  const int NTA = 4 / RP, NTB = 4 / CP;
+  const int ir = r & ((1 << lrh) - 1), jc = r & ((1 << lch) - 1);
+  const int ru = r >> lrh, cu = r >> lch;
+  const bool rowok = ir < NI, colok = jc < NJ;
+  dbl4 a64[16];
+  flt4 a32[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    a64[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    a32[u] = flt4{0.f, 0.f, 0.f, 0.f};
+  }
+  const int32_t c0 = tile_chunk[tile], c1 = tile_chunk[tile + 1];
+  if (c0 < c1) {
+    int32_t wb = c0;
+    int32_t cc = min(wb + lane, c1);
+    int64_t wcl = chunk_lbeg[cc];
+    int32_t wce = chunk_entry[cc];
+    auto cl_at = [&](int32_t x) { return readlane64(wcl, min(x, c1) - wb); };
+    auto ce_at = [&](int32_t x) { return __builtin_amdgcn_readlane(wce, min(x, c1) - wb); };
+    auto list_idx = [&](int64_t lbeg, int32_t nb) { return lbeg + min(w + kWaves * lane, nb - 1); };
+    auto entry_regs = [&](int32_t e0, int32_t e1, uint32_t &m, int32_t &o) {
+      const int32_t e = min(e0 + lane, e1 - 1);
+      m = emask[e];
+      o = ecoff[e];
+    };
+    uint32_t em, emn = 0;
+    int32_t eo, eon = 0;
+    uint64_t lvn = 0;
+    uint32_t lsn = 0;
+    {
+      const int64_t cl0 = cl_at(c0), cl1 = cl_at(c0 + 1);
+      const int64_t i0 = list_idx(cl0, (int32_t)(cl1 - cl0));
+      issue_stage_mx<T>(list64[i0], listls[i0], (int32_t)(cl1 - cl0), P, Q, stage0, w, lane);
+      entry_regs(ce_at(c0), ce_at(c0 + 1), emn, eon);
+      if (c0 + 1 < c1) {
+        const int64_t i1 = list_idx(cl1, (int32_t)(cl_at(c0 + 2) - cl1));
+        lvn = list64[i1];
+        lsn = listls[i1];
+      }
+    }
+    const uint32_t quad_rows = 0xfu << (4 * qr), quad_cols = 0xfu << (4 * qc);
+    for (int32_t c = c0; c < c1; ++c) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      const bool odd = ((c - c0) & 1) != 0;
+      const T *cur = odd ? stage1 : stage0;
+      em = emn;
+      eo = eon;
+      const int32_t ne = ce_at(c + 1) - ce_at(c);
+      if (c + 1 < c1) {
+        const int64_t cl1 = cl_at(c + 1), cl2 = cl_at(c + 2);
+        issue_stage_mx<T>(lvn, lsn, (int32_t)(cl2 - cl1), P, Q, odd ? stage0 : stage1, w, lane);
+        entry_regs(ce_at(c + 1), ce_at(c + 2), emn, eon);
+        if (c + 2 < c1) {
+          const int64_t i2 = list_idx(cl2, (int32_t)(cl_at(c + 3) - cl2));
+          lvn = list64[i2];
+          lsn = listls[i2];
+        }
+      }
+      for (int i = 0; i < ne; ++i) {
+        const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)em, i);
+        const uint32_t ym = m & 0xffu, bm = (m >> 8) & 0xffu;
+        if ((ym & quad_rows) == 0 || (bm & quad_cols) == 0) continue;  // no block pair in this quadrant
+        const int32_t base = __builtin_amdgcn_readlane(eo, i);
+        const int nx = (int)((m >> 16) & 15u) + 1;
+        const int ablk = NI * nx, bblk = nx * NJ;
+        const int bbase = base + ablk * __popc(ym);
+        if constexpr (MODE == 4) {
+          // Split-BF16 x3: one K = 16 step (nx <= 16); lane (r, g) holds
+          // contraction indices 4g..4g+3 of its row (column) as four LDS words
+          // (bits 0-15 hi, 16-31 lo), repacked into the hi and lo operands of
+          // v_mfma_f32_16x16x16_bf16; hi.hi + hi.lo + lo.hi chain in FP32 and
+          // each entry's sum is added into the FP64 accumulator.
+          s16x4 ahi[4], alo[4], bhi[4], blo[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int u = 4 * qr + t * RP + ru, v = 4 * qc + t * CP + cu;
+            const bool oka = t < NTA && rowok && ((ym >> u) & 1u) != 0;
+            const bool okb = t < NTB && colok && ((bm >> v) & 1u) != 0;
+            const int a0 = base + ablk * __popc(ym & ((1u << u) - 1u)) + ir + NI * 4 * g;
+            const int b0 = bbase + bblk * __popc(bm & ((1u << v) - 1u)) + (QT ? jc + NJ * 4 * g : 4 * g + nx * jc);
+            uint32_t wa[4], wb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const bool kin = 4 * g + i < nx;
+              const bool ra = oka && kin, rb = okb && kin;
+              wa[i] = __float_as_uint(cur[ra ? a0 + NI * i : 0]);
+              wb[i] = __float_as_uint(cur[rb ? b0 + (QT ? NJ * i : i) : 0]);
+              if (!ra) wa[i] = 0u;
+              if (!rb) wb[i] = 0u;
+            }
+            ahi[t] = pack_bf16_lo(wa);
+            alo[t] = pack_bf16_hi(wa);
+            bhi[t] = pack_bf16_lo(wb);
+            blo[t] = pack_bf16_hi(wb);
+          }
+#pragma unroll
+          for (int ta = 0; ta < 4; ++ta) {
+            if (ta >= NTA || (ym & (((1u << RP) - 1u) << (4 * qr + ta * RP))) == 0) continue;
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb) {
+              if (tb >= NTB || (bm & (((1u << CP) - 1u) << (4 * qc + tb * CP))) == 0) continue;
+              const int u = ta * 4 + tb;
+              flt4 t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[ta], bhi[tb], flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+              t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[ta], blo[tb], t, 0, 0, 0);
+              t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo[ta], bhi[tb], t, 0, 0, 0);
+#pragma unroll
+              for (int e = 0; e < 4; ++e) a64[u][e] += (double)t[e];
+            }
+          }
+          continue;
+        }
+        // This lane's element of each row unit (ta) and column unit (tb) at k-step 0.
+        int pa[4], pb[4];
+        bool va[4], vb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int u = 4 * qr + t * RP + ru, v = 4 * qc + t * CP + cu;
+          va[t] = t < NTA && rowok && ((ym >> u) & 1u) != 0;
+          vb[t] = t < NTB && colok && ((bm >> v) & 1u) != 0;
+          pa[t] = base + ablk * __popc(ym & ((1u << u) - 1u)) + ir + NI * g;
+          pb[t] = bbase + bblk * __popc(bm & ((1u << v) - 1u)) + (QT ? jc + NJ * g : g + nx * jc);
+        }
+        const int KS = (nx + 3) >> 2;
+        const int sa = 4 * NI, sb = QT ? 4 * NJ : 4;
+        for (int s4 = 0; s4 < KS; ++s4) {
+          const bool kv = s4 + 1 < KS || 4 * s4 + g < nx;
+          T av[4], bv[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const bool oka = va[t] && kv, okb = vb[t] && kv;
+            av[t] = cur[oka ? pa[t] + sa * s4 : 0];
+            bv[t] = cur[okb ? pb[t] + sb * s4 : 0];
+            if (!oka) av[t] = (T)0;
+            if (!okb) bv[t] = (T)0;
+          }
+#pragma unroll
+          for (int ta = 0; ta < 4; ++ta) {
+            if (ta >= NTA || (ym & (((1u << RP) - 1u) << (4 * qr + ta * RP))) == 0) continue;
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb) {
+              if (tb >= NTB || (bm & (((1u << CP) - 1u) << (4 * qc + tb * CP))) == 0) continue;
+              const int u = ta * 4 + tb;
+              if constexpr (MODE == 0) {
+                a64[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ta], bv[tb], a64[u], 0, 0, 0);
+              } else if constexpr (MODE == 1) {
+                a32[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ta], bv[tb], a32[u], 0, 0, 0);
+              } else {
+                const flt4 t = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ta], bv[tb], flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) a64[u][e] += (double)t[e];
+              }
+            }
+          }
+        }
+      }
+      if (c + 4 > wb + 63 && c + 1 < c1) {
+        wb = c + 1;
+        cc = min(wb + lane, c1);
+        wcl = chunk_lbeg[cc];
+        wce = chunk_entry[cc];
+      }
+    }
+  }
+  // Element (R, r) of MFMA tile (ta, tb): row R % RH of quadrant block row
+  // ta RP + R / RH, column r % CH of quadrant block column tb CP + r / CH;
+  // written where it lies inside that block's shape.
+  const int64_t *wo = wave_out + ((int64_t)tile * kWaves + w) * 16;
+#pragma unroll
+  for (int ta = 0; ta < 4; ++ta) {
+    if (ta >= NTA) continue;
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb) {
+      if (tb >= NTB) continue;
+      const int u = ta * 4 + tb;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // FP64 MFMA accumulators hold D[g + 4e][r]; FP32 ones (modes 1 and 2,
+        // whose FP64 sums add FP32 results element for element) D[4g + e][r].
+        const int R = MODE == 0 ? g + 4 * e : 4 * g + e;
+        const int bu = ta * RP + (R >> lrh), rowi = R & ((1 << lrh) - 1);
+        const int bv = tb * CP + cu;
+        const int64_t off = wo[bu * 4 + bv];
+        if (off >= 0 && rowi < NI && colok) O[off + rowi + (int64_t)NI * jc] = MODE == 1 ? (T)a32[u][e] : (T)a64[u][e];
+      }
+    }
+  }
+}
+
+// Global tile number of a launch -> the product's arguments and its tile.
+__device__ inline int32_t tile_at(const TileArgs &arg, int32_t t, TileArgs &a) {
+  a = arg;
+  return t;
+}
+__device__ inline int32_t tile_at(const TileGroup &g, int32_t t, TileArgs &a) {
+  int32_t j = 0;
+#pragma unroll
+  for (int k = 1; k < kMaxGroup; ++k) j += (k < g.ngrp && g.a[k].tile_base <= t) ? 1 : 0;
+  a = g.a[j];
+  return t - a.tile_base;
+}
+__device__ inline int32_t total_tiles(const TileArgs &a) { return a.n_tiles; }
+__device__ inline int32_t total_tiles(const TileGroup &g) { return g.total; }
+
+// Persistent launch of class tiles: the grid (a multiple of 8 workgroups, at
+// most one per tile) splits the tiles into 8 contiguous ranges, one per XCD
+// (workgroup b runs on XCD b % 8), and each workgroup takes every (grid/8)-th
+// tile of its XCD's range, so neighbouring tiles (same rows) share an L2.
+// With one workgroup per tile this is the one-tile-per-workgroup order of the
+// other tile kernels.
+template <int MODE, bool QT, int STG, class ARG>
+__global__ void __launch_bounds__(kTileThreads) k_tile_quad(const ARG arg) {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
+  constexpr int STAGE = STG + kLdsPad * (int)(8 / sizeof(T));
+  __shared__ T stage0[STAGE];
+  __shared__ T stage1[STAGE];
+  const int32_t total = total_tiles(arg);
+  const int32_t per = (total + 7) >> 3;
+  const int32_t x = (int32_t)(blockIdx.x & 7), step = (int32_t)(gridDim.x >> 3);
+  const int32_t tend = min(total, (x + 1) * per);
+  for (int32_t tg = x * per + (int32_t)(blockIdx.x >> 3); tg < tend; tg += step) {
+    __syncthreads();  // the previous tile's last chunk is read before this tile's first copies
+    TileArgs a;
+    const int32_t tile = tile_at(arg, tg, a);
+    quad_tile<MODE, QT>(a, tile, stage0, stage1);
+  }
+}
+
 template <int MODE, bool QT, class ARG>
-int launch_kind(int kind, const ARG &arg, int32_t n, hipStream_t s) {
-  const unsigned gridx = 8u * (unsigned)((n + 7) / 8);
+int launch_kind(int kind, const ARG &arg, int32_t n, int32_t wgs, hipStream_t s) {
+  // One workgroup per tile; the class-tile kernel also runs persistent grids
+  // of wgs workgroups (rounded up to a multiple of 8).
+  unsigned gridx = 8u * (unsigned)((n + 7) / 8);
+  if ((kind == kKindQuadSmall || kind == kKindQuadLarge) && wgs > 0)
+    gridx = std::min(gridx, 8u * (unsigned)((wgs + 7) / 8));
   const dim3 blk(kTileThreads);
   switch (kind) {
     case kKindDense4: hipLaunchKernelGGL((k_tile_dense4<MODE, QT, ARG>), dim3(gridx), blk, 0, s, arg); break;
     case kKindDenseMx: hipLaunchKernelGGL((k_tile_dense_mx<MODE, QT, ARG>), dim3(gridx), blk, 0, s, arg); break;
     case kKindMixedSmall: hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgSmall, ARG>), dim3(gridx), blk, 0, s, arg); break;
     case kKindMixedLarge: hipLaunchKernelGGL((k_tile_mixed<MODE, QT, kStgLarge, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindQuadSmall: hipLaunchKernelGGL((k_tile_quad<MODE, QT, kStgSmall, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindQuadLarge: hipLaunchKernelGGL((k_tile_quad<MODE, QT, kStgLarge, ARG>), dim3(gridx), blk, 0, s, arg); break;
     case kKindUniform + 4: hipLaunchKernelGGL((k_tile_products<MODE, QT, 4, ARG>), dim3(gridx), blk, 0, s, arg); break;
     case kKindUniform + 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9, ARG>), dim3(gridx), blk, 0, s, arg); break;
     case kKindUniform + 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13, ARG>), dim3(gridx), blk, 0, s, arg); break;
@@ -1311,18 +1616,33 @@ int launch_kind(int kind, const ARG &arg, int32_t n, hipStream_t s) {
   return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
 }
 
+template <int MODE, bool QT, class ARG>
+int launch_quad(int kind, const ARG &arg, int32_t n, int32_t wgs, hipStream_t s) {
+  unsigned gridx = 8u * (unsigned)((n + 7) / 8);
+  if (wgs > 0) gridx = std::min(gridx, 8u * (unsigned)((wgs + 7) / 8));
+  if (kind == kKindQuadSmall) hipLaunchKernelGGL((k_tile_quad<MODE, QT, kStgSmall, ARG>), dim3(gridx), dim3(kTileThreads), 0, s, arg);
+  else hipLaunchKernelGGL((k_tile_quad<MODE, QT, kStgLarge, ARG>), dim3(gridx), dim3(kTileThreads), 0, s, arg);
+  return cq::hip_check(hipGetLastError(), "class-tile kernel launch");
+}
+
 template <class ARG>
-int launch_any(int mode, bool qt, int kind, const ARG &arg, int32_t n, hipStream_t s) {
+int launch_any(int mode, bool qt, int kind, const ARG &arg, int32_t n, int32_t wgs, hipStream_t s) {
   switch (mode * 2 + (qt ? 1 : 0)) {
-    case 0: return launch_kind<0, false, ARG>(kind, arg, n, s);
-    case 1: return launch_kind<0, true, ARG>(kind, arg, n, s);
-    case 2: return launch_kind<1, false, ARG>(kind, arg, n, s);
-    case 3: return launch_kind<1, true, ARG>(kind, arg, n, s);
-    case 4: return launch_kind<2, false, ARG>(kind, arg, n, s);
-    case 5: return launch_kind<2, true, ARG>(kind, arg, n, s);
+    case 0: return launch_kind<0, false, ARG>(kind, arg, n, wgs, s);
+    case 1: return launch_kind<0, true, ARG>(kind, arg, n, wgs, s);
+    case 2: return launch_kind<1, false, ARG>(kind, arg, n, wgs, s);
+    case 3: return launch_kind<1, true, ARG>(kind, arg, n, wgs, s);
+    case 4: return launch_kind<2, false, ARG>(kind, arg, n, wgs, s);
+    case 5: return launch_kind<2, true, ARG>(kind, arg, n, wgs, s);
     // Mode 3 has its own instance in the uniform-size kernel only.
-    case 6: return kind >= kKindUniform ? launch_kind<3, false, ARG>(kind, arg, n, s) : launch_kind<2, false, ARG>(kind, arg, n, s);
-    case 7: return kind >= kKindUniform ? launch_kind<3, true, ARG>(kind, arg, n, s) : launch_kind<2, true, ARG>(kind, arg, n, s);
+    case 6: return kind >= kKindUniform ? launch_kind<3, false, ARG>(kind, arg, n, wgs, s) : launch_kind<2, false, ARG>(kind, arg, n, wgs, s);
+    case 7: return kind >= kKindUniform ? launch_kind<3, true, ARG>(kind, arg, n, wgs, s) : launch_kind<2, true, ARG>(kind, arg, n, wgs, s);
+    // Split-BF16 x3 (mode 4) runs in the class-tile kernel only.
+    case 8:
+    case 9:
+      if (kind != kKindQuadSmall && kind != kKindQuadLarge)
+        return cq::set_error(CQMM_ERR_STATE, "split-BF16 products need a class-tile schedule");
+      return qt ? launch_quad<4, true, ARG>(kind, arg, n, wgs, s) : launch_quad<4, false, ARG>(kind, arg, n, wgs, s);
     default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: precision mode %d", mode);
   }
 }
@@ -1330,6 +1650,7 @@ int launch_any(int mode, bool qt, int kind, const ARG &arg, int32_t n, hipStream
 int tile_kind(const cq::TilePlan &tp) {
   if (tp.maxn) {
     if (tp.dense) return kKindDenseMx;
+    if (tp.quad) return stage_elems(tp.maxn) == kStgSmall ? kKindQuadSmall : kKindQuadLarge;
     return stage_elems(tp.maxn) == kStgSmall ? kKindMixedSmall : kKindMixedLarge;
   }
   if (tp.dense) return kKindDense4;
@@ -1343,7 +1664,7 @@ namespace cq {
 int launch_tile_products_f32(const TilePlan &tp, bool q_trans, const float *P, const float *Q, float *O, int mode,
                              hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
-  return launch_any(mode, q_trans, tile_kind(tp), tile_launch_args(tp, P, Q, O), tp.n_tiles, s);
+  return launch_any(mode, q_trans, tile_kind(tp), tile_launch_args(tp, P, Q, O), tp.n_tiles, tp.wgs, s);
 }
 
 TileArgs tile_launch_args(const TilePlan &tp, const void *P, const void *Q, void *O) {
@@ -1373,19 +1694,26 @@ TileArgs tile_launch_args(const TilePlan &tp, const void *P, const void *Q, void
   return a;
 }
 
+int tile_family(const TilePlan &tp) {
+  const int k = tile_kind(tp);
+  if (k == kKindQuadSmall || k == kKindQuadLarge) return CQMM_TILES_CLASS;
+  if (k == kKindMixedSmall || k == kKindMixedLarge) return CQMM_TILES_MIXED;
+  return k;  // dense 1, dense mixed 2, uniform 16 + n
+}
+
 int tile_group_key(const TilePlan &tp, bool q_trans, int mode) {
   return (tile_kind(tp) * 2 + (q_trans ? 1 : 0)) * 4 + mode;
 }
 
 int launch_tile_args(const TilePlan &tp, bool q_trans, int mode, const TileArgs &a, hipStream_t s) {
   if (a.n_tiles <= 0) return CQMM_OK;
-  return launch_any(mode, q_trans, tile_kind(tp), a, a.n_tiles, s);
+  return launch_any(mode, q_trans, tile_kind(tp), a, a.n_tiles, tp.wgs, s);
 }
 
 int launch_tile_group(const TilePlan &tp, bool q_trans, int mode, const TileGroup &g, hipStream_t s) {
   if (g.ngrp <= 0 || g.total <= 0) return CQMM_OK;
   if (g.ngrp > kMaxGroup) return set_errorf(CQMM_ERR_ARG, "grouped tile launch: %d products (at most %d)", g.ngrp, kMaxGroup);
-  return launch_any(mode, q_trans, tile_kind(tp), g, g.total, s);
+  return launch_any(mode, q_trans, tile_kind(tp), g, g.total, tp.wgs, s);
 }
 
 // CQMM_TILES=0 builds plans without the tile schedule (triplet kernels only).
@@ -1395,6 +1723,36 @@ bool tiles_enabled() {
     return !(e && std::string(e) == "0");
   }();
   return v;
+}
+
+// Read at every plan build, so that one process can build both schedules.
+bool class_tiles_enabled() {
+  const char *e = getenv("CQMM_MIXED_TILES");
+  return e && std::string(e) == "class";
+}
+
+bool uniform_class_tiles_enabled() {
+  const char *e = getenv("CQMM_UNIFORM_TILES");
+  return e && std::string(e) == "class";
+}
+
+std::vector<int32_t> group_ranks_by_size(const std::vector<int32_t> &rank, const int32_t *size) {
+  const int32_t n = (int32_t)rank.size();
+  std::vector<int32_t> idx(n), out(n);
+  for (int32_t i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+    return size[a] != size[b] ? size[a] < size[b] : rank[a] < rank[b];
+  });
+  int32_t pos = 0, prev = -1;
+  for (int32_t k = 0; k < n; ++k) {
+    const int32_t i = idx[k];
+    if (size[i] != prev) {
+      pos = (pos + 7) & ~7;
+      prev = size[i];
+    }
+    out[i] = pos++;
+  }
+  return out;
 }
 
 std::vector<int32_t> halo_morton_rank(int32_t np_in_halo, const int32_t *nh_part, const int32_t *lab_hcover,
@@ -1547,7 +1905,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
     pt.mark("  tiles: tile_out");
     if ((st = assign_slots(in, tp.tile_out.as<int32_t>(), d_owner, tp.wave_out, s))) return st;
   }
-  const int32_t max_entries = tp.dense ? kDenseEntries : kChunkEntries;
+  const int32_t max_entries = (tp.dense || tp.quad) ? kDenseEntries : kChunkEntries;
   pt.mark("  tiles: owners");
   // Sort the products by (tile, contraction atom).
   unsigned end_bit = 1;
@@ -1691,7 +2049,7 @@ int build_tiles(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
 int launch_tile_products(const TilePlan &tp, bool q_trans, const double *P, const double *Q, double *O,
                          hipStream_t s) {
   if (tp.n_tiles == 0) return CQMM_OK;
-  return launch_any(0, q_trans, tile_kind(tp), tile_launch_args(tp, P, Q, O), tp.n_tiles, s);
+  return launch_any(0, q_trans, tile_kind(tp), tile_launch_args(tp, P, Q, O), tp.n_tiles, tp.wgs, s);
 }
 
 // Mixed block sizes: the uniform builder's steps with element offsets and
@@ -1706,7 +2064,8 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
   tp.n_tiles = in.n_tiles;
   DevBuf d_owner, d_rdim, d_cdim, d_xdim;
   tp.dense = in.maxn <= kDenseNsf;
-  if (tp.dense) {
+  tp.quad = !tp.dense && in.class_uniform;
+  if (tp.dense || tp.quad) {
     // Blocks of at most 4x4: wavefront w computes quadrant (w / 2, w % 2) of
     // the tile (k_tile_dense_mx); wave_out64 holds its blocks' element offsets.
     const size_t ns = (size_t)in.n_tiles * 64;
@@ -1871,8 +2230,9 @@ int build_tiles_mixed(const TileBuildInput &in, TilePlan &tp, hipStream_t s) {
                      d_esz.as<int32_t>(), (int64_t)nch, d_ecoff.as<int32_t>());
   hipLaunchKernelGGL(k_list_lds_mx, dim3(grid1((int64_t)nch, 256)), dim3(256), 0, s, tp.chunk_lbeg.as<int64_t>(),
                      d_lsz.as<uint16_t>(), (int64_t)nch, tp.listls.as<uint32_t>());
-  if (tp.dense) {
-    // The dense kernel reads the entries and the tile block shapes; no product lists.
+  if (tp.dense || tp.quad) {
+    // The dense and class-tile kernels read the entries and the tile block
+    // shapes; no product lists.
     tp.chunk_entry = std::move(d_centry);
     tp.entry_mask = std::move(d_emask);
     tp.entry_coff = std::move(d_ecoff);

@@ -54,6 +54,7 @@ module cqmm_f
      integer(c_int32_t) :: uniform_nsf, mult_type
      integer(c_int32_t) :: kernel, n_tiles
      integer(c_int64_t) :: n_entries, n_loads
+     integer(c_int32_t) :: tile_family, tile_wgs
   end type cqmm_plan_info
 
   ! type(matrix) index of one range, src/matrix_module.f90:63-82 (include/cqmm_matrix.h)

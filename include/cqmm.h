@@ -41,6 +41,11 @@ extern "C" {
 #define CQMM_FP32_ACC64 8u   /* with CQMM_FP32: sum each block's products in FP64 */
 #define CQMM_FP32_RUN64 16u  /* with CQMM_FP32: sum a block's products within one chunk of the
                                 tile schedule (at most 15) in FP32, the chunks in FP64 */
+#define CQMM_BF16X3 32u      /* with CQMM_FP32: split-BF16 x3 products (each operand split into
+                                two BF16 halves, hi.hi + hi.lo + lo.hi in BF16 MFMA with FP32
+                                accumulation per contraction atom, FP64 block sums); needs a
+                                plan with the class-tile schedule (CQMM_UNIFORM_TILES=class or
+                                CQMM_MIXED_TILES=class), CQMM_ERR_STATE otherwise */
 
 /* type(matrix_halo), src/matrix_module.f90:121-136, built by make_halo
  * (src/matrix_elements_module.f90:560-660). */
@@ -156,7 +161,15 @@ typedef struct cqmm_plan_info {
   int32_t n_tiles;             /* output tiles of the tile schedule (0: none)  */
   int64_t n_entries;           /* (tile, contraction atom) entries            */
   int64_t n_loads;             /* operand blocks the tile kernel reads        */
+  int32_t tile_family;         /* tile kernel: CQMM_TILES_* below (0: none)   */
+  int32_t tile_wgs;            /* class tiles: persistent grid (0: one workgroup per tile) */
 } cqmm_plan_info;
+#define CQMM_TILES_DENSE4 1    /* 4x4 blocks, quadrant MFMA tiles            */
+#define CQMM_TILES_DENSE_MIXED 2  /* mixed blocks up to 4x4, quadrant MFMA tiles */
+#define CQMM_TILES_MIXED 3     /* mixed blocks up to 16, per-product lists    */
+#define CQMM_TILES_CLASS 5     /* mixed blocks up to 16 in class tiles (one row and one
+                                  column size per tile), packed MFMA tiles    */
+#define CQMM_TILES_UNIFORM 16  /* uniform blocks: 16 + block size             */
 int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info);
 
 /* FP64 kernel selection (no reference counterpart). For uniform block sizes
@@ -170,6 +183,12 @@ int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info);
 #define CQMM_KERNEL_TRIPLET 1
 #define CQMM_KERNEL_TILE 2
 int cqmm_plan_set_kernel(cqmm_plan *plan, int32_t kernel);
+/* Grid of the class-tile kernel (no reference counterpart): 0 runs one
+ * workgroup per tile; wgs > 0 a persistent grid of wgs workgroups (rounded up
+ * to a multiple of 8, at most one per tile), each taking tiles of its XCD's
+ * contiguous range in turn. Same products and summation order either way.
+ * cqmm_plan_tune picks it for class-tile plans. Ignored by the other kernels. */
+int cqmm_plan_set_tile_workgroups(cqmm_plan *plan, int32_t wgs);
 /* Kernel autotuning on the caller's operands (no reference counterpart: the
  * reference picks its kernel at compile time, MULT_KERN in src/matrix.obj):
  * runs each kernel the plan has (tile schedule if built, per-output-block

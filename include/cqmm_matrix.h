@@ -73,8 +73,20 @@ int cqmm_matrix_device_ptr(const cqmm_ctx *ctx, int32_t handle, double **ptr);
 int cqmm_matrix_set(cqmm_ctx *ctx, int32_t handle, const double *host);
 int cqmm_matrix_get(cqmm_ctx *ctx, int32_t handle, double *host);
 
-/* matrix_product(A, B, C, mult) (src/mult_module.f90:2556-2606). */
+/* matrix_product(A, B, C, mult) (src/mult_module.f90:2556-2606). Runs in the
+ * context's product precision (FP64 unless set below). */
 int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t mult_id);
+/* Precision of cqmm_matrix_product on this context (no reference counterpart:
+ * the reference multiplies in FP64 only). 0: FP64 (default). 1, 2, 3: FP32
+ * copies of both operands and FP32 MFMA, the result widened into the FP64
+ * output; 1 sums each output block in FP32, 2 adds each block product's FP32
+ * result into FP64, 3 sums a block's products within one chunk of the tile
+ * schedule in FP32 and the chunks in FP64; 4 split-BF16 x3 products (as
+ * CQMM_BF16X3; plans need the class-tile schedule). Sums, traces, transposes and the
+ * matrices themselves stay FP64. cqmm_dmm keeps its electron-number
+ * correction and the final K, energy and electron count in FP64. */
+int cqmm_ctx_set_product_precision(cqmm_ctx *ctx, int32_t precision);
+int cqmm_ctx_get_product_precision(const cqmm_ctx *ctx, int32_t *precision);
 /* matrix_sum(alpha, A, beta, B) (src/mult_module.f90:2677-2715): A = alpha*A + beta*B
  * on the blocks common to both patterns (matrix_add / addscan,
  * src/multiply_module.f90:807-943); blocks of A absent from B are untouched. */
@@ -104,7 +116,10 @@ typedef struct cqmm_mcw_setup {
                               then FP64 refinement: 1 sums each block's products in FP64
                               registers, 2 sums them in the FP32 MFMA accumulator, 3 sums
                               them in FP32 within each chunk of the tile schedule (at most
-                              15 products) and the chunks in FP64                     */
+                              15 products) and the chunks in FP64, 4 split-BF16 x3 products
+                              (operands split into two BF16 halves, hi.hi + hi.lo + lo.hi
+                              per contraction atom in FP32, FP64 sums; the plans need the
+                              class-tile schedule)                                    */
   int32_t n_final_fp64;    /* FP64 iterations after the FP32 phase (>= 2)              */
   double fp32_tol;         /* FP32 phase ends when |dE| < fp32_tol (Ha, per spin)      */
 } cqmm_mcw_setup;

@@ -191,3 +191,37 @@ def test_findmindm_periodic_matches_oracle():
     assert abs(res["band_energy"] - ref["energy"]) / pu.natoms < 1e-8
     assert abs(res["n_electrons"] - ref["electrons"]) < 1e-8
     assert rel(res["K"], ref["K"]) < 1e-5
+
+
+@pytest.mark.parametrize("precision,nsf", [(2, 4), (3, 13)])
+def test_mixed_precision_findmindm_matches_fp64_oracle(precision, nsf):
+    """FindMinDM with FP32 bulk products (FP32 MFMA; block sums in FP64 for
+    precision 2, chunk runs in FP32 for 3) and FP64 electron-number correction
+    and final K / 2Tr[KH] / N_e, on a periodic cell, against the oracle's
+    all-FP64 FindMinDM from the same L and T: band energy within 1e-6 Ha/atom,
+    electron count within 1e-8 of the oracle's, K within 1e-5."""
+    natoms = 24 if nsf == 4 else 12
+    pu = periodic(natoms=natoms, nsf=nsf)
+    print(f"mixed findmindm (precision {precision}) vs oracle: {natoms} atoms x {nsf} SF", flush=True)
+    ne = float(pu.n_orb // 2)
+    pu.hotelling()
+    pu.init_mcweeny(ne)
+    pu.mcweeny(precision=0, max_iter=100)
+    L0 = pu.ctx.get(pu.matL)
+    T = pu.ctx.get(pu.matT)
+    res = pu.dmm(product_precision=precision, n_electrons=ne, tolerance=1e-7, n_L_iterations=100)
+    assert pu.ctx.product_precision == 0
+    sp = mo.Sparse(pu.sys, pu.rid)
+    d = od.Dmm(sp, pu.S_vals, pu.H_vals, T, ne, pu.natoms, tolerance=1e-7, n_L_iterations=100)
+    ref = d.run(L0)
+    dE = abs(res["band_energy"] - ref["energy"]) / pu.natoms
+    dN = abs(res["n_electrons"] - ref["electrons"])
+    print(f"  device mixed: early {res['early_iterations']} late {res['late_iterations']} residual "
+          f"{res['residual']:.3e} E {res['band_energy']:.12f} Ne {res['n_electrons']:.12f}\n"
+          f"  oracle FP64:  early {ref['early_iterations']} late {ref['late_iterations']} residual "
+          f"{ref['residual']:.3e} E {ref['energy']:.12f} Ne {ref['electrons']:.12f}; dE/atom {dE:.2e} dNe {dN:.2e}",
+          flush=True)
+    assert res["done"] == 1 and ref["done"]
+    assert dE <= 1e-6
+    assert dN <= 1e-8
+    assert rel(res["K"], ref["K"]) < 1e-5

@@ -368,3 +368,177 @@ def test_cfg2_full_size_properties():
     a2 = np.full(n_out, np.nan)
     plan.mat_mult(a2, b, 2.0 * c)
     np.testing.assert_array_equal(a2, 2.0 * out[cqmm.KERNEL_TILE])
+
+
+# Species mixes for the class-tile kernel: every row/column packing (blocks up
+# to 4 pack four to an MFMA tile, 5-8 two, 9-16 one), both product types,
+# rectangular blocks, a 16x16 species and an isolated species (one atom).
+CLASS_CASES = [
+    (64, ((4, .5), (9, .25), (13, .25)), 6.0, 8.0, 12.0, 1),
+    (64, ((4, .5), (9, .25), (13, .25)), 8.0, 8.0, 6.0, 2),
+    (48, ((8, .5), (13, .5)), 6.0, 7.0, 10.0, 1),
+    (48, ((5, .3), (16, .3), (2, .4)), 6.0, 7.0, 9.0, 1),
+    (40, ((6, .5), (9, .5)), 7.0, 8.0, 6.0, 2),
+    (40, ((1, .45), (4, .45), (13, .1)), 6.0, 8.0, 12.0, 1),
+]
+
+
+@pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", CLASS_CASES)
+def test_class_tiles_match_oracle_general_tiles_and_persistent_grids(natoms, nsf, ra, rb, rc, mt, monkeypatch):
+    """Class tiles (tile rows and columns grouped by block size, packed MFMA
+    tiles, entry lists) against the oracle (1e-12 relative) and against the
+    general mixed-size schedule (1e-13); persistent grids of 8 to 1024
+    workgroups give the one-workgroup-per-tile result bit for bit; FP32 modes
+    within their bars of the oracle."""
+    case = Case(natoms, nsf, ra, rb, rc, mt, seed=47)
+    print(f"class tiles: natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    ref = c_r if mt == 1 else a_r
+    out_of = lambda x, z: z if mt == 1 else x
+    monkeypatch.setenv("CQMM_MIXED_TILES", "class")
+    plan = cqmm.Plan(case.desc)
+    info = plan.info()
+    print(f"  products {info.n_triplets}, tiles {info.n_tiles}, family {info.tile_family}", flush=True)
+    assert info.n_tiles > 0 and info.tile_family == cqmm.TILES_CLASS and info.kernel == cqmm.KERNEL_TILE
+    x, y, z = a.copy(), b.copy(), c.copy()
+    out_of(x, z)[:] = np.nan
+    plan.mat_mult(x, y, z)
+    got = out_of(x, z).copy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"  class tiles vs oracle: rel err {err:.3e}", flush=True)
+    assert np.all(np.isfinite(got)) and err <= TOL
+    assert np.all(got[ref == 0.0] == 0.0)
+    for wgs in (8, 64, 1024):
+        plan.set_tile_workgroups(wgs)
+        x, y, z = a.copy(), b.copy(), c.copy()
+        out_of(x, z)[:] = np.nan
+        plan.mat_mult(x, y, z)
+        np.testing.assert_array_equal(out_of(x, z), got)
+    plan.set_tile_workgroups(0)
+    for flags, bar in ((cqmm.CQMM_FP32, 2e-5), (cqmm.CQMM_FP32 | cqmm.CQMM_FP32_ACC64, 2e-6)):
+        x, y, z = a.copy(), b.copy(), c.copy()
+        out_of(x, z)[:] = np.nan
+        plan.mat_mult(x, y, z, flags)
+        e32 = np.abs(out_of(x, z) - ref).max() / np.abs(ref).max()
+        print(f"  flags {flags}: rel err {e32:.3e}", flush=True)
+        assert e32 <= bar
+    plan.close()
+    monkeypatch.setenv("CQMM_MIXED_TILES", "general")
+    gen = cqmm.Plan(case.desc)
+    assert gen.info().tile_family in (cqmm.TILES_MIXED, cqmm.TILES_DENSE_MIXED)
+    x, y, z = a.copy(), b.copy(), c.copy()
+    gen.mat_mult(x, y, z)
+    d = np.abs(out_of(x, z) - got).max() / np.abs(ref).max()
+    print(f"  class vs general mixed tiles: rel diff {d:.3e}", flush=True)
+    assert d <= 1e-13
+    gen.close()
+
+
+def test_cfg3_full_size_class_tiles_properties():
+    """The batched uneven-blocks workload at full size (8192 atoms, 50% 4 /
+    25% 9 / 25% 13 SF, C(16 A) = A(8 A).B(12 A), one launch over all block
+    shapes), checked through size-independent properties: class tiles
+    (CQMM_MIXED_TILES=class), the general mixed-size tiles (the default) and
+    the per-output-block kernel agree to 1e-13
+    relative; a persistent grid gives the same result bit for bit; doubling an
+    operand doubles the result bit for bit."""
+    import os
+    nsf = ((4, .5), (9, .25), (13, .25))
+    xyz, edge = cqmm.random_structure(8192, 0.05, 2.0, 1)
+    rng = np.random.default_rng(1 + 7919)
+    sp = np.array([4, 9, 13], np.int32)[rng.choice(3, size=8192, p=[.5, .25, .25])]
+    print("cfg3 full size: building the system", flush=True)
+    sysm = cqmm.System(xyz, edge, sp, 20.0, cqmm.default_grid(edge, 4.0))
+    ids = {r: sysm.add_range(r) for r in (8.0, 12.0, 16.0)}
+    desc = sysm.mult_desc(1, ids[8.0], ids[12.0], ids[16.0])
+    a = sysm.fill(ids[8.0], 0, 1, 1, (2.0,))
+    b = sysm.fill(ids[12.0], 0, 1, 2, (2.0,))
+    n_out = sysm.length(ids[16.0])
+    os.environ["CQMM_MIXED_TILES"] = "class"
+    try:
+        plan = cqmm.Plan(desc)
+    finally:
+        del os.environ["CQMM_MIXED_TILES"]
+    info = plan.info()
+    print(f"  products {info.n_triplets}, tiles {info.n_tiles}, family {info.tile_family}", flush=True)
+    assert info.tile_family == cqmm.TILES_CLASS
+    out = {}
+    for kernel in (cqmm.KERNEL_TILE, cqmm.KERNEL_TRIPLET):
+        plan.set_kernel(kernel)
+        c = np.full(n_out, np.nan)
+        plan.mat_mult(a, b, c)
+        assert np.isfinite(c).all()
+        out[kernel] = c
+    ref = out[cqmm.KERNEL_TRIPLET]
+    err = np.abs(out[cqmm.KERNEL_TILE] - ref).max() / np.abs(ref).max()
+    print(f"  class tiles vs per-block kernel: rel err {err:.3e}", flush=True)
+    assert err <= 1e-13
+    plan.set_kernel(cqmm.KERNEL_TILE)
+    plan.set_tile_workgroups(768)
+    c = np.full(n_out, np.nan)
+    plan.mat_mult(a, b, c)
+    np.testing.assert_array_equal(c, out[cqmm.KERNEL_TILE])
+    c2 = np.full(n_out, np.nan)
+    plan.mat_mult(2.0 * a, b, c2)
+    np.testing.assert_array_equal(c2, 2.0 * out[cqmm.KERNEL_TILE])
+    plan.close()
+    gen = cqmm.Plan(desc)
+    assert gen.info().tile_family == cqmm.TILES_MIXED
+    c = np.full(n_out, np.nan)
+    gen.mat_mult(a, b, c)
+    err = np.abs(c - ref).max() / np.abs(ref).max()
+    print(f"  general mixed tiles vs per-block kernel: rel err {err:.3e}", flush=True)
+    assert err <= 1e-13
+    gen.close()
+
+
+@pytest.mark.parametrize("natoms,nsf,ra,rb,rc,mt", [(48, 13, 6.0, 8.0, 8.0, 2), (40, 9, 6.0, 8.0, 12.0, 1),
+                                                    (48, ((4, .5), (9, .25), (13, .25)), 6.0, 8.0, 12.0, 1)])
+def test_class_tiles_uniform_blocks_and_split_bf16(natoms, nsf, ra, rb, rc, mt, monkeypatch):
+    """Uniform 9 and 13 blocks run as one class of the class-tile schedule
+    (CQMM_UNIFORM_TILES=class): FP64 against the oracle (1e-12) and against the
+    fixed-size tile kernel (1e-13). Split-BF16 x3 products (CQMM_BF16X3: each
+    operand rounded to BF16 plus its BF16 remainder, hi.hi + hi.lo + lo.hi per
+    contraction atom in FP32, FP64 sums) against the oracle: relative error
+    bounded by 5e-5 (16 significant bits per operand; the dropped lo.lo term is
+    2^-16 of a product); a plan without class tiles refuses the mode."""
+    case = Case(natoms, nsf, ra, rb, rc, mt, seed=53)
+    print(f"class uniform / bf16x3: natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
+    a, b, c = case.values()
+    a_r, b_r, c_r = a.copy(), b.copy(), c.copy()
+    oracle.mat_mult(case.desc, a_r, b_r, c_r)
+    ref = c_r if mt == 1 else a_r
+    out_of = lambda x, z: z if mt == 1 else x
+    fixed = cqmm.Plan(case.desc)
+    x, y, z = a.copy(), b.copy(), c.copy()
+    fixed.mat_mult(x, y, z)
+    got_fixed = out_of(x, z).copy()
+    if fixed.info().tile_family != cqmm.TILES_CLASS:
+        with pytest.raises(cqmm.CqmmError, match="class-tile"):
+            fixed.mat_mult(a.copy(), b.copy(), c.copy(), cqmm.CQMM_FP32 | cqmm.CQMM_BF16X3)
+    fixed.close()
+    monkeypatch.setenv("CQMM_UNIFORM_TILES", "class")
+    monkeypatch.setenv("CQMM_MIXED_TILES", "class")
+    plan = cqmm.Plan(case.desc)
+    info = plan.info()
+    assert info.tile_family == cqmm.TILES_CLASS
+    x, y, z = a.copy(), b.copy(), c.copy()
+    out_of(x, z)[:] = np.nan
+    plan.mat_mult(x, y, z)
+    got = out_of(x, z).copy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    d = np.abs(got - got_fixed).max() / np.abs(ref).max()
+    print(f"  class tiles vs oracle {err:.3e}, vs default tile kernel {d:.3e}", flush=True)
+    assert err <= TOL and d <= 1e-13
+    x, y, z = a.copy(), b.copy(), c.copy()
+    out_of(x, z)[:] = np.nan
+    plan.mat_mult(x, y, z, cqmm.CQMM_FP32 | cqmm.CQMM_BF16X3)
+    got16 = out_of(x, z)
+    e16 = np.abs(got16 - ref).max() / np.abs(ref).max()
+    f16 = np.linalg.norm(got16 - ref) / np.linalg.norm(ref)
+    print(f"  split-BF16 x3 vs oracle: max rel {e16:.3e}, Frobenius rel {f16:.3e}", flush=True)
+    assert np.all(np.isfinite(got16)) and e16 <= 5e-5
+    assert np.all(got16[ref == 0.0] == 0.0)
+    plan.close()
