@@ -152,7 +152,8 @@ class InfoGlobal(C.Structure):
 EXPORTS = [
     "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
     "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_device_copy", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
-    "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_plan_get_info",
+    "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_mat_mult_fp32_noalloc",
+    "cqmm_noalloc_scratch_bytes", "cqmm_plan_get_info",
     "cqmm_plan_get_triplets", "cqmm_plan_set_kernel", "cqmm_plan_set_tile_workgroups", "cqmm_plan_tune",
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
     "cqmm_sys_get_info", "cqmm_sys_mult_desc", "cqmm_sys_blocks", "cqmm_sys_primary_order",
@@ -217,6 +218,9 @@ def lib():
                                  C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.cqmm_mat_mult.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                 C.c_int64, C.c_uint32]
+    L.cqmm_mat_mult_fp32_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                C.c_int64, C.c_uint32]
+    L.cqmm_noalloc_scratch_bytes.restype = C.c_int64
     L.cqmm_device_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_int64]
     L.cqmm_device_free.argtypes = [C.c_void_p]
     L.cqmm_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
@@ -570,3 +574,9 @@ class Plan:
         """Device pointers (e.g. torch tensors' data_ptr())."""
         check(lib().cqmm_mat_mult(self.h, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags | CQMM_DEVICE_PTRS),
               "cqmm_mat_mult")
+
+    def mat_mult_fp32_noalloc_dev(self, a_ptr, lena, b_ptr, lenb, c_ptr, lenc, flags=0):
+        """cqmm_mat_mult's FP32 products on device pointers with the FP32 copies
+        in the library's static scratch array (no device allocation)."""
+        check(lib().cqmm_mat_mult_fp32_noalloc(self.h, a_ptr, lena, b_ptr, lenb, c_ptr, lenc,
+                                               flags | CQMM_DEVICE_PTRS), "cqmm_mat_mult_fp32_noalloc")

@@ -17,6 +17,9 @@ struct Runtime {
 
 Runtime &runtime();
 int ensure_init();
+// Device address of the no-allocation scratch array on `device` (the device
+// current), looked up once at cqmm_init (cq_noalloc.hip).
+int noalloc_setup(int device);
 // Stream for the calling thread's library work: the thread's own stream when
 // one was set with set_thread_stream, else the library stream.
 hipStream_t stream();

@@ -72,6 +72,8 @@ int cqmm_init(int device) {
     if (dev >= n) return cq::set_errorf(CQMM_ERR_ARG, "cqmm_init: device %d of %d", dev, n);
     st = cq::hip_check(hipSetDevice(dev), "hipSetDevice");
     if (st) return st;
+    st = cq::noalloc_setup(dev);
+    if (st) return st;
     // Default-priority stream of the library (no priority other than the default).
     st = cq::hip_check(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking), "hipStreamCreate");
     if (st) return st;

@@ -149,6 +149,21 @@ int cqmm_plan_destroy(cqmm_plan *plan);
 int cqmm_mat_mult(cqmm_plan *plan, double *a, int64_t lena, const double *b,
                   int64_t lenb, double *c, int64_t lenc, uint32_t flags);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP stream capture, serving loops that forbid hipMalloc).
+ * cqmm_mat_mult with CQMM_FP32 (and CQMM_FP32_ACC64 / CQMM_FP32_RUN64 as
+ * given) on device pointers (CQMM_DEVICE_PTRS required), the same kernels and
+ * results, but the FP32 operand and result copies live in one static device
+ * array of cqmm_noalloc_scratch_bytes() (64 MiB) per device instead of
+ * buffers the plan allocates: a product whose FP32 copies (lengths of both
+ * operands and the result, each rounded up to 64 elements) exceed it is
+ * refused with CQMM_ERR_ARG. Calls no hipMalloc/hipFree. cqmm_init must have
+ * run (CQMM_ERR_STATE otherwise). Every call on a device shares that device's
+ * array: calls that overlap on one device (other threads or streams) race. */
+int cqmm_mat_mult_fp32_noalloc(cqmm_plan *plan, double *a, int64_t lena, const double *b,
+                               int64_t lenb, double *c, int64_t lenc, uint32_t flags);
+int64_t cqmm_noalloc_scratch_bytes(void);
+
 /* Plan introspection (no reference counterpart; used for checks and reports). */
 typedef struct cqmm_plan_info {
   int64_t n_triplets;          /* contributing (i,k,j) block products         */
