@@ -386,15 +386,17 @@ int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t 
     if ((st = get_scratch(ctx, "prodP", tp.len_p * 4, &fp)) || (st = get_scratch(ctx, "prodQ", tp.len_q * 4, &fq)) ||
         (st = get_scratch(ctx, "prodO", tp.len_out * 4, &fo)))
       return st;
-    if (ctx->product_precision == 4) {
+    // Split-BF16 x3 where the plan's kernel has it, else FP32 MFMA with FP64 sums.
+    const int mode = (ctx->product_precision == 4 && !(cq::uses_tiles(tp) && cq::tiles_take_bf16x3(tp.tiles)))
+                         ? 2 : ctx->product_precision;
+    if (mode == 4) {
       if ((st = cq::op_to_bf16x2(tp.len_p, dptr(ctx, A), (uint32_t *)fp, s))) return st;
       if ((st = cq::op_to_bf16x2(tp.len_q, dptr(ctx, B), (uint32_t *)fq, s))) return st;
     } else {
       if ((st = cq::op_to_f32(tp.len_p, dptr(ctx, A), (float *)fp, s))) return st;
       if ((st = cq::op_to_f32(tp.len_q, dptr(ctx, B), (float *)fq, s))) return st;
     }
-    if ((st = cq::launch_block_products_f32(tp, (const float *)fp, (const float *)fq, (float *)fo,
-                                            ctx->product_precision, s)))
+    if ((st = cq::launch_block_products_f32(tp, (const float *)fp, (const float *)fq, (float *)fo, mode, s)))
       return st;
     return cq::op_to_f64(tp.len_out, (const float *)fo, dptr(ctx, C), s);
   });

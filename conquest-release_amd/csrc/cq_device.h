@@ -207,6 +207,9 @@ int launch_tile_args(const TilePlan &tp, bool q_trans, int mode, const TileArgs 
 bool uses_tiles(const TripletPlan &tp);
 // Tile kernel family of a schedule (CQMM_TILES_* of cqmm_plan_info).
 int tile_family(const TilePlan &tp);
+// True when the schedule's kernel has the split-BF16 x3 mode (class tiles,
+// uniform 9, 13, 16).
+bool tiles_take_bf16x3(const TilePlan &tp);
 // Launch the FP64 (or FP32) block-product kernel over a plan.
 int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O,
                           bool fp32, hipStream_t stream);

@@ -80,6 +80,7 @@ struct McwJob {
   int32_t oldL = 0, rho = 0, LS = 0, LSL = 0, LSLSL = 0, top = 0, bottom = 0, botH = 0;
   float *L32 = nullptr, *S32 = nullptr, *LS32 = nullptr, *LSL32 = nullptr, *LSLSL32 = nullptr;
   bool mixed = false, fp32_phase = false, done = false;
+  int mode = 0;  // FP32 product mode of the reduced-precision phase (1-4)
   int fp64_done = 0, n = 0, stop = 4, max_iter = 50, nfinal = 2, status = 0;
   double oldE = 1.0e30, lastE = 0.0, lastNe = 0.0, lastC = 0.0, t0 = 0.0;
   int slot = -1;
@@ -152,6 +153,16 @@ int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S
   MTRY(t.alloc(su.range_S, &j->bottom));
   MTRY(t.alloc(su.range_H, &j->botH));
   j->mixed = su.precision != 0;
+  // Product mode of the reduced-precision phase: precision 1 -> FP32 MFMA with
+  // FP64 sums, 2 -> FP32 sums, 3 -> FP32 chunk runs, 4 -> split-BF16 x3 where
+  // all three products' kernels have it (blocks of 9 to 16 or class tiles),
+  // else FP32 MFMA with FP64 sums (e.g. 4x4 blocks).
+  j->mode = su.precision == 1 ? 2 : su.precision == 2 ? 1 : su.precision;
+  if (j->mode == 4)
+    for (int32_t m : {su.mult_L_S_LS, su.mult_LS_L_LSL, su.mult_LSL_SL_L}) {
+      const TripletPlan &tp = ctx->mults.at(m).plan->tp;
+      if (!uses_tiles(tp) || !tiles_take_bf16x3(tp.tiles)) j->mode = 2;
+    }
   j->fp32_phase = j->mixed;
   j->max_iter = su.max_iter > 0 ? su.max_iter : 50;
   j->nfinal = su.n_final_fp64 > 0 ? su.n_final_fp64 : 2;
@@ -169,7 +180,7 @@ int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S
     j->LSL32 = (float *)p;
     MTRY(get_scratch(ctx, "LSLSL32" + k, hlen(ctx, j->LSLSL) * 4, &p));
     j->LSLSL32 = (float *)p;
-    if (su.precision == 4) MTRY(op_to_bf16x2(hlen(ctx, S), dptr(ctx, S), (uint32_t *)j->S32, s_));
+    if (j->mode == 4) MTRY(op_to_bf16x2(hlen(ctx, S), dptr(ctx, S), (uint32_t *)j->S32, s_));
     else MTRY(op_to_f32(hlen(ctx, S), dptr(ctx, S), j->S32, s_));
   }
   // Range maps of every sum of the step, built now (not during a capture).
@@ -232,7 +243,7 @@ int McwEngine::products(int which) {
       P = A32;
       Q = B32;
       O = C32;
-      mode = j.su.precision == 1 ? 2 : j.su.precision == 3 ? 3 : j.su.precision == 4 ? 4 : 1;
+      mode = j.mode;
     } else {
       P = dptr(ctx, A);
       Q = dptr(ctx, B);
@@ -279,7 +290,7 @@ int McwEngine::enqueue_step() {
   for (const auto &jp : jobs_) {
     McwJob &j = *jp;
     if (!j.fp32_phase) continue;
-    if (j.su.precision == 4) MTRY(op_to_bf16x2(hlen(j.ctx, j.L), dptr(j.ctx, j.L), (uint32_t *)j.L32, s_));
+    if (j.mode == 4) MTRY(op_to_bf16x2(hlen(j.ctx, j.L), dptr(j.ctx, j.L), (uint32_t *)j.L32, s_));
     else MTRY(op_to_f32(hlen(j.ctx, j.L), dptr(j.ctx, j.L), j.L32, s_));
   }
   // Split-BF16 systems take LS and LSL as operands in split-BF16 words: each
@@ -288,7 +299,7 @@ int McwEngine::enqueue_step() {
   auto split_bf16 = [&](int which) -> int {
     for (const auto &jp : jobs_) {
       McwJob &j = *jp;
-      if (!j.fp32_phase || j.su.precision != 4) continue;
+      if (!j.fp32_phase || j.mode != 4) continue;
       if (which == 0) MTRY(op_f32_to_bf16x2(hlen(j.ctx, j.LS), (uint32_t *)j.LS32, s_));
       if (which == 1) {
         MTRY(op_to_f64(hlen(j.ctx, j.LSL), j.LSL32, dptr(j.ctx, j.LSL), s_));
@@ -306,7 +317,7 @@ int McwEngine::enqueue_step() {
     cqmm_ctx *ctx = j.ctx;
     double *sc = dsc(j);
     if (j.fp32_phase) {
-      if (j.su.precision != 4) MTRY(op_to_f64(hlen(ctx, j.LSL), j.LSL32, dptr(ctx, j.LSL), s_));
+      if (j.mode != 4) MTRY(op_to_f64(hlen(ctx, j.LSL), j.LSL32, dptr(ctx, j.LSL), s_));
       MTRY(op_to_f64(hlen(ctx, j.LSLSL), j.LSLSL32, dptr(ctx, j.LSLSL), s_));
     }
     // c1 = Tr[S (LSL - LSLSL)] and c2 = Tr[S (L - LSL)] on the S range.

@@ -363,7 +363,7 @@ int launch_block_products_f32(const TripletPlan &tp, const float *P, const float
   if (mode < 1 || mode > 4) return set_errorf(CQMM_ERR_ARG, "block products: FP32 mode %d", mode);
   if (uses_tiles(tp))
     return launch_tile_products_f32(tp.tiles, tp.q_trans, P, Q, O, mode, s);
-  if (mode == 4) return set_error(CQMM_ERR_STATE, "split-BF16 products need the class-tile schedule");
+  if (mode == 4) return set_error(CQMM_ERR_STATE, "split-BF16 products need a tile schedule of blocks 9 to 16");
   const bool acc64 = mode != 1;
   if (acc64) {
     if (tp.off64) return tp.q_trans ? launch_f32_t<int64_t, true, true>(tp, P, Q, O, s) : launch_f32_t<int64_t, false, true>(tp, P, Q, O, s);

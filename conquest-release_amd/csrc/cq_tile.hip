@@ -280,6 +280,34 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_products(
         flt4 run = {0.f, 0.f, 0.f, 0.f};  // MODE 3: this block's products of the chunk
         for (int i = 0; i < n; ++i, ++p) {
           const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)plc, p);
+          if constexpr (MODE == 4) {
+            // Split-BF16 x3: the whole contraction (NSF <= 16) in one K = 16
+            // step; lane (r, g) holds contraction indices 4g..4g+3 as four
+            // LDS words (bits 0-15 hi, 16-31 lo; indices >= NSF zeroed),
+            // repacked into the hi and lo operands of v_mfma_f32_16x16x16_bf16.
+            // hi.hi + hi.lo + lo.hi chain in FP32; each product's sum is
+            // added into the FP64 accumulator (D[4g + e][r]).
+            const T *la = cur + (int)(v & 0xffffu) + r + NSF * 4 * g;
+            const T *lq = cur + (int)(v >> 16) + (QT ? r + NSF * 4 * g : 4 * g + NSF * r);
+            uint32_t wa[4], wb[4];
+#pragma unroll
+            for (int i4 = 0; i4 < 4; ++i4) {
+              wa[i4] = __float_as_uint(la[NSF * i4]);
+              wb[i4] = __float_as_uint(lq[QT ? NSF * i4 : i4]);
+              if (NSF < 16 && 4 * g + i4 >= NSF) {
+                wa[i4] = 0u;
+                wb[i4] = 0u;
+              }
+            }
+            const s16x4 ahi = pack_bf16_lo(wa), alo = pack_bf16_hi(wa);
+            const s16x4 bhi = pack_bf16_lo(wb), blo = pack_bf16_hi(wb);
+            flt4 t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, flt4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, t, 0, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) a64[u][e] += (double)t[e];
+            continue;
+          }
           if constexpr (REM == 1) {
             // NSF = 4m + 1: the last contraction index k = NSF - 1 is a
             // rank-1 update on the VALU, D[row][r] += A[row][k] * B[k][r]
@@ -1617,6 +1645,19 @@ int launch_kind(int kind, const ARG &arg, int32_t n, int32_t wgs, hipStream_t s)
 }
 
 template <int MODE, bool QT, class ARG>
+int launch_uniform(int kind, const ARG &arg, int32_t n, hipStream_t s) {
+  const unsigned gridx = 8u * (unsigned)((n + 7) / 8);
+  const dim3 blk(kTileThreads);
+  switch (kind) {
+    case kKindUniform + 9: hipLaunchKernelGGL((k_tile_products<MODE, QT, 9, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindUniform + 13: hipLaunchKernelGGL((k_tile_products<MODE, QT, 13, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    case kKindUniform + 16: hipLaunchKernelGGL((k_tile_products<MODE, QT, 16, ARG>), dim3(gridx), blk, 0, s, arg); break;
+    default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: kernel kind %d has no instance", kind);
+  }
+  return cq::hip_check(hipGetLastError(), "tile-product kernel launch");
+}
+
+template <int MODE, bool QT, class ARG>
 int launch_quad(int kind, const ARG &arg, int32_t n, int32_t wgs, hipStream_t s) {
   unsigned gridx = 8u * (unsigned)((n + 7) / 8);
   if (wgs > 0) gridx = std::min(gridx, 8u * (unsigned)((wgs + 7) / 8));
@@ -1637,12 +1678,15 @@ int launch_any(int mode, bool qt, int kind, const ARG &arg, int32_t n, int32_t w
     // Mode 3 has its own instance in the uniform-size kernel only.
     case 6: return kind >= kKindUniform ? launch_kind<3, false, ARG>(kind, arg, n, wgs, s) : launch_kind<2, false, ARG>(kind, arg, n, wgs, s);
     case 7: return kind >= kKindUniform ? launch_kind<3, true, ARG>(kind, arg, n, wgs, s) : launch_kind<2, true, ARG>(kind, arg, n, wgs, s);
-    // Split-BF16 x3 (mode 4) runs in the class-tile kernel only.
+    // Split-BF16 x3 (mode 4) runs in the class-tile kernel and the fixed-size
+    // kernels of 9x9, 13x13 and 16x16 blocks.
     case 8:
     case 9:
-      if (kind != kKindQuadSmall && kind != kKindQuadLarge)
-        return cq::set_error(CQMM_ERR_STATE, "split-BF16 products need a class-tile schedule");
-      return qt ? launch_quad<4, true, ARG>(kind, arg, n, wgs, s) : launch_quad<4, false, ARG>(kind, arg, n, wgs, s);
+      if (kind == kKindQuadSmall || kind == kKindQuadLarge)
+        return qt ? launch_quad<4, true, ARG>(kind, arg, n, wgs, s) : launch_quad<4, false, ARG>(kind, arg, n, wgs, s);
+      if (kind == kKindUniform + 9 || kind == kKindUniform + 13 || kind == kKindUniform + 16)
+        return qt ? launch_uniform<4, true, ARG>(kind, arg, n, s) : launch_uniform<4, false, ARG>(kind, arg, n, s);
+      return cq::set_error(CQMM_ERR_STATE, "split-BF16 products need blocks of 9 to 16 or a class-tile schedule");
     default: return cq::set_errorf(CQMM_ERR_ARG, "tile products: precision mode %d", mode);
   }
 }
@@ -1692,6 +1736,13 @@ TileArgs tile_launch_args(const TilePlan &tp, const void *P, const void *Q, void
   a.n_tiles = tp.n_tiles;
   a.tile_base = 0;
   return a;
+}
+
+bool tiles_take_bf16x3(const TilePlan &tp) {
+  if (tp.n_tiles == 0) return false;
+  const int k = tile_kind(tp);
+  return k == kKindQuadSmall || k == kKindQuadLarge || k == kKindUniform + 9 || k == kKindUniform + 13 ||
+         k == kKindUniform + 16;
 }
 
 int tile_family(const TilePlan &tp) {

@@ -43,9 +43,9 @@ extern "C" {
                                 tile schedule (at most 15) in FP32, the chunks in FP64 */
 #define CQMM_BF16X3 32u      /* with CQMM_FP32: split-BF16 x3 products (each operand split into
                                 two BF16 halves, hi.hi + hi.lo + lo.hi in BF16 MFMA with FP32
-                                accumulation per contraction atom, FP64 block sums); needs a
-                                plan with the class-tile schedule (CQMM_UNIFORM_TILES=class or
-                                CQMM_MIXED_TILES=class), CQMM_ERR_STATE otherwise */
+                                accumulation per block product, FP64 block sums); needs a tile
+                                schedule of 9x9 to 16x16 blocks or class tiles
+                                (CQMM_MIXED_TILES=class), CQMM_ERR_STATE otherwise */
 
 /* type(matrix_halo), src/matrix_module.f90:121-136, built by make_halo
  * (src/matrix_elements_module.f90:560-660). */

@@ -82,7 +82,7 @@ int cqmm_matrix_product(cqmm_ctx *ctx, int32_t A, int32_t B, int32_t C, int32_t 
  * output; 1 sums each output block in FP32, 2 adds each block product's FP32
  * result into FP64, 3 sums a block's products within one chunk of the tile
  * schedule in FP32 and the chunks in FP64; 4 split-BF16 x3 products (as
- * CQMM_BF16X3; plans need the class-tile schedule). Sums, traces, transposes and the
+ * CQMM_BF16X3) where the plan's kernel has them, else mode 2. Sums, traces, transposes and the
  * matrices themselves stay FP64. cqmm_dmm keeps its electron-number
  * correction and the final K, energy and electron count in FP64. */
 int cqmm_ctx_set_product_precision(cqmm_ctx *ctx, int32_t precision);
@@ -118,8 +118,8 @@ typedef struct cqmm_mcw_setup {
                               them in FP32 within each chunk of the tile schedule (at most
                               15 products) and the chunks in FP64, 4 split-BF16 x3 products
                               (operands split into two BF16 halves, hi.hi + hi.lo + lo.hi
-                              per contraction atom in FP32, FP64 sums; the plans need the
-                              class-tile schedule)                                    */
+                              per block product in FP32, FP64 sums) where all three plans'
+                              kernels have them (blocks 9 to 16 or class tiles), else 1 */
   int32_t n_final_fp64;    /* FP64 iterations after the FP32 phase (>= 2)              */
   double fp32_tol;         /* FP32 phase ends when |dE| < fp32_tol (Ha, per spin)      */
 } cqmm_mcw_setup;

@@ -501,9 +501,10 @@ def test_class_tiles_uniform_blocks_and_split_bf16(natoms, nsf, ra, rb, rc, mt, 
     (CQMM_UNIFORM_TILES=class): FP64 against the oracle (1e-12) and against the
     fixed-size tile kernel (1e-13). Split-BF16 x3 products (CQMM_BF16X3: each
     operand rounded to BF16 plus its BF16 remainder, hi.hi + hi.lo + lo.hi per
-    contraction atom in FP32, FP64 sums) against the oracle: relative error
-    bounded by 5e-5 (16 significant bits per operand; the dropped lo.lo term is
-    2^-16 of a product); a plan without class tiles refuses the mode."""
+    block product in FP32, FP64 sums) in the class-tile kernel and in the
+    fixed-size kernel of 9x9 and 13x13 blocks against the oracle: relative
+    error bounded by 5e-5 (16 significant bits per operand; the dropped lo.lo
+    term is 2^-16 of a product); the per-product mixed-size kernel refuses it."""
     case = Case(natoms, nsf, ra, rb, rc, mt, seed=53)
     print(f"class uniform / bf16x3: natoms={natoms} nsf={nsf} ranges={ra},{rb},{rc} type={mt}", flush=True)
     a, b, c = case.values()
@@ -515,9 +516,19 @@ def test_class_tiles_uniform_blocks_and_split_bf16(natoms, nsf, ra, rb, rc, mt, 
     x, y, z = a.copy(), b.copy(), c.copy()
     fixed.mat_mult(x, y, z)
     got_fixed = out_of(x, z).copy()
-    if fixed.info().tile_family != cqmm.TILES_CLASS:
-        with pytest.raises(cqmm.CqmmError, match="class-tile"):
-            fixed.mat_mult(a.copy(), b.copy(), c.copy(), cqmm.CQMM_FP32 | cqmm.CQMM_BF16X3)
+    x, y, z = a.copy(), b.copy(), c.copy()
+    if fixed.info().tile_family == cqmm.TILES_MIXED:
+        # The per-product mixed-size kernel has no split-BF16 mode.
+        with pytest.raises(cqmm.CqmmError, match="split-BF16"):
+            fixed.mat_mult(x, y, z, cqmm.CQMM_FP32 | cqmm.CQMM_BF16X3)
+    else:
+        # Uniform 9 and 13: the fixed-size tile kernel's split-BF16 mode.
+        out_of(x, z)[:] = np.nan
+        fixed.mat_mult(x, y, z, cqmm.CQMM_FP32 | cqmm.CQMM_BF16X3)
+        e16 = np.abs(out_of(x, z) - ref).max() / np.abs(ref).max()
+        print(f"  fixed-size kernel split-BF16 x3 vs oracle: max rel {e16:.3e}", flush=True)
+        assert np.all(np.isfinite(out_of(x, z))) and e16 <= 5e-5
+        assert np.all(out_of(x, z)[ref == 0.0] == 0.0)
     fixed.close()
     monkeypatch.setenv("CQMM_UNIFORM_TILES", "class")
     monkeypatch.setenv("CQMM_MIXED_TILES", "class")
