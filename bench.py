@@ -306,8 +306,15 @@ def served_stream(args):
     batching) against batched FP64 and batched mixed precision."""
     from cqmm import serve as sv
     cqmm.init(0)
-    n = args.requests
-    reqs = sv.poisson_stream(n, args.rate, args.seed)
+    reqs = sv.poisson_stream(args.requests, args.rate, args.seed)
+    if args.request_range:
+        # A contiguous part of the stream (requests a..b-1, arrivals shifted to
+        # start at 0): the full stream can be served in parts and the parts'
+        # per-request results combined.
+        a, b = (int(x) for x in args.request_range.split(":"))
+        t0 = reqs[a]["arrival_s"]
+        reqs = [dict(q, arrival_s=q["arrival_s"] - t0, index=a + k) for k, q in enumerate(reqs[a:b])]
+    n = len(reqs)
     runs = {
         "plain_fp64": dict(precision=0, batch=0, slots=1),
         "batched_fp64": dict(precision=0, batch=1, slots=args.slots),
@@ -330,10 +337,19 @@ def served_stream(args):
                           p50_service_s=float(np.percentile(svc, 50)), p99_service_s=float(np.percentile(svc, 99)),
                           max_ne_error=max(r["ne_error"] for r in ok))
         energies[key] = np.array([r["band_energy"] for r in res])
+        if args.serve_out:
+            rows = [dict(index=q.get("index", k), natoms=q["natoms"], nsf=q["nsf"], seed=q["seed"],
+                         arrival_s=q["arrival_s"], **{f: r[f] for f in ("status", "band_energy", "n_electrons",
+                                                                        "ne_error", "iterations", "latency_s",
+                                                                        "admit_s", "done_s")})
+                    for k, (q, r) in enumerate(zip(reqs, res))]
+            with open(f"{args.serve_out}.{key}.json", "w") as f:
+                json.dump({"mode": key, "wall_s": wall, "precision": m["precision"], "requests": rows}, f)
         log(f"{key}: {len(ok)}/{n} solved in {wall:.1f} s ({lines[key]['solves_per_s']:.4f}/s), "
             f"latency p50 {lines[key]['p50_latency_s']:.2f} s p99 {lines[key]['p99_latency_s']:.2f} s")
     na = np.array([q["natoms"] for q in reqs])
-    cfg = {"workload": f"served stream of {n} purification requests (atoms log-uniform [64, 2048], 4 or 13 SF, "
+    part = f" (requests {args.request_range} of {args.requests})" if args.request_range else ""
+    cfg = {"workload": f"served stream of {n} purification requests{part} (atoms log-uniform [64, 2048], 4 or 13 SF, "
                        f"S/H/L 8/8/12 A), Poisson {args.rate}/s, HBM budget {args.hbm_gb} GiB, "
                        f"max batch {args.max_batch}",
            **lines}
@@ -362,6 +378,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS) + ["cfg4", "cfg5"])
     ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--request-range", default="", help="serve only requests a:b of the stream (arrivals shifted)")
+    ap.add_argument("--serve-out", default="", help="write per-request results to <prefix>.<mode>.json")
     ap.add_argument("--rate", type=float, default=0.05)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=16)
