@@ -22,6 +22,14 @@ iteration (src/S_matrix_module.f90:746-1034):
     the no-mubar case, and, with T = S^-1, Tr[L0 S] = Ne and the S-metric
     occupancies of L0 being the linear map (lambda (mubar - e) + Ne)/No of the
     generalised eigenvalues (tests/test_oracle_initmcw.py).
+  * Sparse.mcwmin_spin (McWMin for two spin channels, fixed and variable
+    populations, src/McWeeny.f90:144-256, 772-890) and init_mcw_dense_spin
+    (the direct-sum InitMcW for variable populations, :380-395, 460-565) —
+    pinned by: identical channels reducing to the unpolarised loop; fixed
+    populations converging to each spin's own S-metric projector; variable
+    populations converging to the lowest ne_up + ne_dn levels of both spectra
+    together (electrons move to the lower spin); a hand-worked two-level known
+    answer of the direct-sum mubar and lambda (tests/test_oracle_mcweeny_spin.py).
 """
 from __future__ import annotations
 
@@ -148,6 +156,38 @@ def init_mcw_dense(H, S, T, ne, nsf, r_L_mask=None):
     else:
         raise RuntimeError("InitMcW: cannot find mubar")
     return (lam * mubar + ne) / n_o * T - lam / n_o * THT, mubar, lam
+
+
+def init_mcw_dense_spin(Hs, S, T, nes, nsf):
+    """InitMcW for variable spin populations (src/McWeeny.f90:380-395,460-565) on
+    dense matrices: the direct sum of both spin channels, with n_o_ds = 2 n_o,
+    n_e_ds = ne_up + ne_dn (spin_factor 1), SXHX_ds the sum over spins, the
+    limits hmin_ds / hmax_ds the extremes over spins, A_ds = 1 - 2 SX / n_o_ds,
+    one mubar and lambda for both spins, and L0_s = ((lambda mubar + n_e_ds) /
+    n_o_ds) T - (lambda / n_o_ds) T H_s T. The second branch's test is the
+    reference's as written there: n_e/(hmax - mu2) < (n_o - n_e)/(mu2 - hmin)."""
+    n_o = float(np.sum(nsf))
+    n_o_ds = 2.0 * n_o
+    n_e = float(nes[0] + nes[1])
+    SX = np.trace(T @ S)
+    THTs = [T @ H @ T for H in Hs]
+    SXHX = sum(float(np.sum(S * THT)) for THT in THTs)
+    lims = [hlimits_dense_blocks(H, nsf) for H in Hs]
+    hmin, hmax = min(l[0] for l in lims), max(l[1] for l in lims)
+    A = 1.0 - (SX + SX) / n_o_ds
+    if n_o_ds == n_e:
+        mubar, lam = 0.0, 0.0
+    else:
+        mu1 = SXHX / n_o_ds + hmax * A
+        mu2 = (hmin * A * n_e / (n_o_ds - n_e) - SXHX / n_o_ds) / (n_o_ds * A / (n_o_ds - n_e) - 1.0)
+        if n_e / (hmax - mu1) < (n_o_ds - n_e) / (mu1 - hmin):
+            mubar, lam = mu1, n_e / (hmax - mu1)
+        elif n_e / (hmax - mu2) < (n_o_ds - n_e) / (mu2 - hmin):
+            mubar, lam = mu2, (n_o_ds - n_e) / (mu2 - hmin)
+        else:
+            raise RuntimeError("InitMcW: cannot find mubar")
+    Ls = [(lam * mubar + n_e) / n_o_ds * T - lam / n_o_ds * THT for THT in THTs]
+    return Ls, mubar, lam
 
 
 def densify(blocks, vals, nsf):
@@ -281,6 +321,98 @@ class Sparse:
         if abs(c2) < 1e-6:
             c = -0.01
         return rho, E, Ne, c
+
+    def step_traces(self, L, S):
+        """The products and c1, c2 of McW_matrix_multiply_ds for one spin
+        (src/McWeeny.f90:806-840): (LSL, LSLSL, c1, c2)."""
+        LS = self.zeros("LS")
+        self.product(1, "L", "S", "LS", L, S, LS)
+        LSL = self.zeros("LSL")
+        self.product(1, "LS", "L", "LSL", LS, L, LSL)
+        LSLSL = self.zeros("L")
+        self.product(2, "L", "LS", "LSL", LSLSL, LS, LSL)
+        top = self.zeros("S")
+        top = self.sum(0.0, top, "S", 1.0, LSL, "LSL")
+        top = self.sum(1.0, top, "S", -1.0, LSLSL, "L")
+        bottom = self.zeros("S")
+        bottom = self.sum(0.0, bottom, "S", 1.0, L, "L")
+        bottom = self.sum(1.0, bottom, "S", -1.0, LSL, "LSL")
+        return LSL, LSLSL, float(np.dot(S, top)), float(np.dot(S, bottom))
+
+    def rho_of(self, L, LSL, LSLSL, c):
+        """rho' = ((1+c)LSL - LSLSL + (1-2c)L)/(1-c) for c < 1/2, else ((1+c)LSL - LSLSL)/c."""
+        rho = self.zeros("L")
+        rho = self.sum(0.0, rho, "L", -1.0, LSLSL, "L")
+        rho = self.sum(1.0, rho, "L", 1.0 + c, LSL, "LSL")
+        if c < 0.5:
+            rho = self.sum(1.0, rho, "L", 1.0 - 2.0 * c, L, "L")
+            rho *= 1.0 / (1.0 - c)
+        else:
+            rho *= 1.0 / c
+        return rho
+
+    def traces_of(self, rho, S, H):
+        botH = self.zeros("H")
+        botH = self.sum(0.0, botH, "H", 1.0, rho, "L")
+        top = self.zeros("S")
+        top = self.sum(0.0, top, "S", 1.0, rho, "L")
+        return float(np.dot(botH, H)), float(np.dot(top, S))
+
+    def mcwmin_spin(self, Ls, S, Hs, max_iter=100, fixed=False):
+        """McWMin for two spin channels sharing S (src/McWeeny.f90:144-256).
+
+        fixed=True (fixed spin populations): McW_matrix_multiply_nonds per
+        spin, each spin's stop test marks it done, the loop ends when both are
+        done, and until then both spins take every step. fixed=False (variable
+        spin, McW_matrix_multiply_ds :772-890): one c = (c1_up + c1_dn) /
+        (c2_up + c2_dn) for both spins (spin_factor 1), one stop test on the
+        total energy. Fewer than 3 steps restore both L. Returns
+        (Ls, energies per spin, electrons per spin, steps)."""
+        Ls = [L.copy() for L in Ls]
+        Lold = [L.copy() for L in Ls]
+        E = [None, None]
+        Ne = [None, None]
+        n = 0
+        if fixed:
+            oldE = [1e30, 1e30]
+            done = [False, False]
+            for n in range(1, max_iter + 1):
+                res = [self.step(Ls[s], S, Hs[s]) for s in range(2)]
+                for s in range(2):
+                    _, om, _, c = res[s]
+                    if om > oldE[s] or abs(om - oldE[s]) < 1e-8 or c > 1 or c < 0:
+                        done[s] = True
+                if done[0] and done[1]:
+                    break
+                for s in range(2):
+                    rho, om, ne, _ = res[s]
+                    Ls[s] = self.symmetrise(rho, "L")
+                    oldE[s], E[s], Ne[s] = om, om, ne
+            else:
+                n = max_iter + 1
+        else:
+            oldE = 2e30
+            for n in range(1, max_iter + 1):
+                parts = [self.step_traces(Ls[s], S) for s in range(2)]
+                c1 = parts[0][2] + parts[1][2]
+                c2 = parts[0][3] + parts[1][3]
+                if not abs(c2) > 1e-15:
+                    raise ZeroDivisionError("McW_matrix_multiply_ds: c2 is zero")
+                c = c1 / c2
+                rhos = [self.rho_of(Ls[s], parts[s][0], parts[s][1], c) for s in range(2)]
+                tr = [self.traces_of(rhos[s], S, Hs[s]) for s in range(2)]
+                om = tr[0][0] + tr[1][0]
+                if om > oldE or abs(om - oldE) < 1e-8 or c > 1 or c < 0:
+                    break
+                for s in range(2):
+                    Ls[s] = self.symmetrise(rhos[s], "L")
+                    E[s], Ne[s] = tr[s]
+                oldE = om
+            else:
+                n = max_iter + 1
+        if n < 3:
+            Ls = Lold
+        return Ls, E, Ne, n
 
     def mcwmin(self, L, S, H, max_iter=100):
         oldE = 1e30
