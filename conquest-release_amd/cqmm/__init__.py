@@ -167,6 +167,7 @@ EXPORTS = [
     "cqmm_matrix_device_ptr", "cqmm_matrix_set", "cqmm_matrix_get", "cqmm_matrix_product", "cqmm_matrix_sum",
     "cqmm_matrix_scale", "cqmm_matrix_product_trace", "cqmm_matrix_trace", "cqmm_matrix_symmetrise",
     "cqmm_matrix_transpose", "cqmm_mcweeny", "cqmm_mcweeny_batch", "cqmm_init_mcweeny", "cqmm_hotelling",
+    "cqmm_mcweeny_spin", "cqmm_init_mcweeny_spin",
     "cqmm_lnv", "cqmm_correct_electrons", "cqmm_dmm",
     "cqmm_purify_default_params", "cqmm_purify_hbm_estimate", "cqmm_purify", "cqmm_serve",
 ]
@@ -256,6 +257,9 @@ def lib():
                                      C.POINTER(McwResult), i32p]
     L.cqmm_init_mcweeny.argtypes = [vp, C.POINTER(InitMcwSetup), C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p]
     L.cqmm_hotelling.argtypes = [vp, C.POINTER(HotellingSetup), C.c_int32, C.c_int32, i32p, f64p]
+    L.cqmm_mcweeny_spin.argtypes = [vp, C.POINTER(McwSetup), C.c_int32, i32p, i32p, i32p, C.POINTER(McwResult)]
+    L.cqmm_init_mcweeny_spin.argtypes = [vp, C.POINTER(InitMcwSetup), C.c_int32, i32p, i32p, i32p, i32p, f64p,
+                                         f64p]
     _lib = L
     return L
 

@@ -278,6 +278,38 @@ class Purifier:
               "cqmm_mcweeny")
         return mcw_result_dict(res)
 
+    # -- two spin channels (src/McWeeny.f90, nspin = 2) ----------------------
+    def add_spin_down(self, H_dn_vals):
+        """Second spin channel: its own H (values on the H range) and L; both
+        channels share S and T (support functions that do not depend on spin)."""
+        self.H_dn_vals = np.ascontiguousarray(H_dn_vals, np.float64)
+        self.matH_dn = self.ctx.allocate(self.rid["H"])
+        self.matL_dn = self.ctx.allocate(self.rid["L"])
+        self.ctx.set(self.matH_dn, self.H_dn_vals)
+
+    def _pair(self, a, b):
+        return (C.c_int32 * 2)(a, b)
+
+    def init_mcweeny_spin(self, ne_up, ne_dn, fixed):
+        """InitMcW for both channels: ((mubar_up, lambda_up), (mubar_dn, lambda_dn))."""
+        su = InitMcwSetup(self.rid["T"], self.rid["S"], self.rid["H"], self.rid["L"], self.rid["TS"], self.rid["TH"],
+                          T_S_TS, T_H_TH, TH_T_L, 0.0)
+        ml = (C.c_double * 4)()
+        ne = (C.c_double * 2)(ne_up, ne_dn)
+        check(lib().cqmm_init_mcweeny_spin(self.ctx.h, C.byref(su), int(bool(fixed)), self._pair(self.matH, self.matH_dn),
+                                           self._pair(self.matS, self.matS), self._pair(self.matT, self.matT),
+                                           self._pair(self.matL, self.matL_dn), ne, ml), "cqmm_init_mcweeny_spin")
+        return (ml[0], ml[1]), (ml[2], ml[3])
+
+    def mcweeny_spin(self, fixed, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6):
+        """McWMin over both channels: [result of spin up, result of spin down]."""
+        su = self.mcw_setup(precision, max_iter, n_final_fp64, fp32_tol)
+        res = (McwResult * 2)()
+        check(lib().cqmm_mcweeny_spin(self.ctx.h, C.byref(su), int(bool(fixed)), self._pair(self.matH, self.matH_dn),
+                                      self._pair(self.matS, self.matS), self._pair(self.matL, self.matL_dn), res),
+              "cqmm_mcweeny_spin")
+        return [mcw_result_dict(res[0]), mcw_result_dict(res[1])]
+
     # -- LNV / DMM (src/mult_module.f90:1478-1823, src/DMMinModule.f90) ----
     def lnv_setup(self):
         R = self.rid

@@ -505,6 +505,99 @@ namespace {
     if (st_) return st_;     \
   } while (0)
 
+// Checks of InitMcW's handles against its setup.
+int initmcw_check(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH, int32_t matS, int32_t matT,
+                  int32_t matL, const char *who) {
+  CQ_TRY(ctx_check(ctx, matH, who));
+  CQ_TRY(ctx_check(ctx, matS, who));
+  CQ_TRY(ctx_check(ctx, matT, who));
+  CQ_TRY(ctx_check(ctx, matL, who));
+  for (int32_t r : {su->range_T, su->range_S, su->range_H, su->range_L, su->range_TS, su->range_TH})
+    if (r < 0 || r >= kMaxRanges || !ctx->ranges[r].set) return cq::set_errorf(CQMM_ERR_ARG, "%s: range %d not registered", who, r);
+  if (hrange(ctx, matH) != su->range_H || hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T ||
+      hrange(ctx, matL) != su->range_L)
+    return cq::set_errorf(CQMM_ERR_ARG, "%s: handle ranges differ from the setup", who);
+  return CQMM_OK;
+}
+
+// n_o: support functions of the cell (the row dimensions of the H range).
+double initmcw_orbitals(const cqmm_ctx *ctx, const cqmm_initmcw_setup *su) {
+  const RangeRec &rh = ctx->ranges[su->range_H];
+  double n_o = 0.0;
+  for (int32_t p = 0; p < rh.n_prim; ++p) n_o += rh.ndimi[p];
+  return n_o;
+}
+
+// What InitMcW needs of one spin (src/McWeeny.f90:413-452): SX = Tr[T S];
+// McWXHX's XHX = T H T at L range (a temporary of `tmp`, left allocated) and
+// SXHX = Tr[S (THT)_S]; GetHLimits' hmin, hmax (:1020-1109) from a host copy of H.
+struct InitMcwParts {
+  double SX = 0.0, SXHX = 0.0, hmin = 1e30, hmax = -1e30;
+  int32_t XHX = 0;
+};
+
+int initmcw_parts(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH, int32_t matS, int32_t matT,
+                  cq::Temps &tmp, InitMcwParts *q) {
+  int32_t matTS, matTH, matTmp;
+  CQ_TRY(tmp.alloc(su->range_TS, &matTS));
+  CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
+  CQ_TRY(cqmm_matrix_trace(ctx, matTS, &q->SX));
+  CQ_TRY(tmp.alloc(su->range_TH, &matTH));
+  CQ_TRY(tmp.alloc(su->range_L, &q->XHX));
+  CQ_TRY(tmp.alloc(su->range_S, &matTmp));
+  CQ_TRY(cqmm_matrix_product(ctx, matT, matH, matTH, su->mult_T_H_TH));
+  CQ_TRY(cqmm_matrix_product(ctx, matTH, matT, q->XHX, su->mult_TH_T_L));
+  CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTmp, 1.0, q->XHX));
+  CQ_TRY(cqmm_matrix_product_trace(ctx, matS, matTmp, &q->SXHX));
+  const RangeRec &rh = ctx->ranges[su->range_H];
+  std::vector<double> H(rh.length);
+  CQ_TRY(cqmm_matrix_get(ctx, matH, H.data()));
+  for (int32_t p = 0; p < rh.n_prim; ++p) {
+    const int32_t nsfi = rh.ndimi[p];
+    std::vector<double> lo(64, 0.0), hi(64, 0.0);
+    int64_t off = rh.row_off[p];
+    for (int32_t nb = 0; nb < rh.n_nab[p]; ++nb) {
+      const int32_t nsfj = rh.ndimj[rh.row_nab[p] + nb];
+      const bool onsite = rh.onsite[p] > 0 && off == rh.onsite[p] - 1;
+      for (int32_t i = 0; i < nsfj; ++i)
+        for (int32_t j = 0; j < nsfi; ++j) {
+          const double v = std::fabs(H[off + j + (int64_t)nsfi * i]);
+          lo[i] += onsite ? v : -v;
+          hi[i] += v;
+        }
+      off += (int64_t)nsfi * nsfj;
+    }
+    for (int32_t i = 0; i < nsfi; ++i) {
+      q->hmin = std::min(q->hmin, lo[i]);
+      q->hmax = std::max(q->hmax, hi[i]);
+    }
+  }
+  return CQMM_OK;
+}
+
+// InitMcW's choice of mubar and lambda (src/McWeeny.f90:476-507 per spin,
+// :522-545 on the direct sum of both spins, whose second branch tests `<`
+// where the per-spin one tests `>`); n_o == n_e gives mubar = lambda = 0.
+int initmcw_mubar(double n_o, double ne, double SXHX, double hmin, double hmax, double A, bool direct_sum,
+                  double *mubar, double *lambda) {
+  *mubar = 0.0;
+  *lambda = 0.0;
+  if (n_o == ne) return CQMM_OK;
+  const double mu1 = SXHX / n_o + hmax * A;
+  const double mu2 = (hmin * A * ne / (n_o - ne) - SXHX / n_o) / (n_o * A / (n_o - ne) - 1.0);
+  const double lhs2 = ne / (hmax - mu2), rhs2 = (n_o - ne) / (mu2 - hmin);
+  if (ne / (hmax - mu1) < (n_o - ne) / (mu1 - hmin)) {
+    *mubar = mu1;
+    *lambda = ne / (hmax - mu1);
+  } else if (direct_sum ? lhs2 < rhs2 : lhs2 > rhs2) {
+    *mubar = mu2;
+    *lambda = (n_o - ne) / (mu2 - hmin);
+  } else {
+    return cq::set_error(CQMM_ERR_NOTCONV, "InitMcW: cannot find mubar");
+  }
+  return CQMM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -568,84 +661,69 @@ int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t matH,
                       int32_t matL, double out_mu_lambda[2]) {
   return cq::guarded(__func__, [&]() -> int {
     if (!ctx || !su) return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny: null argument");
-    int st;
-    if ((st = ctx_check(ctx, matH, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matS, "cqmm_init_mcweeny")) ||
-        (st = ctx_check(ctx, matT, "cqmm_init_mcweeny")) || (st = ctx_check(ctx, matL, "cqmm_init_mcweeny")))
-      return st;
-    for (int32_t r : {su->range_T, su->range_S, su->range_H, su->range_L, su->range_TS, su->range_TH})
-      if (r < 0 || r >= kMaxRanges || !ctx->ranges[r].set)
-        return cq::set_errorf(CQMM_ERR_ARG, "cqmm_init_mcweeny: range %d not registered", r);
-    if (hrange(ctx, matH) != su->range_H || hrange(ctx, matS) != su->range_S || hrange(ctx, matT) != su->range_T ||
-        hrange(ctx, matL) != su->range_L)
-      return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny: handle ranges differ from the setup");
-    const RangeRec &rh = ctx->ranges[su->range_H];
-    double n_o = 0.0;
-    for (int32_t p = 0; p < rh.n_prim; ++p) n_o += rh.ndimi[p];
+    CQ_TRY(initmcw_check(ctx, su, matH, matS, matT, matL, "cqmm_init_mcweeny"));
+    const double n_o = initmcw_orbitals(ctx, su);
     const double ne = su->n_electrons;
-    // SX = Tr[T S]
     cq::Temps tmp(ctx);
-    int32_t matTS, matTH, matXHX, matTmp;
-    CQ_TRY(tmp.alloc(su->range_TS, &matTS));
-    CQ_TRY(cqmm_matrix_product(ctx, matT, matS, matTS, su->mult_T_S_TS));
-    double SX = 0.0;
-    CQ_TRY(cqmm_matrix_trace(ctx, matTS, &SX));
-    // McWXHX: XHX = T H T at L range, truncated copy at S range, SXHX = Tr[S (THT)_S].
-    CQ_TRY(tmp.alloc(su->range_TH, &matTH));
-    CQ_TRY(tmp.alloc(su->range_L, &matXHX));
-    CQ_TRY(tmp.alloc(su->range_S, &matTmp));
-    CQ_TRY(cqmm_matrix_product(ctx, matT, matH, matTH, su->mult_T_H_TH));
-    CQ_TRY(cqmm_matrix_product(ctx, matTH, matT, matXHX, su->mult_TH_T_L));
-    CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matTmp, 1.0, matXHX));
-    double SXHX = 0.0;
-    CQ_TRY(cqmm_matrix_product_trace(ctx, matS, matTmp, &SXHX));
-    // GetHLimits (src/McWeeny.f90:1020-1109) on a host copy of H.
-    std::vector<double> H(rh.length);
-    CQ_TRY(cqmm_matrix_get(ctx, matH, H.data()));
-    double hmin = 1e30, hmax = -1e30;
-    for (int32_t p = 0; p < rh.n_prim; ++p) {
-      const int32_t nsfi = rh.ndimi[p];
-      std::vector<double> lo(64, 0.0), hi(64, 0.0);
-      int64_t off = rh.row_off[p];
-      for (int32_t nb = 0; nb < rh.n_nab[p]; ++nb) {
-        const int32_t nsfj = rh.ndimj[rh.row_nab[p] + nb];
-        const bool onsite = rh.onsite[p] > 0 && off == rh.onsite[p] - 1;
-        for (int32_t i = 0; i < nsfj; ++i)
-          for (int32_t j = 0; j < nsfi; ++j) {
-            const double v = std::fabs(H[off + j + (int64_t)nsfi * i]);
-            lo[i] += onsite ? v : -v;
-            hi[i] += v;
-          }
-        off += (int64_t)nsfi * nsfj;
-      }
-      for (int32_t i = 0; i < nsfi; ++i) {
-        hmin = std::min(hmin, lo[i]);
-        hmax = std::max(hmax, hi[i]);
-      }
-    }
-    const double A = 1.0 - SX / n_o;
+    InitMcwParts q;
+    CQ_TRY(initmcw_parts(ctx, su, matH, matS, matT, tmp, &q));
     double mubar = 0.0, lambda = 0.0;
-    if (n_o != ne) {
-      const double mu1 = SXHX / n_o + hmax * A;
-      const double mu2 = (hmin * A * ne / (n_o - ne) - SXHX / n_o) / (n_o * A / (n_o - ne) - 1.0);
-      if (ne / (hmax - mu1) < (n_o - ne) / (mu1 - hmin)) {
-        mubar = mu1;
-        lambda = ne / (hmax - mu1);
-      } else if (ne / (hmax - mu2) > (n_o - ne) / (mu2 - hmin)) {
-        mubar = mu2;
-        lambda = (n_o - ne) / (mu2 - hmin);
-      } else {
-        return cq::set_error(CQMM_ERR_NOTCONV, "cqmm_init_mcweeny: cannot find mubar");
-      }
-    }
+    CQ_TRY(initmcw_mubar(n_o, ne, q.SXHX, q.hmin, q.hmax, 1.0 - q.SX / n_o, false, &mubar, &lambda));
     // McWRho0 (src/McWeeny.f90:961-1000): L = ((l m + Ne)/No) T - (l/No) XHX.
     CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL, (lambda * mubar + ne) / n_o, matT));
-    CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matL, -lambda / n_o, matXHX));
+    CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matL, -lambda / n_o, q.XHX));
     if (out_mu_lambda) {
       out_mu_lambda[0] = mubar;
       out_mu_lambda[1] = lambda;
     }
     CQ_TRY(tmp.release());
     return cq::hip_check(hipStreamSynchronize(cq::stream()), "cqmm_init_mcweeny");
+  });
+}
+
+int cqmm_init_mcweeny_spin(cqmm_ctx *ctx, const cqmm_initmcw_setup *su, int32_t fixed_populations,
+                           const int32_t matH[2], const int32_t matS[2], const int32_t matT[2], const int32_t matL[2],
+                           const double n_electrons[2], double out_mu_lambda[4]) {
+  return cq::guarded(__func__, [&]() -> int {
+    if (!ctx || !su || !matH || !matS || !matT || !matL || !n_electrons)
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny_spin: null argument");
+    if (matL[0] == matL[1]) return cq::set_error(CQMM_ERR_ARG, "cqmm_init_mcweeny_spin: one L handle for both spins");
+    for (int sp = 0; sp < 2; ++sp)
+      CQ_TRY(initmcw_check(ctx, su, matH[sp], matS[sp], matT[sp], matL[sp], "cqmm_init_mcweeny_spin"));
+    if (fixed_populations) {
+      // Each spin as the spin-unpolarised InitMcW with its own population.
+      for (int sp = 0; sp < 2; ++sp) {
+        cqmm_initmcw_setup one = *su;
+        one.n_electrons = n_electrons[sp];
+        CQ_TRY(cqmm_init_mcweeny(ctx, &one, matH[sp], matS[sp], matT[sp], matL[sp],
+                                 out_mu_lambda ? out_mu_lambda + 2 * sp : nullptr));
+      }
+      return CQMM_OK;
+    }
+    // Variable populations: InitMcW on the direct sum of the two spin channels
+    // (src/McWeeny.f90:380-395, 460-474, 516-554): twice the orbitals, all the
+    // electrons, SXHX summed over spins, the extremes of both spins' limits,
+    // one mubar and lambda (its second branch test as written there, with <).
+    const double n_o = initmcw_orbitals(ctx, su);
+    const double n_o_ds = n_o + n_o, n_e_ds = n_electrons[0] + n_electrons[1];
+    cq::Temps tmp(ctx);
+    InitMcwParts q[2];
+    for (int sp = 0; sp < 2; ++sp) CQ_TRY(initmcw_parts(ctx, su, matH[sp], matS[sp], matT[sp], tmp, &q[sp]));
+    const double SXHX_ds = q[0].SXHX + q[1].SXHX;
+    const double hmin_ds = std::min(q[0].hmin, q[1].hmin), hmax_ds = std::max(q[0].hmax, q[1].hmax);
+    const double A_ds = 1.0 - (q[0].SX + q[1].SX) / n_o_ds;
+    double mubar = 0.0, lambda = 0.0;
+    CQ_TRY(initmcw_mubar(n_o_ds, n_e_ds, SXHX_ds, hmin_ds, hmax_ds, A_ds, true, &mubar, &lambda));
+    for (int sp = 0; sp < 2; ++sp) {
+      CQ_TRY(cqmm_matrix_sum(ctx, 0.0, matL[sp], (lambda * mubar + n_e_ds) / n_o_ds, matT[sp]));
+      CQ_TRY(cqmm_matrix_sum(ctx, 1.0, matL[sp], -lambda / n_o_ds, q[sp].XHX));
+      if (out_mu_lambda) {
+        out_mu_lambda[2 * sp] = mubar;
+        out_mu_lambda[2 * sp + 1] = lambda;
+      }
+    }
+    CQ_TRY(tmp.release());
+    return cq::hip_check(hipStreamSynchronize(cq::stream()), "cqmm_init_mcweeny_spin");
   });
 }
 

@@ -31,6 +31,10 @@ class McwEngine {
   void set_grouping(bool on) { group_ = on; }
   // Admit a system (McWMin's set-up); it joins from the next step.
   int add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S, int32_t L, cqmm_mcw_result *res);
+  // Admit both spin channels of one McWMin (nspin = 2, one context): fixed or
+  // variable spin populations.
+  int add_spin(cqmm_ctx *ctx, const cqmm_mcw_setup &su, bool fixed, const int32_t H[2], const int32_t S[2],
+               const int32_t L[2], cqmm_mcw_result res[2]);
   // One McWeeny step of every admitted system; systems that stopped are
   // appended to `finished` (ownership passes to the caller: job_delete).
   int step(std::vector<McwJob *> *finished);
@@ -42,14 +46,18 @@ class McwEngine {
   static void job_delete(McwJob *j);
 
  private:
+  int add_one(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S, int32_t L, cqmm_mcw_result *res,
+              const cqmm_ctx *shared);
   int enqueue_step();
   int products(int which);
   int sum(cqmm_ctx *ctx, double a, const double *ap, int32_t A, double b, const double *bp, int32_t B);
   int trace(cqmm_ctx *ctx, int32_t A, int32_t B, double *d_out);
   int accept(McwJob &j);
-  int after_step(McwJob &j, const double *sc);
+  int after_step(McwJob &j);
+  int after_fixed_pair(McwJob &a);
   int finish(McwJob &j);
   double *dsc(const McwJob &j) const;
+  const double *hsc(const McwJob &j) const;
   std::string key() const;
   void drop_graph();
 

@@ -49,10 +49,10 @@ namespace {
 constexpr int kScal = 8;
 
 // rho' = ((1+c)LSL - LSLSL + (1-2c)L)/(1-c) for c < 1/2, else ((1+c)LSL - LSLSL)/c
-// (McW_matrix_multiply_nonds, src/McWeeny.f90:700-720).
-__global__ void k_mcw_coef(double *sc) {
-  if (threadIdx.x != 0) return;
-  const double c1 = sc[0], c2 = sc[1];
+// (McW_matrix_multiply_nonds, src/McWeeny.f90:700-720); `nonds` applies its
+// rule that a vanishing c2 reports c = -0.01 (:737), which
+// McW_matrix_multiply_ds does not have.
+__device__ void mcw_coef(double c1, double c2, bool nonds, double *sc) {
   const double cn = c1 / c2;
   sc[4] = 1.0 + cn;
   if (cn < 0.5) {
@@ -62,7 +62,21 @@ __global__ void k_mcw_coef(double *sc) {
     sc[5] = 0.0;
     sc[6] = 1.0 / cn;
   }
-  sc[7] = (fabs(c2) < 1e-6) ? -0.01 : cn;
+  sc[7] = (nonds && fabs(c2) < 1e-6) ? -0.01 : cn;
+}
+
+__global__ void k_mcw_coef(double *sc) {
+  if (threadIdx.x != 0) return;
+  mcw_coef(sc[0], sc[1], true, sc);
+}
+
+// Variable spin populations (McW_matrix_multiply_ds, src/McWeeny.f90:806-858):
+// one c = (c1_up + c1_dn)/(c2_up + c2_dn) for both spin channels.
+__global__ void k_mcw_coef_pair(double *a, double *b) {
+  if (threadIdx.x != 0) return;
+  const double c1 = a[0] + b[0], c2 = a[1] + b[1];
+  mcw_coef(c1, c2, false, a);
+  mcw_coef(c1, c2, false, b);
 }
 
 double now_s() {
@@ -85,11 +99,21 @@ struct McwJob {
   double oldE = 1.0e30, lastE = 0.0, lastNe = 0.0, lastC = 0.0, t0 = 0.0;
   int slot = -1;
   uint64_t serial = 0;
+  // Spin pairs: both channels of one McWMin (nspin = 2). `partner` is the other
+  // channel, `lead` marks the first (it holds the loop state of variable
+  // populations); spin 0 = a spin-unpolarised system, 1 = fixed populations,
+  // 2 = variable populations; spin_done = this channel's stop test fired.
+  McwJob *partner = nullptr;
+  bool lead = true, spin_done = false;
+  int spin = 0;
 };
 
 McwEngine::McwEngine() = default;
 
 McwEngine::~McwEngine() {
+  // Temporaries of systems still admitted (an error ended the loop), newest
+  // first: two channels of a spin pair share a context's stack.
+  for (auto it = jobs_.rbegin(); it != jobs_.rend(); ++it) (void)(*it)->tmp->release();
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (hsc_) (void)hipHostFree(hsc_);
 }
@@ -105,11 +129,46 @@ int McwEngine::init(hipStream_t s, bool use_graph) {
 }
 
 double *McwEngine::dsc(const McwJob &j) const { return dsc_.as<double>() + (size_t)j.slot * kScal; }
+const double *McwEngine::hsc(const McwJob &j) const { return hsc_ + (size_t)j.slot * kScal; }
 
 // McWMin's set-up for one system (src/McWeeny.f90:107-140): temporaries, the
 // saved L for the restore rule, FP32 copies for the mixed-precision phase, and
 // every map and buffer the step needs, so that no step allocates (graph capture).
 int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S, int32_t L, cqmm_mcw_result *res) {
+  return add_one(ctx, su, H, S, L, res, nullptr);
+}
+
+// Both spin channels of one McWMin (nspin = 2) in one context: two systems
+// whose steps the host couples (one c for variable populations, the joint stop
+// rule for fixed ones).
+int McwEngine::add_spin(cqmm_ctx *ctx, const cqmm_mcw_setup &su, bool fixed, const int32_t H[2], const int32_t S[2],
+                        const int32_t L[2], cqmm_mcw_result res[2]) {
+  if (fixed && su.precision != 0)
+    return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny_spin: fixed spin populations take precision 0 (not %d)",
+                      su.precision);
+  if (L[0] == L[1]) return set_error(CQMM_ERR_ARG, "cqmm_mcweeny_spin: one L handle for both spins");
+  MTRY(add_one(ctx, su, H[0], S[0], L[0], &res[0], nullptr));
+  McwJob *a = jobs_.back().get();
+  const int st = add_one(ctx, su, H[1], S[1], L[1], &res[1], ctx);
+  if (st) {
+    (void)a->tmp->release();
+    slot_used_[a->slot] = false;
+    jobs_.pop_back();
+    return st;
+  }
+  McwJob *b = jobs_.back().get();
+  a->partner = b;
+  b->partner = a;
+  b->lead = false;
+  a->spin = b->spin = fixed ? 1 : 2;
+  // Variable populations compare the total energy; McWMin starts it at
+  // spin_factor (oldE_up + oldE_dn) = 2e30 (src/McWeeny.f90:145-146).
+  if (!fixed) a->oldE = 2.0e30;
+  return CQMM_OK;
+}
+
+int McwEngine::add_one(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S, int32_t L, cqmm_mcw_result *res,
+                       const cqmm_ctx *shared) {
   if (!ctx || !res) return set_error(CQMM_ERR_ARG, "cqmm_mcweeny: null argument");
   MTRY(ctx_check(ctx, H, "cqmm_mcweeny"));
   MTRY(ctx_check(ctx, S, "cqmm_mcweeny"));
@@ -124,8 +183,11 @@ int McwEngine::add(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32_t S
   if (!ctx->ranges[su.range_L].has_pairs)
     return set_errorf(CQMM_ERR_STATE, "cqmm_mcweeny: range %d has no transpose pairs", su.range_L);
   if (su.precision < 0 || su.precision > 4) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: precision %d", su.precision);
+  // Systems of one batch have their own contexts (their temporaries follow
+  // the context's LIFO rule); the second channel of a spin pair shares the first's.
   for (const auto &j : jobs_)
-    if (j->ctx == ctx) return set_error(CQMM_ERR_ARG, "cqmm_mcweeny: two systems of one batch share a context");
+    if (j->ctx == ctx && ctx != shared)
+      return set_error(CQMM_ERR_ARG, "cqmm_mcweeny: two systems of one batch share a context");
   int slot = -1;
   for (int k = 0; k < kMaxJobs; ++k)
     if (!slot_used_[k]) {
@@ -329,8 +391,20 @@ int McwEngine::enqueue_step() {
     MTRY(sum(ctx, 0.0, nullptr, j.bottom, 1.0, nullptr, j.L));
     MTRY(sum(ctx, 1.0, nullptr, j.bottom, -1.0, nullptr, j.LSL));
     MTRY(trace(ctx, j.S, j.bottom, sc + 1));
-    hipLaunchKernelGGL(k_mcw_coef, dim3(1), dim3(64), 0, s_, sc);
+  }
+  for (const auto &jp : jobs_) {
+    McwJob &j = *jp;
+    if (j.spin == 2) {
+      if (j.lead) hipLaunchKernelGGL(k_mcw_coef_pair, dim3(1), dim3(64), 0, s_, dsc(j), dsc(*j.partner));
+    } else {
+      hipLaunchKernelGGL(k_mcw_coef, dim3(1), dim3(64), 0, s_, dsc(j));
+    }
     MTRY(hip_check(hipGetLastError(), "McWeeny coefficient kernel"));
+  }
+  for (const auto &jp : jobs_) {
+    McwJob &j = *jp;
+    cqmm_ctx *ctx = j.ctx;
+    double *sc = dsc(j);
     // rho' on the L range.
     MTRY(op_scale(hlen(ctx, j.rho), 0.0, dptr(ctx, j.rho), s_));
     MTRY(sum(ctx, 0.0, nullptr, j.rho, -1.0, nullptr, j.LSLSL));
@@ -357,32 +431,47 @@ int McwEngine::accept(McwJob &j) {
 // mixed-precision phase rules: the FP32 phase ends at the first stop test (or
 // when |dE| falls below fp32_tol |E|); a rejected FP32 step restarts from the
 // last accepted L in FP64; the first n_final_fp64 FP64 steps are kept even if
-// FP32-level noise trips the energy tests.
-int McwEngine::after_step(McwJob &j, const double *sc) {
-  const double c2 = sc[1];
+// FP32-level noise trips the energy tests. `j` is a system or the first
+// channel of a variable-population spin pair, whose loop tests the total
+// energy and the common c (:207-237) and accepts or rejects both channels.
+int McwEngine::after_step(McwJob &j) {
+  McwJob *u[2] = {&j, j.partner};
+  const int nu = j.partner ? 2 : 1;
+  const double *sc[2] = {hsc(j), nu == 2 ? hsc(*j.partner) : nullptr};
+  double c2 = sc[0][1], omega = sc[0][2];
+  if (nu == 2) {
+    c2 = sc[0][1] + sc[1][1];
+    omega = sc[0][2] + sc[1][2];
+  }
   if (!(std::fabs(c2) > 1e-15)) return set_errorf(CQMM_ERR_NOTCONV, "McWeeny: c2 is near zero (%g)", c2);
-  const double omega = sc[2], ne = sc[3], cn = sc[7];
-  cqmm_mcw_result *res = j.res;
+  const double cn = sc[0][7];
   const int n = ++j.n;
-  if (n - 1 < CQMM_MCW_TRACE) res->energy_trace[n - 1] = omega;
-  if (j.fp32_phase) ++res->fp32_iterations;
-  else ++j.fp64_done;
+  for (int k = 0; k < nu; ++k) {
+    u[k]->n = n;
+    if (n - 1 < CQMM_MCW_TRACE) u[k]->res->energy_trace[n - 1] = sc[k][2];
+    if (j.fp32_phase) ++u[k]->res->fp32_iterations;
+  }
+  if (!j.fp32_phase) ++j.fp64_done;
   int why = 0;
   if (omega > j.oldE) why = 1;
   else if (std::fabs(omega - j.oldE) < 1.0e-8) why = 2;
   else if (cn > 1.0 || cn < 0.0) why = 3;
+  auto keep = [&]() -> int {
+    for (int k = 0; k < nu; ++k) {
+      MTRY(accept(*u[k]));
+      u[k]->lastE = sc[k][2];
+      u[k]->lastNe = sc[k][3];
+      u[k]->lastC = cn;
+    }
+    return CQMM_OK;
+  };
   bool handled = false;
   if (j.fp32_phase) {
     const bool flat = std::fabs(omega - j.oldE) < j.su.fp32_tol * std::max(1.0, std::fabs(omega));
     if (why || flat) {
-      j.fp32_phase = false;
-      if (!(why == 1 || why == 3)) {
-        MTRY(accept(j));
-        j.lastE = omega;
-        j.lastNe = ne;
-        j.lastC = cn;
-      }
-      j.oldE = 1.0e30;
+      for (int k = 0; k < nu; ++k) u[k]->fp32_phase = false;
+      if (!(why == 1 || why == 3)) MTRY(keep());
+      j.oldE = 1.0e30 * nu;
       handled = true;
     }
   } else if (why && j.fp64_done <= j.nfinal && j.mixed && why != 3) {
@@ -390,20 +479,63 @@ int McwEngine::after_step(McwJob &j, const double *sc) {
   }
   if (!handled) {
     if (why) {
-      j.stop = why;
-      j.done = true;
+      for (int k = 0; k < nu; ++k) {
+        u[k]->stop = why;
+        u[k]->done = true;
+      }
       return CQMM_OK;
     }
-    MTRY(accept(j));
+    MTRY(keep());
     j.oldE = omega;
-    j.lastE = omega;
-    j.lastNe = ne;
-    j.lastC = cn;
   }
-  if (n >= j.max_iter) {
-    j.n = j.max_iter + 1;
-    j.done = true;
+  if (n >= j.max_iter)
+    for (int k = 0; k < nu; ++k) {
+      u[k]->n = j.max_iter + 1;
+      u[k]->done = true;
+    }
+  return CQMM_OK;
+}
+
+// Fixed spin populations (McWMin, src/McWeeny.f90:156-203): each channel has
+// its own c and energy test; a channel whose test fires is marked done for the
+// rest of the loop; the loop ends when both are done, and until then both
+// channels take every step.
+int McwEngine::after_fixed_pair(McwJob &a) {
+  McwJob *u[2] = {&a, a.partner};
+  const double *sc[2] = {hsc(a), hsc(*a.partner)};
+  for (int k = 0; k < 2; ++k)
+    if (!(std::fabs(sc[k][1]) > 1e-15))
+      return set_errorf(CQMM_ERR_NOTCONV, "McWeeny: c2 of spin %d is near zero (%g)", k + 1, sc[k][1]);
+  const int n = ++a.n;
+  a.partner->n = n;
+  for (int k = 0; k < 2; ++k) {
+    if (n - 1 < CQMM_MCW_TRACE) u[k]->res->energy_trace[n - 1] = sc[k][2];
+    const double omega = sc[k][2], cn = sc[k][7], oldE = u[k]->oldE;
+    int why = 0;
+    if (omega > oldE) why = 1;
+    else if (std::fabs(omega - oldE) < 1.0e-8) why = 2;
+    else if (cn > 1.0 || cn < 0.0) why = 3;
+    if (why && !u[k]->spin_done) {
+      u[k]->spin_done = true;
+      u[k]->stop = why;
+    }
   }
+  if (u[0]->spin_done && u[1]->spin_done) {
+    u[0]->done = u[1]->done = true;
+    return CQMM_OK;
+  }
+  for (int k = 0; k < 2; ++k) {
+    MTRY(accept(*u[k]));
+    u[k]->oldE = sc[k][2];
+    u[k]->lastE = sc[k][2];
+    u[k]->lastNe = sc[k][3];
+    u[k]->lastC = sc[k][7];
+  }
+  if (n >= a.max_iter)
+    for (int k = 0; k < 2; ++k) {
+      u[k]->n = a.max_iter + 1;
+      u[k]->done = true;
+    }
   return CQMM_OK;
 }
 
@@ -476,20 +608,35 @@ int McwEngine::step(std::vector<McwJob *> *finished) {
   MTRY(hip_check(hipMemcpyAsync(hsc_, dsc_.p, (size_t)kMaxJobs * kScal * sizeof(double), hipMemcpyDeviceToHost, s_),
                  "D2H McWeeny scalars"));
   MTRY(hip_check(hipStreamSynchronize(s_), "McWeeny step"));
-  for (size_t i = 0; i < jobs_.size();) {
-    McwJob &j = *jobs_[i];
-    int st = after_step(j, hsc_ + (size_t)j.slot * kScal);
-    if (st) {
-      j.status = st;
-      j.done = true;
+  for (const auto &jp : jobs_) {
+    McwJob &j = *jp;
+    if (!j.lead) continue;
+    const int st = j.spin == 1 ? after_fixed_pair(j) : after_step(j);
+    if (st)
+      for (McwJob *m : {&j, j.partner})
+        if (m) {
+          m->status = st;
+          m->done = true;
+        }
+  }
+  // Finish what stopped; a spin pair's second channel first, as its
+  // temporaries are the newer on the shared context's stack.
+  std::vector<McwJob *> order;
+  for (const auto &jp : jobs_)
+    if (jp->lead && jp->done) {
+      if (jp->partner) order.push_back(jp->partner);
+      order.push_back(jp.get());
     }
-    if (!j.done) {
+  for (McwJob *j : order) {
+    if (!j->status) j->status = finish(*j);
+    else (void)j->tmp->release();
+    slot_used_[j->slot] = false;
+  }
+  for (size_t i = 0; i < jobs_.size();) {
+    if (!jobs_[i]->done) {
       ++i;
       continue;
     }
-    if (!j.status) j.status = finish(j);
-    else (void)j.tmp->release();
-    slot_used_[j.slot] = false;
     if (finished) finished->push_back(jobs_[i].release());
     jobs_.erase(jobs_.begin() + (long)i);
   }
@@ -511,6 +658,29 @@ int cqmm_mcweeny(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t matH, int32_t 
   int32_t status = 0;
   const int st = cqmm_mcweeny_batch(1, &ctx, su, &matH, &matS, &matL, 0u, res, &status);
   return st ? st : status;
+}
+
+int cqmm_mcweeny_spin(cqmm_ctx *ctx, const cqmm_mcw_setup *su, int32_t fixed_populations, const int32_t matH[2],
+                      const int32_t matS[2], const int32_t matL[2], cqmm_mcw_result result[2]) {
+  return guarded(__func__, [&]() -> int {
+    if (!ctx || !su || !matH || !matS || !matL || !result)
+      return set_error(CQMM_ERR_ARG, "cqmm_mcweeny_spin: null argument");
+    MTRY(ensure_init());
+    McwEngine eng;
+    MTRY(eng.init(stream(), true));
+    MTRY(eng.add_spin(ctx, *su, fixed_populations != 0, matH, matS, matL, result));
+    std::vector<McwJob *> fin;
+    int status = CQMM_OK;
+    while (eng.active() > 0) {
+      MTRY(eng.step(&fin));
+      for (McwJob *j : fin) {
+        if (!status) status = McwEngine::job_status(j);
+        McwEngine::job_delete(j);
+      }
+      fin.clear();
+    }
+    return status;
+  });
 }
 
 int cqmm_mcweeny_batch(int32_t n, cqmm_ctx *const *ctxs, const cqmm_mcw_setup *setups, const int32_t *matH,

@@ -332,6 +332,27 @@ module cqmm_f
        type(cqmm_mcw_result) :: res
        integer(c_int) :: st
      end function cqmm_mcweeny
+     function cqmm_mcweeny_spin(ctx, setup, fixed_populations, math, mats, matl, res) &
+         bind(c, name="cqmm_mcweeny_spin") result(st)
+       import :: c_ptr, c_int, c_int32_t, cqmm_mcw_setup, cqmm_mcw_result
+       type(c_ptr), value :: ctx
+       type(cqmm_mcw_setup), intent(in) :: setup
+       integer(c_int32_t), value :: fixed_populations
+       integer(c_int32_t), intent(in) :: math(2), mats(2), matl(2)
+       type(cqmm_mcw_result) :: res(2)
+       integer(c_int) :: st
+     end function cqmm_mcweeny_spin
+     function cqmm_init_mcweeny_spin(ctx, setup, fixed_populations, math, mats, matt, matl, n_electrons, &
+         mu_lambda) bind(c, name="cqmm_init_mcweeny_spin") result(st)
+       import :: c_ptr, c_int, c_int32_t, c_double, cqmm_initmcw_setup
+       type(c_ptr), value :: ctx
+       type(cqmm_initmcw_setup), intent(in) :: setup
+       integer(c_int32_t), value :: fixed_populations
+       integer(c_int32_t), intent(in) :: math(2), mats(2), matt(2), matl(2)
+       real(c_double), intent(in) :: n_electrons(2)
+       real(c_double) :: mu_lambda(4)
+       integer(c_int) :: st
+     end function cqmm_init_mcweeny_spin
      function cqmm_lnv(ctx, setup, flags, math, mats, matl, matk, matm3, matphi, electrons, energy) &
          bind(c, name="cqmm_lnv") result(st)
        import :: c_ptr, c_int, c_int32_t, c_double, cqmm_lnv_setup

@@ -173,6 +173,34 @@ typedef struct cqmm_initmcw_setup {
 int cqmm_init_mcweeny(cqmm_ctx *ctx, const cqmm_initmcw_setup *setup, int32_t matH, int32_t matS, int32_t matT,
                       int32_t matL, double out_mu_lambda[2]);
 
+/* McWMin with two spin channels (src/McWeeny.f90:107-272, nspin = 2,
+ * spin_factor 1). Spin s is (matH[s], matS[s], matL[s]) of one context; pass one
+ * S handle twice unless the support functions depend on spin
+ * (flag_SpinDependentSF). fixed_populations = 1 (flag_fix_spin_population):
+ * each spin steps with McW_matrix_multiply_nonds (:620-740) and its own c, a
+ * spin's stop test marks it done (for good), and both spins take every step
+ * until both are done; setup->precision must be 0. fixed_populations = 0:
+ * McW_matrix_multiply_ds (:772-890), one c = (c1_up + c1_dn)/(c2_up + c2_dn)
+ * and one stop test on E_up + E_dn; every precision of cqmm_mcw_setup applies,
+ * both spins changing phase together. result[s] receives spin s's energy,
+ * electrons and energy_trace; iterations, restored and c are the loop's;
+ * stop_reason is the loop's (variable) or why spin s was marked done (fixed).
+ * Fewer than 3 steps restore both L. */
+int cqmm_mcweeny_spin(cqmm_ctx *ctx, const cqmm_mcw_setup *setup, int32_t fixed_populations, const int32_t matH[2],
+                      const int32_t matS[2], const int32_t matL[2], cqmm_mcw_result result[2]);
+
+/* InitMcW with two spin channels (src/McWeeny.f90:340-570, nspin = 2). Fixed
+ * populations: each spin as cqmm_init_mcweeny with n_electrons[s]. Variable
+ * populations: the direct sum of both channels, n_o_ds = 2 n_o, n_e_ds =
+ * n_electrons[0] + n_electrons[1], SXHX summed over spins, the limits of H the
+ * extremes over both spins, A_ds = 1 - (SX_up + SX_dn)/n_o_ds, one mubar and
+ * lambda, L_s = ((lambda mubar + n_e_ds)/n_o_ds) T_s - (lambda/n_o_ds) T_s H_s T_s.
+ * setup->n_electrons is not read. out_mu_lambda (may be NULL) receives
+ * {mubar_up, lambda_up, mubar_dn, lambda_dn}. */
+int cqmm_init_mcweeny_spin(cqmm_ctx *ctx, const cqmm_initmcw_setup *setup, int32_t fixed_populations,
+                           const int32_t matH[2], const int32_t matS[2], const int32_t matT[2], const int32_t matL[2],
+                           const double n_electrons[2], double out_mu_lambda[4]);
+
 /* Hotelling iteration for T ~ S^-1 (Iter_Hott_InvS, src/S_matrix_module.f90:746-963):
  * T <- 2T - T.S.T with products T_S_TS (type 1) and TS_T_T (type 2, result at T
  * range); stops when the change of Tr[TS]/n_orb is below tol or after max_iter. */
