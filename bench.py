@@ -258,23 +258,27 @@ def purification_pair(args, natoms=2048, nsf=13):
     precs = [int(p) for p in args.precisions.split(",")]
     if 0 not in precs:
         precs = [0] + precs
-    for prec in precs:
-        r = sv.purify(natoms, nsf, args.seed, sv.default_params(precision=prec))
+    # Symmetric-result plans for LSL and LSLSL (half the blocks computed, the
+    # rest mirrored) in the listed precisions.
+    sym = [int(p) for p in args.symmetric_precisions.split(",")] if args.symmetric_precisions else []
+    runs = [(prec, 0) for prec in precs] + [(prec, 1) for prec in sym]
+    for prec, symm in runs:
+        r = sv.purify(natoms, nsf, args.seed, sv.default_params(precision=prec, symmetric=symm))
         if r["status"] != 0:
             raise SystemExit(f"purify failed: {r}")
-        out[prec] = r
-        log(f"purification precision {prec}: solve {r['solve_s']:.2f} s (McWeeny {r['mcw_s']:.2f} s), "
-            f"{r['iterations']} it ({r['fp32_iterations']} reduced precision), E_band {r['band_energy']:.10f} Ha, "
-            f"Ne error {r['ne_error']:.2e}")
-    e64 = out[0]
-    names = {1: "fp32_mfma_fp64_sums", 2: "fp32_accumulate", 3: "fp32_chunk_runs", 4: "split_bf16x3"}
+        out[(prec, symm)] = r
+        log(f"purification precision {prec}{' symmetric' if symm else ''}: solve {r['solve_s']:.2f} s "
+            f"(McWeeny {r['mcw_s']:.2f} s), {r['iterations']} it ({r['fp32_iterations']} reduced precision), "
+            f"E_band {r['band_energy']:.10f} Ha, Ne error {r['ne_error']:.2e}")
+    e64 = out[(0, 0)]
+    names = {0: "fp64", 1: "fp32_mfma_fp64_sums", 2: "fp32_accumulate", 3: "fp32_chunk_runs", 4: "split_bf16x3"}
     modes = {}
-    for prec in precs:
-        if prec == 0:
+    for prec, symm in runs:
+        if (prec, symm) == (0, 0):
             continue
-        e = out[prec]
-        modes[names[prec]] = {
-            "precision": prec, "solves_per_s": round(1.0 / e["solve_s"], 5),
+        e = out[(prec, symm)]
+        modes[("symmetric_" if symm else "") + names[prec]] = {
+            "precision": prec, "symmetric": symm, "solves_per_s": round(1.0 / e["solve_s"], 5),
             "speedup_vs_fp64": round(e64["solve_s"] / e["solve_s"], 3), "solve_s": round(e["solve_s"], 3),
             "mcw_s": round(e["mcw_s"], 3), "iterations": e["iterations"],
             "reduced_precision_iterations": e["fp32_iterations"], "band_energy": e["band_energy"],
@@ -325,7 +329,7 @@ def served_stream(args):
     energies = {}
     for key in only:
         m = runs[key]
-        params = sv.default_params(precision=m["precision"])
+        params = sv.default_params(precision=m["precision"], symmetric=args.serve_symmetric if m["batch"] else 0)
         log(f"serving {n} requests at {args.rate}/s: {key} ({m['slots']} slots, batch {m['batch']})")
         res, wall = sv.serve(reqs, slots=m["slots"], hbm_budget=int(args.hbm_gb * (1 << 30)), params=params,
                              log_progress=True, batch=m["batch"], max_batch=args.max_batch)
@@ -352,6 +356,7 @@ def served_stream(args):
     cfg = {"workload": f"served stream of {n} purification requests{part} (atoms log-uniform [64, 2048], 4 or 13 SF, "
                        f"S/H/L 8/8/12 A), Poisson {args.rate}/s, HBM budget {args.hbm_gb} GiB, "
                        f"max batch {args.max_batch}",
+           "symmetric_plans_in_batched_modes": bool(args.serve_symmetric),
            **lines}
     if "batched_mixed" in energies and "plain_fp64" in energies:
         cfg["max_band_energy_err_per_atom"] = float(np.max(np.abs(energies["batched_mixed"] -
@@ -392,6 +397,10 @@ def main():
                     help="precision of the served stream's mixed solves (1 FP32 MFMA + FP64 sums, 4 split-BF16 x3)")
     ap.add_argument("--precisions", default="0,1,4",
                     help="purification solves: 0 all FP64, 1 FP32 MFMA + FP64 sums, 4 split-BF16 x3 (comma list)")
+    ap.add_argument("--symmetric-precisions", default="0,4",
+                    help="purification: also solve with symmetric-result plans for LSL and LSLSL in these precisions")
+    ap.add_argument("--serve-symmetric", type=int, default=0,
+                    help="served stream: batched modes use symmetric-result plans for LSL and LSLSL")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run the streaming copy kernel once over this many MiB before the timed region "

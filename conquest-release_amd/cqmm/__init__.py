@@ -67,7 +67,7 @@ class PlanInfo(C.Structure):
     _fields_ = [("n_triplets", C.c_int64), ("n_out_blocks", C.c_int64), ("n_out_total", C.c_int64),
                 ("flops", C.c_double), ("uniform_nsf", C.c_int32), ("mult_type", C.c_int32),
                 ("kernel", C.c_int32), ("n_tiles", C.c_int32), ("n_entries", C.c_int64), ("n_loads", C.c_int64),
-                ("tile_family", C.c_int32), ("tile_wgs", C.c_int32)]
+                ("tile_family", C.c_int32), ("tile_wgs", C.c_int32), ("n_mirror", C.c_int64)]
 
 
 KERNEL_AUTO, KERNEL_TRIPLET, KERNEL_TILE = 0, 1, 2
@@ -152,7 +152,8 @@ class InfoGlobal(C.Structure):
 EXPORTS = [
     "cqmm_last_error", "cqmm_init", "cqmm_finalize", "cqmm_device_alloc", "cqmm_device_free",
     "cqmm_memcpy_h2d", "cqmm_memcpy_d2h", "cqmm_device_copy", "cqmm_synchronize", "cqmm_timer_start", "cqmm_timer_stop",
-    "cqmm_plan_create", "cqmm_plan_destroy", "cqmm_mat_mult", "cqmm_mat_mult_fp32_noalloc",
+    "cqmm_plan_create", "cqmm_plan_create_symmetric", "cqmm_plan_destroy", "cqmm_mat_mult",
+    "cqmm_mat_mult_fp32_noalloc", "cqmm_sys_plan_create_symmetric",
     "cqmm_noalloc_scratch_bytes", "cqmm_plan_get_info",
     "cqmm_plan_get_triplets", "cqmm_plan_set_kernel", "cqmm_plan_set_tile_workgroups", "cqmm_plan_tune",
     "cqmm_sys_create", "cqmm_sys_destroy", "cqmm_sys_add_range", "cqmm_sys_range_info",
@@ -210,6 +211,9 @@ def lib():
     L.cqmm_grab_matrix_text.argtypes = [C.c_char_p, f64p, C.c_int64]
     L.cqmm_init.argtypes = [C.c_int]
     L.cqmm_plan_create.argtypes = [C.POINTER(MultDesc), C.POINTER(C.c_void_p)]
+    L.cqmm_plan_create_symmetric.argtypes = [C.POINTER(MultDesc), C.c_int64, C.POINTER(C.c_int64),
+                                             C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_void_p)]
     L.cqmm_plan_destroy.argtypes = [C.c_void_p]
     L.cqmm_plan_get_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
     L.cqmm_plan_get_triplets.argtypes = [C.c_void_p, i64p, i64p, i64p]
@@ -230,6 +234,7 @@ def lib():
     L.cqmm_timer_stop.argtypes = [C.POINTER(C.c_float)]
     vp = C.c_void_p
     L.cqmm_sys_plan_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.cqmm_sys_plan_create_symmetric.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.cqmm_sys_matrix_index.argtypes = [vp, C.c_int32, C.POINTER(MatrixIndex)]
     L.cqmm_sys_transpose_pairs.argtypes = [vp, C.c_int32, i64p, i64p, i64p, i32p, i32p]
     L.cqmm_ctx_create.argtypes = [C.POINTER(vp)]
@@ -392,6 +397,18 @@ class System:
     def length(self, rid):
         return self.range_info(rid).length
 
+    def transpose_pairs(self, rid):
+        """Transpose pairs of range rid: (p, q, n1, n2), p <= q 1-based block
+        offsets of blocks (i,j,s) and (j,i,-s), n1 x n2 the dimensions of p."""
+        n = C.c_int64()
+        check(lib().cqmm_sys_transpose_pairs(self.h, rid, C.byref(n), None, None, None, None),
+              "cqmm_sys_transpose_pairs")
+        p, q = np.zeros(n.value, np.int64), np.zeros(n.value, np.int64)
+        a, b = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        check(lib().cqmm_sys_transpose_pairs(self.h, rid, C.byref(n), _p(p, C.c_int64), _p(q, C.c_int64),
+                                             _p(a, C.c_int32), _p(b, C.c_int32)), "cqmm_sys_transpose_pairs")
+        return p, q, a, b
+
     def mult_desc(self, mult_type, ra, rb, rc):
         d = MultDesc()
         check(lib().cqmm_sys_mult_desc(self.h, mult_type, ra, rb, rc, C.byref(d)), "cqmm_sys_mult_desc")
@@ -516,11 +533,22 @@ def init(device=-1):
 class Plan:
     """A triplet plan for one product (one CONQUEST mult(x) entry)."""
 
-    def __init__(self, desc: MultDesc):
+    def __init__(self, desc: MultDesc, pairs=None):
+        """pairs = (p, q, n1, n2): the result is symmetric with these transpose
+        pairs of its blocks (cqmm_plan_create_symmetric); None: every block."""
         init()
         self.desc = desc
         h = C.c_void_p()
-        check(lib().cqmm_plan_create(C.byref(desc), C.byref(h)), "cqmm_plan_create")
+        if pairs is None:
+            check(lib().cqmm_plan_create(C.byref(desc), C.byref(h)), "cqmm_plan_create")
+        else:
+            p, q, n1, n2 = (np.ascontiguousarray(x, t) for x, t in
+                            zip(pairs, (np.int64, np.int64, np.int32, np.int32)))
+            self._pairs = (p, q, n1, n2)
+            ptr = lambda x, t: x.ctypes.data_as(C.POINTER(t))
+            check(lib().cqmm_plan_create_symmetric(C.byref(desc), len(p), ptr(p, C.c_int64), ptr(q, C.c_int64),
+                                                   ptr(n1, C.c_int32), ptr(n2, C.c_int32), C.byref(h)),
+                  "cqmm_plan_create_symmetric")
         self.h = h
         self.mult_type = desc.mult_type
         self.lena, self.lenb, self.lenc = desc.lena, desc.lenb, desc.lenc

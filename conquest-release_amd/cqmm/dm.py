@@ -48,23 +48,18 @@ class Context:
         check(lib().cqmm_sys_matrix_index(self.sys.h, rid, C.byref(idx)), "cqmm_sys_matrix_index")
         check(lib().cqmm_ctx_add_range(self.h, rid, C.byref(idx)), "cqmm_ctx_add_range")
         if transpose:
-            n = C.c_int64()
-            check(lib().cqmm_sys_transpose_pairs(self.sys.h, rid, C.byref(n), None, None, None, None),
-                  "cqmm_sys_transpose_pairs")
-            p = np.zeros(n.value, np.int64)
-            q = np.zeros(n.value, np.int64)
-            a = np.zeros(n.value, np.int32)
-            b = np.zeros(n.value, np.int32)
+            p, q, a, b = self.sys.transpose_pairs(rid)
             ptr = lambda x, t: x.ctypes.data_as(C.POINTER(t))
-            check(lib().cqmm_sys_transpose_pairs(self.sys.h, rid, C.byref(n), ptr(p, C.c_int64), ptr(q, C.c_int64),
-                                                 ptr(a, C.c_int32), ptr(b, C.c_int32)), "cqmm_sys_transpose_pairs")
-            check(lib().cqmm_ctx_set_transpose(self.h, rid, n.value, ptr(p, C.c_int64), ptr(q, C.c_int64),
+            check(lib().cqmm_ctx_set_transpose(self.h, rid, len(p), ptr(p, C.c_int64), ptr(q, C.c_int64),
                                                ptr(a, C.c_int32), ptr(b, C.c_int32)), "cqmm_ctx_set_transpose")
 
-    def add_mult(self, mult_id, mult_type, ra, rb, rc):
-        """mult(mult_id): type 1 C(rc) = A(ra).B(rb); type 2 A(ra) = C(rc).B(rb)^T."""
+    def add_mult(self, mult_id, mult_type, ra, rb, rc, symmetric=False):
+        """mult(mult_id): type 1 C(rc) = A(ra).B(rb); type 2 A(ra) = C(rc).B(rb)^T.
+        symmetric: the result is symmetric; half of its blocks are computed and
+        the others written as their transposes (cqmm_sys_plan_create_symmetric)."""
         plan = C.c_void_p()
-        check(lib().cqmm_sys_plan_create(self.sys.h, mult_type, ra, rb, rc, C.byref(plan)), "cqmm_sys_plan_create")
+        fn = lib().cqmm_sys_plan_create_symmetric if symmetric else lib().cqmm_sys_plan_create
+        check(fn(self.sys.h, mult_type, ra, rb, rc, C.byref(plan)), "cqmm_sys_plan_create")
         check(lib().cqmm_ctx_add_mult(self.h, mult_id, ra, rb, rc, plan), "cqmm_ctx_add_mult")
 
     # -- matrices ----------------------------------------------------------
@@ -170,10 +165,13 @@ class Purifier:
 
     def __init__(self, xyz, box, nsf, r_S=8.0, r_H=8.0, r_L=12.0, r_T=None, grid=None, seed=1,
                  s_amp=0.1, s_decay=1.5, h_amp=-0.2, h_decay=2.0, h_onsite=0.5, h_split=1.5, h_shift=3.0,
-                 device=True, lnv=False):
+                 device=True, lnv=False, symmetric=False):
         r_T = r_S if r_T is None else r_T
         self.r = dict(S=r_S, H=r_H, L=r_L, T=r_T, LS=r_L + r_S, LSL=2 * r_L + r_S, TS=r_T + r_S, TH=r_T + r_H)
         self.lnv_ranges = lnv
+        # LSL = LS.L and LSLSL = LSL.(LS)^T are symmetric: with `symmetric`
+        # their plans compute half of the blocks and mirror the rest.
+        self.symmetric = symmetric
         if lnv:  # LNV / DMM ranges (src/dimens_module.f90:342-355)
             self.r.update(LH=r_L + r_H, SLS=2 * r_S + r_L, TL=r_T + r_L)
         rcover = max(self.r["LSL"], self.r["L"] + self.r["LS"], self.r["LS"] + self.r["L"], self.r["TH"] + r_T,
@@ -226,8 +224,8 @@ class Purifier:
             self.ctx.add_range(name, transpose=(name in trans))
         R = self.rid
         self.ctx.add_mult(L_S_LS, 1, R["L"], R["S"], R["LS"])
-        self.ctx.add_mult(LS_L_LSL, 1, R["LS"], R["L"], R["LSL"])
-        self.ctx.add_mult(LSL_SL_L, 2, R["L"], R["LS"], R["LSL"])
+        self.ctx.add_mult(LS_L_LSL, 1, R["LS"], R["L"], R["LSL"], symmetric=self.symmetric)
+        self.ctx.add_mult(LSL_SL_L, 2, R["L"], R["LS"], R["LSL"], symmetric=self.symmetric)
         self.ctx.add_mult(T_S_TS, 1, R["T"], R["S"], R["TS"])
         self.ctx.add_mult(TS_T_T, 2, R["T"], R["T"], R["TS"])
         self.ctx.add_mult(T_H_TH, 1, R["T"], R["H"], R["TH"])

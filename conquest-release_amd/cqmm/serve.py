@@ -12,7 +12,8 @@ class PurifyParams(C.Structure):
     _fields_ = [("r_S", C.c_double), ("r_H", C.c_double), ("r_L", C.c_double), ("density", C.c_double),
                 ("dmin", C.c_double), ("s_amp", C.c_double), ("s_decay", C.c_double), ("h_amp", C.c_double),
                 ("h_decay", C.c_double), ("h_onsite", C.c_double), ("h_split", C.c_double), ("h_shift", C.c_double),
-                ("precision", C.c_int32), ("max_iter", C.c_int32), ("fp32_tol", C.c_double)]
+                ("precision", C.c_int32), ("max_iter", C.c_int32), ("fp32_tol", C.c_double),
+                ("symmetric", C.c_int32)]
 
 
 class PurifyResult(C.Structure):

@@ -147,6 +147,11 @@ struct TripletPlan {
   DevBuf tk;                // uint8 [n_trip] inner dimension (non-uniform only)
   TilePlan tiles;           // FP64 tile schedule (uniform block sizes), n_tiles == 0 if absent
   int kernel = 0;           // CQMM_KERNEL_* selection for the FP64 path
+  // Symmetric results (cqmm_plan_create_symmetric): of each transpose pair of
+  // result blocks (p, q), p < q, only p is computed; the launchers then write
+  // block q as the transpose of block p (n_mirror pairs, 0 for other plans).
+  int64_t n_mirror = 0;
+  DevBuf mir_p, mir_q, mir_n1, mir_n2;  // int64 element offsets, int64, uint8 rows, uint8 columns of p
 };
 
 // Tile schedule construction from the builder's per-triplet tile keys.
@@ -217,6 +222,12 @@ int launch_block_products(const TripletPlan &tp, const double *P, const double *
 // `mode` (1, 2 or 3, as for launch_tile_products_f32).
 int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, int mode,
                               hipStream_t stream);
+// Symmetric-result plans: write each mirrored block as the transpose of its
+// computed partner (FP64 or FP32 result); nothing for other plans.
+// launch_block_products(_f32) call it; callers that launch a plan's tile
+// schedule themselves call it after the products.
+int mirror_result(const TripletPlan &tp, double *O, hipStream_t stream);
+int mirror_result_f32(const TripletPlan &tp, float *O, hipStream_t stream);
 
 }  // namespace cq
 

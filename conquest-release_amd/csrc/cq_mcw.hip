@@ -286,6 +286,13 @@ int McwEngine::products(int which) {
     std::vector<TileArgs> a;
   };
   std::map<int, Grp> groups;
+  // Symmetric-result plans on the tile path: mirrored after the launches.
+  struct Mirror {
+    const TripletPlan *tp;
+    void *O;
+    bool f32;
+  };
+  std::vector<Mirror> mirrors;
   for (const auto &jp : jobs_) {
     McwJob &j = *jp;
     cqmm_ctx *ctx = j.ctx;
@@ -316,6 +323,7 @@ int McwEngine::products(int which) {
       else MTRY(launch_block_products_f32(tp, (const float *)P, (const float *)Q, (float *)O, mode, s_));
       continue;
     }
+    if (tp.n_mirror > 0) mirrors.push_back({&tp, O, mode != 0});
     if (tp.tiles.n_tiles == 0) continue;
     if (!group_) {
       MTRY(launch_tile_args(tp.tiles, tp.q_trans, mode, tile_launch_args(tp.tiles, P, Q, O), s_));
@@ -344,6 +352,8 @@ int McwEngine::products(int which) {
       MTRY(launch_tile_group(*g.tp, g.qt, g.mode, tg, s_));
     }
   }
+  for (const Mirror &m : mirrors)
+    MTRY(m.f32 ? mirror_result_f32(*m.tp, (float *)m.O, s_) : mirror_result(*m.tp, (double *)m.O, s_));
   return CQMM_OK;
 }
 

@@ -357,8 +357,24 @@ bool uses_tiles(const TripletPlan &tp) {
   return tp.tiles.n_tiles > 0 && tp.kernel != CQMM_KERNEL_TRIPLET && !force_general();
 }
 
+// The launches of a plan's products (tile schedule or per-output-block kernels)
+// without the symmetric-result mirror.
+int products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, int mode, hipStream_t s);
+int products_f64(const TripletPlan &tp, const double *P, const double *Q, double *O, bool fp32, hipStream_t s);
+
 int launch_block_products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, int mode,
                               hipStream_t s) {
+  const int st = products_f32(tp, P, Q, O, mode, s);
+  return st ? st : mirror_result_f32(tp, O, s);
+}
+
+int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O, bool fp32,
+                          hipStream_t s) {
+  const int st = products_f64(tp, P, Q, O, fp32, s);
+  return st ? st : mirror_result(tp, O, s);
+}
+
+int products_f32(const TripletPlan &tp, const float *P, const float *Q, float *O, int mode, hipStream_t s) {
   if (tp.n_seg == 0) return CQMM_OK;
   if (mode < 1 || mode > 4) return set_errorf(CQMM_ERR_ARG, "block products: FP32 mode %d", mode);
   if (uses_tiles(tp))
@@ -373,8 +389,7 @@ int launch_block_products_f32(const TripletPlan &tp, const float *P, const float
   return tp.q_trans ? launch_f32_t<int32_t, true, false>(tp, P, Q, O, s) : launch_f32_t<int32_t, false, false>(tp, P, Q, O, s);
 }
 
-int launch_block_products(const TripletPlan &tp, const double *P, const double *Q, double *O, bool fp32,
-                          hipStream_t s) {
+int products_f64(const TripletPlan &tp, const double *P, const double *Q, double *O, bool fp32, hipStream_t s) {
   if (fp32) return set_error(CQMM_ERR_ARG, "block products: FP32 path not built in this library");
   if (tp.n_seg == 0) return CQMM_OK;
   if (tp.uniform != 0 && tp.uniform != 4 && tp.uniform != 9 && tp.uniform != 13 && tp.uniform != 16) {

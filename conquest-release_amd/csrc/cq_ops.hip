@@ -22,6 +22,23 @@ inline unsigned grid_for(int64_t n, int per_thread = 1) {
   return (unsigned)g;
 }
 
+// Symmetric results: block q (n x m) = transpose of block p (m x n), one
+// wavefront per pair; blocks are column-major at their element offsets.
+template <class T>
+__global__ void k_mirror(int64_t npairs, const int64_t *__restrict__ poff, const int64_t *__restrict__ qoff,
+                         const uint8_t *__restrict__ n1, const uint8_t *__restrict__ n2, T *__restrict__ A) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < npairs;
+       r += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int64_t p = poff[r], q = qoff[r];
+    const int m = n1[r], n = n2[r];
+    for (int e = lane; e < m * n; e += 64) {
+      const int a = e % m, b = e / m;
+      A[q + b + (int64_t)n * a] = A[p + a + (int64_t)m * b];
+    }
+  }
+}
+
 // A coefficient given by value, or (pointer non-null) read from device memory
 // where an earlier kernel on the stream computed it (no host round trip).
 __device__ inline double coef(double v, const double *p) { return p ? *p : v; }
@@ -257,6 +274,17 @@ int op_symmetrise(const PairMap &m, double *a, hipStream_t s) {
                      m.poff.as<int64_t>(), m.qoff.as<int64_t>(), m.n1.as<uint8_t>(), m.n2.as<uint8_t>(), a);
   return hip_check(hipGetLastError(), "symmetrise launch");
 }
+
+template <class T>
+int mirror_launch(const TripletPlan &tp, T *O, hipStream_t s) {
+  if (tp.n_mirror <= 0) return CQMM_OK;
+  hipLaunchKernelGGL(k_mirror<T>, dim3(grid_for(tp.n_mirror * 64, 1)), dim3(kBS), 0, s, tp.n_mirror,
+                     tp.mir_p.as<int64_t>(), tp.mir_q.as<int64_t>(), tp.mir_n1.as<uint8_t>(), tp.mir_n2.as<uint8_t>(), O);
+  return hip_check(hipGetLastError(), "mirror launch");
+}
+
+int mirror_result(const TripletPlan &tp, double *O, hipStream_t s) { return mirror_launch(tp, O, s); }
+int mirror_result_f32(const TripletPlan &tp, float *O, hipStream_t s) { return mirror_launch(tp, O, s); }
 
 int op_transpose(const PairMap &m, const double *a, double *b, hipStream_t s) {
   if (m.npairs <= 0) return CQMM_OK;

@@ -127,6 +127,11 @@ struct DescV {
   const int32_t *i_cc2seq = nullptr;
   int32_t n_cell_parts = 0;
   int64_t lena = 0, lenb = 0, lenc = 0;
+  // Symmetric result: transpose pairs of the result's blocks (1-based element
+  // offsets, n1 x n2 = dimensions of block p); npairs < 0 for other plans.
+  int64_t npairs = -1;
+  const int64_t *pair_p = nullptr, *pair_q = nullptr;
+  const int32_t *pair_n1 = nullptr, *pair_n2 = nullptr;
   int64_t b_nd_offset(int32_t p) const { return b_nd_offset64 ? b_nd_offset64[p] : (int64_t)b_nd_offset32[p]; }
 };
 
@@ -201,7 +206,7 @@ __global__ void __launch_bounds__(256) k_items(
     const int64_t *__restrict__ item_off, int32_t *__restrict__ key_out, int64_t *__restrict__ p_out,
     int64_t *__restrict__ q_out, uint8_t *__restrict__ k_out, DevErr *err, const int32_t *__restrict__ crank,
     const int32_t *__restrict__ tmap, int32_t ncg, int64_t nx, uint64_t *__restrict__ tkey,
-    uint8_t *__restrict__ tslot, const int32_t *__restrict__ rrank) {
+    uint8_t *__restrict__ tslot, const int32_t *__restrict__ rrank, const uint8_t *__restrict__ omask) {
   const int lane = threadIdx.x & 63;
   const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (item >= n_items) return;
@@ -216,6 +221,9 @@ __global__ void __launch_bounds__(256) k_items(
   // Type 1: left operand is A(i,k), the same for every j of the item.
   int64_t pa = 0;
   if (mult_type == 1) pa = (int64_t)a_i_h2d[(int64_t)(i - 1) * ni_a + kh] - 1;
+  // Symmetric results skip the mirrored blocks: type 2's result block (i, k)
+  // is the item's own; type 1's (i, j) is checked per j.
+  const bool item_masked = omask && mult_type == 2 && omask[(int64_t)(i - 1) * ni_a + kh];
   for (int32_t j0 = 0; j0 < m.nbnab; j0 += 64) {
     const int32_t j = j0 + lane;
     bool valid = false;
@@ -226,7 +234,7 @@ __global__ void __launch_bounds__(256) k_items(
       if (jh != 0) {
         pos = (int32_t)(crow + jh - 1);
         ncbeg = c_i_h2d[crow + jh - 1];
-        valid = ncbeg != 0;
+        valid = ncbeg != 0 && !item_masked && !(omask && mult_type == 1 && omask[pos]);
         nd2 = nd2_arr[m.jrow + j];
       }
     }
@@ -451,6 +459,69 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   for (int32_t k = 0; k < ni_a; ++k) maxn = std::max(maxn, A.ndimj[k]);
   for (int32_t k = 0; k < ni_c; ++k) maxn = std::max(maxn, C.ndimj[k]);
 
+  // Symmetric result: of each transpose pair (p < q) of result blocks, block q
+  // is not computed (its products are dropped) and is written afterwards as
+  // the transpose of p. omask marks the result's halo-table positions of the
+  // q blocks (type 1: C's table, type 2: A's).
+  std::vector<uint8_t> omask;
+  std::vector<int64_t> mir_p, mir_q;
+  std::vector<uint8_t> mir_n1, mir_n2;
+  std::vector<int64_t> skip;  // sorted 0-based offsets of the q blocks
+  if (d->npairs >= 0) {
+    const int64_t len_out = d->mult_type == 1 ? d->lenc : d->lena;
+    if (d->npairs > 0 && (!d->pair_p || !d->pair_q || !d->pair_n1 || !d->pair_n2))
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create_symmetric: null pair array");
+    for (int64_t t = 0; t < d->npairs; ++t) {
+      int64_t pp = d->pair_p[t] - 1, qq = d->pair_q[t] - 1;
+      int32_t a = d->pair_n1[t], b = d->pair_n2[t];
+      if (a < 1 || b < 1 || a > 64 || b > 64 || pp < 0 || qq < 0 || pp + (int64_t)a * b > len_out ||
+          qq + (int64_t)a * b > len_out)
+        return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create_symmetric: pair %lld outside the result", (long long)t + 1);
+      if (pp == qq) {
+        if (a != b) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create_symmetric: non-square self pair");
+        continue;
+      }
+      if (pp > qq) {
+        std::swap(pp, qq);
+        std::swap(a, b);
+      }
+      mir_p.push_back(pp);
+      mir_q.push_back(qq);
+      mir_n1.push_back((uint8_t)a);
+      mir_n2.push_back((uint8_t)b);
+      skip.push_back(qq);
+    }
+    std::sort(skip.begin(), skip.end());
+    if (std::adjacent_find(skip.begin(), skip.end()) != skip.end())
+      return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create_symmetric: a block is the second block of two pairs");
+    std::vector<int64_t> firsts(mir_p);
+    std::sort(firsts.begin(), firsts.end());
+    const HaloV<H> &R = d->mult_type == 1 ? C : A;
+    const int32_t nr = R.ni_in_halo;
+    const int64_t npos = (int64_t)d->n_prim * nr;
+    omask.assign((size_t)npos, 0);
+    std::atomic<int64_t> found_q(0), found_p(0);
+    cq::parallel_for(npos, [&](int64_t b0, int64_t b1) {
+      int64_t fq = 0, fp = 0;
+      for (int64_t x = b0; x < b1; ++x) {
+        const int64_t off = (int64_t)R.i_h2d[x] - 1;
+        if (off < 0) continue;
+        if (std::binary_search(skip.begin(), skip.end(), off)) {
+          omask[x] = 1;
+          ++fq;
+        }
+        if (std::binary_search(firsts.begin(), firsts.end(), off)) ++fp;
+      }
+      found_q += fq;
+      found_p += fp;
+    });
+    // Every pair must name two blocks of the result, or the mirror would read
+    // or write outside them.
+    if (found_q.load() != (int64_t)skip.size() || found_p.load() != (int64_t)mir_p.size())
+      return cq::set_errorf(CQMM_ERR_ARG,
+                            "cqmm_plan_create_symmetric: %lld pairs, of which %lld / %lld blocks are blocks of the result",
+                            (long long)mir_p.size(), (long long)found_p.load(), (long long)found_q.load());
+  }
   pt.mark("host index checks");
   // ---- device work ----
   DevBuf d_km, d_bindex, d_ci_halo, d_ci_hbeg, d_ci_h2d, d_ai_h2d, d_iprim, d_itemk, d_ndimi, d_err;
@@ -465,6 +536,8 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   if ((st = upload(d_ndimi, A.ndimi, (size_t)d->n_prim, s))) return st;
   DevErr herr = {0, 0, 0, 0};
   if ((st = upload(d_err, &herr, 1, s))) return st;
+  DevBuf d_omask;
+  if (!omask.empty() && (st = upload(d_omask, omask.data(), omask.size(), s))) return st;
 
   DevBuf d_jh, d_nd2, d_boff;
   if ((st = d_jh.alloc(std::max<int64_t>(jrow, 1) * 4))) return st;
@@ -484,7 +557,8 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
                      d_iprim.as<int32_t>(), n_items, d_jh.as<int32_t>(), d_nd2.as<int32_t>(), d_boff.as<int64_t>(),
                      d_ci_h2d.as<H>(), ni_c, d_ai_h2d.as<H>(), ni_a, d_ndimi.as<int32_t>(),
                      d->mult_type, len_p, d_counts.as<int32_t>(), d_flops.as<int64_t>(), nullptr, nullptr, nullptr,
-                     nullptr, nullptr, d_err.as<DevErr>(), nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr);
+                     nullptr, nullptr, d_err.as<DevErr>(), nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr,
+                     omask.empty() ? nullptr : d_omask.as<uint8_t>());
   if ((st = cq::hip_check(hipGetLastError(), "builder kernel launch"))) return st;
   std::vector<int32_t> counts(n_items);
   std::vector<int64_t> iflops(n_items);
@@ -616,6 +690,19 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
         }
       }, 64);
     }
+    if (!skip.empty()) {
+      // Mirrored blocks are no tile's outputs.
+      size_t w = 0;
+      for (size_t b = 0; b < ob_o.size(); ++b) {
+        if (std::binary_search(skip.begin(), skip.end(), ob_o[b])) continue;
+        ob_r[w] = ob_r[b];
+        ob_c[w] = ob_c[b];
+        ob_o[w++] = ob_o[b];
+      }
+      ob_r.resize(w);
+      ob_c.resize(w);
+      ob_o.resize(w);
+    }
     pt.mark("  tile map: output blocks");
     for (int64_t b = 0; b < (int64_t)ob_o.size() && tiles && uniform; ++b) tiles = ob_o[b] % ns2 == 0;
     if (tiles && nrg * ncg > INT32_MAX) tiles = false;
@@ -689,7 +776,8 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
                      d_p.as<int64_t>(), d_q.as<int64_t>(), uniform ? nullptr : d_k.as<uint8_t>(), d_err.as<DevErr>(),
                      tiles ? d_crank.as<int32_t>() : nullptr, tiles ? d_tmap.as<int32_t>() : nullptr, ncg, nx,
                      tiles ? d_tkey.as<uint64_t>() : nullptr, tiles ? d_tslot.as<uint8_t>() : nullptr,
-                     tiles && !rrank.empty() ? d_rrank.as<int32_t>() : nullptr);
+                     tiles && !rrank.empty() ? d_rrank.as<int32_t>() : nullptr,
+                     omask.empty() ? nullptr : d_omask.as<uint8_t>());
   if ((st = cq::hip_check(hipGetLastError(), "builder fill launch"))) return st;
   if (tiles) {
     cq::TileBuildInput tin;
@@ -841,6 +929,14 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     return cq::set_errorf(CQMM_ERR_RANGE, "cqmm_plan_create: %s (halo atom %d, B neighbour %d, value %lld)",
                           err_name(herr.code), herr.a + 1, herr.b + 1, herr.c);
   pt.mark("segments");
+  if (!mir_p.empty()) {
+    tp.n_mirror = (int64_t)mir_p.size();
+    if ((st = upload(tp.mir_p, mir_p.data(), mir_p.size(), s))) return st;
+    if ((st = upload(tp.mir_q, mir_q.data(), mir_q.size(), s))) return st;
+    if ((st = upload(tp.mir_n1, mir_n1.data(), mir_n1.size(), s))) return st;
+    if ((st = upload(tp.mir_n2, mir_n2.data(), mir_n2.size(), s))) return st;
+    if ((st = cq::hip_check(hipStreamSynchronize(s), "mirror pairs"))) return st;
+  }
   *out = plan.release();
   return CQMM_OK;
 }
@@ -861,14 +957,28 @@ void halo_view64(const cq::RangeIndex &r, HaloV<int64_t> &v) {
   v.i_h2d = r.i_h2d64.data(); v.len_i_halo = (int64_t)r.i_halo.size(); v.len_i_hbeg = (int64_t)r.i_hbeg.size();
 }
 
+// Plan from a system's indices; symmetric: with the result range's transpose pairs.
+int sys_plan(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc, bool symmetric,
+             cqmm_plan **out);
+
 }  // namespace
 
 extern "C" {
 
 int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
+  return cqmm_plan_create_symmetric(d, -1, nullptr, nullptr, nullptr, nullptr, out);
+}
+
+int cqmm_plan_create_symmetric(const cqmm_mult_desc *d, int64_t npairs, const int64_t *p, const int64_t *q,
+                               const int32_t *n1, const int32_t *n2, cqmm_plan **out) {
   return cq::guarded(__func__, [&]() -> int {
     if (!d || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: null argument");
     DescV<int32_t> v;
+    v.npairs = npairs;
+    v.pair_p = p;
+    v.pair_q = q;
+    v.pair_n1 = n1;
+    v.pair_n2 = n2;
     v.mult_type = d->mult_type; v.n_prim = d->n_prim; v.ncovery = d->ncovery; v.ncoverz = d->ncoverz;
     halo_view(d->ahalo, v.ahalo);
     halo_view(d->chalo, v.chalo);
@@ -884,7 +994,21 @@ int cqmm_plan_create(const cqmm_mult_desc *d, cqmm_plan **out) {
 
 int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
                          cqmm_plan **out) {
-  return cq::guarded(__func__, [&]() -> int {
+  return sys_plan(sys, mult_type, ra, rb, rc, false, out);
+}
+
+int cqmm_sys_plan_create_symmetric(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
+                                   cqmm_plan **out) {
+  return sys_plan(sys, mult_type, ra, rb, rc, true, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+int sys_plan(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc, bool symmetric,
+             cqmm_plan **out) {
+  return cq::guarded("cqmm_sys_plan_create", [&]() -> int {
     if (!sys || !out) return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_plan_create: null argument");
     const cq::System &s = sys->s;
     const int32_t nr = (int32_t)s.ranges.size();
@@ -904,9 +1028,32 @@ int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, 
     v.b_len_bindex = (int64_t)Bm.bindex.size();
     v.i_cc2seq = s.i_cc2seq.data(); v.n_cell_parts = s.n_parts;
     v.lena = A.length; v.lenb = Bm.length; v.lenc = Cm.length;
+    std::vector<int64_t> pp, qq;
+    std::vector<int32_t> a, b;
+    if (symmetric) {
+      // The transpose pairs of the result's range (C for type 1, A for type 2).
+      const int32_t rr = mult_type == 1 ? rc : ra;
+      int64_t n = 0;
+      int st = cq::transpose_pairs(s, rr, &n, nullptr, nullptr, nullptr, nullptr);
+      if (st) return st;
+      pp.resize(n);
+      qq.resize(n);
+      a.resize(n);
+      b.resize(n);
+      if ((st = cq::transpose_pairs(s, rr, &n, pp.data(), qq.data(), a.data(), b.data()))) return st;
+      v.npairs = n;
+      v.pair_p = pp.data();
+      v.pair_q = qq.data();
+      v.pair_n1 = a.data();
+      v.pair_n2 = b.data();
+    }
     return build_plan(&v, out);
   });
 }
+
+}  // namespace
+
+extern "C" {
 
 int cqmm_plan_destroy(cqmm_plan *plan) {
   return cq::guarded(__func__, [&]() -> int {
@@ -930,6 +1077,7 @@ int cqmm_plan_get_info(const cqmm_plan *plan, cqmm_plan_info *info) {
     info->n_entries = plan->tp.tiles.n_entries;
     info->n_loads = plan->tp.tiles.n_loads;
     info->tile_family = plan->tp.tiles.n_tiles > 0 ? cq::tile_family(plan->tp.tiles) : 0;
+    info->n_mirror = plan->tp.n_mirror;
     info->tile_wgs = plan->tp.tiles.wgs;
     info->kernel = (plan->tp.tiles.n_tiles > 0 && plan->tp.kernel != CQMM_KERNEL_TRIPLET) ? CQMM_KERNEL_TILE
                                                                                             : CQMM_KERNEL_TRIPLET;

@@ -176,7 +176,11 @@ int prepare(int32_t natoms, int32_t nsf, uint64_t seed, const cqmm_purify_params
                       {TH_T_L, 1, id[TH], id[T], id[Lr]}};
   for (const M &m : mults) {
     cqmm_plan *plan = nullptr;
-    PTRY(cqmm_sys_plan_create(sv.sys, m.type, m.ra, m.rb, m.rc, &plan));
+    // LSL = LS.L and LSLSL = LSL.(LS)^T are symmetric (L and S are).
+    if (p->symmetric && (m.mid == LS_L_LSL || m.mid == LSL_SL_L))
+      PTRY(cqmm_sys_plan_create_symmetric(sv.sys, m.type, m.ra, m.rb, m.rc, &plan));
+    else
+      PTRY(cqmm_sys_plan_create(sv.sys, m.type, m.ra, m.rb, m.rc, &plan));
     PTRY(cqmm_ctx_add_mult(sv.ctx, m.mid, m.ra, m.rb, m.rc, plan));
   }
   cqmm_range_info ri;

@@ -347,6 +347,47 @@ uint64_t block_key(uint64_t seed, uint64_t tag, int32_t gi, int32_t gj, const in
 
 }  // namespace
 
+namespace cq {
+
+// Transpose pairs (p <= q, 1-based offsets) of range rid: block (i,j,s) and
+// block (j,i,-s); p == NULL counts them only.
+int transpose_pairs(const System &s, int32_t rid, int64_t *npairs, int64_t *p, int64_t *q, int32_t *n1, int32_t *n2) {
+  const RangeIndex &r = s.ranges[rid];
+  auto key = [&](int32_t gi, int32_t gj, int32_t sx, int32_t sy, int32_t sz) {
+    return (((((uint64_t)(uint32_t)gi * (uint64_t)s.natoms + (uint32_t)gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32));
+  };
+  std::unordered_map<uint64_t, int64_t> where;
+  where.reserve((size_t)r.row_beg[s.natoms] * 2);
+  std::vector<uint64_t> kt(r.row_beg[s.natoms]);
+  for (int32_t pp = 0; pp < s.natoms; ++pp)
+    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+      const int32_t m = r.nb_cover[e], ic = s.c_part[m];
+      const int32_t *sh = &s.cs_shift[3 * ic];
+      const int32_t gi = s.prim_gid[pp], gj = s.prim_gid[s.c_prim[m]];
+      where[key(gi, gj, sh[0], sh[1], sh[2])] = e;
+      kt[e] = key(gj, gi, -sh[0], -sh[1], -sh[2]);
+    }
+  int64_t n = 0;
+  for (int32_t pp = 0; pp < s.natoms; ++pp)
+    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+      auto it = where.find(kt[e]);
+      if (it == where.end()) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
+      const int64_t f = it->second;
+      if (f < e) continue;
+      if (p) {
+        p[n] = r.nb_off[e] + 1;
+        q[n] = r.nb_off[f] + 1;
+        n1[n] = s.pnsf[pp];
+        n2[n] = r.nb_ndimj[e];
+      }
+      ++n;
+    }
+  *npairs = n;
+  return CQMM_OK;
+}
+
+}  // namespace cq
+
 extern "C" {
 
 int cqmm_random_structure(int32_t natoms, double density, double dmin, uint64_t seed,
@@ -652,41 +693,10 @@ int cqmm_sys_transpose_pairs(const cqmm_system *sys, int32_t rid, int64_t *npair
   return cq::guarded(__func__, [&]() -> int {
     if (!sys || !npairs || rid < 0 || rid >= (int32_t)sys->s.ranges.size())
       return cq::set_error(CQMM_ERR_ARG, "cqmm_sys_transpose_pairs: bad argument");
-    const System &s = sys->s;
-    const RangeIndex &r = s.ranges[rid];
-    auto key = [&](int32_t gi, int32_t gj, int32_t sx, int32_t sy, int32_t sz) {
-      return (((((uint64_t)(uint32_t)gi * (uint64_t)s.natoms + (uint32_t)gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32));
-    };
-    std::unordered_map<uint64_t, int64_t> where;
-    where.reserve((size_t)r.row_beg[s.natoms] * 2);
-    std::vector<uint64_t> kt(r.row_beg[s.natoms]);
-    for (int32_t pp = 0; pp < s.natoms; ++pp)
-      for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
-        const int32_t m = r.nb_cover[e], ic = s.c_part[m];
-        const int32_t *sh = &s.cs_shift[3 * ic];
-        const int32_t gi = s.prim_gid[pp], gj = s.prim_gid[s.c_prim[m]];
-        where[key(gi, gj, sh[0], sh[1], sh[2])] = e;
-        kt[e] = key(gj, gi, -sh[0], -sh[1], -sh[2]);
-      }
-    int64_t n = 0;
-    for (int32_t pp = 0; pp < s.natoms; ++pp)
-      for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
-        auto it = where.find(kt[e]);
-        if (it == where.end()) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
-        const int64_t f = it->second;
-        if (f < e) continue;
-        if (p) {
-          p[n] = r.nb_off[e] + 1;
-          q[n] = r.nb_off[f] + 1;
-          n1[n] = s.pnsf[pp];
-          n2[n] = r.nb_ndimj[e];
-        }
-        ++n;
-      }
-    *npairs = n;
-    return CQMM_OK;
+    return cq::transpose_pairs(sys->s, rid, npairs, p, q, n1, n2);
   });
 }
+
 
 int cqmm_sys_primary_order(const cqmm_system *sys, int32_t *prim_gid) {
   return cq::guarded(__func__, [&]() -> int {

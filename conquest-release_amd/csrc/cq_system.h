@@ -78,6 +78,10 @@ struct System {
   std::vector<RangeIndex> ranges;
 };
 
+// Transpose pairs (p <= q, 1-based offsets) of range rid of the system (the
+// work of cqmm_sys_transpose_pairs); p == NULL counts them only.
+int transpose_pairs(const System &s, int32_t rid, int64_t *npairs, int64_t *p, int64_t *q, int32_t *n1, int32_t *n2);
+
 }  // namespace cq
 
 struct cqmm_system {

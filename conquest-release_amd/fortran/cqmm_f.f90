@@ -55,6 +55,7 @@ module cqmm_f
      integer(c_int32_t) :: kernel, n_tiles
      integer(c_int64_t) :: n_entries, n_loads
      integer(c_int32_t) :: tile_family, tile_wgs
+     integer(c_int64_t) :: n_mirror
   end type cqmm_plan_info
 
   ! type(matrix) index of one range, src/matrix_module.f90:63-82 (include/cqmm_matrix.h)
@@ -138,6 +139,16 @@ module cqmm_f
        type(c_ptr), intent(out) :: plan
        integer(c_int) :: st
      end function cqmm_plan_create
+     function cqmm_plan_create_symmetric(desc, npairs, p, q, n1, n2, plan) &
+         bind(c, name="cqmm_plan_create_symmetric") result(st)
+       import :: c_int, c_ptr, c_int32_t, c_int64_t, cqmm_mult_desc
+       type(cqmm_mult_desc), intent(in) :: desc
+       integer(c_int64_t), value :: npairs
+       integer(c_int64_t), intent(in) :: p(*), q(*)
+       integer(c_int32_t), intent(in) :: n1(*), n2(*)
+       type(c_ptr), intent(out) :: plan
+       integer(c_int) :: st
+     end function cqmm_plan_create_symmetric
      function cqmm_plan_destroy(plan) bind(c, name="cqmm_plan_destroy") result(st)
        import :: c_int, c_ptr
        type(c_ptr), value :: plan
@@ -212,6 +223,14 @@ module cqmm_f
        type(c_ptr) :: plan
        integer(c_int) :: st
      end function cqmm_sys_plan_create
+     function cqmm_sys_plan_create_symmetric(sys, mult_type, ra, rb, rc, plan) &
+         bind(c, name="cqmm_sys_plan_create_symmetric") result(st)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: sys
+       integer(c_int32_t), value :: mult_type, ra, rb, rc
+       type(c_ptr) :: plan
+       integer(c_int) :: st
+     end function cqmm_sys_plan_create_symmetric
      function cqmm_sys_matrix_index(sys, range_id, idx) bind(c, name="cqmm_sys_matrix_index") result(st)
        import :: c_ptr, c_int, c_int32_t, cqmm_matrix_index
        type(c_ptr), value :: sys

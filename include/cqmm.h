@@ -139,6 +139,18 @@ int cqmm_timer_stop(float *ms);
  * same patterns (CONQUEST's mult(x) table entry, src/mult_module.f90:271-1102).
  * Block dimensions (support functions per atom) 1..64; CQMM_ERR_ARG outside. */
 int cqmm_plan_create(const cqmm_mult_desc *desc, cqmm_plan **plan);
+/* A plan for a product whose result is symmetric (no reference counterpart:
+ * CONQUEST computes every block; the caller asserts the symmetry, e.g. LS.L =
+ * LSL and LSL.(LS)^T = LSLSL for symmetric L and S). The npairs transpose
+ * pairs of the result's blocks (1-based element offsets p, q of blocks (i,j,s)
+ * and (j,i,-s), n1 x n2 the dimensions of block p, as cqmm_ctx_set_transpose
+ * takes them; either order) must all be blocks of the result. Of each pair
+ * with p != q only the block at the lower offset is computed; every product
+ * through the plan then writes the other block as its transpose. Half the
+ * block products for a result that differs from the full product only by
+ * rounding (the mirrored blocks are transposes bit for bit). */
+int cqmm_plan_create_symmetric(const cqmm_mult_desc *desc, int64_t npairs, const int64_t *p, const int64_t *q,
+                               const int32_t *n1, const int32_t *n2, cqmm_plan **plan);
 int cqmm_plan_destroy(cqmm_plan *plan);
 
 /* mat_mult (src/multiply_module.f90:115-359) for one process.
@@ -178,6 +190,7 @@ typedef struct cqmm_plan_info {
   int64_t n_loads;             /* operand blocks the tile kernel reads        */
   int32_t tile_family;         /* tile kernel: CQMM_TILES_* below (0: none)   */
   int32_t tile_wgs;            /* class tiles: persistent grid (0: one workgroup per tile) */
+  int64_t n_mirror;            /* symmetric-result plans: blocks written as transposes (0: none) */
 } cqmm_plan_info;
 #define CQMM_TILES_DENSE4 1    /* 4x4 blocks, quadrant MFMA tiles            */
 #define CQMM_TILES_DENSE_MIXED 2  /* mixed blocks up to 4x4, quadrant MFMA tiles */

@@ -27,6 +27,9 @@ typedef struct cqmm_purify_params {
   int32_t max_iter;         /* McWeeny iterations at most                     */
   double fp32_tol;          /* mixed: FP32 phase also ends when |dE| < fp32_tol*|E|
                                (0: only the McWeeny stop tests end it)        */
+  int32_t symmetric;        /* 1: LSL and LSLSL through symmetric-result plans
+                               (cqmm_plan_create_symmetric: half the blocks
+                               computed, the rest mirrored); 0: every block    */
 } cqmm_purify_params;
 
 typedef struct cqmm_purify_result {

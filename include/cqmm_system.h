@@ -70,6 +70,10 @@ int cqmm_sys_mult_desc(const cqmm_system *sys, int32_t mult_type, int32_t ra,
  * products and order as cqmm_plan_create on cqmm_sys_mult_desc. */
 int cqmm_sys_plan_create(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
                          cqmm_plan **plan);
+/* The same as cqmm_plan_create_symmetric with the transpose pairs of the
+ * result's range (rc for type 1, ra for type 2). */
+int cqmm_sys_plan_create_symmetric(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, int32_t rc,
+                                   cqmm_plan **plan);
 
 /* type(matrix) index of a range as cqmm_ctx_add_range takes it (pointers stay
  * valid while the system lives). Requires cqmm_matrix.h for the struct. */
