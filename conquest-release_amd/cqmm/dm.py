@@ -234,9 +234,11 @@ class Purifier:
             self.ctx.add_mult(L_H_LH, 1, R["L"], R["H"], R["LH"])
             self.ctx.add_mult(HL_S_LSL, 1, R["LH"], R["S"], R["LSL"])
             self.ctx.add_mult(HLS_LS_L, 2, R["L"], R["LS"], R["LSL"])
-            self.ctx.add_mult(SL_S_SLS, 1, R["LS"], R["S"], R["SLS"])
-            self.ctx.add_mult(SLS_LS_L, 2, R["L"], R["LS"], R["SLS"])
-            self.ctx.add_mult(LSL_SL_H, 2, R["H"], R["LS"], R["LSL"])
+            # SLS = SL.S, SLSLS = SLS.(SL)^T and LSLSL = LSL.(LS)^T are
+            # symmetric too; HLS and (2HLS + SLH).(SL)^T are not.
+            self.ctx.add_mult(SL_S_SLS, 1, R["LS"], R["S"], R["SLS"], symmetric=self.symmetric)
+            self.ctx.add_mult(SLS_LS_L, 2, R["L"], R["LS"], R["SLS"], symmetric=self.symmetric)
+            self.ctx.add_mult(LSL_SL_H, 2, R["H"], R["LS"], R["LSL"], symmetric=self.symmetric)
             self.ctx.add_mult(T_L_TL, 1, R["T"], R["L"], R["TL"])
             self.ctx.add_mult(TL_T_L, 2, R["L"], R["T"], R["TL"])
         self.matS = self.ctx.allocate(R["S"])

@@ -28,11 +28,12 @@ def cluster(natoms=9, nsf=4, seed=3):
     return Purifier(xyz, box, nsf, r_S=8.0, r_H=8.0, r_L=12.0, grid=(5, 5, 5), seed=seed, lnv=True)
 
 
-def periodic(natoms=24, nsf=4, seed=5):
+def periodic(natoms=24, nsf=4, seed=5, symmetric=False):
     xyz, edge = cqmm.random_structure(natoms, 0.05, 2.0, seed)
     if isinstance(nsf, list):  # a species pattern repeated over the atoms (mixed block sizes)
         nsf = (nsf * natoms)[:natoms]
-    return Purifier(xyz, edge, nsf, r_S=4.5, r_H=4.5, r_L=6.0, grid=(2, 2, 2), seed=seed, lnv=True)
+    return Purifier(xyz, edge, nsf, r_S=4.5, r_H=4.5, r_L=6.0, grid=(2, 2, 2), seed=seed, lnv=True,
+                    symmetric=symmetric)
 
 
 def dense_of(pu, name, vals):
@@ -163,11 +164,14 @@ def test_findmindm_cluster_converges_to_projector(nsf):
     assert np.abs(K - P).max() < 1e-4
 
 
-def test_findmindm_periodic_matches_oracle():
+@pytest.mark.parametrize("symmetric", [False, True], ids=["full", "symmetric_plans"])
+def test_findmindm_periodic_matches_oracle(symmetric):
     """Truncated ranges on a periodic cell: the device loop and the oracle's,
-    started from the same L and T, take the same path to the residual bar."""
-    pu = periodic(natoms=24, nsf=4)
-    print("findmindm periodic vs oracle", flush=True)
+    started from the same L and T, take the same path to the residual bar
+    (also with symmetric-result plans for the symmetric products of McWeeny
+    and LNV: LSL, LSLSL, SLS, SLSLS, which change results by rounding only)."""
+    pu = periodic(natoms=24, nsf=4, symmetric=symmetric)
+    print(f"findmindm periodic vs oracle (symmetric plans: {symmetric})", flush=True)
     ne = float(pu.n_orb // 2)
     pu.hotelling()
     pu.init_mcweeny(ne)
