@@ -242,6 +242,7 @@ def multiply_workload(name, args, fp32_modes=True, cpu=False):
     for x in (da, db, dc):
         x.free()
     plan.close()
+    # The descriptor's arrays live in the system, which it keeps alive.
     host = (desc, a, b, c) if cpu else None
     return res, host
 

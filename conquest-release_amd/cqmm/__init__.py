@@ -410,8 +410,11 @@ class System:
         return p, q, a, b
 
     def mult_desc(self, mult_type, ra, rb, rc):
+        """Descriptor of a product; its arrays point into this system's memory,
+        so the descriptor keeps the system alive."""
         d = MultDesc()
         check(lib().cqmm_sys_mult_desc(self.h, mult_type, ra, rb, rc, C.byref(d)), "cqmm_sys_mult_desc")
+        d._system = self
         return d
 
     def blocks(self, rid):

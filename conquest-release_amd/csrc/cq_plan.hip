@@ -501,20 +501,35 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     const int64_t npos = (int64_t)d->n_prim * nr;
     omask.assign((size_t)npos, 0);
     std::atomic<int64_t> found_q(0), found_p(0);
-    cq::parallel_for(npos, [&](int64_t b0, int64_t b1) {
+    // Row by row: the row's blocks lie in [lo, hi] of the result, so each
+    // lookup searches only that row's part of the sorted offsets.
+    cq::parallel_for(d->n_prim, [&](int64_t r0, int64_t r1) {
       int64_t fq = 0, fp = 0;
-      for (int64_t x = b0; x < b1; ++x) {
-        const int64_t off = (int64_t)R.i_h2d[x] - 1;
-        if (off < 0) continue;
-        if (std::binary_search(skip.begin(), skip.end(), off)) {
-          omask[x] = 1;
-          ++fq;
+      for (int64_t row = r0; row < r1; ++row) {
+        const H *h2d = R.i_h2d + row * nr;
+        int64_t lo = INT64_MAX, hi = -1;
+        for (int32_t j = 0; j < nr; ++j)
+          if (h2d[j] != 0) {
+            lo = std::min<int64_t>(lo, (int64_t)h2d[j] - 1);
+            hi = std::max<int64_t>(hi, (int64_t)h2d[j] - 1);
+          }
+        if (hi < 0) continue;
+        const auto sb = std::lower_bound(skip.begin(), skip.end(), lo), se = std::upper_bound(sb, skip.end(), hi);
+        const auto fb = std::lower_bound(firsts.begin(), firsts.end(), lo),
+                   fe = std::upper_bound(fb, firsts.end(), hi);
+        for (int32_t j = 0; j < nr; ++j) {
+          const int64_t off = (int64_t)h2d[j] - 1;
+          if (off < 0) continue;
+          if (std::binary_search(sb, se, off)) {
+            omask[row * nr + j] = 1;
+            ++fq;
+          }
+          if (std::binary_search(fb, fe, off)) ++fp;
         }
-        if (std::binary_search(firsts.begin(), firsts.end(), off)) ++fp;
       }
       found_q += fq;
       found_p += fp;
-    });
+    }, 1);
     // Every pair must name two blocks of the result, or the mirror would read
     // or write outside them.
     if (found_q.load() != (int64_t)skip.size() || found_p.load() != (int64_t)mir_p.size())
@@ -1033,15 +1048,9 @@ int sys_plan(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, 
     if (symmetric) {
       // The transpose pairs of the result's range (C for type 1, A for type 2).
       const int32_t rr = mult_type == 1 ? rc : ra;
-      int64_t n = 0;
-      int st = cq::transpose_pairs(s, rr, &n, nullptr, nullptr, nullptr, nullptr);
+      const int st = cq::transpose_pairs(s, rr, pp, qq, a, b);
       if (st) return st;
-      pp.resize(n);
-      qq.resize(n);
-      a.resize(n);
-      b.resize(n);
-      if ((st = cq::transpose_pairs(s, rr, &n, pp.data(), qq.data(), a.data(), b.data()))) return st;
-      v.npairs = n;
+      v.npairs = (int64_t)pp.size();
       v.pair_p = pp.data();
       v.pair_q = qq.data();
       v.pair_n1 = a.data();

@@ -350,39 +350,85 @@ uint64_t block_key(uint64_t seed, uint64_t tag, int32_t gi, int32_t gj, const in
 namespace cq {
 
 // Transpose pairs (p <= q, 1-based offsets) of range rid: block (i,j,s) and
-// block (j,i,-s); p == NULL counts them only.
+// block (j,i,-s); p == NULL counts them only. Each row's blocks are sorted by
+// (column atom, image shift); the partner of block e = (i, j, s) is found by a
+// binary search for (i, -s) in the row of j. Pairs come out in storage order
+// of their first block.
 int transpose_pairs(const System &s, int32_t rid, int64_t *npairs, int64_t *p, int64_t *q, int32_t *n1, int32_t *n2) {
+  std::vector<int64_t> vp, vq;
+  std::vector<int32_t> va, vb;
+  const int st = transpose_pairs(s, rid, vp, vq, va, vb);
+  if (st) return st;
+  *npairs = (int64_t)vp.size();
+  if (p) {
+    std::copy(vp.begin(), vp.end(), p);
+    std::copy(vq.begin(), vq.end(), q);
+    std::copy(va.begin(), va.end(), n1);
+    std::copy(vb.begin(), vb.end(), n2);
+  }
+  return CQMM_OK;
+}
+
+int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::vector<int64_t> &q,
+                    std::vector<int32_t> &n1, std::vector<int32_t> &n2) {
   const RangeIndex &r = s.ranges[rid];
-  auto key = [&](int32_t gi, int32_t gj, int32_t sx, int32_t sy, int32_t sz) {
-    return (((((uint64_t)(uint32_t)gi * (uint64_t)s.natoms + (uint32_t)gj) * 64 + (sx + 32)) * 64 + (sy + 32)) * 64 + (sz + 32));
+  const int32_t na = s.natoms;
+  const int64_t nb = r.row_beg[na];
+  auto key = [](int32_t gj, const int32_t *sh) {
+    return (((uint64_t)(uint32_t)gj * 64 + (uint64_t)(sh[0] + 32)) * 64 + (uint64_t)(sh[1] + 32)) * 64 +
+           (uint64_t)(sh[2] + 32);
   };
-  std::unordered_map<uint64_t, int64_t> where;
-  where.reserve((size_t)r.row_beg[s.natoms] * 2);
-  std::vector<uint64_t> kt(r.row_beg[s.natoms]);
-  for (int32_t pp = 0; pp < s.natoms; ++pp)
+  std::vector<int32_t> prim_of(na, -1);
+  for (int32_t pp = 0; pp < na; ++pp) prim_of[s.prim_gid[pp]] = pp;
+  // Per row: (key of the block's column atom and shift, block), sorted by key.
+  std::vector<std::pair<uint64_t, int64_t>> rowkey(nb);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int32_t pp = 0; pp < na; ++pp) {
     for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
-      const int32_t m = r.nb_cover[e], ic = s.c_part[m];
-      const int32_t *sh = &s.cs_shift[3 * ic];
-      const int32_t gi = s.prim_gid[pp], gj = s.prim_gid[s.c_prim[m]];
-      where[key(gi, gj, sh[0], sh[1], sh[2])] = e;
-      kt[e] = key(gj, gi, -sh[0], -sh[1], -sh[2]);
+      const int32_t m = r.nb_cover[e];
+      rowkey[e] = {key(s.prim_gid[s.c_prim[m]], &s.cs_shift[3 * s.c_part[m]]), e};
     }
+    std::sort(rowkey.begin() + r.row_beg[pp], rowkey.begin() + r.row_beg[pp + 1]);
+  }
+  // Partner of every block (-1: none).
+  std::vector<int64_t> partner(nb, -1);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int32_t pp = 0; pp < na; ++pp) {
+    const int32_t gi = s.prim_gid[pp];
+    for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
+      const int32_t m = r.nb_cover[e];
+      const int32_t *sh = &s.cs_shift[3 * s.c_part[m]];
+      const int32_t neg[3] = {-sh[0], -sh[1], -sh[2]};
+      const int32_t pj = prim_of[s.prim_gid[s.c_prim[m]]];
+      if (pj < 0) continue;
+      const uint64_t k = key(gi, neg);
+      auto b = rowkey.begin() + r.row_beg[pj], en = rowkey.begin() + r.row_beg[pj + 1];
+      auto it = std::lower_bound(b, en, std::make_pair(k, (int64_t)-1));
+      if (it != en && it->first == k) partner[e] = it->second;
+    }
+  }
   int64_t n = 0;
-  for (int32_t pp = 0; pp < s.natoms; ++pp)
+  for (int64_t e = 0; e < nb; ++e) {
+    const int64_t f = partner[e];
+    if (f < 0) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
+    if (f < e) continue;
+    ++n;
+  }
+  p.resize(n);
+  q.resize(n);
+  n1.resize(n);
+  n2.resize(n);
+  int64_t t = 0;
+  for (int32_t pp = 0; pp < na; ++pp)
     for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
-      auto it = where.find(kt[e]);
-      if (it == where.end()) return cq::set_error(CQMM_ERR_RANGE, "cqmm_sys_transpose_pairs: pattern not symmetric");
-      const int64_t f = it->second;
+      const int64_t f = partner[e];
       if (f < e) continue;
-      if (p) {
-        p[n] = r.nb_off[e] + 1;
-        q[n] = r.nb_off[f] + 1;
-        n1[n] = s.pnsf[pp];
-        n2[n] = r.nb_ndimj[e];
-      }
-      ++n;
+      p[t] = r.nb_off[e] + 1;
+      q[t] = r.nb_off[f] + 1;
+      n1[t] = s.pnsf[pp];
+      n2[t] = r.nb_ndimj[e];
+      ++t;
     }
-  *npairs = n;
   return CQMM_OK;
 }
 
