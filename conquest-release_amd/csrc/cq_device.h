@@ -175,6 +175,10 @@ bool tiles_enabled();
 // (CQMM_MIXED_TILES=class; default: primary/Morton order and the per-product
 // mixed-size kernel, which keeps tiles spatially compact).
 bool class_tiles_enabled();
+// Tile rows of plans built from a system in the column (Morton) order of the
+// atoms' own copies (CQMM_TILE_ROWS=morton; symmetric-result plans always).
+bool morton_rows_enabled();
+bool primary_rows_forced();
 // Uniform 9, 13, 16 blocks as one class of the class-tile schedule
 // (CQMM_UNIFORM_TILES=class; default: the fixed-size tile kernel).
 bool uniform_class_tiles_enabled();

@@ -132,6 +132,9 @@ struct DescV {
   int64_t npairs = -1;
   const int64_t *pair_p = nullptr, *pair_q = nullptr;
   const int32_t *pair_n1 = nullptr, *pair_n2 = nullptr;
+  // Tile row order (rank of each primary atom, 0..n_prim-1), or null for
+  // primary order.
+  const int32_t *row_rank = nullptr;
   int64_t b_nd_offset(int32_t p) const { return b_nd_offset64 ? b_nd_offset64[p] : (int64_t)b_nd_offset32[p]; }
 };
 
@@ -481,10 +484,6 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
         if (a != b) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create_symmetric: non-square self pair");
         continue;
       }
-      if (pp > qq) {
-        std::swap(pp, qq);
-        std::swap(a, b);
-      }
       mir_p.push_back(pp);
       mir_q.push_back(qq);
       mir_n1.push_back((uint8_t)a);
@@ -646,12 +645,24 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
     const int32_t ncol = CH.ni_in_halo;
     const int32_t *cdims = d->mult_type == 1 ? C.ndimj : A.ndimj;
     int64_t max_rrank = d->n_prim - 1;
+    bool class_rows = false;
+    if (d->row_rank) {
+      // A permutation of the primary atoms (each tile slot holds one block).
+      std::vector<uint8_t> seen(d->n_prim, 0);
+      for (int32_t p = 0; p < d->n_prim; ++p) {
+        const int32_t v = d->row_rank[p];
+        if (v < 0 || v >= d->n_prim || seen[v]) return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create: bad row order");
+        seen[v] = 1;
+      }
+      rrank.assign(d->row_rank, d->row_rank + d->n_prim);
+    }
     if (!uniform && cq::class_tiles_enabled()) {
+      class_rows = true;
       // Species mixes: tile rows (primary order) and columns (Morton order)
       // grouped by block size, each group starting at a multiple of 8, so that
       // every tile has one row block size and one column block size.
       std::vector<int32_t> prim(d->n_prim);
-      for (int32_t p = 0; p < d->n_prim; ++p) prim[p] = p;
+      for (int32_t p = 0; p < d->n_prim; ++p) prim[p] = rrank.empty() ? p : rrank[p];
       rrank = cq::group_ranks_by_size(prim, A.ndimi);
       crank = cq::group_ranks_by_size(crank, cdims);
       max_rrank = 0;
@@ -750,7 +761,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
           tcdim[(size_t)tile * 8 + (cr & 7)] = (uint8_t)cdims[ob_c[b]];
         }
         // One row size and one column size per tile (the class-tile kernel's condition).
-        class_uniform = uclass || !rrank.empty();
+        class_uniform = uclass || class_rows;
         for (int32_t t = 0; t < n_tiles && class_uniform; ++t) {
           int rsz = 0, csz = 0;
           for (int k = 0; k < 8; ++k) {
@@ -1043,13 +1054,56 @@ int sys_plan(const cqmm_system *sys, int32_t mult_type, int32_t ra, int32_t rb, 
     v.b_len_bindex = (int64_t)Bm.bindex.size();
     v.i_cc2seq = s.i_cc2seq.data(); v.n_cell_parts = s.n_parts;
     v.lena = A.length; v.lenb = Bm.length; v.lenc = Cm.length;
-    std::vector<int64_t> pp, qq;
-    std::vector<int32_t> a, b;
+    std::vector<int64_t> pp, qq, blk;
+    std::vector<int32_t> a, b, row_rank;
+    const int32_t rr = mult_type == 1 ? rc : ra;  // the result's range
+    const cq::RangeIndex &O = s.ranges[rr];
+    std::vector<int32_t> crank;
+    const bool morton = cq::morton_rows_enabled() || (symmetric && !cq::primary_rows_forced());
+    if (symmetric || morton) crank = cq::halo_morton_rank(O.np_in_halo, O.nh_part.data(), O.lab_hcover.data(),
+                                                          s.ncover[1], s.ncover[2]);
+    if (morton) {
+      // Tile rows in the order of the result's columns: each primary atom at
+      // the column rank (Morton order of the halo, as the tile columns) of its
+      // own unshifted copy, so that a tile's rows and columns are the same
+      // compact groups of atoms.
+      std::vector<int64_t> home(s.natoms, -1);
+      for (int32_t h = 0; h < O.ni_in_halo; ++h) {
+        const int32_t m = O.halo_cover[h];
+        const int32_t *sh = &s.cs_shift[3 * s.c_part[m]];
+        if (sh[0] == 0 && sh[1] == 0 && sh[2] == 0) home[s.c_prim[m]] = crank[h];
+      }
+      if (std::find(home.begin(), home.end(), (int64_t)-1) == home.end()) {
+        std::vector<int32_t> order(s.natoms);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return home[x] < home[y]; });
+        row_rank.resize(s.natoms);
+        for (int32_t k = 0; k < s.natoms; ++k) row_rank[order[k]] = k;
+        v.row_rank = row_rank.data();
+      }
+    }
     if (symmetric) {
       // The transpose pairs of the result's range (C for type 1, A for type 2).
-      const int32_t rr = mult_type == 1 ? rc : ra;
-      const int st = cq::transpose_pairs(s, rr, pp, qq, a, b);
+      const int st = cq::transpose_pairs(s, rr, pp, qq, a, b, &blk);
       if (st) return st;
+      {
+        // Of each pair compute the block in the lower tile (row group, column
+        // group): with the rows ordered like the columns, a tile whose rows
+        // and columns are different atoms keeps all or none of its blocks.
+        auto tile_key = [&](int64_t e) {
+          const int32_t row = (int32_t)(std::upper_bound(O.row_beg.begin(), O.row_beg.end(), e) - O.row_beg.begin()) - 1;
+          const int32_t h = O.i_halo[O.nb_cover[e]] - 1;
+          const int32_t rk = row_rank.empty() ? row : row_rank[row];
+          return ((uint64_t)(uint32_t)(rk >> 3) << 32) | (uint32_t)(crank[h] >> 3);
+        };
+        cq::parallel_for((int64_t)pp.size(), [&](int64_t t0, int64_t t1) {
+          for (int64_t t = t0; t < t1; ++t)
+            if (tile_key(blk[2 * t + 1]) < tile_key(blk[2 * t])) {
+              std::swap(pp[t], qq[t]);
+              std::swap(a[t], b[t]);
+            }
+        });
+      }
       v.npairs = (int64_t)pp.size();
       v.pair_p = pp.data();
       v.pair_q = qq.data();

@@ -370,7 +370,7 @@ int transpose_pairs(const System &s, int32_t rid, int64_t *npairs, int64_t *p, i
 }
 
 int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::vector<int64_t> &q,
-                    std::vector<int32_t> &n1, std::vector<int32_t> &n2) {
+                    std::vector<int32_t> &n1, std::vector<int32_t> &n2, std::vector<int64_t> *blocks) {
   const RangeIndex &r = s.ranges[rid];
   const int32_t na = s.natoms;
   const int64_t nb = r.row_beg[na];
@@ -418,6 +418,7 @@ int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::
   q.resize(n);
   n1.resize(n);
   n2.resize(n);
+  if (blocks) blocks->resize(2 * n);
   int64_t t = 0;
   for (int32_t pp = 0; pp < na; ++pp)
     for (int64_t e = r.row_beg[pp]; e < r.row_beg[pp + 1]; ++e) {
@@ -427,6 +428,10 @@ int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::
       q[t] = r.nb_off[f] + 1;
       n1[t] = s.pnsf[pp];
       n2[t] = r.nb_ndimj[e];
+      if (blocks) {
+        (*blocks)[2 * t] = e;
+        (*blocks)[2 * t + 1] = f;
+      }
       ++t;
     }
   return CQMM_OK;

@@ -81,8 +81,10 @@ struct System {
 // Transpose pairs (p <= q, 1-based offsets) of range rid of the system (the
 // work of cqmm_sys_transpose_pairs); p == NULL counts them only.
 int transpose_pairs(const System &s, int32_t rid, int64_t *npairs, int64_t *p, int64_t *q, int32_t *n1, int32_t *n2);
+// The same into vectors; blocks (if given) receives the two blocks' indices
+// in storage order of each pair.
 int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::vector<int64_t> &q,
-                    std::vector<int32_t> &n1, std::vector<int32_t> &n2);
+                    std::vector<int32_t> &n1, std::vector<int32_t> &n2, std::vector<int64_t> *blocks = nullptr);
 
 }  // namespace cq
 

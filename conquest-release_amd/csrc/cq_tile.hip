@@ -1782,6 +1782,19 @@ bool class_tiles_enabled() {
   return e && std::string(e) == "class";
 }
 
+// CQMM_TILE_ROWS=morton: plans built from a system order their tile rows like
+// the tile columns (symmetric-result plans always do).
+bool morton_rows_enabled() {
+  const char *e = getenv("CQMM_TILE_ROWS");
+  return e && std::string(e) == "morton";
+}
+
+// CQMM_TILE_ROWS=primary: symmetric-result plans keep primary-order tile rows.
+bool primary_rows_forced() {
+  const char *e = getenv("CQMM_TILE_ROWS");
+  return e && std::string(e) == "primary";
+}
+
 bool uniform_class_tiles_enabled() {
   const char *e = getenv("CQMM_UNIFORM_TILES");
   return e && std::string(e) == "class";

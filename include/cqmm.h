@@ -144,11 +144,12 @@ int cqmm_plan_create(const cqmm_mult_desc *desc, cqmm_plan **plan);
  * LSL and LSL.(LS)^T = LSLSL for symmetric L and S). The npairs transpose
  * pairs of the result's blocks (1-based element offsets p, q of blocks (i,j,s)
  * and (j,i,-s), n1 x n2 the dimensions of block p, as cqmm_ctx_set_transpose
- * takes them; either order) must all be blocks of the result. Of each pair
- * with p != q only the block at the lower offset is computed; every product
- * through the plan then writes the other block as its transpose. Half the
- * block products for a result that differs from the full product only by
- * rounding (the mirrored blocks are transposes bit for bit). */
+ * takes them) must all be blocks of the result. Of each pair with p != q only
+ * block p is computed; every product through the plan then writes block q as
+ * its transpose. Half the block products for a result that differs from the
+ * full product only by rounding (the mirrored blocks are transposes bit for
+ * bit). cqmm_sys_plan_create_symmetric orders each pair so that the computed
+ * blocks fill whole tiles of the schedule. */
 int cqmm_plan_create_symmetric(const cqmm_mult_desc *desc, int64_t npairs, const int64_t *p, const int64_t *q,
                                const int32_t *n1, const int32_t *n2, cqmm_plan **plan);
 int cqmm_plan_destroy(cqmm_plan *plan);

@@ -6,7 +6,8 @@ for fixed populations, the direct sum for variable ones); the McWMin loop over
 both channels matches the oracle's two-channel loop (stop step within one,
 energies to 1e-8 relative, L to 1e-8); identical channels give the
 spin-unpolarised device loop bit for bit (fixed populations: every step;
-variable populations: c1 and c2 both double, so c is the same); with spin down
+variable populations: c1 and c2 both double, so c is the same); the same
+against the oracle with symmetric-result plans for LSL and LSLSL; with spin down
 shifted above the gap the variable-population loop moves electrons to spin up
 and ends on the lowest levels of both spectra; mixed precision on a
 variable-population pair stays within 1e-6 Ha/atom of the FP64 oracle.
@@ -22,9 +23,9 @@ from oracle import mcweeny as mo
 pytestmark = pytest.mark.gpu
 
 
-def spin_cluster(natoms=8, nsf=4, seed=3, shift=0.25):
+def spin_cluster(natoms=8, nsf=4, seed=3, shift=0.25, symmetric=False):
     xyz, box = cluster_structure(natoms, seed=seed)
-    pu = Purifier(xyz, box, nsf, r_S=8.0, r_H=8.0, r_L=12.0, grid=(5, 5, 5), seed=seed)
+    pu = Purifier(xyz, box, nsf, r_S=8.0, r_H=8.0, r_L=12.0, grid=(5, 5, 5), seed=seed, symmetric=symmetric)
     s, R = pu.sys, pu.rid
     bS, bH = s.blocks(R["S"]), s.blocks(R["H"])
     S = mo.densify(bS, pu.S_vals, pu.nsf)
@@ -61,9 +62,10 @@ def test_spin_init_matches_dense_oracle(fixed):
     np.testing.assert_allclose(L0d, ref_d, atol=1e-12)
 
 
+@pytest.mark.parametrize("symmetric", [False, True], ids=["full", "symmetric_plans"])
 @pytest.mark.parametrize("fixed", [True, False], ids=["fixed", "variable"])
-def test_spin_mcweeny_matches_sparse_oracle(fixed):
-    pu, S, Hu, Hd = spin_cluster(shift=4.0)
+def test_spin_mcweeny_matches_sparse_oracle(fixed, symmetric):
+    pu, S, Hu, Hd = spin_cluster(shift=4.0, symmetric=symmetric)
     ne = pu.n_orb // 2
     pu.init_mcweeny_spin(ne, ne, fixed)
     L0 = [pu.ctx.get(pu.matL), pu.ctx.get(pu.matL_dn)]
