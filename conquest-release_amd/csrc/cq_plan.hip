@@ -462,7 +462,7 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
   for (int32_t k = 0; k < ni_a; ++k) maxn = std::max(maxn, A.ndimj[k]);
   for (int32_t k = 0; k < ni_c; ++k) maxn = std::max(maxn, C.ndimj[k]);
 
-  // Symmetric result: of each transpose pair (p < q) of result blocks, block q
+  // Symmetric result: of each transpose pair (p, q) of result blocks, block q
   // is not computed (its products are dropped) and is written afterwards as
   // the transpose of p. omask marks the result's halo-table positions of the
   // q blocks (type 1: C's table, type 2: A's).
@@ -495,6 +495,10 @@ int build_plan(const DescV<H> *d, cqmm_plan **out) {
       return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create_symmetric: a block is the second block of two pairs");
     std::vector<int64_t> firsts(mir_p);
     std::sort(firsts.begin(), firsts.end());
+    // A computed block must not be mirrored by another pair.
+    for (int64_t x : firsts)
+      if (std::binary_search(skip.begin(), skip.end(), x))
+        return cq::set_error(CQMM_ERR_ARG, "cqmm_plan_create_symmetric: a block is in two pairs");
     const HaloV<H> &R = d->mult_type == 1 ? C : A;
     const int32_t nr = R.ni_in_halo;
     const int64_t npos = (int64_t)d->n_prim * nr;

@@ -117,7 +117,10 @@ def test_symmetric_plan_refuses_pairs_outside_the_result():
         Plan(desc, pairs=(p[off] + 1, q[off], n1[off], n2[off]))
     with pytest.raises(CqmmError):
         Plan(desc, pairs=(p, q + pu.sys.length(R["LSL"]), n1, n2))
-    Plan(desc, pairs=(q, p, n2, n1))  # either order of a pair is accepted
+    k = int(np.nonzero(off)[0][0])
+    with pytest.raises(CqmmError):  # one block computed by one pair and mirrored by another
+        Plan(desc, pairs=(np.append(p, q[k]), np.append(q, p[k]), np.append(n1, n2[k]), np.append(n2, n1[k])))
+    Plan(desc, pairs=(q, p, n2, n1))  # either block of a pair may be the computed one
 
 
 @pytest.mark.parametrize("structure", ["random24", "mix48"])
