@@ -272,7 +272,8 @@ def purification_pair(args, natoms=2048, nsf=13):
             f"(McWeeny {r['mcw_s']:.2f} s), {r['iterations']} it ({r['fp32_iterations']} reduced precision), "
             f"E_band {r['band_energy']:.10f} Ha, Ne error {r['ne_error']:.2e}")
     e64 = out[(0, 0)]
-    names = {0: "fp64", 1: "fp32_mfma_fp64_sums", 2: "fp32_accumulate", 3: "fp32_chunk_runs", 4: "split_bf16x3"}
+    names = {0: "fp64", 1: "fp32_mfma_fp64_sums", 2: "fp32_accumulate", 3: "fp32_chunk_runs", 4: "split_bf16x3",
+             5: "split_bf16x3_then_fp32"}
     modes = {}
     for prec, symm in runs:
         if (prec, symm) == (0, 0):

@@ -95,6 +95,7 @@ struct McwJob {
   float *L32 = nullptr, *S32 = nullptr, *LS32 = nullptr, *LSL32 = nullptr, *LSLSL32 = nullptr;
   bool mixed = false, fp32_phase = false, done = false;
   int mode = 0;  // FP32 product mode of the reduced-precision phase (1-4)
+  int mode2 = 0; // a second reduced-precision phase's mode (precision 5: 2), 0 for none
   int fp64_done = 0, n = 0, stop = 4, max_iter = 50, nfinal = 2, status = 0;
   double oldE = 1.0e30, lastE = 0.0, lastNe = 0.0, lastC = 0.0, t0 = 0.0;
   int slot = -1;
@@ -182,7 +183,7 @@ int McwEngine::add_one(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32
     if (!ctx->mults.count(m)) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: mult %d not registered", m);
   if (!ctx->ranges[su.range_L].has_pairs)
     return set_errorf(CQMM_ERR_STATE, "cqmm_mcweeny: range %d has no transpose pairs", su.range_L);
-  if (su.precision < 0 || su.precision > 4) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: precision %d", su.precision);
+  if (su.precision < 0 || su.precision > 5) return set_errorf(CQMM_ERR_ARG, "cqmm_mcweeny: precision %d", su.precision);
   // Systems of one batch have their own contexts (their temporaries follow
   // the context's LIFO rule); the second channel of a spin pair shares the first's.
   for (const auto &j : jobs_)
@@ -218,13 +219,15 @@ int McwEngine::add_one(cqmm_ctx *ctx, const cqmm_mcw_setup &su, int32_t H, int32
   // Product mode of the reduced-precision phase: precision 1 -> FP32 MFMA with
   // FP64 sums, 2 -> FP32 sums, 3 -> FP32 chunk runs, 4 -> split-BF16 x3 where
   // all three products' kernels have it (blocks of 9 to 16 or class tiles),
-  // else FP32 MFMA with FP64 sums (e.g. 4x4 blocks).
-  j->mode = su.precision == 1 ? 2 : su.precision == 2 ? 1 : su.precision;
+  // else FP32 MFMA with FP64 sums (e.g. 4x4 blocks); 5 -> split-BF16 x3, then
+  // a second phase in FP32 MFMA with FP64 sums before the FP64 steps.
+  j->mode = su.precision == 1 ? 2 : su.precision == 2 ? 1 : su.precision == 5 ? 4 : su.precision;
   if (j->mode == 4)
     for (int32_t m : {su.mult_L_S_LS, su.mult_LS_L_LSL, su.mult_LSL_SL_L}) {
       const TripletPlan &tp = ctx->mults.at(m).plan->tp;
       if (!uses_tiles(tp) || !tiles_take_bf16x3(tp.tiles)) j->mode = 2;
     }
+  j->mode2 = su.precision == 5 && j->mode == 4 ? 2 : 0;
   j->fp32_phase = j->mixed;
   j->max_iter = su.max_iter > 0 ? su.max_iter : 50;
   j->nfinal = su.n_final_fp64 > 0 ? su.n_final_fp64 : 2;
@@ -479,7 +482,16 @@ int McwEngine::after_step(McwJob &j) {
   if (j.fp32_phase) {
     const bool flat = std::fabs(omega - j.oldE) < j.su.fp32_tol * std::max(1.0, std::fabs(omega));
     if (why || flat) {
-      for (int k = 0; k < nu; ++k) u[k]->fp32_phase = false;
+      if (j.mode2 && j.mode != j.mode2) {
+        // The next reduced-precision phase (its S copy in its operand format),
+        // from the last accepted L or this step's.
+        for (int k = 0; k < nu; ++k) {
+          u[k]->mode = u[k]->mode2;
+          MTRY(op_to_f32(hlen(u[k]->ctx, u[k]->S), dptr(u[k]->ctx, u[k]->S), u[k]->S32, s_));
+        }
+      } else {
+        for (int k = 0; k < nu; ++k) u[k]->fp32_phase = false;
+      }
       if (!(why == 1 || why == 3)) MTRY(keep());
       j.oldE = 1.0e30 * nu;
       handled = true;
@@ -579,7 +591,7 @@ void McwEngine::drop_graph() {
 // system admitted after one finishes) and its precision phase.
 std::string McwEngine::key() const {
   std::string k;
-  for (const auto &jp : jobs_) k += std::to_string(jp->serial) + (jp->fp32_phase ? "f" : "d") + ",";
+  for (const auto &jp : jobs_) k += std::to_string(jp->serial) + (jp->fp32_phase ? (char)('0' + jp->mode) : 'd') + ",";
   return k;
 }
 

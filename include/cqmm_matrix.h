@@ -119,7 +119,9 @@ typedef struct cqmm_mcw_setup {
                               15 products) and the chunks in FP64, 4 split-BF16 x3 products
                               (operands split into two BF16 halves, hi.hi + hi.lo + lo.hi
                               per block product in FP32, FP64 sums) where all three plans'
-                              kernels have them (blocks 9 to 16 or class tiles), else 1 */
+                              kernels have them (blocks 9 to 16 or class tiles), else 1;
+                              5 as 4, then a second reduced-precision phase as 1 (from the
+                              first phase's end to its own stop test), then FP64 */
   int32_t n_final_fp64;    /* FP64 iterations after the FP32 phase (>= 2)              */
   double fp32_tol;         /* FP32 phase ends when |dE| < fp32_tol (Ha, per spin)      */
 } cqmm_mcw_setup;

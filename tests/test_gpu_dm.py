@@ -246,6 +246,31 @@ def test_replacing_a_range_in_use_is_refused():
         ctx.add_range(case.idb)
 
 
+def test_bf16_then_fp32_mcweeny_phases_against_fp64_oracle():
+    """Precision 5: a split-BF16 x3 phase, then an FP32 MFMA phase with FP64
+    sums from where the first stopped, then FP64 steps; against the oracle's
+    FP64 McWMin from the same L0 (4-atom 13-SF cell at 8/8/12 A): within
+    1e-6 Ha/atom and 1e-8 electrons, with at least the two final FP64 steps."""
+    xyz, edge = cqmm.random_structure(4, 0.05, 2.0, 11)
+    pu = Purifier(xyz, edge, 13, r_S=8.0, r_H=8.0, r_L=12.0, seed=11)
+    sp = mo.Sparse(pu.sys, pu.rid, ompgemm=True)
+    pu.hotelling()
+    pu.init_mcweeny()
+    L0 = pu.ctx.get(pu.matL)
+    print("precision 5 McWeeny vs oracle: 4 atoms x 13", flush=True)
+    r = pu.mcweeny(precision=5, max_iter=100, fp32_tol=0.0)
+    pu.ctx.set(pu.matL, L0)
+    r4 = pu.mcweeny(precision=4, max_iter=100, fp32_tol=0.0)
+    L_ref, E_ref, Ne_ref, n_ref = sp.mcwmin(L0, pu.S_vals, pu.H_vals, 100)
+    dE = abs(2.0 * r["energy"] - 2.0 * E_ref) / pu.natoms
+    dN = abs(2.0 * r["electrons"] - 2.0 * Ne_ref)
+    print(f"precision 5: {r['fp32_iterations']} reduced + {r['iterations'] - r['fp32_iterations']} FP64 steps "
+          f"(precision 4: {r4['fp32_iterations']} + {r4['iterations'] - r4['fp32_iterations']}); dE/atom {dE:.2e}, "
+          f"dNe {dN:.2e}", flush=True)
+    assert r["fp32_iterations"] >= r4["fp32_iterations"] and r["iterations"] - r["fp32_iterations"] >= 2
+    assert dE <= 1e-6 and dN <= 1e-8
+
+
 @pytest.mark.parametrize("tiles", ["fixed", "class"])
 def test_split_bf16_mcweeny_error_against_fp64_oracle(tiles, monkeypatch):
     """Split-BF16 x3 bulk products (precision 4: the fixed-size 13x13 tile
