@@ -396,10 +396,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-purification", action="store_true", help="default run: skip the purification pair")
     ap.add_argument("--mixed-precision", type=int, default=1,
-                    help="precision of the served stream's mixed solves (1 FP32 MFMA + FP64 sums, 4 split-BF16 x3)")
+                    help="precision of the served stream's mixed solves (1 FP32 MFMA + FP64 sums, 4 split-BF16 x3, "
+                         "5 split-BF16 x3 then FP32 MFMA + FP64 sums)")
     ap.add_argument("--precisions", default="0,1,4",
-                    help="purification solves: 0 all FP64, 1 FP32 MFMA + FP64 sums, 4 split-BF16 x3 (comma list)")
-    ap.add_argument("--symmetric-precisions", default="0,4",
+                    help="purification solves: 0 all FP64, 1 FP32 MFMA + FP64 sums, 4 split-BF16 x3, 5 split-BF16 x3 "
+                         "then FP32 MFMA + FP64 sums (comma list)")
+    ap.add_argument("--symmetric-precisions", default="0,4,5",
                     help="purification: also solve with symmetric-result plans for LSL and LSLSL in these precisions")
     ap.add_argument("--serve-symmetric", type=int, default=0,
                     help="served stream: batched modes use symmetric-result plans for LSL and LSLSL")
