@@ -23,7 +23,9 @@ typedef struct cqmm_purify_params {
   double density, dmin;     /* structure: atoms/A^3 and minimum separation    */
   double s_amp, s_decay;    /* S off-diagonal amplitude and decay length      */
   double h_amp, h_decay, h_onsite, h_split, h_shift;  /* H recipe             */
-  int32_t precision;        /* 0 FP64; 1 FP32 MFMA + FP64 block sums; 2 FP32 sums */
+  int32_t precision;        /* cqmm_mcw_setup.precision: 0 FP64; 1 FP32 MFMA + FP64 block
+                               sums; 2 FP32 sums; 3 FP32 chunk runs; 4 split-BF16 x3; 5 split-
+                               BF16 x3, then FP32 MFMA + FP64 sums; FP64 final steps   */
   int32_t max_iter;         /* McWeeny iterations at most                     */
   double fp32_tol;          /* mixed: FP32 phase also ends when |dE| < fp32_tol*|E|
                                (0: only the McWeeny stop tests end it)        */
