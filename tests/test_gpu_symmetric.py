@@ -158,3 +158,27 @@ def test_symmetric_split_bf16_mcweeny_against_fp64_oracle():
     dN = abs(2.0 * r["electrons"] - 2.0 * Ne_ref)
     print(f"symmetric split-BF16 McWeeny: dE/atom {dE:.2e}, dNe {dN:.2e}", flush=True)
     assert r["fp32_iterations"] > 0 and dE <= 1e-6 and dN <= 1e-8
+
+
+def test_morton_tile_rows_give_the_same_products(monkeypatch):
+    """Full plans with tile rows in the columns' Morton order
+    (CQMM_TILE_ROWS=morton): other tiles, the same products in the same order
+    for every output block, so the result is the primary-order plan's bit for
+    bit (both product types, species mix)."""
+    nsf = np.random.default_rng(5).choice([4, 9, 13], size=48).astype(np.int32)
+    pu = system(nsf)
+    R, c = pu.rid, pu.ctx
+    monkeypatch.setenv("CQMM_TILE_ROWS", "morton")
+    c.add_mult(42, 1, R["LS"], R["L"], R["LSL"])
+    c.add_mult(43, 2, R["L"], R["LS"], R["LSL"])
+    monkeypatch.delenv("CQMM_TILE_ROWS")
+    L = symmetric_L(pu)
+    LS, A, B = c.allocate(R["LS"]), c.allocate(R["LSL"]), c.allocate(R["LSL"])
+    X, Y = c.allocate(R["L"]), c.allocate(R["L"])
+    c.product(L, pu.matS, LS, L_S_LS)
+    c.product(LS, L, A, LS_L_LSL)
+    c.product(LS, L, B, 42)
+    np.testing.assert_array_equal(c.get(A), c.get(B))
+    c.product(A, LS, X, LSL_SL_L)
+    c.product(A, LS, Y, 43)
+    np.testing.assert_array_equal(c.get(X), c.get(Y))
