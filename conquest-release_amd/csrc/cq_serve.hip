@@ -314,7 +314,12 @@ int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_confi
       ++n_done;
       log_done(i, how);
     };
-    // Next admissible request (FIFO; a request that can never fit fails at once).
+    // Next admissible request: the oldest, if it fits the budget; if not,
+    // the first of the next kLookahead that fits (backfill), at most
+    // kMaxBypass times while the same request waits at the head, after which
+    // admission waits for it. A request that can never fit fails at once.
+    constexpr int32_t kLookahead = 64, kMaxBypass = 64;
+    int32_t bypassed = 0;
     auto admit_locked = [&]() -> int32_t {
       while (!pending.empty()) {
         const int32_t h = pending.front();
@@ -325,12 +330,27 @@ int cqmm_serve(const cqmm_serve_request *reqs, int32_t n, const cqmm_serve_confi
           in_use += est[h];
           finish_locked(h, "refused");
           cv.notify_all();
+          bypassed = 0;
           continue;
         }
-        if (in_use + est[h] > cfg->hbm_budget) return -1;
-        pending.pop_front();
-        in_use += est[h];
-        return h;
+        if (in_use + est[h] <= cfg->hbm_budget) {
+          pending.pop_front();
+          in_use += est[h];
+          bypassed = 0;
+          return h;
+        }
+        if (bypassed >= kMaxBypass) return -1;
+        const int32_t look = std::min<int32_t>((int32_t)pending.size(), kLookahead + 1);
+        for (int32_t k = 1; k < look; ++k) {
+          const int32_t r = pending[k];
+          if (est[r] <= cfg->hbm_budget && in_use + est[r] <= cfg->hbm_budget) {
+            pending.erase(pending.begin() + k);
+            in_use += est[r];
+            ++bypassed;
+            return r;
+          }
+        }
+        return -1;
       }
       return -1;
     };

@@ -5,8 +5,9 @@ bit for bit; in the request loop, with requests solved on their own slots or
 batched (McWeeny steps of all prepared requests in lockstep, grouped product
 launches, hipGraph replay), every result equals the same request solved alone
 (bitwise: each request's arithmetic is deterministic and independent of what
-else runs), requests overlap on the device, admission respects the HBM budget,
-and a request that cannot fit fails alone. A Poisson stream shaped like the
+else runs), requests overlap on the device, admission respects the HBM budget
+(a smaller request behind one that does not fit is admitted meanwhile), and a
+request that cannot fit fails alone. A Poisson stream shaped like the
 served workload (log-uniform atom counts, 4 or 13 SF, S/H/L 8/8/12 A and its
 S/H recipe, mixed precision) holds every electron count to 1e-8, and its
 smallest request matches the oracle's FP64 solve to 1e-6 Ha/atom.
@@ -81,6 +82,20 @@ def test_admission_respects_hbm_budget_and_isolates_failures():
     assert res[3]["status"] == -3  # CQMM_ERR_NOMEM: never admitted, the others unaffected
     iv = sorted((m["admit_s"], m["done_s"]) for m in res[:3])
     assert all(iv[k + 1][0] >= iv[k][1] - 1e-3 for k in range(2))
+
+
+def test_admission_backfills_a_smaller_request():
+    """Two large requests that do not fit together and a small one behind them:
+    the small one is admitted while the first large one runs (bounded
+    backfill), the second large one when the first is done."""
+    params = sv.default_params(precision=0, **SMALL)
+    eb, es = sv.hbm_estimate(48, 4, params), sv.hbm_estimate(12, 4, params)
+    assert es + 1 < eb
+    reqs = [dict(natoms=n, nsf=4, seed=s, arrival_s=0.0) for n, s in ((48, 1), (48, 2), (12, 3))]
+    res, wall = sv.serve(reqs, slots=3, hbm_budget=eb + es + 1, params=params)
+    assert all(m["status"] == 0 for m in res)
+    assert res[2]["admit_s"] < res[1]["admit_s"]
+    assert res[1]["admit_s"] >= res[0]["done_s"] - 1e-3
 
 
 def _workload_stream():
