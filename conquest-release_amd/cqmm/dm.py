@@ -120,7 +120,8 @@ class Context:
 
     @product_precision.setter
     def product_precision(self, precision):
-        """0 FP64; 1, 2, 3 FP32 MFMA products (FP32 sums, FP64 block sums, FP32 chunk runs)."""
+        """0 FP64; 1, 2, 3 FP32 MFMA products (FP32 sums, FP64 block sums, FP32 chunk runs); 4 split-BF16 x3
+        products where the plan's kernel has them (else as 2)."""
         check(lib().cqmm_ctx_set_product_precision(self.h, int(precision)), "cqmm_ctx_set_product_precision")
 
 
@@ -271,6 +272,9 @@ class Purifier:
                         LS_L_LSL, LSL_SL_L, max_iter, precision, n_final_fp64, fp32_tol)
 
     def mcweeny(self, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6):
+        """McWMin from the current L (cqmm_mcweeny). precision: 0 all FP64; 1 FP32 MFMA with FP64 block
+        sums, 2 FP32 sums, 3 FP32 chunk runs, 4 split-BF16 x3, 5 split-BF16 x3 then FP32 MFMA with FP64
+        sums, each followed by FP64 steps (at least n_final_fp64)."""
         su = McwSetup(self.rid["L"], self.rid["S"], self.rid["H"], self.rid["LS"], self.rid["LSL"], L_S_LS, LS_L_LSL,
                       LSL_SL_L, max_iter, precision, n_final_fp64, fp32_tol)
         res = McwResult()
@@ -345,7 +349,7 @@ class Purifier:
 
     def dmm(self, product_precision=0, **kw):
         """FindMinDM from the current L; K receives 3LSL - 2LSLSL. product_precision
-        (0 FP64; 1, 2, 3 FP32 modes) applies to the bulk products of the
+        (0 FP64; 1, 2, 3 FP32 modes; 4 split-BF16 x3) applies to the bulk products of the
         minimisation; the electron-number correction and the final K, energy and
         electron count use FP64 products."""
         su = self.dmm_setup(**kw)
