@@ -378,8 +378,6 @@ int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::
     return (((uint64_t)(uint32_t)gj * 64 + (uint64_t)(sh[0] + 32)) * 64 + (uint64_t)(sh[1] + 32)) * 64 +
            (uint64_t)(sh[2] + 32);
   };
-  std::vector<int32_t> prim_of(na, -1);
-  for (int32_t pp = 0; pp < na; ++pp) prim_of[s.prim_gid[pp]] = pp;
   // Per row: (key of the block's column atom and shift, block), sorted by key.
   std::vector<std::pair<uint64_t, int64_t>> rowkey(nb);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -399,8 +397,7 @@ int transpose_pairs(const System &s, int32_t rid, std::vector<int64_t> &p, std::
       const int32_t m = r.nb_cover[e];
       const int32_t *sh = &s.cs_shift[3 * s.c_part[m]];
       const int32_t neg[3] = {-sh[0], -sh[1], -sh[2]};
-      const int32_t pj = prim_of[s.prim_gid[s.c_prim[m]]];
-      if (pj < 0) continue;
+      const int32_t pj = s.c_prim[m];  // the row of the column atom
       const uint64_t k = key(gi, neg);
       auto b = rowkey.begin() + r.row_beg[pj], en = rowkey.begin() + r.row_beg[pj + 1];
       auto it = std::lower_bound(b, en, std::make_pair(k, (int64_t)-1));
