@@ -29,7 +29,9 @@ iteration (src/S_matrix_module.f90:746-1034):
     populations converging to each spin's own S-metric projector; variable
     populations converging to the lowest ne_up + ne_dn levels of both spectra
     together (electrons move to the lower spin); a hand-worked two-level known
-    answer of the direct-sum mubar and lambda (tests/test_oracle_mcweeny_spin.py).
+    answer of the direct-sum mubar and lambda; per-spin overlaps (spin-dependent
+    support functions) reducing to the shared one and converging to each
+    spin's projector in its own metric (tests/test_oracle_mcweeny_spin.py).
 """
 from __future__ import annotations
 
@@ -359,7 +361,9 @@ class Sparse:
         return float(np.dot(botH, H)), float(np.dot(top, S))
 
     def mcwmin_spin(self, Ls, S, Hs, max_iter=100, fixed=False):
-        """McWMin for two spin channels sharing S (src/McWeeny.f90:144-256).
+        """McWMin for two spin channels (src/McWeeny.f90:144-256). S is one
+        overlap for both spins, or a pair (S_up, S_dn) when the support
+        functions depend on spin (flag_SpinDependentSF: matS(spin)).
 
         fixed=True (fixed spin populations): McW_matrix_multiply_nonds per
         spin, each spin's stop test marks it done, the loop ends when both are
@@ -368,6 +372,7 @@ class Sparse:
         (c2_up + c2_dn) for both spins (spin_factor 1), one stop test on the
         total energy. Fewer than 3 steps restore both L. Returns
         (Ls, energies per spin, electrons per spin, steps)."""
+        Ss = list(S) if isinstance(S, (list, tuple)) else [S, S]
         Ls = [L.copy() for L in Ls]
         Lold = [L.copy() for L in Ls]
         E = [None, None]
@@ -377,7 +382,7 @@ class Sparse:
             oldE = [1e30, 1e30]
             done = [False, False]
             for n in range(1, max_iter + 1):
-                res = [self.step(Ls[s], S, Hs[s]) for s in range(2)]
+                res = [self.step(Ls[s], Ss[s], Hs[s]) for s in range(2)]
                 for s in range(2):
                     _, om, _, c = res[s]
                     if om > oldE[s] or abs(om - oldE[s]) < 1e-8 or c > 1 or c < 0:
@@ -393,14 +398,14 @@ class Sparse:
         else:
             oldE = 2e30
             for n in range(1, max_iter + 1):
-                parts = [self.step_traces(Ls[s], S) for s in range(2)]
+                parts = [self.step_traces(Ls[s], Ss[s]) for s in range(2)]
                 c1 = parts[0][2] + parts[1][2]
                 c2 = parts[0][3] + parts[1][3]
                 if not abs(c2) > 1e-15:
                     raise ZeroDivisionError("McW_matrix_multiply_ds: c2 is zero")
                 c = c1 / c2
                 rhos = [self.rho_of(Ls[s], parts[s][0], parts[s][1], c) for s in range(2)]
-                tr = [self.traces_of(rhos[s], S, Hs[s]) for s in range(2)]
+                tr = [self.traces_of(rhos[s], Ss[s], Hs[s]) for s in range(2)]
                 om = tr[0][0] + tr[1][0]
                 if om > oldE or abs(om - oldE) < 1e-8 or c > 1 or c < 0:
                     break

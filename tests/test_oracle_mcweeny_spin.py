@@ -165,3 +165,28 @@ def test_variable_spin_init_known_answer():
     assert abs(mubar - 3.0) < 1e-14 and abs(lam - 2.0 / 3.0) < 1e-14
     np.testing.assert_allclose(np.diag(L0s[0]), [5 / 6, 1 / 2], atol=1e-14)
     np.testing.assert_allclose(np.diag(L0s[1]), [2 / 3, 0.0], atol=1e-14)
+
+
+def test_spin_dependent_overlaps():
+    """S given per spin: the same S twice is the shared-S loop bit for bit; with
+    a different (well-conditioned) overlap for spin down, fixed populations
+    converge to each spin's projector in its own metric."""
+    pu, sp, S, Hu, Hd, Hd_vals, T, bL = setup()
+    ne = pu.n_orb // 2
+    L0, _, _ = l0(Hu, S, T, ne, pu, bL)
+    a = sp.mcwmin_spin([L0, L0], pu.S_vals, [pu.H_vals, Hd_vals], 60, fixed=True)
+    b = sp.mcwmin_spin([L0, L0], (pu.S_vals, pu.S_vals), [pu.H_vals, Hd_vals], 60, fixed=True)
+    assert a[3] == b[3] and a[1] == b[1]
+    np.testing.assert_array_equal(a[0][1], b[0][1])
+    # Spin down with S' = S + 0.02 (S - I) (same pattern, still positive definite).
+    bS = pu.sys.blocks(pu.rid["S"])
+    S2 = S + 0.02 * (S - np.eye(S.shape[0]))
+    S2_vals = mo.sparsify(S2, bS, pu.nsf, pu.sys.length(pu.rid["S"]))
+    L0d, _, _ = mo.init_mcw_dense(Hd, S2, np.linalg.inv(S2), ne, pu.nsf)
+    L0d = mo.sparsify(L0d, bL, pu.nsf, pu.sys.length(pu.rid["L"]))
+    Ls, E, Ne, n = sp.mcwmin_spin([L0, L0d], (pu.S_vals, S2_vals), [pu.H_vals, Hd_vals], 300, fixed=True)
+    Pu, _ = mo.projector(Hu, S, ne)
+    Pd, _ = mo.projector(Hd, S2, ne)
+    np.testing.assert_allclose(mo.densify(bL, Ls[0], pu.nsf), Pu, atol=2e-5)
+    np.testing.assert_allclose(mo.densify(bL, Ls[1], pu.nsf), Pd, atol=2e-5)
+    assert abs(Ne[1] - ne) < 1e-6
