@@ -305,12 +305,14 @@ class Purifier:
                                            self._pair(self.matL, self.matL_dn), ne, ml), "cqmm_init_mcweeny_spin")
         return (ml[0], ml[1]), (ml[2], ml[3])
 
-    def mcweeny_spin(self, fixed, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6):
-        """McWMin over both channels: [result of spin up, result of spin down]."""
+    def mcweeny_spin(self, fixed, precision=0, max_iter=100, n_final_fp64=2, fp32_tol=1e-6, matS_dn=None):
+        """McWMin over both channels: [result of spin up, result of spin down].
+        matS_dn: spin down's own overlap (spin-dependent support functions)."""
         su = self.mcw_setup(precision, max_iter, n_final_fp64, fp32_tol)
         res = (McwResult * 2)()
+        s_dn = self.matS if matS_dn is None else matS_dn
         check(lib().cqmm_mcweeny_spin(self.ctx.h, C.byref(su), int(bool(fixed)), self._pair(self.matH, self.matH_dn),
-                                      self._pair(self.matS, self.matS), self._pair(self.matL, self.matL_dn), res),
+                                      self._pair(self.matS, s_dn), self._pair(self.matL, self.matL_dn), res),
               "cqmm_mcweeny_spin")
         return [mcw_result_dict(res[0]), mcw_result_dict(res[1])]
 

@@ -9,7 +9,8 @@ spin-unpolarised device loop bit for bit (fixed populations: every step;
 variable populations: c1 and c2 both double, so c is the same); the same
 against the oracle with symmetric-result plans for LSL and LSLSL; with spin down
 shifted above the gap the variable-population loop moves electrons to spin up
-and ends on the lowest levels of both spectra; mixed precision on a
+and ends on the lowest levels of both spectra; spin-dependent overlaps (one S
+per spin) match the oracle's per-spin-metric loop; mixed precision on a
 variable-population pair stays within 1e-6 Ha/atom of the FP64 oracle.
 """
 import numpy as np
@@ -114,6 +115,36 @@ def test_identical_spin_channels_match_the_unpolarised_device_loop(fixed):
         np.testing.assert_array_equal(pu.ctx.get(pu.matL), L1)
         np.testing.assert_array_equal(pu.ctx.get(pu.matL_dn), L1)
         assert res[0]["energy"] == single["energy"] and res[1]["energy"] == single["energy"]
+
+
+def test_spin_dependent_overlaps_against_oracle():
+    """Spin down with its own overlap S' (matS[1] != matS[0], spin-dependent
+    support functions): fixed populations against the oracle's two-channel
+    McWMin with per-spin overlaps."""
+    pu, S, Hu, Hd = spin_cluster(shift=4.0)
+    ne = pu.n_orb // 2
+    bS, bL = pu.sys.blocks(pu.rid["S"]), pu.sys.blocks(pu.rid["L"])
+    S2 = S + 0.02 * (S - np.eye(S.shape[0]))
+    S2_vals = mo.sparsify(S2, bS, pu.nsf, pu.sys.length(pu.rid["S"]))
+    matS2 = pu.ctx.allocate(pu.rid["S"])
+    pu.ctx.set(matS2, S2_vals)
+    T = dense(pu, "T", pu.ctx.get(pu.matT))
+    L0u, _, _ = mo.init_mcw_dense(Hu, S, T, ne, pu.nsf)
+    L0d, _, _ = mo.init_mcw_dense(Hd, S2, np.linalg.inv(S2), ne, pu.nsf)
+    L0 = [mo.sparsify(L0u, bL, pu.nsf, pu.sys.length(pu.rid["L"])),
+          mo.sparsify(L0d, bL, pu.nsf, pu.sys.length(pu.rid["L"]))]
+    pu.ctx.set(pu.matL, L0[0])
+    pu.ctx.set(pu.matL_dn, L0[1])
+    print(f"spin-dependent overlaps: {pu.natoms} atoms", flush=True)
+    res = pu.mcweeny_spin(True, precision=0, max_iter=300, matS_dn=matS2)
+    sp = mo.Sparse(pu.sys, pu.rid)
+    L_ref, E_ref, Ne_ref, n_ref = sp.mcwmin_spin(L0, (pu.S_vals, S2_vals), [pu.H_vals, pu.H_dn_vals], 300,
+                                                 fixed=True)
+    assert abs(res[0]["iterations"] - n_ref) <= 1
+    for k, h in enumerate((pu.matL, pu.matL_dn)):
+        assert abs(res[k]["energy"] - E_ref[k]) < 1e-8 * max(1.0, abs(E_ref[k]))
+        assert abs(res[k]["electrons"] - Ne_ref[k]) < 1e-8
+        assert np.abs(pu.ctx.get(h) - L_ref[k]).max() < 1e-8
 
 
 def test_variable_spin_mixed_precision_against_fp64_oracle():
